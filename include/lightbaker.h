@@ -1,0 +1,176 @@
+/*
+ * lightbaker_amd: the ray-cast bake of jer-nc/lightbaker on one AMD MI355X (gfx950).
+ *
+ * This header is the drop-in boundary. The first group of entry points replaces the wgpu
+ * `Gpu` of the reference (crates/lightbaker/src/gpu/mod.rs); a Rust crate would bind it with
+ * `extern "C"` declarations over the same #[repr(C)] records. Every record here is byte-identical
+ * to the reference's Pod type it names (little-endian, no padding).
+ *
+ * Conventions: a return of 0 is success; any other value is an error whose text
+ * lb_last_error() returns (thread-local, valid until the next call on the same thread).
+ * Handles are owned by the library; input arrays are borrowed for the duration of a call;
+ * output arrays are allocated by the caller. Calls on one lb_gpu are serialized by the caller,
+ * as the reference's single-threaded driver does (bake.rs:151,302, volume.rs:77).
+ */
+#ifndef LIGHTBAKER_H
+#define LIGHTBAKER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- records (byte-identical to the reference's Pod types) ---------------------------------- */
+
+/* bvh.rs:12-22 `Node` (32 B): box; inner: first child (the second follows it); leaf: first
+ * triangle and count (count 0: inner node). */
+typedef struct lb_node {
+    float min[3];
+    uint32_t first;
+    float max[3];
+    uint32_t count;
+} lb_node;
+
+/* bvh.rs:24-33 `Tri` (48 B): v0 (w: material index as bits), e1 = v1 - v0 (w: chart index as
+ * bits, 0xffffffff none), e2 = v2 - v0 (w: 0). */
+typedef struct lb_tri {
+    float v0[4];
+    float e1[4];
+    float e2[4];
+} lb_tri;
+
+/* raster.rs:13-22 `Texel` (64 B): pos (w: 1 covered), normal (w: chart index as bits; for a
+ * volume probe the probe's index as bits), dx, dy (metres per texel along x and y). */
+typedef struct lb_texel {
+    float pos[4];
+    float normal[4];
+    float dx[4];
+    float dy[4];
+} lb_texel;
+
+/* gpu/mod.rs:11-23 `Params` (144 B), filled as bake.rs:109-120 does. counts = (sun grid side g,
+ * ao rays, gather or volume rays, items); reach = (ao reach, gather reach, lift, seed);
+ * atlas = (width, height, first item, inside share as f32 bits). counts[3] and atlas[2] are
+ * written by the library per launch, as Gpu::run does (gpu/mod.rs:153-154). */
+typedef struct lb_params {
+    float sun_dir[4];
+    float sun_irr[4];
+    float zenith[4];
+    float horizon[4];
+    float ground[4];
+    float fallback[4];
+    uint32_t counts[4];
+    float reach[4];
+    uint32_t atlas[4];
+} lb_params;
+
+/* gpu/mod.rs:25-33 `Pass`; the discriminants index the pipelines as in the reference. */
+enum lb_pass { LB_PASS_DIRECT = 0, LB_PASS_GATHER = 1, LB_PASS_VOLUME = 2 };
+
+/* Kernel form. PLAIN: one item per lane, the reference's serial ray loop, FP32 node records.
+ * TUNED: rays spread over the lanes of a wave with Kahan-compensated wave reductions, an LDS
+ * traversal stack, and the node/triangle layout chosen by lb_set_tuning. */
+enum lb_form { LB_FORM_PLAIN = 0, LB_FORM_TUNED = 1 };
+
+/* Node layout for the TUNED form. WIDE: both children's FP32 boxes stored in the parent (64 B
+ * per inner node visit instead of the reference's 96 B). QUANT: children's boxes as 8-bit
+ * offsets in the parent's box, rounded outward (32 B per visit). */
+enum lb_nodes { LB_NODES_WIDE = 0, LB_NODES_QUANT = 1 };
+
+typedef struct lb_gpu lb_gpu;
+typedef struct lb_scene lb_scene;
+
+/* ---- the reference's Gpu ---------------------------------------------------------------------- */
+
+/* Gpu::new (gpu/mod.rs:74-127): opens device 0 and loads the kernels. Errors: "no GPU adapter"
+ * when no HIP device exists, "no GPU device" when it cannot be opened. */
+int lb_open(lb_gpu **out);
+/* Gpu.adapter (gpu/mod.rs:62,126): "<device name> (HIP <arch>)". */
+const char *lb_adapter_name(const lb_gpu *gpu);
+void lb_close(lb_gpu *gpu);
+const char *lb_last_error(void);
+
+/* Gpu::scene (gpu/mod.rs:129-136): uploads the BVH, the materials (albedo rgb, opacity) and the
+ * charts (two affine rows a chart). No charts becomes one zero chart, as the reference does.
+ * Also derives the TUNED form's node layouts from the same BVH. */
+int lb_scene_create(lb_gpu *gpu, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+                    size_t n_tris, const float *materials /* [n_materials][4] */,
+                    size_t n_materials, const float *charts /* [n_charts][2][4] */,
+                    size_t n_charts, lb_scene **out);
+void lb_scene_destroy(lb_scene *scene);
+
+/* Gpu::run (gpu/mod.rs:140-196): runs `pass` over `n` items (texels, or probes for the volume)
+ * reading `light` (rgba per atlas texel; none becomes one zero texel), writes n * width vec4s in
+ * item order to `out` (width 4 for the volume, else 1), and returns when they are there. Item i's
+ * ray keys derive from i (its index in `items`), exactly as the reference's batched `first + i`.
+ * Zero items is a no-op. `out_cap` is the number of vec4s `out` holds. */
+int lb_run(lb_gpu *gpu, int pass, const lb_scene *scene, const lb_params *params,
+           const lb_texel *items, size_t n, const float *light /* [n_light][4] */, size_t n_light,
+           float *out /* [out_cap][4] */, size_t out_cap);
+
+/* ---- device-resident extensions (no counterpart in the reference) -------------------------- */
+
+/* Device buffers owned by the library, for callers that keep inputs resident in HBM. */
+int lb_buffer_alloc(lb_gpu *gpu, size_t bytes, void **dev_out);
+int lb_buffer_free(lb_gpu *gpu, void *dev);
+int lb_upload(lb_gpu *gpu, void *dev_dst, const void *host_src, size_t bytes);
+int lb_download(lb_gpu *gpu, void *host_dst, const void *dev_src, size_t bytes);
+int lb_sync(lb_gpu *gpu);
+
+/* As lb_run over device pointers, queued on the gpu's stream without waiting. `key_base` is added
+ * to every item's index to form its ray key, so a slice of a longer item list keeps the keys of
+ * the whole list (gpu/mod.rs:153-154 `atlas.z = first`). */
+int lb_run_device(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_params *params,
+                  const lb_texel *items_dev, size_t n, uint32_t key_base, const float *light_dev,
+                  size_t n_light, float *out_dev);
+
+/* Tuning of the TUNED form: workgroup size (64..1024, multiple of 64), rays per lane (1..8), LDS
+ * stack depth (8..48 entries; deeper entries spill to a per-lane scratch area) and node layout. */
+int lb_set_tuning(lb_gpu *gpu, int workgroup, int rays_per_lane, int lds_stack, int nodes);
+int lb_get_tuning(const lb_gpu *gpu, int *workgroup, int *rays_per_lane, int *lds_stack,
+                  int *nodes);
+
+/* Traversal counters of the last lb_run_stats call: rays traced, inner-node visits (child-box
+ * pairs tested), triangle tests, material loads and bounced() light gathers. */
+typedef struct lb_stats {
+    uint64_t rays;
+    uint64_t inner_visits;
+    uint64_t tri_tests;
+    uint64_t material_loads;
+    uint64_t light_gathers;
+} lb_stats;
+
+/* As lb_run_device with the traversal counted (a separate instantiation of the kernels; the
+ * counting adds atomics, so it is not the timed form). Waits for completion. */
+int lb_run_stats(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_params *params,
+                 const lb_texel *items_dev, size_t n, uint32_t key_base, const float *light_dev,
+                 size_t n_light, float *out_dev, lb_stats *stats);
+
+/* Times the last `reps` launches queued by lb_run_device on the gpu's stream with HIP events:
+ * lb_timer_begin records a start event, lb_timer_end a stop event and waits; returns ms. */
+int lb_timer_begin(lb_gpu *gpu);
+int lb_timer_end(lb_gpu *gpu, float *ms);
+
+/* Selects the form lb_run uses (LB_FORM_PLAIN by default). */
+int lb_set_form(lb_gpu *gpu, int form);
+
+/* ---- host substrate (the reference's Rust host, restated in C++) ------------------------------ */
+
+typedef struct lb_bvh lb_bvh;
+
+/* Bvh::build (bvh.rs:50-106) over non-indexed triangles: positions [n_tris][3][3], each
+ * triangle's material and chart (0xffffffff: none); triangles whose material does not cast are
+ * left out, as are degenerate ones. */
+int lb_bvh_build(const float *positions, size_t n_tris, const uint32_t *material,
+                 const uint32_t *chart, const uint8_t *casts, size_t n_materials, lb_bvh **out);
+size_t lb_bvh_nodes(const lb_bvh *bvh, const lb_node **nodes);
+size_t lb_bvh_tris(const lb_bvh *bvh, const lb_tri **tris);
+void lb_bvh_destroy(lb_bvh *bvh);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIGHTBAKER_H */

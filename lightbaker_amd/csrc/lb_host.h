@@ -1,0 +1,72 @@
+// Host-side state behind the C-ABI handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "lb_kernels.h"
+
+namespace lbk {
+
+int fail(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+
+// A device allocation that only grows.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    int reserve(size_t bytes);
+    int upload(const void *src, size_t bytes, hipStream_t stream);
+    void release();
+};
+
+}  // namespace lbk
+
+struct lb_gpu {
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string name;
+    unsigned long long *counters = nullptr;  // lb_stats block
+    uint32_t *queue = nullptr;                // TUNED: the work queue's head
+    size_t lds_per_block = 0;
+    lbk::DevBuf run_items, run_light, run_out;  // lb_run's staging
+    lbk::DevBuf zero;                           // one zero vec4 (no light)
+    lbk::DevBuf spill;                          // TUNED: stack entries beyond the LDS part
+    int form = LB_FORM_PLAIN;
+    lbk::Tuning tuning;
+};
+
+struct lb_scene {
+    lb_gpu *gpu = nullptr;  // valid during lb_scene_create only
+    int device = 0;
+    lbk::DevBuf nodes, tris, materials, charts;
+    lbk::DevBuf wide, quant, leaves;  // TUNED node layouts
+    uint32_t n_nodes = 0, n_tris = 0, n_materials = 0, n_charts = 0, n_inner = 0;
+    uint32_t root_ref = 0;  // the root as a TUNED child reference
+};
+
+extern "C" int lb_set_form(lb_gpu *gpu, int form);
+
+namespace lbk {
+
+bool validate_bvh(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, size_t n_tris,
+                  size_t n_materials, size_t n_charts, std::string *err);
+int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+                      size_t n_tris);
+SceneView scene_view(const lb_scene *s, const float *light);
+int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_params *params,
+             const lb_texel *items, size_t n, uint32_t key_base, const float *light,
+             size_t n_light, float *out, bool stats);
+bool is_empty_bvh(const lb_node *nodes, size_t n_nodes);
+hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
+                        const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
+                        float4 *out);
+
+}  // namespace lbk

@@ -1,0 +1,175 @@
+// Checks a BVH before any kernel walks it, and derives the TUNED form's node layouts from it.
+//
+// Both layouts are indexed by the reference node index and hold, for an inner node, the boxes of
+// its two children (first, first + 1) so one fetch serves the pair the reference reads as three
+// 32 B records (trace.wgsl:90,111-112):
+//   WIDE   4 x 16 B: a.min xyz a.max.x | a.max.yz b.min.xy | b.min.z b.max xyz | first, leaf flags
+//   QUANT  2 x 16 B: origin xyz, exponents | 12 bytes of children's boxes, first + leaf flags
+// QUANT boxes are 8-bit multiples of a power of two per axis above the parent's min corner, each
+// rounded outward and checked against the device's own decode (origin + q * 2^e in FP32), so
+// every decoded box contains its FP32 box and culling stays conservative.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lb_host.h"
+
+namespace lbk {
+
+namespace {
+inline uint32_t bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+inline float fl(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+}  // namespace
+
+bool is_empty_bvh(const lb_node *nodes, size_t n_nodes) {
+    return n_nodes == 1 && nodes[0].count == 0;
+}
+
+bool validate_bvh(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, size_t n_tris,
+                  size_t n_materials, size_t n_charts, std::string *err) {
+    char buf[256];
+    size_t charts = n_charts ? n_charts : 1;
+    for (size_t t = 0; t < n_tris; t++) {
+        uint32_t m = bits(tris[t].v0[3]), c = bits(tris[t].e1[3]);
+        if (m >= n_materials) {
+            snprintf(buf, sizeof buf, "triangle %zu: material %u of %zu", t, m, n_materials);
+            *err = buf;
+            return false;
+        }
+        if (c != 0xffffffffu && c >= charts) {
+            snprintf(buf, sizeof buf, "triangle %zu: chart %u of %zu", t, c, charts);
+            *err = buf;
+            return false;
+        }
+    }
+    if (is_empty_bvh(nodes, n_nodes)) return true;
+    // a tree: every inner node's children lie after it, every node but the root has one parent
+    std::vector<uint8_t> parents(n_nodes, 0);
+    for (size_t i = 0; i < n_nodes; i++) {
+        const lb_node &n = nodes[i];
+        if (n.count > 0) {
+            if ((uint64_t)n.first + n.count > n_tris) {
+                snprintf(buf, sizeof buf, "leaf %zu: triangles %u+%u of %zu", i, n.first, n.count,
+                         n_tris);
+                *err = buf;
+                return false;
+            }
+        } else {
+            if (n.first <= i || (uint64_t)n.first + 1 >= n_nodes) {
+                snprintf(buf, sizeof buf, "node %zu: children %u,%u of %zu", i, n.first,
+                         n.first + 1, n_nodes);
+                *err = buf;
+                return false;
+            }
+            if (parents[n.first]++ || parents[n.first + 1]++) {
+                snprintf(buf, sizeof buf, "node %zu: a child with two parents", i);
+                *err = buf;
+                return false;
+            }
+        }
+    }
+    for (size_t i = 1; i < n_nodes; i++)
+        if (parents[i] != 1) {
+            snprintf(buf, sizeof buf, "node %zu: not reached from the root", i);
+            *err = buf;
+            return false;
+        }
+    return true;
+}
+
+// The power-of-two step of a quantized axis and the outward-rounded 8-bit corners of a child.
+// Returns false if 255 steps do not reach (the caller widens the step).
+static bool quantize_axis(float origin, uint32_t ebits, float lo, float hi, uint8_t *qlo,
+                          uint8_t *qhi) {
+    float step = fl(ebits << 23);
+    auto dec = [&](int q) { return origin + (float)q * step; };
+    int a = (int)std::floor((lo - origin) / step);
+    if (a < 0) a = 0;
+    while (a > 0 && dec(a) > lo) a--;
+    if (dec(a) > lo) return false;
+    int b = (int)std::ceil((hi - origin) / step);
+    if (b < 0) b = 0;
+    while (b <= 255 && dec(b) < hi) b++;
+    if (b > 255) return false;
+    *qlo = (uint8_t)a;
+    *qhi = (uint8_t)b;
+    return true;
+}
+
+int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+                      size_t n_tris) {
+    (void)tris;
+    (void)n_tris;
+    std::vector<float> wide(n_nodes * 16, 0.0f);
+    std::vector<uint32_t> quant(n_nodes * 8, 0u);
+    std::vector<uint32_t> leafinfo(n_nodes * 2, 0u);
+    bool empty = is_empty_bvh(nodes, n_nodes);
+    for (size_t i = 0; i < n_nodes && !empty; i++) {
+        const lb_node &n = nodes[i];
+        if (n.count > 0) continue;
+        const lb_node &a = nodes[n.first], &b = nodes[n.first + 1];
+        uint32_t flags = (a.count > 0 ? 1u : 0u) | (b.count > 0 ? 2u : 0u);
+        float *w = &wide[i * 16];
+        const float vals[12] = {a.min[0], a.min[1], a.min[2], a.max[0], a.max[1], a.max[2],
+                                b.min[0], b.min[1], b.min[2], b.max[0], b.max[1], b.max[2]};
+        std::memcpy(w, vals, sizeof vals);
+        w[12] = fl(n.first);
+        w[13] = fl(flags);
+        // quantized: origin at the union's min corner, one exponent an axis
+        uint32_t *q = &quant[i * 8];
+        uint8_t qb[12];
+        uint32_t ex[3];
+        for (int ax = 0; ax < 3; ax++) {
+            float lo = std::fmin(a.min[ax], b.min[ax]);
+            float hi = std::fmax(a.max[ax], b.max[ax]);
+            float origin = lo;
+            float ext = hi - lo;
+            int e = ext > 0.0f ? (int)std::ceil(std::log2((double)ext / 250.0)) : -100;
+            if (e < -100) e = -100;
+            for (;; e++) {
+                uint32_t eb = (uint32_t)(e + 127);
+                if (quantize_axis(origin, eb, a.min[ax], a.max[ax], &qb[ax], &qb[3 + ax]) &&
+                    quantize_axis(origin, eb, b.min[ax], b.max[ax], &qb[6 + ax], &qb[9 + ax])) {
+                    ex[ax] = eb;
+                    break;
+                }
+                if (e > 120) return fail("scene: node %zu is too large to quantize", i);
+            }
+            q[ax] = bits(origin);
+        }
+        q[3] = ex[0] | (ex[1] << 8) | (ex[2] << 16);
+        std::memcpy(&q[4], qb, 12);
+        q[7] = n.first | (flags << 30);
+        if (n.first >= (1u << 30)) return fail("scene: %zu nodes exceed 30-bit indices", n_nodes);
+    }
+    // leaves: (first triangle, count) at the leaf's own index
+    for (size_t i = 0; i < n_nodes; i++) {
+        leafinfo[i * 2] = nodes[i].first;
+        leafinfo[i * 2 + 1] = nodes[i].count;
+    }
+    if (empty) {
+        // the reference's empty marker (bvh.rs:76-80): one leaf over the zero triangle, which no
+        // ray meets (its determinant is 0)
+        leafinfo[0] = 0;
+        leafinfo[1] = 1;
+    }
+    s->root_ref = (empty || nodes[0].count > 0) ? 0x80000000u : 0u;
+    hipStream_t st = s->gpu->stream;
+    if (s->wide.upload(wide.data(), wide.size() * 4, st)) return 1;
+    if (s->quant.upload(quant.data(), quant.size() * 4, st)) return 1;
+    if (s->leaves.upload(leafinfo.data(), leafinfo.size() * 4, st)) return 1;
+    return 0;
+}
+
+}  // namespace lbk

@@ -1,0 +1,588 @@
+// The TUNED form of the three ray passes (crates/lightbaker/src/gpu/trace.wgsl) for gfx950.
+//
+// Mapping: a wave64 takes work units from one global atomic queue until it is empty (a persistent
+// grid sized to the device's residency). A unit is 64 / L items; L lanes of the wave (a group,
+// L a power of two in 8..64) share one item and spread its rays over themselves, ray r on lane
+// r mod L, so a pass over uneven items keeps every lane busy. Each lane sums its rays with
+// Kahan-Neumaier compensation and the group combines the (sum, compensation) pairs with
+// ds_swizzle butterflies (xor 1..16) and one cross-half exchange (xor 32). Counts (shadow bits,
+// back faces, sun hits) are exact integers, so they equal the reference's whatever the order.
+//
+// Traversal: the reference's order exactly (near child first on ta <= tb, far child pushed,
+// 48 entries, a push beyond them dropped) over a node record that holds both children's boxes,
+// FP32 (WIDE, 64 B a visit) or 8-bit outward-rounded (QUANT, 32 B a visit). The stack's first
+// `lds_stack` entries live in LDS, laid out entry-major so a wave's lanes hit distinct banks;
+// deeper entries spill to a per-lane global area.
+#include "lb_device.h"
+#include "lb_host.h"
+
+namespace lbk {
+
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+
+struct StackView {
+    uint32_t *lds;    // this lane's entry 0; entry k at lds[k * lds_stride]
+    uint32_t *spill;  // this lane's spilled entry D; entry k at spill[(k - D) * spill_stride]
+    uint32_t lds_stride, spill_stride, depth;
+    __device__ __forceinline__ void put(uint32_t k, uint32_t v) const {
+        if (k < depth)
+            lds[k * lds_stride] = v;
+        else
+            spill[(k - depth) * spill_stride] = v;
+    }
+    __device__ __forceinline__ uint32_t get(uint32_t k) const {
+        return k < depth ? lds[k * lds_stride] : spill[(k - depth) * spill_stride];
+    }
+};
+
+// The children of inner node n: boxes, first child's index and which of the two are leaves.
+template <int LAYOUT>
+__device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn, f3 &amx,
+                                         f3 &bmn, f3 &bmx, uint32_t &first, uint32_t &flags) {
+    if (LAYOUT == LB_NODES_WIDE) {
+        const float4 *w = S.wide + n * 4u;
+        float4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+        amn = mk(w0.x, w0.y, w0.z);
+        amx = mk(w0.w, w1.x, w1.y);
+        bmn = mk(w1.z, w1.w, w2.x);
+        bmx = mk(w2.y, w2.z, w2.w);
+        first = fbits(w3.x);
+        flags = fbits(w3.y);
+    } else {
+        const uint4 *q = S.quant + n * 2u;
+        uint4 q0 = q[0], q1 = q[1];
+        float ox = bitsf(q0.x), oy = bitsf(q0.y), oz = bitsf(q0.z);
+        float sx = bitsf((q0.w & 0xffu) << 23), sy = bitsf(((q0.w >> 8) & 0xffu) << 23),
+              sz = bitsf(((q0.w >> 16) & 0xffu) << 23);
+        auto b = [](uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); };
+        amn = mk(ox + b(q1.x, 0) * sx, oy + b(q1.x, 1) * sy, oz + b(q1.x, 2) * sz);
+        amx = mk(ox + b(q1.x, 3) * sx, oy + b(q1.y, 0) * sy, oz + b(q1.y, 1) * sz);
+        bmn = mk(ox + b(q1.y, 2) * sx, oy + b(q1.y, 3) * sy, oz + b(q1.z, 0) * sz);
+        bmx = mk(ox + b(q1.z, 1) * sx, oy + b(q1.z, 2) * sy, oz + b(q1.z, 3) * sz);
+        first = q1.w & 0x3fffffffu;
+        flags = q1.w >> 30;
+    }
+}
+
+// trace.wgsl:83-130 over the tuned node records
+template <int LAYOUT, bool STATS>
+__device__ Hit trace_t(const SceneView &S, const lb_params &P, uint32_t root, f3 o, f3 d,
+                       float tmax, bool any, uint32_t ray, const StackView &st, Counts &cnt) {
+    Hit hit{tmax, NONE, false};
+    f3 inv = inverse_dir(d);
+    uint32_t sp = 0, n = root;
+    if (STATS) cnt.rays++;
+    for (;;) {
+        if (n & LEAF_BIT) {
+            uint2 li = S.leaves[n & ~LEAF_BIT];
+            for (uint32_t i = li.x; i < li.x + li.y; i++) {
+                float4 v0 = S.tris[i * 3u], e1 = S.tris[i * 3u + 1u], e2 = S.tris[i * 3u + 2u];
+                if (STATS) cnt.tris++;
+                float det;
+                float t = mt(o, d, v0, e1, e2, &det);
+                if (t > 1e-4f && t < hit.t) {
+                    if (STATS) cnt.mats++;
+                    if (stops(S, P, i, v0, ray)) {
+                        hit.t = t;
+                        hit.tri = i;
+                        hit.back = det < 0.0f;
+                        if (any) return hit;
+                    }
+                }
+            }
+        } else {
+            f3 amn, amx, bmn, bmx;
+            uint32_t first, flags;
+            children<LAYOUT>(S, n, amn, amx, bmn, bmx, first, flags);
+            if (STATS) cnt.inner++;
+            float ta = slab(o, inv, amn, amx, hit.t);
+            float tb = slab(o, inv, bmn, bmx, hit.t);
+            uint32_t ra = first | ((flags & 1u) ? LEAF_BIT : 0u);
+            uint32_t rb = (first + 1u) | ((flags & 2u) ? LEAF_BIT : 0u);
+            bool an = ta <= tb;
+            uint32_t near = an ? ra : rb, far = an ? rb : ra;
+            float tn = vmin(ta, tb), tf = vmax(ta, tb);
+            if (tn < FAR) {
+                if (tf < FAR && sp < (uint32_t)STACK) {
+                    st.put(sp, far);
+                    sp++;
+                }
+                n = near;
+                continue;
+            }
+        }
+        if (sp == 0u) break;
+        sp--;
+        n = st.get(sp);
+    }
+    return hit;
+}
+
+// ---- group reductions -------------------------------------------------------------------------
+
+template <int M>
+__device__ __forceinline__ float swz(float v) {
+    // ds_swizzle bitmask mode: and 0x1f, or 0, xor M (within each 32-lane half)
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1f | (M << 10)));
+}
+template <int M>
+__device__ __forceinline__ uint32_t swzu(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1f | (M << 10));
+}
+__device__ __forceinline__ float x32(float v) { return __shfl_xor(v, 32); }
+__device__ __forceinline__ uint32_t x32u(uint32_t v) { return (uint32_t)__shfl_xor((int)v, 32); }
+
+// Kahan-Neumaier running sum
+struct KSum {
+    float s, c;
+    __device__ __forceinline__ void add(float x) {
+        float t = s + x;
+        if (fabsf(s) >= fabsf(x))
+            c += (s - t) + x;
+        else
+            c += (x - t) + s;
+        s = t;
+    }
+    // TwoSum merge of another compensated sum
+    __device__ __forceinline__ void merge(float s2, float c2) {
+        float t = s + s2;
+        float bp = t - s;
+        float err = (s - (t - bp)) + (s2 - bp);
+        s = t;
+        c = c + c2 + err;
+    }
+    __device__ __forceinline__ float value() const { return s + c; }
+};
+
+template <int M>
+__device__ __forceinline__ void kstep(KSum &k) {
+    float s2 = swz<M>(k.s), c2 = swz<M>(k.c);
+    k.merge(s2, c2);
+}
+
+// Every lane of an aligned group of L lanes ends with the group's compensated total.
+__device__ __forceinline__ void group_ksum(KSum &k, int L) {
+    if (L > 1) kstep<1>(k);
+    if (L > 2) kstep<2>(k);
+    if (L > 4) kstep<4>(k);
+    if (L > 8) kstep<8>(k);
+    if (L > 16) kstep<16>(k);
+    if (L > 32) {
+        float s2 = x32(k.s), c2 = x32(k.c);
+        k.merge(s2, c2);
+    }
+}
+__device__ __forceinline__ float group_sum(float v, int L) {
+    if (L > 1) v += swz<1>(v);
+    if (L > 2) v += swz<2>(v);
+    if (L > 4) v += swz<4>(v);
+    if (L > 8) v += swz<8>(v);
+    if (L > 16) v += swz<16>(v);
+    if (L > 32) v += x32(v);
+    return v;
+}
+__device__ __forceinline__ uint32_t group_or(uint32_t v, int L) {
+    if (L > 1) v |= swzu<1>(v);
+    if (L > 2) v |= swzu<2>(v);
+    if (L > 4) v |= swzu<4>(v);
+    if (L > 8) v |= swzu<8>(v);
+    if (L > 16) v |= swzu<16>(v);
+    if (L > 32) v |= x32u(v);
+    return v;
+}
+
+// ---- the persistent frame -----------------------------------------------------------------------
+
+struct TunedArgs {
+    SceneView S;
+    lb_params P;
+    const lb_texel *items;
+    uint32_t n_items, key_base;
+    float4 *out;
+    unsigned long long *stats;
+    uint32_t *queue;
+    uint32_t *spill;
+    uint32_t lanes_per_item, depth;
+};
+
+__device__ __forceinline__ StackView make_stack(const TunedArgs &A) {
+    extern __shared__ uint32_t lds_stack[];
+    StackView st;
+    st.lds = lds_stack + threadIdx.x;
+    st.lds_stride = blockDim.x;
+    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    st.spill = A.spill + gtid;
+    st.spill_stride = gridDim.x * blockDim.x;
+    st.depth = A.depth;
+    return st;
+}
+
+// Takes the next unit for the whole wave; UINT_MAX when the queue is empty.
+__device__ __forceinline__ uint32_t next_unit(uint32_t *queue, uint32_t n_units) {
+    uint32_t u = 0;
+    if ((threadIdx.x & 63u) == 0u) u = atomicAdd(queue, 1u);
+    u = __builtin_amdgcn_readfirstlane(u);
+    return u < n_units ? u : 0xffffffffu;
+}
+
+struct ItemIn {
+    f3 pos, n, dx, dy;
+    uint32_t w;  // normal.w bits
+};
+__device__ __forceinline__ ItemIn load_item(const lb_texel *items, uint32_t i) {
+    const float4 *tp = reinterpret_cast<const float4 *>(items + i);
+    float4 a = tp[0], b = tp[1], c = tp[2], d = tp[3];
+    return ItemIn{xyz(a), xyz(b), xyz(c), xyz(d), fbits(b.w)};
+}
+
+// trace.wgsl:159-168 with the 16 rays over the group
+template <int LAYOUT, bool STATS>
+__device__ __forceinline__ bool inside_g(const TunedArgs &A, const StackView &st, f3 o,
+                                         const frame_t &f, uint32_t key, uint32_t lig, int L,
+                                         Counts &cnt) {
+    float back = 0.0f;
+    for (uint32_t r = lig; r < 16u; r += (uint32_t)L) {
+        f3 d = cosine(((float)r + 0.5f) / 16.0f, radical(r), f);
+        Hit h = trace_t<LAYOUT, STATS>(A.S, A.P, A.S.root, o, d, A.P.ground[3], false,
+                                       key + r + 7000u, st, cnt);
+        if (h.tri != NONE && h.back) back += 1.0f;
+    }
+    back = group_sum(back, L);
+    return back / 16.0f > bitsf(A.P.atlas[3]);
+}
+
+// trace.wgsl:170-243
+template <int LAYOUT, bool STATS>
+__global__ void k_direct_tuned(TunedArgs A) {
+    StackView st = make_stack(A);
+    const lb_params &P = A.P;
+    int L = (int)A.lanes_per_item;
+    uint32_t lane = threadIdx.x & 63u;
+    uint32_t per_wave = 64u / (uint32_t)L;
+    uint32_t grp = lane / (uint32_t)L, lig = lane % (uint32_t)L;
+    uint32_t n_units = (A.n_items + per_wave - 1u) / per_wave;
+    uint32_t seed = fbits(P.reach[3]);
+    uint32_t g = P.counts[0], gg = g * g;
+    f3 sun = mk(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]);
+    frame_t sf = frame(sun);
+    Counts cnt{0, 0, 0, 0, 0};
+    for (;;) {
+        uint32_t unit = next_unit(A.queue, n_units);
+        if (unit == 0xffffffffu) break;
+        uint32_t i = unit * per_wave + grp;
+        if (i >= A.n_items) continue;
+        ItemIn tx = load_item(A.items, i);
+        f3 n = tx.n;
+        uint32_t key = (A.key_base + i) * 7919u;
+        f3 lift = scl(n, P.reach[2]);
+        frame_t f = frame(n);
+        f3 at = tx.pos;
+        uint32_t pick = 0u;
+        float span = 1.0f;
+        if (inside_g<LAYOUT, STATS>(A, st, add(at, lift), f, key, lig, L, cnt)) {
+            for (uint32_t q = 1u; q < 10u; q++) {
+                f3 p;
+                uint32_t kq;
+                if (q < 9u) {
+                    float cx, cy;
+                    offset(q, &cx, &cy);
+                    p = add(add(tx.pos, scl(tx.dx, cx)), scl(tx.dy, cy));
+                    kq = key + q * 31u;
+                } else {
+                    p = add(tx.pos, scl(n, PUSH));
+                    kq = key + 331u;
+                }
+                if (!inside_g<LAYOUT, STATS>(A, st, add(p, lift), f, kq, lig, L, cnt)) {
+                    at = p;
+                    pick = q;
+                    span = 0.4f;
+                    break;
+                }
+            }
+        }
+        bool lit = dot3(sun, n) > 0.0f;
+        uint32_t n_sun = lit ? 2u * gg : 0u;
+        uint32_t rays = P.counts[1];
+        uint32_t total = n_sun + rays;
+        float s1 = rnd(key, 3u, seed), s2 = rnd(key, 5u, seed);
+        float kr = rnd(key, 17u, seed);
+        f3 o_ao = add(at, lift);
+        uint32_t hard = 0u;
+        float soft = 0.0f, back = 0.0f;
+        KSum occ{0.0f, 0.0f};
+        for (uint32_t slot = lig; slot < total; slot += (uint32_t)L) {
+            if (slot < n_sun) {
+                uint32_t s = slot < gg ? slot : slot - gg;
+                float cx = (((float)(s % g) + 0.5f) / (float)g - 0.5f) * span;
+                float cy = (((float)(s / g) + 0.5f) / (float)g - 0.5f) * span;
+                f3 o = add(add(add(at, scl(tx.dx, cx)), scl(tx.dy, cy)), lift);
+                if (slot < gg) {
+                    if (trace_t<LAYOUT, STATS>(A.S, P, A.S.root, o, sun, FAR, true, key + s,
+                                               st, cnt)
+                            .tri == NONE)
+                        hard |= 1u << s;
+                } else {
+                    uint32_t k = (s + (uint32_t)(kr * (float)gg)) % gg;
+                    float a = (float)k * 2.399963f;
+                    float r = P.sun_dir[3] * sqrtf(((float)k + 0.5f) / (float)gg);
+                    float sa, ca;
+                    sincos_lb(a, &sa, &ca);
+                    f3 d = normalize3(add(add(sun, scl(scl(sf.t, ca), r)), scl(scl(sf.b, sa), r)));
+                    if (trace_t<LAYOUT, STATS>(A.S, P, A.S.root, o, d, FAR, true,
+                                               key + s + 1000u, st, cnt)
+                            .tri == NONE)
+                        soft += 1.0f;
+                }
+            } else {
+                uint32_t r = slot - n_sun;
+                f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2),
+                              f);
+                Hit h = trace_t<LAYOUT, STATS>(A.S, P, A.S.root, o_ao, d, P.reach[0], false,
+                                               key + r + 2000u, st, cnt);
+                if (h.tri != NONE) {
+                    float k = h.t / P.reach[0];
+                    occ.add(1.0f - k * k);
+                    if (h.back && h.t < P.ground[3]) back += 1.0f;
+                }
+            }
+        }
+        hard = group_or(hard, L);
+        soft = group_sum(soft, L);
+        back = group_sum(back, L);
+        group_ksum(occ, L);
+        if (lig == 0u) {
+            float sv = lit ? soft / (float)gg : 0.0f;
+            A.out[i] = make_float4(sv, bitsf(hard | (pick << 24u)), occ.value() / (float)rays,
+                                   back / (float)rays);
+        }
+    }
+    if (STATS) flush_counts(cnt, A.stats);
+}
+
+// trace.wgsl:245-270
+template <int LAYOUT, bool STATS>
+__global__ void k_gather_tuned(TunedArgs A) {
+    StackView st = make_stack(A);
+    const lb_params &P = A.P;
+    int L = (int)A.lanes_per_item;
+    uint32_t lane = threadIdx.x & 63u;
+    uint32_t per_wave = 64u / (uint32_t)L;
+    uint32_t grp = lane / (uint32_t)L, lig = lane % (uint32_t)L;
+    uint32_t n_units = (A.n_items + per_wave - 1u) / per_wave;
+    uint32_t seed = fbits(P.reach[3]);
+    uint32_t rays = P.counts[2];
+    Counts cnt{0, 0, 0, 0, 0};
+    for (;;) {
+        uint32_t unit = next_unit(A.queue, n_units);
+        if (unit == 0xffffffffu) break;
+        uint32_t i = unit * per_wave + grp;
+        if (i >= A.n_items) continue;
+        ItemIn tx = load_item(A.items, i);
+        f3 n = tx.n;
+        frame_t f = frame(n);
+        uint32_t key = (A.key_base + i) * 7919u;
+        float s1 = rnd(key, 11u, seed), s2 = rnd(key, 13u, seed);
+        KSum sr{0, 0}, sg{0, 0}, sb{0, 0};
+        float back = 0.0f;
+        for (uint32_t r = lig; r < rays; r += (uint32_t)L) {
+            float jx = rnd(key + r, 19u, seed) - 0.5f;
+            float jy = rnd(key + r, 23u, seed) - 0.5f;
+            f3 o = add(add(add(tx.pos, scl(tx.dx, jx)), scl(tx.dy, jy)), scl(n, P.reach[2]));
+            f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
+            Hit h = trace_t<LAYOUT, STATS>(A.S, P, A.S.root, o, d, P.reach[1], false,
+                                           key + r + 4000u, st, cnt);
+            f3 l;
+            if (h.tri == NONE) {
+                l = sky(P, d);
+            } else if (h.back) {
+                back += 1.0f;
+                continue;
+            } else {
+                if (STATS) {
+                    cnt.mats++;
+                    cnt.gathers++;
+                }
+                l = bounced(A.S, P, o, d, h);
+            }
+            sr.add(l.x);
+            sg.add(l.y);
+            sb.add(l.z);
+        }
+        group_ksum(sr, L);
+        group_ksum(sg, L);
+        group_ksum(sb, L);
+        back = group_sum(back, L);
+        if (lig == 0u)
+            A.out[i] = make_float4(sr.value() * PI / (float)rays, sg.value() * PI / (float)rays,
+                                   sb.value() * PI / (float)rays, back / (float)rays);
+    }
+    if (STATS) flush_counts(cnt, A.stats);
+}
+
+// trace.wgsl:272-324
+template <int LAYOUT, bool STATS>
+__global__ void k_volume_tuned(TunedArgs A) {
+    StackView st = make_stack(A);
+    const lb_params &P = A.P;
+    int L = (int)A.lanes_per_item;
+    uint32_t lane = threadIdx.x & 63u;
+    uint32_t per_wave = 64u / (uint32_t)L;
+    uint32_t grp = lane / (uint32_t)L, lig = lane % (uint32_t)L;
+    uint32_t n_units = (A.n_items + per_wave - 1u) / per_wave;
+    uint32_t seed = fbits(P.reach[3]);
+    uint32_t rays = P.counts[2];
+    uint32_t g = P.counts[0], gg = g * g;
+    f3 sund = mk(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]);
+    frame_t sf = frame(sund);
+    Counts cnt{0, 0, 0, 0, 0};
+    for (;;) {
+        uint32_t unit = next_unit(A.queue, n_units);
+        if (unit == 0xffffffffu) break;
+        uint32_t i = unit * per_wave + grp;
+        if (i >= A.n_items) continue;
+        ItemIn pr = load_item(A.items, i);
+        f3 o = pr.pos;
+        uint32_t key = pr.w * 7919u + 77u;
+        float s1 = rnd(key, 29u, seed), s2 = rnd(key, 31u, seed);
+        KSum c[12];
+#pragma unroll
+        for (int k = 0; k < 12; k++) c[k] = KSum{0.0f, 0.0f};
+        float back = 0.0f, sun = 0.0f;
+        uint32_t total = rays + gg;
+        for (uint32_t slot = lig; slot < total; slot += (uint32_t)L) {
+            if (slot < rays) {
+                uint32_t r = slot;
+                float u = fract(((float)r + 0.5f) / (float)rays + s1);
+                float v = fract(radical(r) + s2);
+                float z = 1.0f - 2.0f * u;
+                float rr = sqrtf(vmax(0.0f, 1.0f - z * z));
+                float a = v * 2.0f * PI;
+                float sa, ca;
+                sincos_lb(a, &sa, &ca);
+                f3 d = mk(rr * ca, z, rr * sa);
+                Hit h = trace_t<LAYOUT, STATS>(A.S, P, A.S.root, o, d, P.reach[1], false,
+                                               key + r, st, cnt);
+                f3 l = sky(P, d);
+                if (h.tri != NONE) {
+                    if (h.back) {
+                        back += 1.0f;
+                        l = mk(0.0f, 0.0f, 0.0f);
+                    } else {
+                        if (STATS) {
+                            cnt.mats++;
+                            cnt.gathers++;
+                        }
+                        l = bounced(A.S, P, o, d, h);
+                    }
+                }
+                f3 l0 = scl(l, 0.282095f);
+                f3 lx = scl(scl(l, 0.488603f), d.x);
+                f3 ly = scl(scl(l, 0.488603f), d.y);
+                f3 lz = scl(scl(l, 0.488603f), d.z);
+                c[0].add(l0.x); c[1].add(l0.y); c[2].add(l0.z);
+                c[3].add(lx.x); c[4].add(lx.y); c[5].add(lx.z);
+                c[6].add(ly.x); c[7].add(ly.y); c[8].add(ly.z);
+                c[9].add(lz.x); c[10].add(lz.y); c[11].add(lz.z);
+            } else {
+                uint32_t s = slot - rays;
+                float a = (float)s * 2.399963f;
+                float r = P.sun_dir[3] * sqrtf(((float)s + 0.5f) / (float)gg);
+                float sa, ca;
+                sincos_lb(a, &sa, &ca);
+                f3 d = normalize3(add(add(sund, scl(scl(sf.t, ca), r)), scl(scl(sf.b, sa), r)));
+                if (trace_t<LAYOUT, STATS>(A.S, P, A.S.root, o, d, FAR, true, key + s + 9000u,
+                                           st, cnt)
+                        .tri == NONE)
+                    sun += 1.0f;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 12; k++) group_ksum(c[k], L);
+        back = group_sum(back, L);
+        sun = group_sum(sun, L);
+        if (lig == 0u) {
+            float w = 4.0f * PI / (float)rays;
+            A.out[i * 4u] = make_float4(c[0].value() * w, c[1].value() * w, c[2].value() * w,
+                                        sun / (float)gg);
+            A.out[i * 4u + 1u] = make_float4(c[3].value() * w, c[4].value() * w,
+                                             c[5].value() * w, back / (float)rays);
+            A.out[i * 4u + 2u] =
+                make_float4(c[6].value() * w, c[7].value() * w, c[8].value() * w, 0.0f);
+            A.out[i * 4u + 3u] =
+                make_float4(c[9].value() * w, c[10].value() * w, c[11].value() * w, 0.0f);
+        }
+    }
+    if (STATS) flush_counts(cnt, A.stats);
+}
+
+__global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
+
+using KernelFn = void (*)(TunedArgs);
+
+static KernelFn pick_kernel(int pass, int layout, bool stats) {
+#define LB_PICK(K)                                                                  \
+    if (layout == LB_NODES_WIDE)                                                    \
+        return stats ? K<LB_NODES_WIDE, true> : K<LB_NODES_WIDE, false>;            \
+    return stats ? K<LB_NODES_QUANT, true> : K<LB_NODES_QUANT, false>;
+    if (pass == LB_PASS_DIRECT) {
+        LB_PICK(k_direct_tuned)
+    } else if (pass == LB_PASS_GATHER) {
+        LB_PICK(k_gather_tuned)
+    } else {
+        LB_PICK(k_volume_tuned)
+    }
+#undef LB_PICK
+}
+
+// Lanes sharing one item: the rays an item casts over the rays a lane should take, as a power
+// of two in 8..64.
+uint32_t lanes_per_item(int pass, const lb_params &P, int rays_per_lane) {
+    uint32_t gg = P.counts[0] * P.counts[0];
+    uint32_t rays = pass == LB_PASS_DIRECT   ? 2u * gg + P.counts[1]
+                    : pass == LB_PASS_GATHER ? P.counts[2]
+                                             : P.counts[2] + gg;
+    uint32_t want = (rays + (uint32_t)rays_per_lane - 1u) / (uint32_t)rays_per_lane;
+    uint32_t L = 8;
+    while (L < want && L < 64) L <<= 1;
+    return L;
+}
+
+hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
+                        const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
+                        float4 *out) {
+    const Tuning &T = gpu->tuning;
+    KernelFn fn = pick_kernel(pass, T.nodes, stats);
+    uint32_t block = (uint32_t)T.workgroup;
+    size_t lds = (size_t)T.lds_stack * block * 4u;
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void *>(fn), (int)block, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    uint32_t L = lanes_per_item(pass, P, T.rays_per_lane);
+    uint32_t per_block = (block / 64u) * (64u / L);
+    uint32_t need = (n + per_block - 1u) / per_block;
+    uint32_t grid = (uint32_t)gpu->cus * (uint32_t)per_cu;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    size_t spill_bytes = (size_t)grid * block * (size_t)(STACK - T.lds_stack) * 4u;
+    if (spill_bytes < 16) spill_bytes = 16;
+    if (gpu->spill.reserve(spill_bytes)) return hipErrorOutOfMemory;
+    TunedArgs A;
+    A.S = S;
+    A.P = P;
+    A.items = items;
+    A.n_items = n;
+    A.key_base = key_base;
+    A.out = out;
+    A.stats = gpu->counters;
+    A.queue = gpu->queue;
+    A.spill = static_cast<uint32_t *>(gpu->spill.ptr);
+    A.lanes_per_item = L;
+    A.depth = (uint32_t)T.lds_stack;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, gpu->stream, gpu->queue);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), lds, gpu->stream, A);
+    return hipGetLastError();
+}
+
+}  // namespace lbk

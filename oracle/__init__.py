@@ -1,0 +1,107 @@
+"""ORACLE — test infrastructure, not product code.
+
+ctypes access to liboracle.so, the scalar FP32 C restatement of the reference's ray passes
+(crates/lightbaker/src/gpu/trace.wgsl) and CPU texel passes (filter.rs), plus the FP64 tracer.
+Only tests/, the smoke check and bench.py's cpu_baseline leg use it, as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+
+class Counters(C.Structure):
+    _fields_ = [("rays", C.c_uint64), ("inner_visits", C.c_uint64), ("tri_tests", C.c_uint64),
+                ("material_loads", C.c_uint64), ("light_gathers", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            from . import build as _b
+            _b.build()
+        L = C.CDLL(str(LIB_PATH))
+        VP, SZ, U32, F, I = C.c_void_p, C.c_size_t, C.c_uint32, C.c_float, C.c_int
+        L.lb_sincos.argtypes = [F, C.POINTER(F), C.POINTER(F)]
+        L.lb_sincos.restype = None
+        L.orc_pcg.argtypes = [U32]
+        L.orc_pcg.restype = U32
+        L.orc_rand.argtypes = [U32, U32, F]
+        L.orc_rand.restype = F
+        L.orc_radical.argtypes = [U32]
+        L.orc_radical.restype = F
+        L.orc_cosine.argtypes = [F, F, VP, VP]
+        L.orc_cosine.restype = None
+        L.orc_run.argtypes = [I, VP, SZ, VP, VP, VP, VP, VP, SZ, U32, VP, VP, C.POINTER(Counters)]
+        L.orc_run.restype = I
+        L.orc_trace.argtypes = [VP, SZ, VP, VP, VP, VP, VP, F, I, U32, C.POINTER(F),
+                                C.POINTER(U32), C.POINTER(I), C.POINTER(Counters)]
+        L.orc_trace.restype = None
+        L.orc_trace_brute.argtypes = [VP, SZ, VP, VP, VP, VP, F, U32, C.POINTER(F),
+                                      C.POINTER(U32), C.POINTER(I)]
+        L.orc_trace_brute.restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def run(pass_: int, nodes, tris, materials, charts, params, items, key_base: int = 0,
+        light=None, counters: bool = False):
+    """orc_run: each item's results as (n * width, 4) float32; with counters, also the counts."""
+    L = lib()
+    items = np.ascontiguousarray(items)
+    nodes = np.ascontiguousarray(nodes)
+    tris = np.ascontiguousarray(tris)
+    params = np.ascontiguousarray(params)
+    n = len(items)
+    width = 4 if pass_ == 2 else 1
+    out = np.zeros((n * width, 4), dtype=np.float32)
+    mats = np.ascontiguousarray(materials, dtype=np.float32)
+    ch = None if charts is None or len(charts) == 0 else np.ascontiguousarray(charts,
+                                                                               dtype=np.float32)
+    lt = None if light is None or len(light) == 0 else np.ascontiguousarray(light,
+                                                                             dtype=np.float32)
+    cnt = Counters() if counters else None
+    rc = L.orc_run(pass_, _p(nodes), len(nodes), _p(tris), _p(mats), _p(ch), _p(params),
+                   _p(items), n, key_base, _p(lt), _p(out), C.byref(cnt) if cnt else None)
+    assert rc == 0
+    if counters:
+        return out, {k: getattr(cnt, k) for k, _ in Counters._fields_}
+    return out
+
+
+def trace(nodes, tris, materials, params, o, d, tmax, any_hit, ray):
+    L = lib()
+    nodes, tris = np.ascontiguousarray(nodes), np.ascontiguousarray(tris)
+    params = np.ascontiguousarray(params)
+    t, tri, back = C.c_float(), C.c_uint32(), C.c_int()
+    o = np.ascontiguousarray(o, dtype=np.float32)
+    d = np.ascontiguousarray(d, dtype=np.float32)
+    mats = np.ascontiguousarray(materials, dtype=np.float32)
+    L.orc_trace(_p(nodes), len(nodes), _p(tris), _p(mats), _p(params), _p(o), _p(d), tmax,
+                int(any_hit), ray, C.byref(t), C.byref(tri), C.byref(back), None)
+    return t.value, tri.value, bool(back.value)
+
+
+def trace_brute(tris, materials, params, o, d, tmax, ray):
+    L = lib()
+    tris, params = np.ascontiguousarray(tris), np.ascontiguousarray(params)
+    t, tri, back = C.c_float(), C.c_uint32(), C.c_int()
+    o = np.ascontiguousarray(o, dtype=np.float32)
+    d = np.ascontiguousarray(d, dtype=np.float32)
+    mats = np.ascontiguousarray(materials, dtype=np.float32)
+    L.orc_trace_brute(_p(tris), len(tris), _p(mats), _p(params), _p(o), _p(d), tmax, ray,
+                      C.byref(t), C.byref(tri), C.byref(back))
+    return t.value, tri.value, bool(back.value)

@@ -1,0 +1,47 @@
+/*
+ * ORACLE — test infrastructure, not product code: the CPU restatement that the HIP kernels are
+ * checked against. Nothing in lightbaker_amd/ includes or links it.
+ */
+#ifndef LB_ORACLE_H
+#define LB_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/lightbaker.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_counters {
+    uint64_t rays;
+    uint64_t inner_visits;
+    uint64_t tri_tests;
+    uint64_t material_loads;
+    uint64_t light_gathers;
+} orc_counters;
+
+void lb_sincos(float x, float *s, float *c);
+uint32_t orc_pcg(uint32_t v);
+float orc_rand(uint32_t a, uint32_t b, float seed);
+float orc_radical(uint32_t i);
+void orc_cosine(float u, float v, const float n[3], float out[3]);
+int orc_sat_floor(float x, int hi);
+
+int orc_run(int pass, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+            const float *materials, const float *charts, const lb_params *params,
+            const lb_texel *items, size_t n, uint32_t key_base, const float *light, float *out,
+            orc_counters *cnt);
+void orc_trace(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, const float *materials,
+               const lb_params *params, const float o[3], const float d[3], float tmax, int any,
+               uint32_t ray, float *t_out, uint32_t *tri_out, int *back_out, orc_counters *cnt);
+void orc_trace_brute(const lb_tri *tris, size_t n_tris, const float *materials,
+                     const lb_params *params, const float o[3], const float d[3], float tmax,
+                     uint32_t ray, float *t_out, uint32_t *tri_out, int *back_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

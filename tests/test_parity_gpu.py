@@ -1,0 +1,143 @@
+"""Parity of the HIP passes with the oracle (the FP32 C restatement of trace.wgsl) on the same
+items, keys and scene, through the C-ABI.
+
+Bar: the PLAIN form reproduces every output bit for bit (hit-derived bits, counts and FP32 sums in
+the reference's order). The TUNED form reproduces every hit-derived quantity exactly (shadow
+bits, pick, sun/soft/back counts) and the ray sums within 2e-6 relative + 1e-6 absolute (its
+sums are Kahan-compensated in a different order); with 8-bit quantized nodes the hit sets stay
+the same unless a ray grazes a box edge, so at most 0.5% of items may differ in counts.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from lightbaker_amd import _native as N
+from lightbaker_amd.gpu import Pass
+from lightbaker_amd.scenes import small_scene
+
+from . import kit
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup(gpu):
+    scene = small_scene(1)
+    bvh, pos, mesh = kit.scene_bvh(scene)
+    rng = np.random.default_rng(7)
+    items = kit.sample_items(pos, mesh, 1500, rng)
+    charts = kit.charts_for(scene)
+    light = kit.random_light(rng)
+    mats = scene.materials_array()
+    params = kit.params_for(scene)
+    sb = gpu.scene(bvh, mats, charts)
+    probes = kit.probe_items(scene, 600)
+    return dict(scene=scene, bvh=bvh, items=items, charts=charts, light=light, mats=mats,
+                params=params, sb=sb, probes=probes)
+
+
+def _oracle(s, pass_, items):
+    return oracle.run(int(pass_), s["bvh"].nodes, s["bvh"].tris, s["mats"], s["charts"],
+                      s["params"], items, 0, s["light"])
+
+
+@pytest.mark.parametrize("pass_", [Pass.DIRECT, Pass.GATHER, Pass.VOLUME])
+def test_plain_bit_exact(gpu, setup, pass_):
+    items = setup["probes"] if pass_ == Pass.VOLUME else setup["items"]
+    print(f"plain {pass_.name}: {len(items)} items", flush=True)
+    gpu.set_form(N.FORM_PLAIN)
+    got = gpu.run(pass_, setup["sb"], setup["params"], items, setup["light"])
+    want = _oracle(setup, pass_, items)
+    gb, wb = kit.bits(got), kit.bits(want)
+    bad = np.nonzero((gb != wb).any(axis=1))[0]
+    assert len(bad) == 0, f"{len(bad)} of {len(gb)} rows differ, first {bad[:5]}: " \
+                          f"{got[bad[:3]]} vs {want[bad[:3]]}"
+
+
+def _compare_tuned(pass_, got, want, quant):
+    n = len(want)
+    if pass_ == Pass.DIRECT:
+        exact_cols = [0, 1, 3]
+        sum_cols = [2]
+    elif pass_ == Pass.GATHER:
+        exact_cols = [3]
+        sum_cols = [0, 1, 2]
+    else:
+        got = got.reshape(-1, 16)
+        want = want.reshape(-1, 16)
+        exact_cols = [3, 7, 11, 15]
+        sum_cols = [0, 1, 2, 4, 5, 6, 8, 9, 10, 12, 13, 14]
+    gb, wb = kit.bits(got), kit.bits(want)
+    exact_bad = (gb[:, exact_cols] != wb[:, exact_cols]).any(axis=1)
+    tol = 2e-6 * np.abs(want[:, sum_cols]) + 1e-6
+    sum_bad = (np.abs(got[:, sum_cols] - want[:, sum_cols]) > tol).any(axis=1)
+    bad = exact_bad | sum_bad
+    if quant:
+        assert bad.sum() <= max(1, int(0.005 * n)), f"{bad.sum()} of {n} items differ"
+    else:
+        idx = np.nonzero(bad)[0]
+        assert len(idx) == 0, f"{len(idx)} of {n} differ, first {idx[:5]}: " \
+                              f"{got[idx[:3]]} vs {want[idx[:3]]}"
+
+
+@pytest.mark.parametrize("nodes", [N.NODES_WIDE, N.NODES_QUANT])
+@pytest.mark.parametrize("pass_", [Pass.DIRECT, Pass.GATHER, Pass.VOLUME])
+def test_tuned_matches(gpu, setup, pass_, nodes):
+    items = setup["probes"] if pass_ == Pass.VOLUME else setup["items"]
+    print(f"tuned {pass_.name} nodes={nodes}: {len(items)} items", flush=True)
+    gpu.set_form(N.FORM_TUNED)
+    gpu.set_tuning(256, 2, 16, nodes)
+    try:
+        got = gpu.run(pass_, setup["sb"], setup["params"], items, setup["light"])
+    finally:
+        gpu.set_form(N.FORM_PLAIN)
+    want = _oracle(setup, pass_, items)
+    _compare_tuned(pass_, got, want, nodes == N.NODES_QUANT)
+
+
+@pytest.mark.parametrize("cfg", [(64, 1, 8), (128, 4, 24), (512, 8, 48), (256, 2, 32)])
+def test_tuned_configs(gpu, setup, cfg):
+    """Every autotuner setting gives the same results (workgroup, rays/lane, LDS stack)."""
+    wg, rpl, depth = cfg
+    print(f"tuned gather cfg={cfg}", flush=True)
+    gpu.set_form(N.FORM_TUNED)
+    gpu.set_tuning(wg, rpl, depth, N.NODES_WIDE)
+    try:
+        got = gpu.run(Pass.GATHER, setup["sb"], setup["params"], setup["items"], setup["light"])
+    finally:
+        gpu.set_form(N.FORM_PLAIN)
+        gpu.set_tuning(256, 2, 16, N.NODES_WIDE)
+    want = _oracle(setup, Pass.GATHER, setup["items"])
+    _compare_tuned(Pass.GATHER, got, want, False)
+
+
+def test_key_base_slices(gpu, setup):
+    """A slice run with key_base = its offset gives the rows of the whole run (gpu/mod.rs:153-154)."""
+    items = setup["items"]
+    d_items = __import__("lightbaker_amd.gpu", fromlist=["DeviceArray"]).DeviceArray
+    whole = gpu.run(Pass.GATHER, setup["sb"], setup["params"], items, setup["light"])
+    part = items[500:900].copy()
+    di = d_items.of(gpu, part)
+    dl = d_items.of(gpu, setup["light"])
+    do = d_items(gpu, len(part) * 16)
+    gpu.run_device(Pass.GATHER, N.FORM_PLAIN, setup["sb"], setup["params"], di, len(part), dl,
+                   len(setup["light"]), do, key_base=500)
+    got = do.get(np.float32, (len(part), 4))
+    assert np.array_equal(kit.bits(got), kit.bits(whole[500:900]))
+
+
+def test_counters_match_oracle(gpu, setup):
+    """The kernels' traversal counters equal the oracle's on the same rays (PLAIN form)."""
+    from lightbaker_amd.gpu import DeviceArray
+    items = setup["items"][:300]
+    di = DeviceArray.of(gpu, items)
+    dl = DeviceArray.of(gpu, setup["light"])
+    do = DeviceArray(gpu, len(items) * 16)
+    st = gpu.run_stats(Pass.GATHER, N.FORM_PLAIN, setup["sb"], setup["params"], di, len(items),
+                       dl, len(setup["light"]), do)
+    _, want = oracle.run(1, setup["bvh"].nodes, setup["bvh"].tris, setup["mats"],
+                         setup["charts"], setup["params"], items, 0, setup["light"],
+                         counters=True)
+    assert st == want
