@@ -158,6 +158,61 @@ int lb_set_form(lb_gpu *gpu, int form);
 
 /* ---- host substrate (the reference's Rust host, restated in C++) ------------------------------ */
 
+/* scene.rs:53-64 `Material` */
+typedef struct lb_material_desc {
+    float albedo[3];
+    float opacity;
+    uint32_t receives;
+    uint32_t casts;
+} lb_material_desc;
+
+/* scene.rs:66-72 `MeshInfo` with its positions (triangles * 3 vertices * xyz, not indexed) */
+typedef struct lb_mesh_desc {
+    uint32_t material;
+    uint32_t triangles;
+    const float *positions;
+} lb_mesh_desc;
+
+/* scene.rs:17-128 `SceneHeader` (sun, sky, materials, meshes, lightmap, bake, volume) */
+typedef struct lb_scene_desc {
+    float sun_direction[3];
+    float sun_radius_deg;
+    float sun_irradiance[3];
+    float zenith[3];
+    float horizon[3];
+    float ground[3];
+    const lb_material_desc *materials;
+    size_t n_materials;
+    const lb_mesh_desc *meshes;
+    size_t n_meshes;
+    /* lightmap (scene.rs:85-97) */
+    float texels_per_meter;
+    float tile_size;
+    float chart_angle_deg;
+    uint32_t padding;
+    uint32_t atlas_width;
+    uint32_t indirect_downscale;
+    /* bake (scene.rs:99-118) */
+    uint32_t sun_samples;
+    uint32_t ao_rays;
+    float ao_reach;
+    float ao_strength;
+    float ao_floor;
+    float ao_contrast;
+    float sdf_radius;
+    uint32_t gi_rays;
+    uint32_t gi_bounces;
+    float gi_reach;
+    float inside_share;
+    uint32_t gi_filter;
+    /* volume (scene.rs:120-128); has_volume 0: none */
+    uint32_t has_volume;
+    float volume_min[3];
+    float volume_max[3];
+    float volume_step;
+    uint32_t volume_rays;
+} lb_scene_desc;
+
 typedef struct lb_bvh lb_bvh;
 
 /* Bvh::build (bvh.rs:50-106) over non-indexed triangles: positions [n_tris][3][3], each
@@ -168,6 +223,56 @@ int lb_bvh_build(const float *positions, size_t n_tris, const uint32_t *material
 size_t lb_bvh_nodes(const lb_bvh *bvh, const lb_node **nodes);
 size_t lb_bvh_tris(const lb_bvh *bvh, const lb_tri **tris);
 void lb_bvh_destroy(lb_bvh *bvh);
+
+/* ---- the bake (bake.rs:94-432, volume.rs:34-148) ------------------------------------------- */
+
+typedef struct lb_bake lb_bake;
+
+typedef struct lb_bake_options {
+    int form;      /* LB_FORM_* of the ray passes */
+    int graph;     /* nonzero: the bounce loop (gather + texel passes) replayed from a hipGraph */
+    int threads;   /* host threads for the rasterizer (0: all) */
+} lb_bake_options;
+
+/* bake() (bake.rs:94): layout, BVH, the tiles' texels, the direct pass, the texel passes, the
+ * bounces, the volume, the atlas. Every tile is baked (no cache: tile hashing and the cache are
+ * the reference's host bookkeeping). Errors as the reference's: a tile wider than the atlas, a
+ * failing GPU call. */
+int lb_bake_run(lb_gpu *gpu, const lb_scene_desc *scene, const lb_bake_options *options,
+                lb_bake **out);
+void lb_bake_destroy(lb_bake *bake);
+
+typedef struct lb_bake_info {
+    uint32_t atlas_width, atlas_height;        /* layout.rs:56-66 */
+    uint32_t indirect_width, indirect_height;  /* bake.rs:365 */
+    uint32_t tiles, tiles_baked, charts;       /* bake.rs:49-58 Stats */
+    uint64_t texels, texels_baked, gather_items, probes;
+    uint32_t volume_size[3];
+    float volume_origin[3];
+    float volume_step;
+    uint32_t bvh_nodes, bvh_tris;
+    /* device time of each stage (HIP events on the bake's stream), ms */
+    float ms_direct, ms_direct_texel, ms_bounces, ms_volume;
+    /* host time of each stage, s */
+    double s_layout, s_bvh, s_raster, s_gather_set, s_finish, s_total;
+} lb_bake_info;
+
+int lb_bake_info_get(const lb_bake *bake, lb_bake_info *info);
+
+enum lb_bake_array {
+    LB_BAKE_DIRECT = 0,      /* f32 x2 per atlas texel: sun field, occlusion (bake.rs:38-39) */
+    LB_BAKE_INDIRECT = 1,    /* f32 x3 per indirect texel (bake.rs:40-42) */
+    LB_BAKE_VALIDITY = 2,    /* u8 per atlas texel (output.rs:29-40) */
+    LB_BAKE_VOLUME = 3,      /* f32 x16 per probe, x fastest then z then y (volume.rs:25-31) */
+    LB_BAKE_CHARTS = 4,      /* f32 x8 per chart: the affine rows (layout.rs:77-85) */
+    LB_BAKE_TRI_CHARTS = 5,  /* u32 per triangle, meshes in order (layout.rs:64-65) */
+    LB_BAKE_LIGHT = 6,       /* f32 x4 per atlas texel: the light the last bounce left */
+    LB_BAKE_CHART_TILES = 7, /* u32 per chart: its tile */
+    LB_BAKE_TILES = 8        /* i32 x6 per tile: key x, key z, w, h, atlas x, atlas y */
+};
+
+/* The bake's array `which`: its bytes, with *data pointing at them (owned by the bake). */
+size_t lb_bake_array(const lb_bake *bake, int which, const void **data);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,55 @@ class Stats(C.Structure):
                 ("material_loads", C.c_uint64), ("light_gathers", C.c_uint64)]
 
 
+class MaterialDesc(C.Structure):
+    _fields_ = [("albedo", C.c_float * 3), ("opacity", C.c_float), ("receives", C.c_uint32),
+                ("casts", C.c_uint32)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("material", C.c_uint32), ("triangles", C.c_uint32),
+                ("positions", C.POINTER(C.c_float))]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("sun_direction", C.c_float * 3), ("sun_radius_deg", C.c_float),
+        ("sun_irradiance", C.c_float * 3), ("zenith", C.c_float * 3), ("horizon", C.c_float * 3),
+        ("ground", C.c_float * 3), ("materials", C.POINTER(MaterialDesc)),
+        ("n_materials", C.c_size_t), ("meshes", C.POINTER(MeshDesc)), ("n_meshes", C.c_size_t),
+        ("texels_per_meter", C.c_float), ("tile_size", C.c_float), ("chart_angle_deg", C.c_float),
+        ("padding", C.c_uint32), ("atlas_width", C.c_uint32), ("indirect_downscale", C.c_uint32),
+        ("sun_samples", C.c_uint32), ("ao_rays", C.c_uint32), ("ao_reach", C.c_float),
+        ("ao_strength", C.c_float), ("ao_floor", C.c_float), ("ao_contrast", C.c_float),
+        ("sdf_radius", C.c_float), ("gi_rays", C.c_uint32), ("gi_bounces", C.c_uint32),
+        ("gi_reach", C.c_float), ("inside_share", C.c_float), ("gi_filter", C.c_uint32),
+        ("has_volume", C.c_uint32), ("volume_min", C.c_float * 3), ("volume_max", C.c_float * 3),
+        ("volume_step", C.c_float), ("volume_rays", C.c_uint32),
+    ]
+
+
+class BakeOptions(C.Structure):
+    _fields_ = [("form", C.c_int), ("graph", C.c_int), ("threads", C.c_int)]
+
+
+class BakeInfo(C.Structure):
+    _fields_ = [
+        ("atlas_width", C.c_uint32), ("atlas_height", C.c_uint32),
+        ("indirect_width", C.c_uint32), ("indirect_height", C.c_uint32),
+        ("tiles", C.c_uint32), ("tiles_baked", C.c_uint32), ("charts", C.c_uint32),
+        ("texels", C.c_uint64), ("texels_baked", C.c_uint64), ("gather_items", C.c_uint64),
+        ("probes", C.c_uint64), ("volume_size", C.c_uint32 * 3), ("volume_origin", C.c_float * 3),
+        ("volume_step", C.c_float), ("bvh_nodes", C.c_uint32), ("bvh_tris", C.c_uint32),
+        ("ms_direct", C.c_float), ("ms_direct_texel", C.c_float), ("ms_bounces", C.c_float),
+        ("ms_volume", C.c_float), ("s_layout", C.c_double), ("s_bvh", C.c_double),
+        ("s_raster", C.c_double), ("s_gather_set", C.c_double), ("s_finish", C.c_double),
+        ("s_total", C.c_double),
+    ]
+
+
+BAKE_DIRECT, BAKE_INDIRECT, BAKE_VALIDITY, BAKE_VOLUME, BAKE_CHARTS = 0, 1, 2, 3, 4
+BAKE_TRI_CHARTS, BAKE_LIGHT, BAKE_CHART_TILES, BAKE_TILES = 5, 6, 7, 8
+
 # Every symbol include/lightbaker.h declares, with its ctypes signature.
 _VP, _SZ, _U32, _I = C.c_void_p, C.c_size_t, C.c_uint32, C.c_int
 SIGNATURES = {
@@ -61,6 +110,10 @@ SIGNATURES = {
     "lb_bvh_nodes": (_SZ, [_VP, C.POINTER(_VP)]),
     "lb_bvh_tris": (_SZ, [_VP, C.POINTER(_VP)]),
     "lb_bvh_destroy": (None, [_VP]),
+    "lb_bake_run": (_I, [_VP, C.POINTER(SceneDesc), C.POINTER(BakeOptions), C.POINTER(_VP)]),
+    "lb_bake_destroy": (None, [_VP]),
+    "lb_bake_info_get": (_I, [_VP, C.POINTER(BakeInfo)]),
+    "lb_bake_array": (_SZ, [_VP, _I, C.POINTER(_VP)]),
 }
 
 

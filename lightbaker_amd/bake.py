@@ -1,0 +1,117 @@
+"""bake() (crates/lightbaker/src/bake.rs:94) over the C-ABI: the scene in, the lightmap atlas,
+the indirect atlas, the validity, the irradiance volume and the layout's chart maps out."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+from .gpu import Gpu
+from .scenes import Scene
+
+
+@dataclass
+class Bake:
+    """bake.rs:36-47 `Bake` plus the layout facts the tests and tools need."""
+    info: dict
+    direct: np.ndarray      # (ah, aw, 2): sun field, occlusion
+    indirect: np.ndarray    # (ih, iw, 3)
+    validity: np.ndarray    # (ah, aw) u8
+    volume: np.ndarray | None  # (ny, nz, nx, 4, 4)
+    charts: np.ndarray      # (charts, 2, 4) affine rows
+    tri_charts: np.ndarray  # (triangles,) u32
+    light: np.ndarray       # (ah, aw, 4)
+    chart_tiles: np.ndarray
+    tiles: np.ndarray       # (tiles, 6): key x, key z, w, h, atlas x, atlas y
+
+
+def scene_desc(scene: Scene):
+    """The C descriptor of a scene, and the arrays it borrows (keep them alive while it is used)."""
+    keep = []
+    mats = (N.MaterialDesc * len(scene.materials))()
+    for i, m in enumerate(scene.materials):
+        mats[i].albedo[:] = list(m.albedo)
+        mats[i].opacity = m.opacity
+        mats[i].receives = int(m.receives)
+        mats[i].casts = int(m.casts)
+    meshes = (N.MeshDesc * len(scene.meshes))()
+    for i, m in enumerate(scene.meshes):
+        p = np.ascontiguousarray(m.positions, dtype=np.float32).reshape(-1)
+        keep.append(p)
+        meshes[i].material = m.material
+        meshes[i].triangles = len(p) // 9
+        meshes[i].positions = p.ctypes.data_as(C.POINTER(C.c_float))
+    st = scene.settings
+    d = N.SceneDesc()
+    d.sun_direction[:] = list(scene.sun_direction)
+    d.sun_radius_deg = scene.sun_radius_deg
+    d.sun_irradiance[:] = list(scene.sun_irradiance)
+    d.zenith[:] = list(scene.zenith)
+    d.horizon[:] = list(scene.horizon)
+    d.ground[:] = list(scene.ground)
+    d.materials = mats
+    d.n_materials = len(scene.materials)
+    d.meshes = meshes
+    d.n_meshes = len(scene.meshes)
+    for k in ("texels_per_meter", "tile_size", "chart_angle_deg", "padding", "atlas_width",
+              "indirect_downscale", "sun_samples", "ao_rays", "ao_reach", "ao_strength",
+              "ao_floor", "ao_contrast", "sdf_radius", "gi_rays", "gi_bounces", "gi_reach",
+              "inside_share", "gi_filter"):
+        setattr(d, k, getattr(st, k))
+    if scene.volume is not None:
+        d.has_volume = 1
+        d.volume_min[:] = list(scene.volume.min)
+        d.volume_max[:] = list(scene.volume.max)
+        d.volume_step = scene.volume.step
+        d.volume_rays = scene.volume.rays
+    keep += [mats, meshes]
+    return d, keep
+
+
+def _array(lib, h, which, dtype):
+    p = C.c_void_p()
+    n = lib.lb_bake_array(h, which, C.byref(p))
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.frombuffer(C.string_at(p, n), dtype=dtype).copy()
+
+
+def bake(scene: Scene, gpu: Gpu, form: int = N.FORM_PLAIN, graph: bool = False,
+         threads: int = 0) -> Bake:
+    lib = N.lib()
+    d, keep = scene_desc(scene)
+    opt = N.BakeOptions(int(form), int(graph), int(threads))
+    h = C.c_void_p()
+    N.check(lib.lb_bake_run(gpu.handle, C.byref(d), C.byref(opt), C.byref(h)))
+    try:
+        inf = N.BakeInfo()
+        N.check(lib.lb_bake_info_get(h, C.byref(inf)))
+        info = {}
+        for k, _ in N.BakeInfo._fields_:
+            v = getattr(inf, k)
+            info[k] = list(v) if hasattr(v, "__len__") else v
+        aw, ah = inf.atlas_width, inf.atlas_height
+        iw, ih = inf.indirect_width, inf.indirect_height
+        vol = _array(lib, h, N.BAKE_VOLUME, np.float32)
+        if len(vol):
+            nx, ny, nz = info["volume_size"]
+            vol = vol.reshape(ny, nz, nx, 4, 4)
+        else:
+            vol = None
+        return Bake(
+            info=info,
+            direct=_array(lib, h, N.BAKE_DIRECT, np.float32).reshape(ah, aw, 2),
+            indirect=_array(lib, h, N.BAKE_INDIRECT, np.float32).reshape(ih, iw, 3),
+            validity=_array(lib, h, N.BAKE_VALIDITY, np.uint8).reshape(ah, aw),
+            volume=vol,
+            charts=_array(lib, h, N.BAKE_CHARTS, np.float32).reshape(-1, 2, 4),
+            tri_charts=_array(lib, h, N.BAKE_TRI_CHARTS, np.uint32),
+            light=_array(lib, h, N.BAKE_LIGHT, np.float32).reshape(ah, aw, 4),
+            chart_tiles=_array(lib, h, N.BAKE_CHART_TILES, np.uint32),
+            tiles=_array(lib, h, N.BAKE_TILES, np.int32).reshape(-1, 6),
+        )
+    finally:
+        lib.lb_bake_destroy(h)
+        del keep

@@ -20,12 +20,8 @@ namespace {
 constexpr int BINS = 16;   // bvh.rs:9
 constexpr size_t LEAF = 4;  // bvh.rs:10
 
-struct V3 {
-    float x, y, z;
-    float operator[](int a) const { return a == 0 ? x : (a == 1 ? y : z); }
-};
-inline V3 vmin3(V3 a, V3 b) { return {std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)}; }
-inline V3 vmax3(V3 a, V3 b) { return {std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)}; }
+inline V3 vmin3(V3 a, V3 b) { return vmin(a, b); }
+inline V3 vmax3(V3 a, V3 b) { return vmax(a, b); }
 
 struct Prim {
     V3 min, max, centre;

@@ -65,6 +65,8 @@ int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_pa
              const lb_texel *items, size_t n, uint32_t key_base, const float *light,
              size_t n_light, float *out, bool stats);
 bool is_empty_bvh(const lb_node *nodes, size_t n_nodes);
+// sizes the tuned form's spill area for a launch over n items (before a graph capture)
+hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n);
 hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                         const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
                         float4 *out);

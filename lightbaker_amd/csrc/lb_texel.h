@@ -1,0 +1,62 @@
+// The device texel passes between the ray passes (texel_passes.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/lightbaker.h"
+
+namespace lbk {
+
+// output.rs:29-40: how a texel got its light
+enum : uint8_t {
+    VALIDITY_NONE = 0,
+    VALIDITY_CENTRE = 1,
+    VALIDITY_OFFSET = 2,
+    VALIDITY_PUSHED = 3,
+    VALIDITY_FILLED = 4,
+    VALIDITY_VOLUME = 5,
+};
+
+// The dirty tiles' texels back to back, and their per-texel state (bake.rs:61-92 TileLight/Work).
+struct TexelView {
+    uint32_t n;              // texels
+    const uint32_t *tile_of; // texel -> tile
+    const uint4 *tiles;      // per tile: first texel, w, h, atlas corner (x | y << 16)
+    lb_texel *texels;        // the rasterized texels (sample points move on decode)
+    const uint32_t *chart;   // chart, ~0 where empty
+    const float4 *normal;    // face normal (0 where empty)
+    uint8_t *valid;          // covered and not inside a solid
+    uint8_t *known;          // 2 own light, 1 filled, 0 none
+    uint8_t *state;          // VALIDITY_*
+    uint32_t *hard;          // sun bits (cell s % g, s / g)
+    float *soft;             // sun share
+    float *ao;
+    float *field;
+    float4 *direct;          // rgb
+    float4 *indirect;        // rgb
+};
+
+struct DecodeArgs {
+    float inside_share, ao_strength, ao_contrast, ao_floor;
+};
+
+void tp_decode(const TexelView &V, const float4 *res, const uint32_t *cov, uint32_t n_cov,
+               const DecodeArgs &a, hipStream_t s);
+void tp_smooth(const TexelView &V, int channels, const float *in, float *out, const uint8_t *valid,
+               int r, const float *w, hipStream_t s);
+void tp_sun_field(const TexelView &V, const uint8_t *covered, uint32_t g, float radius,
+                  hipStream_t s);
+void tp_direct_light(const TexelView &V, const float *soft, float3 irr, float3 sun, hipStream_t s);
+void tp_fill_ring(const TexelView &V, int channels, const float *src, float *dst,
+                  const uint8_t *was, uint8_t *now, hipStream_t s);
+void tp_known(const TexelView &V, const uint8_t *filled, hipStream_t s);
+void tp_scatter(const float4 *res, const uint32_t *idx, uint32_t n_items, float4 *e, hipStream_t s);
+void tp_write_light(const TexelView &V, float4 *light, uint32_t atlas_w, hipStream_t s);
+void tp_collect(const lb_texel *texels, const uint32_t *idx, uint32_t n, lb_texel *items,
+                hipStream_t s);
+void tp_fill_f32(float *p, float v, uint32_t n, hipStream_t s);
+void tp_copy_f32(const float *src, float *dst, uint32_t n, hipStream_t s);
+void tp_copy_u8(const uint8_t *src, uint8_t *dst, uint32_t n, hipStream_t s);
+
+}  // namespace lbk

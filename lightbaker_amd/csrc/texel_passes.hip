@@ -1,0 +1,309 @@
+// The texel passes between the ray passes, on the device: the direct pass's decode and AO tone
+// (crates/lightbaker/src/bake.rs:207-241), the edge-aware smoothing, the ring fill and the sun's
+// distance field (filter.rs:13-141), the direct light (bake.rs:247-252), the gathered light's
+// scatter (bake.rs:309-316) and the light atlas write (bake.rs:435-444). In the reference they run
+// on the CPU between dispatches; here they keep the bounce loop on the GPU so it can be captured
+// in a graph.
+//
+// Layout: the dirty tiles' texels back to back (tile-major, row-major within a tile); a texel
+// finds its tile through `tile_of`. One lane a texel. Every output is a pure function of the
+// previous step's arrays in the reference's summation order, so the results are bit-exact with
+// the CPU restatement. Gaussian weights come from a host table (expf is the host's).
+#include "lb_device.h"
+#include "lb_texel.h"
+
+namespace lbk {
+
+namespace {
+
+__device__ __forceinline__ bool same_surface(const TexelView &V, uint32_t i, uint32_t j) {
+    // filter.rs:13-16
+    float4 a = V.normal[i], b = V.normal[j];
+    return V.chart[i] == V.chart[j] && a.x * b.x + a.y * b.y + a.z * b.z >= 0.9f;
+}
+
+struct Where {
+    uint32_t tile, off, w, h, x, y;
+};
+__device__ __forceinline__ Where where(const TexelView &V, uint32_t g) {
+    Where r;
+    r.tile = V.tile_of[g];
+    uint4 t = V.tiles[r.tile];  // off, w, h, atlas corner (x | y << 16)
+    r.off = t.x;
+    r.w = t.y;
+    r.h = t.z;
+    uint32_t i = g - r.off;
+    r.x = i % r.w;
+    r.y = i / r.w;
+    return r;
+}
+
+}  // namespace
+
+// bake.rs:207-241 for the covered item k (texel cov[k])
+__global__ void k_decode(TexelView V, const float4 *__restrict__ res, const uint32_t *cov,
+                         uint32_t n_cov, DecodeArgs a) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_cov) return;
+    uint32_t g = cov[k];
+    float4 r = res[k];
+    if (r.w > a.inside_share) {
+        V.valid[g] = 0;
+        return;
+    }
+    V.soft[g] = r.x;
+    uint32_t bits = fbits(r.y);
+    V.hard[g] = bits & 0x00ffffffu;
+    uint32_t pick = bits >> 24;
+    V.state[g] = pick == 0 ? VALIDITY_CENTRE : (pick == 9 ? VALIDITY_PUSHED : VALIDITY_OFFSET);
+    float4 *t = reinterpret_cast<float4 *>(V.texels + g);
+    if (pick == 9) {
+        float4 p = t[0], n = t[1];
+        t[0] = make_float4(p.x + n.x * PUSH, p.y + n.y * PUSH, p.z + n.z * PUSH, p.w);
+    } else if (pick > 0) {
+        float cx, cy;
+        offset(pick, &cx, &cy);
+        float4 p = t[0], dx = t[2], dy = t[3];
+        // bake.rs:237: pos += dx * cx + dy * cy (the offset summed first)
+        t[0] = make_float4(p.x + (dx.x * cx + dy.x * cy), p.y + (dx.y * cx + dy.y * cy),
+                           p.z + (dx.z * cx + dy.z * cy), p.w);
+    }
+    float v = vmax(1.0f - a.ao_strength * r.z, 0.0f);
+    if (a.ao_contrast != 1.0f) v = powf(v, a.ao_contrast);
+    V.ao[g] = vmax(v, a.ao_floor);
+}
+
+// filter.rs:20-53 over C channels (stride C), weights w[(dy + r) * (2r + 1) + dx + r]
+template <int C>
+__global__ void k_smooth(TexelView V, const float *__restrict__ in, float *__restrict__ out,
+                         const uint8_t *__restrict__ valid, int r, const float *__restrict__ w,
+                         uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    float acc[C];
+    for (int c = 0; c < C; c++) acc[c] = in[g * C + c];
+    if (valid[g] && r > 0) {
+        Where p = where(V, g);
+        float sum[C];
+        for (int c = 0; c < C; c++) sum[c] = 0.0f;
+        float wsum = 0.0f;
+        int y0 = max((int)p.y - r, 0), y1 = min((int)p.y + r + 1, (int)p.h);
+        int x0 = max((int)p.x - r, 0), x1 = min((int)p.x + r + 1, (int)p.w);
+        for (int yy = y0; yy < y1; yy++)
+            for (int xx = x0; xx < x1; xx++) {
+                uint32_t j = p.off + (uint32_t)yy * p.w + (uint32_t)xx;
+                if (!valid[j] || !same_surface(V, g, j)) continue;
+                float k = w[(yy - (int)p.y + r) * (2 * r + 1) + (xx - (int)p.x + r)];
+                for (int c = 0; c < C; c++) sum[c] += in[j * C + c] * k;
+                wsum += k;
+            }
+        for (int c = 0; c < C; c++) acc[c] = sum[c] / wsum;
+    }
+    for (int c = 0; c < C; c++) out[g * C + c] = acc[c];
+}
+
+// filter.rs:106-141: distance from each texel's centre to the nearest sub-sample of the other
+// state on the same surface, over the window the field can reach
+__global__ void k_sun_field(TexelView V, const uint8_t *__restrict__ covered, uint32_t g_side,
+                            float radius, uint32_t n) {
+    uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= n) return;
+    if (!covered[gi]) {
+        V.field[gi] = -1.0f;
+        return;
+    }
+    Where p = where(V, gi);
+    int reach = (int)ceilf(radius) + 1;
+    uint32_t gg = g_side * g_side;
+    bool lit = (uint32_t)__popc(V.hard[gi]) * 2u >= gg;
+    float best = radius;
+    float gf = (float)g_side;
+    float half = 0.5f / gf;
+    int y0 = max((int)p.y - reach, 0), y1 = min((int)p.y + reach + 1, (int)p.h);
+    int x0 = max((int)p.x - reach, 0), x1 = min((int)p.x + reach + 1, (int)p.w);
+    float cxf = (float)p.x + 0.5f, cyf = (float)p.y + 0.5f;
+    for (int yy = y0; yy < y1; yy++)
+        for (int xx = x0; xx < x1; xx++) {
+            uint32_t j = p.off + (uint32_t)yy * p.w + (uint32_t)xx;
+            if (!covered[j] || !same_surface(V, gi, j)) continue;
+            uint32_t bits = V.hard[j];
+            for (uint32_t c = 0; c < gg; c++) {
+                if (((bits >> c) & 1u) == (lit ? 1u : 0u)) continue;
+                float px = (float)xx + ((float)(c % g_side) + 0.5f) / gf - cxf;
+                float py = (float)yy + ((float)(c / g_side) + 0.5f) / gf - cyf;
+                best = vmin(best, sqrtf(px * px + py * py) - half);
+            }
+        }
+    float d = vmin(vmax(best, 0.0f), radius);
+    V.field[gi] = 0.5f + 0.5f * (lit ? d : -d) / radius;
+}
+
+// bake.rs:247-252
+__global__ void k_direct_light(TexelView V, const float *__restrict__ soft, float3 irr, float3 sun,
+                               uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    float4 nr = V.normal[g];
+    float c = vmax(nr.x * sun.x + nr.y * sun.y + nr.z * sun.z, 0.0f);
+    float s = soft[g];
+    V.direct[g] = make_float4(irr.x * c * s, irr.y * c * s, irr.z * c * s, 0.0f);
+}
+
+// filter.rs:58-94, one ring: texels without a value take the mean of their 3 x 3 neighbours that
+// had one (their own chart's first; an empty texel any chart's). C channels in `v` (stride S,
+// first C used), read from `src`, written to `dst`; `was` in, `now` out.
+template <int C, int S>
+__global__ void k_fill_ring(TexelView V, const float *__restrict__ src, float *__restrict__ dst,
+                            const uint8_t *__restrict__ was, uint8_t *__restrict__ now, uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    float out[C];
+    for (int c = 0; c < C; c++) out[c] = src[g * S + c];
+    uint8_t known = was[g];
+    if (!known) {
+        Where p = where(V, g);
+        float own[C], any[C];
+        for (int c = 0; c < C; c++) own[c] = any[c] = 0.0f;
+        int no = 0, na = 0;
+        uint32_t ci = V.chart[g];
+        int y0 = max((int)p.y - 1, 0), y1 = min((int)p.y + 2, (int)p.h);
+        int x0 = max((int)p.x - 1, 0), x1 = min((int)p.x + 2, (int)p.w);
+        for (int yy = y0; yy < y1; yy++)
+            for (int xx = x0; xx < x1; xx++) {
+                uint32_t j = p.off + (uint32_t)yy * p.w + (uint32_t)xx;
+                if (!was[j]) continue;
+                if (ci != NONE && V.chart[j] == ci) {
+                    for (int c = 0; c < C; c++) own[c] += src[j * S + c];
+                    no++;
+                } else {
+                    for (int c = 0; c < C; c++) any[c] += src[j * S + c];
+                    na++;
+                }
+            }
+        if (no > 0) {
+            for (int c = 0; c < C; c++) out[c] = own[c] / (float)no;
+            known = 1;
+        } else if (na > 0 && ci == NONE) {
+            for (int c = 0; c < C; c++) out[c] = any[c] / (float)na;
+            known = 1;
+        }
+    }
+    for (int c = 0; c < C; c++) dst[g * S + c] = out[c];
+    now[g] = known;
+}
+
+// bake.rs:257-265: the state of covered texels filled from neighbours, and known = 2 own, 1 filled
+__global__ void k_known(TexelView V, const uint8_t *__restrict__ filled, uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    bool covered = V.texels[g].pos[3] > 0.5f;
+    if (covered && V.state[g] == VALIDITY_NONE && filled[g]) V.state[g] = VALIDITY_FILLED;
+    V.known[g] = V.valid[g] ? 2 : filled[g];
+}
+
+// bake.rs:309-316: the gathered items' irradiance at their texels, zero elsewhere
+__global__ void k_scatter(const float4 *__restrict__ res, const uint32_t *__restrict__ idx,
+                          uint32_t n_items, float4 *__restrict__ e) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_items) return;
+    float4 r = res[k];
+    e[idx[k]] = make_float4(r.x, r.y, r.z, 0.0f);
+}
+
+// bake.rs:435-444 for every texel of the dirty tiles
+__global__ void k_write_light(TexelView V, float4 *__restrict__ light, uint32_t atlas_w,
+                              uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    Where p = where(V, g);
+    uint32_t corner = V.tiles[p.tile].w;
+    uint32_t ax = (corner & 0xffffu) + p.x, ay = (corner >> 16) + p.y;
+    float4 d = V.direct[g], e = V.indirect[g];
+    light[ay * atlas_w + ax] = V.known[g] >= 1
+                                   ? make_float4(d.x + e.x, d.y + e.y, d.z + e.z, 1.0f)
+                                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// items[k] = texels[idx[k]]
+__global__ void k_collect(const lb_texel *__restrict__ texels, const uint32_t *__restrict__ idx,
+                          uint32_t n, lb_texel *__restrict__ items) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float4 *s = reinterpret_cast<const float4 *>(texels + idx[k]);
+    float4 *d = reinterpret_cast<float4 *>(items + k);
+    d[0] = s[0];
+    d[1] = s[1];
+    d[2] = s[2];
+    d[3] = s[3];
+}
+
+__global__ void k_fill_f32(float *p, float v, uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) p[g] = v;
+}
+
+__global__ void k_copy_f32(const float *s, float *d, uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) d[g] = s[g];
+}
+
+__global__ void k_copy_u8(const uint8_t *s, uint8_t *d, uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) d[g] = s[g];
+}
+
+// ---- launchers ---------------------------------------------------------------------------------
+
+static inline dim3 grid_for(uint32_t n) { return dim3((n + 255u) / 256u); }
+
+void tp_decode(const TexelView &V, const float4 *res, const uint32_t *cov, uint32_t n_cov,
+               const DecodeArgs &a, hipStream_t s) {
+    if (n_cov) hipLaunchKernelGGL(k_decode, grid_for(n_cov), dim3(256), 0, s, V, res, cov, n_cov, a);
+}
+void tp_smooth(const TexelView &V, int channels, const float *in, float *out,
+               const uint8_t *valid, int r, const float *w, hipStream_t s) {
+    if (!V.n) return;
+    if (channels == 1)
+        hipLaunchKernelGGL(k_smooth<1>, grid_for(V.n), dim3(256), 0, s, V, in, out, valid, r, w, V.n);
+    else
+        hipLaunchKernelGGL(k_smooth<4>, grid_for(V.n), dim3(256), 0, s, V, in, out, valid, r, w, V.n);
+}
+void tp_sun_field(const TexelView &V, const uint8_t *covered, uint32_t g, float radius,
+                  hipStream_t s) {
+    if (V.n) hipLaunchKernelGGL(k_sun_field, grid_for(V.n), dim3(256), 0, s, V, covered, g, radius, V.n);
+}
+void tp_direct_light(const TexelView &V, const float *soft, float3 irr, float3 sun, hipStream_t s) {
+    if (V.n) hipLaunchKernelGGL(k_direct_light, grid_for(V.n), dim3(256), 0, s, V, soft, irr, sun, V.n);
+}
+void tp_fill_ring(const TexelView &V, int channels, const float *src, float *dst,
+                  const uint8_t *was, uint8_t *now, hipStream_t s) {
+    if (!V.n) return;
+    if (channels == 1)
+        hipLaunchKernelGGL((k_fill_ring<1, 1>), grid_for(V.n), dim3(256), 0, s, V, src, dst, was, now, V.n);
+    else
+        hipLaunchKernelGGL((k_fill_ring<3, 4>), grid_for(V.n), dim3(256), 0, s, V, src, dst, was, now, V.n);
+}
+void tp_known(const TexelView &V, const uint8_t *filled, hipStream_t s) {
+    if (V.n) hipLaunchKernelGGL(k_known, grid_for(V.n), dim3(256), 0, s, V, filled, V.n);
+}
+void tp_scatter(const float4 *res, const uint32_t *idx, uint32_t n_items, float4 *e, hipStream_t s) {
+    if (n_items) hipLaunchKernelGGL(k_scatter, grid_for(n_items), dim3(256), 0, s, res, idx, n_items, e);
+}
+void tp_write_light(const TexelView &V, float4 *light, uint32_t atlas_w, hipStream_t s) {
+    if (V.n) hipLaunchKernelGGL(k_write_light, grid_for(V.n), dim3(256), 0, s, V, light, atlas_w, V.n);
+}
+void tp_collect(const lb_texel *texels, const uint32_t *idx, uint32_t n, lb_texel *items,
+                hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_collect, grid_for(n), dim3(256), 0, s, texels, idx, n, items);
+}
+void tp_fill_f32(float *p, float v, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill_f32, grid_for(n), dim3(256), 0, s, p, v, n);
+}
+void tp_copy_f32(const float *src, float *dst, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_copy_f32, grid_for(n), dim3(256), 0, s, src, dst, n);
+}
+void tp_copy_u8(const uint8_t *src, uint8_t *dst, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_copy_u8, grid_for(n), dim3(256), 0, s, src, dst, n);
+}
+
+}  // namespace lbk

@@ -547,27 +547,49 @@ uint32_t lanes_per_item(int pass, const lb_params &P, int rays_per_lane) {
     return L;
 }
 
-hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
-                        const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
-                        float4 *out) {
+struct TunedGrid {
+    KernelFn fn;
+    uint32_t block, grid, lanes;
+    size_t lds;
+};
+
+static hipError_t tuned_grid(lb_gpu *gpu, int pass, bool stats, const lb_params &P, uint32_t n,
+                             TunedGrid *out) {
     const Tuning &T = gpu->tuning;
-    KernelFn fn = pick_kernel(pass, T.nodes, stats);
-    uint32_t block = (uint32_t)T.workgroup;
-    size_t lds = (size_t)T.lds_stack * block * 4u;
+    out->fn = pick_kernel(pass, T.nodes, stats);
+    out->block = (uint32_t)T.workgroup;
+    out->lds = (size_t)T.lds_stack * out->block * 4u;
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void *>(fn), (int)block, lds);
+        &per_cu, reinterpret_cast<const void *>(out->fn), (int)out->block, out->lds);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
-    uint32_t L = lanes_per_item(pass, P, T.rays_per_lane);
-    uint32_t per_block = (block / 64u) * (64u / L);
+    out->lanes = lanes_per_item(pass, P, T.rays_per_lane);
+    uint32_t per_block = (out->block / 64u) * (64u / out->lanes);
     uint32_t need = (n + per_block - 1u) / per_block;
     uint32_t grid = (uint32_t)gpu->cus * (uint32_t)per_cu;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
-    size_t spill_bytes = (size_t)grid * block * (size_t)(STACK - T.lds_stack) * 4u;
+    out->grid = grid;
+    size_t spill_bytes = (size_t)grid * out->block * (size_t)(STACK - T.lds_stack) * 4u;
     if (spill_bytes < 16) spill_bytes = 16;
     if (gpu->spill.reserve(spill_bytes)) return hipErrorOutOfMemory;
+    return hipSuccess;
+}
+
+hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n) {
+    TunedGrid tg;
+    hipError_t e = tuned_grid(gpu, pass, false, P, n, &tg);
+    if (e != hipSuccess) return e;
+    return tuned_grid(gpu, pass, true, P, n, &tg);
+}
+
+hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
+                        const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
+                        float4 *out) {
+    TunedGrid tg;
+    hipError_t e = tuned_grid(gpu, pass, stats, P, n, &tg);
+    if (e != hipSuccess) return e;
     TunedArgs A;
     A.S = S;
     A.P = P;
@@ -578,10 +600,10 @@ hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
     A.stats = gpu->counters;
     A.queue = gpu->queue;
     A.spill = static_cast<uint32_t *>(gpu->spill.ptr);
-    A.lanes_per_item = L;
-    A.depth = (uint32_t)T.lds_stack;
+    A.lanes_per_item = tg.lanes;
+    A.depth = (uint32_t)gpu->tuning.lds_stack;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, gpu->stream, gpu->queue);
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), lds, gpu->stream, A);
+    hipLaunchKernelGGL(tg.fn, dim3(tg.grid), dim3(tg.block), tg.lds, gpu->stream, A);
     return hipGetLastError();
 }
 
