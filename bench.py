@@ -1,0 +1,195 @@
+"""Benchmark: Grays/s of the 100 x 100 m, 3-bounce bake on one MI355X.
+
+A step is the device part of one bake of the map (lightbaker_amd.scenes.map_scene(2): a 256 x 256
+fBm heightfield plus 400 boxes and cylinders, 16 materials, 8 texels/m, 64 AO rays, 16 sun rays,
+256 gather rays, 3 bounces, volume every 2 m to 20 m): the direct pass, its texel passes, the
+gather set, three bounces (gather + texel passes, replayed from a hipGraph) and the volume pass,
+all on inputs already resident in HBM. `value` is the rays those passes trace per second
+(counted by the kernels' own counters in a separate run), over the wall time of K such steps.
+
+Also reported: the roofline of the dominant kernel (the gather pass), the PLAIN form's time per
+step on the same map, and the oracle (the C restatement of the passes) timed on the host's cores
+on a bounded sample of the gather pass.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--form tuned|plain] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def node_bytes(stats: dict, form: int, nodes_layout: int) -> int:
+    """Algorithmic bytes of one pass from its traversal counters, with each form's record sizes:
+    PLAIN reads the node (32 B) at every visit and both children (64 B) at an inner visit
+    (trace.wgsl:90,111-112); TUNED reads one record with both children's boxes (WIDE 64 B, QUANT
+    32 B) at an inner visit and a leaf's (first, count) (8 B) at a leaf; every triangle test 48 B,
+    every material load 16 B, every bounced() 80 B (its triangle's v0 and e1, the chart's two rows,
+    the light texel)."""
+    if form == 0:
+        nb = (stats["inner_visits"] + stats["leaf_visits"]) * 32 + stats["inner_visits"] * 64
+    else:
+        nb = stats["inner_visits"] * (64 if nodes_layout == 0 else 32) + stats["leaf_visits"] * 8
+    return nb + stats["tri_tests"] * 48 + stats["material_loads"] * 16 + stats["light_gathers"] * 80
+
+
+def cpu_baseline(sess, target_s: float = 12.0) -> dict:
+    """The oracle (oracle/trace_ref.c, FP32, OpenMP over items) on a sample of the gather pass."""
+    import oracle
+    from lightbaker_amd import _native as N
+    texels = sess.array(N.BAKE_TEXELS, N.TEXEL)
+    gat = sess.array(N.BAKE_GATHER_INDEX, np.uint32)
+    nodes = sess.array(N.BAKE_NODES, N.NODE)
+    tris = sess.array(N.BAKE_TRIS, N.TRI)
+    mats = sess.array(N.BAKE_MATERIALS, np.float32).reshape(-1, 4)
+    charts = sess.array(N.BAKE_CHARTS, np.float32).reshape(-1, 8)
+    params = sess.array(N.BAKE_PARAMS, N.PARAMS)
+    light = sess.array(N.BAKE_LIGHT, np.float32).reshape(-1, 4)
+    rays_per_item = int(params["counts"][0][2])
+    items = texels[gat]
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(items))
+
+    def run(n):
+        sample = np.ascontiguousarray(items[order[:n]])
+        t0 = time.perf_counter()
+        oracle.run(1, nodes, tris, mats, charts, params, sample, 0, light)
+        return time.perf_counter() - t0
+
+    n = 256
+    dt = run(n)
+    while dt < 1.0 and n < len(items):
+        n = min(len(items), n * 4)
+        dt = run(n)
+    n = int(min(len(items), max(256, n * target_s / max(dt, 1e-3))))
+    dt = run(n)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n * rays_per_item / dt / 1e9, "unit": "Grays/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} gather items x {rays_per_item} rays of the map's first bounce "
+                      f"(random subset, final light atlas), {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--form", choices=["tuned", "plain"], default="tuned")
+    ap.add_argument("--nodes", choices=["wide", "quant"], default="wide")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
+              f"{args.gpus}); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from lightbaker_amd import _native as N
+    from lightbaker_amd.bake import BakeSession
+    from lightbaker_amd.gpu import Gpu, Pass
+    from lightbaker_amd.scenes import map_scene
+
+    form = N.FORM_TUNED if args.form == "tuned" else N.FORM_PLAIN
+    layout = N.NODES_WIDE if args.nodes == "wide" else N.NODES_QUANT
+    t_scene = time.perf_counter()
+    scene = map_scene(2)
+    t_scene = time.perf_counter() - t_scene
+    gpu = Gpu()
+    gpu.set_tuning(256, 4, 16, layout)
+    print(f"# {gpu.adapter}; scene {scene.triangle_count()} triangles ({t_scene:.1f} s)",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    sess = BakeSession(scene, gpu, form=form, graph=True)
+    prep_s = time.perf_counter() - t0
+    stats = sess.execute(stats=True)
+    rays = sum(s["rays"] for s in stats)
+    info = sess.info()
+    print(f"# prepared in {prep_s:.1f} s: {info['texels_baked']} texels, "
+          f"{info['gather_items']} gather items, {info['probes']} probes, "
+          f"{info['bvh_nodes']} nodes; rays/step {rays}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        sess.execute()
+    gpu.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sess.execute()
+    gpu.sync()
+    dt = time.perf_counter() - t0
+    ms_step = dt / args.steps * 1000.0
+    info = sess.info()
+    t0 = time.perf_counter()
+    sess.finish()
+    finish_s = time.perf_counter() - t0
+
+    # the dominant kernel: the gather pass (one launch = one bounce)
+    bounces = scene.settings.gi_bounces
+    ms_gather = sess.time_pass(Pass.GATHER, reps=5)
+    g = stats[1]
+    per_launch = {k: v / bounces for k, v in g.items()}
+    bytes_launch = node_bytes(per_launch, form, layout) + info["gather_items"] * (64 + 16)
+    achieved = bytes_launch / (ms_gather / 1e3) / 1e9
+    print(f"# gather launch {ms_gather:.2f} ms, {bytes_launch / 1e9:.3f} GB algorithmic, "
+          f"{per_launch['rays'] / ms_gather / 1e6:.2f} Grays/s", file=sys.stderr, flush=True)
+
+    plain_ms = None
+    if not args.no_plain and form == N.FORM_TUNED:
+        ps = BakeSession(scene, gpu, form=N.FORM_PLAIN, graph=True)
+        ps.execute()
+        gpu.sync()
+        t0 = time.perf_counter()
+        ps.execute()
+        gpu.sync()
+        plain_ms = (time.perf_counter() - t0) * 1000.0
+        ps.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(sess)
+    out = {
+        "metric": "Grays/s, 100x100 m 3-bounce bake (direct + 3 gather bounces + volume)",
+        "value": rays / (ms_step / 1e3) / 1e9,
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded map generator (fBm terrain + boxes + cylinders), no dataset",
+        "config": {
+            "workload": "map_scene(2): 100x100 m, 8 texels/m, 256 GI rays, 3 bounces, 16 tiles",
+            "form": args.form, "nodes": args.nodes, "graph": True,
+            "triangles": scene.triangle_count(), "bvh_nodes": info["bvh_nodes"],
+            "texels": info["texels_baked"], "gather_items": info["gather_items"],
+            "probes": info["probes"], "rays_per_step": rays,
+            "rays_direct": stats[0]["rays"], "rays_gather": stats[1]["rays"],
+            "rays_volume": stats[2]["rays"],
+            "ms_direct": info["ms_direct"], "ms_direct_texel": info["ms_direct_texel"],
+            "ms_bounces": info["ms_bounces"], "ms_volume": info["ms_volume"],
+            "s_prepare_host": prep_s, "s_finish_host": finish_s,
+            "s_per_bake_end_to_end": prep_s + ms_step / 1e3 + finish_s,
+            "plain_ms_per_step": plain_ms,
+            "tuned_vs_plain": (plain_ms / ms_step) if plain_ms else None,
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "gather", "ms_per_launch": ms_gather,
+                     "bytes_per_launch": bytes_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    sess.close()
+    gpu.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -133,13 +133,14 @@ int lb_get_tuning(const lb_gpu *gpu, int *workgroup, int *rays_per_lane, int *ld
                   int *nodes);
 
 /* Traversal counters of the last lb_run_stats call: rays traced, inner-node visits (child-box
- * pairs tested), triangle tests, material loads and bounced() light gathers. */
+ * pairs tested), triangle tests, material loads, bounced() light gathers and leaf visits. */
 typedef struct lb_stats {
     uint64_t rays;
     uint64_t inner_visits;
     uint64_t tri_tests;
     uint64_t material_loads;
     uint64_t light_gathers;
+    uint64_t leaf_visits;
 } lb_stats;
 
 /* As lb_run_device with the traversal counted (a separate instantiation of the kernels; the
@@ -224,6 +225,21 @@ size_t lb_bvh_nodes(const lb_bvh *bvh, const lb_node **nodes);
 size_t lb_bvh_tris(const lb_bvh *bvh, const lb_tri **tris);
 void lb_bvh_destroy(lb_bvh *bvh);
 
+/* Layout::build (layout.rs:89-115) of a scene, host only: the charts, the tiles and the atlas. */
+typedef struct lb_layout lb_layout;
+int lb_layout_build(const lb_scene_desc *scene, lb_layout **out);
+void lb_layout_destroy(lb_layout *layout);
+/* atlas width and height; the counts of charts and tiles */
+int lb_layout_size(const lb_layout *layout, uint32_t *width, uint32_t *height, size_t *charts,
+                   size_t *tiles);
+/* per chart: its affine rows (layout.rs:77-85) [charts][8] and tile; per tile: key x, key z, w, h,
+ * atlas x, atlas y [tiles][6]; per triangle (meshes in order): its chart. Any may be NULL. */
+int lb_layout_get(const lb_layout *layout, float *affines, uint32_t *chart_tiles, int32_t *tiles,
+                  uint32_t *tri_charts);
+/* rasterize (raster.rs:39-100) tile `tile` into w * h texels */
+int lb_layout_rasterize(const lb_scene_desc *scene, const lb_layout *layout, size_t tile,
+                        lb_texel *out, size_t cap);
+
 /* ---- the bake (bake.rs:94-432, volume.rs:34-148) ------------------------------------------- */
 
 typedef struct lb_bake lb_bake;
@@ -237,10 +253,26 @@ typedef struct lb_bake_options {
 /* bake() (bake.rs:94): layout, BVH, the tiles' texels, the direct pass, the texel passes, the
  * bounces, the volume, the atlas. Every tile is baked (no cache: tile hashing and the cache are
  * the reference's host bookkeeping). Errors as the reference's: a tile wider than the atlas, a
- * failing GPU call. */
+ * failing GPU call. The scene is borrowed until the call returns. */
 int lb_bake_run(lb_gpu *gpu, const lb_scene_desc *scene, const lb_bake_options *options,
                 lb_bake **out);
 void lb_bake_destroy(lb_bake *bake);
+
+/* lb_bake_run in its three steps, for callers that time or repeat the device part. The scene is
+ * borrowed from prepare until finish returns.
+ *   prepare  host work before any ray (bake.rs:98-149): layout, BVH, texels, probes, uploaded;
+ *   execute  every device pass on resident data (direct, texel passes, gather set, bounces,
+ *            volume); repeatable: each call starts from the prepared state. With `per_pass`
+ *            (3 entries: direct, gather over all bounces, volume) the counting kernels run and
+ *            the traversal counters are returned;
+ *   finish   read-back and host work after the rays (volume.rs:86-117, bake.rs:337-421). */
+int lb_bake_prepare(lb_gpu *gpu, const lb_scene_desc *scene, const lb_bake_options *options,
+                    lb_bake **out);
+int lb_bake_execute(lb_bake *bake, lb_stats *per_pass);
+int lb_bake_finish(lb_bake *bake);
+/* Mean device time of `reps` launches of one pass over the bake's own resident items (after an
+ * execute), HIP events on the gpu's stream. */
+int lb_bake_time_pass(lb_bake *bake, int pass, int reps, float *ms_avg);
 
 typedef struct lb_bake_info {
     uint32_t atlas_width, atlas_height;        /* layout.rs:56-66 */
@@ -254,7 +286,7 @@ typedef struct lb_bake_info {
     /* device time of each stage (HIP events on the bake's stream), ms */
     float ms_direct, ms_direct_texel, ms_bounces, ms_volume;
     /* host time of each stage, s */
-    double s_layout, s_bvh, s_raster, s_gather_set, s_finish, s_total;
+    double s_layout, s_bvh, s_raster, s_gather_set, s_execute, s_finish, s_total;
 } lb_bake_info;
 
 int lb_bake_info_get(const lb_bake *bake, lb_bake_info *info);
@@ -268,7 +300,13 @@ enum lb_bake_array {
     LB_BAKE_TRI_CHARTS = 5,  /* u32 per triangle, meshes in order (layout.rs:64-65) */
     LB_BAKE_LIGHT = 6,       /* f32 x4 per atlas texel: the light the last bounce left */
     LB_BAKE_CHART_TILES = 7, /* u32 per chart: its tile */
-    LB_BAKE_TILES = 8        /* i32 x6 per tile: key x, key z, w, h, atlas x, atlas y */
+    LB_BAKE_TILES = 8,       /* i32 x6 per tile: key x, key z, w, h, atlas x, atlas y */
+    LB_BAKE_NODES = 9,       /* lb_node: the BVH the passes walk */
+    LB_BAKE_TRIS = 10,       /* lb_tri */
+    LB_BAKE_MATERIALS = 11,  /* f32 x4 per material */
+    LB_BAKE_PARAMS = 12,     /* lb_params of the direct and gather passes */
+    LB_BAKE_TEXELS = 13,     /* lb_texel per texel of every tile, tile after tile, as rasterized */
+    LB_BAKE_GATHER_INDEX = 14 /* u32 per gather item: its texel (after an execute) */
 };
 
 /* The bake's array `which`: its bytes, with *data pointing at them (owned by the bake). */

@@ -32,7 +32,8 @@ NODES_WIDE, NODES_QUANT = 0, 1
 
 class Stats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("inner_visits", C.c_uint64), ("tri_tests", C.c_uint64),
-                ("material_loads", C.c_uint64), ("light_gathers", C.c_uint64)]
+                ("material_loads", C.c_uint64), ("light_gathers", C.c_uint64),
+                ("leaf_visits", C.c_uint64)]
 
 
 class MaterialDesc(C.Structure):
@@ -76,13 +77,14 @@ class BakeInfo(C.Structure):
         ("volume_step", C.c_float), ("bvh_nodes", C.c_uint32), ("bvh_tris", C.c_uint32),
         ("ms_direct", C.c_float), ("ms_direct_texel", C.c_float), ("ms_bounces", C.c_float),
         ("ms_volume", C.c_float), ("s_layout", C.c_double), ("s_bvh", C.c_double),
-        ("s_raster", C.c_double), ("s_gather_set", C.c_double), ("s_finish", C.c_double),
-        ("s_total", C.c_double),
+        ("s_raster", C.c_double), ("s_gather_set", C.c_double), ("s_execute", C.c_double),
+        ("s_finish", C.c_double), ("s_total", C.c_double),
     ]
 
 
 BAKE_DIRECT, BAKE_INDIRECT, BAKE_VALIDITY, BAKE_VOLUME, BAKE_CHARTS = 0, 1, 2, 3, 4
 BAKE_TRI_CHARTS, BAKE_LIGHT, BAKE_CHART_TILES, BAKE_TILES = 5, 6, 7, 8
+BAKE_NODES, BAKE_TRIS, BAKE_MATERIALS, BAKE_PARAMS, BAKE_TEXELS, BAKE_GATHER_INDEX = 9, 10, 11, 12, 13, 14
 
 # Every symbol include/lightbaker.h declares, with its ctypes signature.
 _VP, _SZ, _U32, _I = C.c_void_p, C.c_size_t, C.c_uint32, C.c_int
@@ -114,6 +116,15 @@ SIGNATURES = {
     "lb_bake_destroy": (None, [_VP]),
     "lb_bake_info_get": (_I, [_VP, C.POINTER(BakeInfo)]),
     "lb_bake_array": (_SZ, [_VP, _I, C.POINTER(_VP)]),
+    "lb_bake_prepare": (_I, [_VP, C.POINTER(SceneDesc), C.POINTER(BakeOptions), C.POINTER(_VP)]),
+    "lb_bake_execute": (_I, [_VP, _VP]),
+    "lb_bake_finish": (_I, [_VP]),
+    "lb_bake_time_pass": (_I, [_VP, _I, _I, C.POINTER(C.c_float)]),
+    "lb_layout_build": (_I, [C.POINTER(SceneDesc), C.POINTER(_VP)]),
+    "lb_layout_destroy": (None, [_VP]),
+    "lb_layout_size": (_I, [_VP, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_SZ), C.POINTER(_SZ)]),
+    "lb_layout_get": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "lb_layout_rasterize": (_I, [C.POINTER(SceneDesc), _VP, _SZ, _VP, _SZ]),
 }
 
 

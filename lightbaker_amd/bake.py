@@ -115,3 +115,104 @@ def bake(scene: Scene, gpu: Gpu, form: int = N.FORM_PLAIN, graph: bool = False,
     finally:
         lib.lb_bake_destroy(h)
         del keep
+
+
+class BakeSession:
+    """lb_bake_prepare / execute / finish: a bake whose device part can be repeated on resident
+    data (the benchmark's step) and timed pass by pass."""
+
+    def __init__(self, scene: Scene, gpu: Gpu, form: int = N.FORM_TUNED, graph: bool = True,
+                 threads: int = 0):
+        self.lib = N.lib()
+        self.gpu = gpu
+        self.scene = scene
+        self._desc, self._keep = scene_desc(scene)
+        opt = N.BakeOptions(int(form), int(graph), int(threads))
+        h = C.c_void_p()
+        N.check(self.lib.lb_bake_prepare(gpu.handle, C.byref(self._desc), C.byref(opt),
+                                         C.byref(h)))
+        self.handle = h
+
+    def info(self) -> dict:
+        inf = N.BakeInfo()
+        N.check(self.lib.lb_bake_info_get(self.handle, C.byref(inf)))
+        out = {}
+        for k, _ in N.BakeInfo._fields_:
+            v = getattr(inf, k)
+            out[k] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+    def execute(self, stats: bool = False):
+        if not stats:
+            N.check(self.lib.lb_bake_execute(self.handle, None))
+            return None
+        st = (N.Stats * 3)()
+        N.check(self.lib.lb_bake_execute(self.handle, C.cast(st, C.c_void_p)))
+        return [{k: getattr(s, k) for k, _ in N.Stats._fields_} for s in st]
+
+    def finish(self) -> None:
+        N.check(self.lib.lb_bake_finish(self.handle))
+
+    def time_pass(self, pass_: int, reps: int = 5) -> float:
+        ms = C.c_float()
+        N.check(self.lib.lb_bake_time_pass(self.handle, int(pass_), int(reps), C.byref(ms)))
+        return ms.value
+
+    def array(self, which: int, dtype) -> np.ndarray:
+        return _array(self.lib, self.handle, which, dtype)
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.lb_bake_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Layout:
+    """Layout::build (layout.rs:89-115) as arrays; `rasterize(t)` gives tile t's texels."""
+    width: int
+    height: int
+    affines: np.ndarray     # (charts, 2, 4)
+    chart_tiles: np.ndarray
+    tiles: np.ndarray       # (tiles, 6): key x, key z, w, h, atlas x, atlas y
+    tri_charts: np.ndarray
+    _scene: Scene = None
+
+    def rasterize(self, t: int) -> np.ndarray:
+        lib = N.lib()
+        d, keep = scene_desc(self._scene)
+        h = C.c_void_p()
+        N.check(lib.lb_layout_build(C.byref(d), C.byref(h)))
+        try:
+            w, hh = int(self.tiles[t, 2]), int(self.tiles[t, 3])
+            out = np.zeros(w * hh, dtype=N.TEXEL)
+            N.check(lib.lb_layout_rasterize(C.byref(d), h, t, out.ctypes.data, len(out)))
+            return out.reshape(hh, w)
+        finally:
+            lib.lb_layout_destroy(h)
+
+
+def layout(scene: Scene) -> Layout:
+    lib = N.lib()
+    d, keep = scene_desc(scene)
+    h = C.c_void_p()
+    N.check(lib.lb_layout_build(C.byref(d), C.byref(h)))
+    try:
+        w, hh = C.c_uint32(), C.c_uint32()
+        nc, nt = C.c_size_t(), C.c_size_t()
+        N.check(lib.lb_layout_size(h, C.byref(w), C.byref(hh), C.byref(nc), C.byref(nt)))
+        aff = np.zeros((nc.value, 8), np.float32)
+        ct = np.zeros(nc.value, np.uint32)
+        tiles = np.zeros((nt.value, 6), np.int32)
+        tc = np.zeros(scene.triangle_count(), np.uint32)
+        N.check(lib.lb_layout_get(h, aff.ctypes.data, ct.ctypes.data, tiles.ctypes.data,
+                                  tc.ctypes.data))
+        return Layout(w.value, hh.value, aff.reshape(-1, 2, 4), ct, tiles, tc, scene)
+    finally:
+        lib.lb_layout_destroy(h)

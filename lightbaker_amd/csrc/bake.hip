@@ -1,7 +1,11 @@
-// The bake (crates/lightbaker/src/bake.rs:94-432 and volume.rs:34-148): the host keeps the
-// layout, the BVH build and the rasterizer, as the reference does; every ray pass and every texel
-// pass between them runs on the device, in one stream, with the bounce loop (gather, scatter,
-// smooth, fill, light write) optionally replayed from a captured hipGraph.
+// The bake (crates/lightbaker/src/bake.rs:94-432 and volume.rs:34-148) in three steps:
+//   prepare  the host's part before any ray: layout, BVH, Params, the tiles' texels, the probes,
+//            all uploaded (the reference's bake.rs:98-149, volume.rs:36-75);
+//   execute  every device pass on resident data, in one stream: the direct pass, its texel
+//            passes, the gather set, the bounces (optionally replayed from a captured hipGraph)
+//            and the volume pass; repeatable (it starts from the prepared state each time);
+//   finish   the read-back and the host's part after the rays: probes inside solids, the
+//            volume's light for texels with none, the atlas (volume.rs:86-117, bake.rs:337-421).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -9,9 +13,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <set>
 #include <thread>
-#include <unordered_set>
 #include <vector>
 
 #include "lb_bake.h"
@@ -20,22 +24,7 @@
 
 using namespace lbk;
 
-struct lb_bake {
-    Layout layout;
-    lb_bake_info info{};
-    std::vector<float> direct;     // [aw * ah][2]
-    std::vector<float> indirect;   // [iw * ih][3]
-    std::vector<uint8_t> validity;
-    std::vector<float> volume;     // [probes][16]
-    std::vector<float> charts;     // [charts][8]
-    std::vector<uint32_t> tri_charts;
-    std::vector<float> light;      // [aw * ah][4]
-    std::vector<uint32_t> chart_tiles;
-    std::vector<int32_t> tiles;    // [tiles][6]
-};
-
 namespace lbk {
-
 namespace {
 
 constexpr float LIFT = 0.01f;          // bake.rs:24
@@ -45,29 +34,18 @@ constexpr float VOLUME_INSIDE = 0.5f;  // volume.rs:20
 using clk = std::chrono::steady_clock;
 double secs(clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); }
 
-struct Dev {
-    std::vector<DevBuf *> all;
-    DevBuf *make() {
-        all.push_back(new DevBuf());
-        return all.back();
-    }
-    ~Dev() {
-        for (auto *b : all) delete b;
-    }
-};
-
 template <typename T>
 int put(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
     return b.upload(v.data(), v.size() * sizeof(T), s);
 }
 
-#define TRYH(expr)                                      \
-    do {                                                \
-        hipError_t e_ = (expr);                         \
+#define TRYH(expr)                                        \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
         if (e_ != hipSuccess) return hip_fail(e_, #expr); \
     } while (0)
-#define TRY(expr)        \
-    do {                 \
+#define TRY(expr)           \
+    do {                    \
         if (expr) return 1; \
     } while (0)
 
@@ -85,8 +63,8 @@ std::vector<float> gauss(int r) {
 }
 
 // volume.rs:125-147
-void volume_irradiance(const std::vector<float> &probes, const uint32_t size[3], const float origin[3],
-                       float step, V3 p, V3 n, float e[3], float *sun) {
+void volume_irradiance(const std::vector<float> &probes, const uint32_t size[3],
+                       const float origin[3], float step, V3 p, V3 n, float e[3], float *sun) {
     size_t nx = size[0], ny = size[1], nz = size[2];
     V3 f = p - V3{origin[0], origin[1], origin[2]};
     f = {f.x / step, f.y / step, f.z / step};
@@ -107,23 +85,81 @@ void volume_irradiance(const std::vector<float> &probes, const uint32_t size[3],
     float a0 = PIF * 0.282095f, a1 = 2.0f * PIF / 3.0f * 0.488603f;
     for (int c = 0; c < 3; c++)
         e[c] = 0.5f * a0 * p4[0][c] +
-               0.5f * std::fmax(a0 * p4[0][c] + a1 * (p4[1][c] * n.x + p4[2][c] * n.y + p4[3][c] * n.z), 0.0f);
+               0.5f * std::fmax(a0 * p4[0][c] +
+                                    a1 * (p4[1][c] * n.x + p4[2][c] * n.y + p4[3][c] * n.z),
+                                0.0f);
     *sun = p4[0][3];
 }
 
 }  // namespace
+}  // namespace lbk
 
-int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, lb_bake *B) {
+struct lb_bake {
+    lb_gpu *gpu = nullptr;
+    lb_bake_options opt{};
+    lb_scene_desc sc{};  // valid while the caller's scene lives (prepare .. finish)
+    Layout layout;
+    lb_bake_info info{};
+    lb_scene *scene = nullptr;
+    lb_params P{}, PV{};
+    V3 sun{};
+    uint32_t g = 0, aw = 0, ah = 0, ds = 1, rg = 0, rings = 0, grings = 0;
+    // host copies
+    std::vector<uint32_t> toff, chart;
+    std::vector<lb_texel> texels;
+    std::vector<float> normal;
+    std::vector<uint8_t> covered;
+    std::vector<uint32_t> cov;
+    uint32_t vsz[3] = {0, 0, 0};
+    size_t T = 0, ncov = 0, ng = 0, np = 0;
+    std::vector<uint32_t> gat;
+    // device state
+    DevBuf d_tex0, d_tex, d_tile_of, d_tiles, d_chart, d_normal, d_valid, d_known, d_state,
+        d_hard, d_soft, d_ao, d_field, d_direct, d_indirect, d_cov, d_items, d_res, d_light, d_w1,
+        d_wg, d_f1, d_f4, d_m1, d_m2, d_covered, d_gat, d_gmask, d_gitems, d_gres, d_filled,
+        d_pitems, d_pres;
+    TexelView V{};
+    hipEvent_t ev[8] = {};
+    hipGraphExec_t graph = nullptr;
+    size_t graph_ng = 0;  // the gather item count the graph was captured with
+    // results
+    std::vector<float> direct, indirect, volume, charts, light;
+    std::vector<uint8_t> validity;
+    std::vector<uint32_t> tri_charts, chart_tiles;
+    std::vector<int32_t> tiles;
+    std::vector<lb_node> nodes;
+    std::vector<lb_tri> tris;
+    std::vector<float> mats;
+
+    ~lb_bake() {
+        if (gpu) hipStreamSynchronize(gpu->stream);
+        if (graph) hipGraphExecDestroy(graph);
+        for (auto &e : ev)
+            if (e) hipEventDestroy(e);
+        if (scene) lb_scene_destroy(scene);
+    }
+};
+
+namespace lbk {
+
+int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, lb_bake *B) {
     auto t_start = clk::now();
     hipStream_t st = gpu->stream;
     hipSetDevice(gpu->device);
+    B->gpu = gpu;
+    B->opt = opt;
+    B->sc = sc;
     lb_bake_info &I = B->info;
     if (!sc.materials || !sc.n_materials) return fail("scene: no materials");
     for (size_t m = 0; m < sc.n_meshes; m++) {
         if (sc.meshes[m].material >= sc.n_materials)
             return fail("mesh %zu: no material %u", m, sc.meshes[m].material);
-        if (sc.meshes[m].triangles && !sc.meshes[m].positions) return fail("mesh %zu: no positions", m);
+        if (sc.meshes[m].triangles && !sc.meshes[m].positions)
+            return fail("mesh %zu: no positions", m);
     }
+    // scene.rs:188-202
+    if (!(sc.texels_per_meter > 0.0f) || !(sc.tile_size > 0.0f))
+        return fail("lightmap: texelsPerMeter and tileSize must be positive");
     uint32_t g = (uint32_t)std::sqrt((float)sc.sun_samples);
     if (g * g != sc.sun_samples || g < 1) return fail("bake: sunSamples must be a square number");
     if (sc.sun_samples > 24) return fail("bake: sunSamples above 24 would reach the pick bits");
@@ -132,6 +168,13 @@ int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, l
         return fail("lightmap: indirectDownscale 1, 2 or 4 (the tiles' grid)");
     if (sc.atlas_width < 64 || sc.atlas_width % 4 || sc.atlas_width > 65535)
         return fail("lightmap: atlasWidth must be a multiple of 4 in 64..65535");
+    if (sc.has_volume) {
+        if (!(sc.volume_step > 0.0f)) return fail("volume: bad bounds or step");
+        for (int a = 0; a < 3; a++)
+            if (!(sc.volume_max[a] >= sc.volume_min[a])) return fail("volume: bad bounds or step");
+        if (sc.volume_rays < 1) return fail("volume: rays at least one");
+    }
+    B->g = g;
 
     // layout, charts, BVH (bake.rs:98-104)
     auto t0 = clk::now();
@@ -141,6 +184,8 @@ int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, l
     if (L.height > 65535) return fail("layout: atlas %u texels high exceeds 65535", L.height);
     I.s_layout = secs(t0);
     uint32_t aw = L.width, ah = L.height;
+    B->aw = aw;
+    B->ah = ah;
     size_t n_charts = L.charts.size();
     B->charts.assign(n_charts * 8, 0.0f);
     for (size_t c = 0; c < n_charts; c++)
@@ -150,12 +195,12 @@ int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, l
     for (size_t m = 0; m < sc.n_meshes; m++) n_tris += sc.meshes[m].triangles;
     std::vector<float> pos(n_tris * 9);
     std::vector<uint32_t> tmat(n_tris), tchart(n_tris);
-    B->tri_charts.resize(n_tris);
     {
         size_t k = 0;
         for (size_t m = 0; m < sc.n_meshes; m++) {
             const lb_mesh_desc &md = sc.meshes[m];
-            std::memcpy(&pos[k * 9], md.positions, (size_t)md.triangles * 9 * sizeof(float));
+            if (md.triangles)
+                std::memcpy(&pos[k * 9], md.positions, (size_t)md.triangles * 9 * sizeof(float));
             for (uint32_t t = 0; t < md.triangles; t++, k++) {
                 tmat[k] = md.material;
                 tchart[k] = L.tri_chart[m][t];
@@ -164,33 +209,31 @@ int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, l
     }
     B->tri_charts = tchart;
     std::vector<uint8_t> casts(sc.n_materials);
-    std::vector<float> mats(sc.n_materials * 4);
+    B->mats.assign(sc.n_materials * 4, 0.0f);
     for (size_t m = 0; m < sc.n_materials; m++) {
         casts[m] = sc.materials[m].casts ? 1 : 0;
-        for (int c = 0; c < 3; c++) mats[m * 4 + c] = sc.materials[m].albedo[c];
-        mats[m * 4 + 3] = sc.materials[m].opacity;
+        for (int c = 0; c < 3; c++) B->mats[m * 4 + c] = sc.materials[m].albedo[c];
+        B->mats[m * 4 + 3] = sc.materials[m].opacity;
     }
-    std::vector<lb_node> nodes;
-    std::vector<lb_tri> tris;
-    build_bvh(pos.data(), n_tris, tmat.data(), tchart.data(), casts.data(), nodes, tris);
-    I.bvh_nodes = (uint32_t)nodes.size();
-    I.bvh_tris = (uint32_t)tris.size();
+    build_bvh(pos.data(), n_tris, tmat.data(), tchart.data(), casts.data(), B->nodes, B->tris);
+    I.bvh_nodes = (uint32_t)B->nodes.size();
+    I.bvh_tris = (uint32_t)B->tris.size();
     I.s_bvh = secs(t0);
-    lb_scene *scene = nullptr;
-    if (lb_scene_create(gpu, nodes.data(), nodes.size(), tris.data(), tris.size(), mats.data(),
-                        sc.n_materials, n_charts ? B->charts.data() : nullptr, n_charts, &scene))
+    if (lb_scene_create(gpu, B->nodes.data(), B->nodes.size(), B->tris.data(), B->tris.size(),
+                        B->mats.data(), sc.n_materials, n_charts ? B->charts.data() : nullptr,
+                        n_charts, &B->scene))
         return 1;
-    struct SceneGuard {
-        lb_scene *s;
-        ~SceneGuard() { lb_scene_destroy(s); }
-    } sg{scene};
 
     // Params (bake.rs:105-120)
     V3 sun{sc.sun_direction[0], sc.sun_direction[1], sc.sun_direction[2]};
     sun = sun * (1.0f / std::sqrt(dot(sun, sun)));
+    B->sun = sun;
     const float PIF = 3.14159265358979323846f;
-    lb_params P{};
-    P.sun_dir[0] = sun.x; P.sun_dir[1] = sun.y; P.sun_dir[2] = sun.z;
+    lb_params &P = B->P;
+    P = lb_params{};
+    P.sun_dir[0] = sun.x;
+    P.sun_dir[1] = sun.y;
+    P.sun_dir[2] = sun.z;
     P.sun_dir[3] = std::tan(sc.sun_radius_deg * (PIF / 180.0f));
     for (int c = 0; c < 3; c++) {
         P.sun_irr[c] = sc.sun_irradiance[c];
@@ -200,10 +243,17 @@ int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, l
         P.fallback[c] = (sc.zenith[c] + sc.horizon[c]) * 0.5f * PIF;
     }
     P.ground[3] = std::fmin(INSIDE_REACH, sc.ao_reach);
-    P.counts[0] = g; P.counts[1] = sc.ao_rays; P.counts[2] = sc.gi_rays; P.counts[3] = 0;
-    P.reach[0] = sc.ao_reach; P.reach[1] = sc.gi_reach; P.reach[2] = LIFT; P.reach[3] = 0.0f;
-    P.atlas[0] = aw; P.atlas[1] = ah; P.atlas[2] = 0;
+    P.counts[0] = g;
+    P.counts[1] = sc.ao_rays;
+    P.counts[2] = sc.gi_rays;
+    P.reach[0] = sc.ao_reach;
+    P.reach[1] = sc.gi_reach;
+    P.reach[2] = LIFT;
+    P.atlas[0] = aw;
+    P.atlas[1] = ah;
     std::memcpy(&P.atlas[3], &sc.inside_share, 4);
+    B->PV = P;
+    B->PV.counts[2] = sc.volume_rays;  // volume.rs:76
 
     // the tiles' texels (bake.rs:139-148): every tile is dirty
     t0 = clk::now();
@@ -212,262 +262,70 @@ int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, l
     I.charts = (uint32_t)n_charts;
     std::vector<std::vector<lb_texel>> tex(nt);
     {
-        unsigned hw = opt.threads > 0 ? (unsigned)opt.threads : std::max(1u, std::thread::hardware_concurrency());
+        unsigned hw = opt.threads > 0 ? (unsigned)opt.threads
+                                      : std::max(1u, std::thread::hardware_concurrency());
         std::vector<std::thread> pool;
         std::atomic<size_t> next{0};
-        for (unsigned w = 0; w < std::min<size_t>(hw, std::max<size_t>(nt, 1)); w++)
+        size_t workers = std::min<size_t>(hw, std::max<size_t>(nt, 1));
+        for (size_t w = 0; w < workers; w++)
             pool.emplace_back([&] {
                 for (size_t t; (t = next.fetch_add(1)) < nt;) rasterize(sc, L, L.tiles[t], tex[t]);
             });
         for (auto &th : pool) th.join();
     }
-    std::vector<uint32_t> toff(nt + 1, 0);
-    for (size_t t = 0; t < nt; t++) toff[t + 1] = toff[t] + L.tiles[t].w * L.tiles[t].h;
-    size_t T = toff[nt];
+    B->chart_tiles = L.chart_tile;
+    B->tiles.resize(nt * 6);
+    for (size_t t = 0; t < nt; t++) {
+        const Tile &tl = L.tiles[t];
+        int32_t v[6] = {tl.kx, tl.kz, (int32_t)tl.w, (int32_t)tl.h, (int32_t)tl.x, (int32_t)tl.y};
+        std::memcpy(&B->tiles[t * 6], v, sizeof v);
+    }
+    B->toff.assign(nt + 1, 0);
+    for (size_t t = 0; t < nt; t++) B->toff[t + 1] = B->toff[t] + L.tiles[t].w * L.tiles[t].h;
+    size_t T = B->toff[nt];
     if (T > 0x7fffffffu) return fail("bake: %zu texels exceed 31-bit indices", T);
-    std::vector<lb_texel> texels(T);
-    std::vector<uint32_t> tile_of(T), chart(T), cov;
-    std::vector<float> normal(T * 4, 0.0f);
-    std::vector<uint8_t> covered(T);
+    B->T = T;
+    B->texels.resize(T);
+    std::vector<uint32_t> tile_of(T);
+    B->chart.resize(T);
+    B->normal.assign(T * 4, 0.0f);
+    B->covered.resize(T);
     std::vector<uint4> tiles(std::max<size_t>(nt, 1));
     for (size_t t = 0; t < nt; t++) {
         const Tile &tl = L.tiles[t];
-        tiles[t] = make_uint4(toff[t], tl.w, tl.h, tl.x | (tl.y << 16));
-        std::memcpy(&texels[toff[t]], tex[t].data(), tex[t].size() * sizeof(lb_texel));
+        uint32_t o = B->toff[t];
+        tiles[t] = make_uint4(o, tl.w, tl.h, tl.x | (tl.y << 16));
+        if (!tex[t].empty()) std::memcpy(&B->texels[o], tex[t].data(), tex[t].size() * sizeof(lb_texel));
         for (uint32_t i = 0; i < tl.w * tl.h; i++) {
-            size_t gi = toff[t] + i;
+            size_t gi = o + i;
             const lb_texel &x = tex[t][i];
             tile_of[gi] = (uint32_t)t;
             bool cv = x.pos[3] > 0.5f;
-            covered[gi] = cv;
+            B->covered[gi] = cv;
             uint32_t c;
             std::memcpy(&c, &x.normal[3], 4);
-            chart[gi] = cv ? c : 0xffffffffu;
-            for (int k = 0; k < 3; k++) normal[gi * 4 + k] = x.normal[k];
-            if (cv) cov.push_back((uint32_t)gi);
+            B->chart[gi] = cv ? c : 0xffffffffu;
+            for (int k = 0; k < 3; k++) B->normal[gi * 4 + k] = x.normal[k];
+            if (cv) B->cov.push_back((uint32_t)gi);
         }
     }
     tex.clear();
+    B->ncov = B->cov.size();
     I.texels = T;
-    I.texels_baked = cov.size();
+    I.texels_baked = B->ncov;
     I.s_raster = secs(t0);
 
-    // device state
-    Dev D;
-    DevBuf &d_tex = *D.make(), &d_tile_of = *D.make(), &d_tiles = *D.make(), &d_chart = *D.make(),
-           &d_normal = *D.make(), &d_valid = *D.make(), &d_known = *D.make(), &d_state = *D.make(),
-           &d_hard = *D.make(), &d_soft = *D.make(), &d_ao = *D.make(), &d_field = *D.make(),
-           &d_direct = *D.make(), &d_indirect = *D.make(), &d_cov = *D.make(),
-           &d_items = *D.make(), &d_res = *D.make(), &d_light = *D.make(), &d_w1 = *D.make(),
-           &d_wg = *D.make(), &d_f1 = *D.make(), &d_f4 = *D.make(), &d_m1 = *D.make(),
-           &d_m2 = *D.make(), &d_covered = *D.make(), &d_gat = *D.make(), &d_gitems = *D.make(),
-           &d_gres = *D.make();
-    size_t Tn = std::max<size_t>(T, 1);
-    TRY(put(d_tex, texels, st));
-    TRY(put(d_tile_of, tile_of, st));
-    TRY(put(d_tiles, tiles, st));
-    TRY(put(d_chart, chart, st));
-    TRY(put(d_normal, normal, st));
-    TRY(put(d_covered, covered, st));
-    TRY(d_valid.reserve(Tn));
-    TRY(d_known.reserve(Tn));
-    TRY(d_state.reserve(Tn));
-    TRY(d_hard.reserve(Tn * 4));
-    TRY(d_soft.reserve(Tn * 4));
-    TRY(d_ao.reserve(Tn * 4));
-    TRY(d_field.reserve(Tn * 4));
-    TRY(d_direct.reserve(Tn * 16));
-    TRY(d_indirect.reserve(Tn * 16));
-    TRY(d_f1.reserve(Tn * 4));
-    TRY(d_f4.reserve(Tn * 16));
-    TRY(d_m1.reserve(Tn));
-    TRY(d_m2.reserve(Tn));
-    TRY(put(d_cov, cov, st));
-    size_t ncov = cov.size();
-    TRY(d_items.reserve(std::max<size_t>(ncov, 1) * sizeof(lb_texel)));
-    TRY(d_res.reserve(std::max<size_t>(ncov, 1) * 16));
-    TRY(d_light.reserve((size_t)aw * ah * 16));
-    std::vector<float> w1 = gauss(1);
-    TRY(put(d_w1, w1, st));
-    int rg = (int)(sc.gi_filter * sc.indirect_downscale);
-    std::vector<float> wg = gauss(std::max(rg, 0));
-    TRY(put(d_wg, wg, st));
-
-    TexelView V;
-    V.n = (uint32_t)T;
-    V.tile_of = (const uint32_t *)d_tile_of.ptr;
-    V.tiles = (const uint4 *)d_tiles.ptr;
-    V.texels = (lb_texel *)d_tex.ptr;
-    V.chart = (const uint32_t *)d_chart.ptr;
-    V.normal = (const float4 *)d_normal.ptr;
-    V.valid = (uint8_t *)d_valid.ptr;
-    V.known = (uint8_t *)d_known.ptr;
-    V.state = (uint8_t *)d_state.ptr;
-    V.hard = (uint32_t *)d_hard.ptr;
-    V.soft = (float *)d_soft.ptr;
-    V.ao = (float *)d_ao.ptr;
-    V.field = (float *)d_field.ptr;
-    V.direct = (float4 *)d_direct.ptr;
-    V.indirect = (float4 *)d_indirect.ptr;
-
-    // initial per-texel state (bake.rs:204-205: soft 0, ao 1, hard 0, state none; valid = covered)
-    TRYH(hipMemsetAsync(d_light.ptr, 0, (size_t)aw * ah * 16, st));
-    TRYH(hipMemcpyAsync(d_valid.ptr, d_covered.ptr, T, hipMemcpyDeviceToDevice, st));
-    TRYH(hipMemsetAsync(d_state.ptr, 0, Tn, st));
-    TRYH(hipMemsetAsync(d_hard.ptr, 0, Tn * 4, st));
-    TRYH(hipMemsetAsync(d_soft.ptr, 0, Tn * 4, st));
-    TRYH(hipMemsetAsync(d_indirect.ptr, 0, Tn * 16, st));
-    tp_fill_f32(V.ao, 1.0f, (uint32_t)T, st);
-
-    hipEvent_t ev[8];
-    for (auto &e : ev) TRYH(hipEventCreate(&e));
-    struct EvGuard {
-        hipEvent_t *e;
-        ~EvGuard() {
-            for (int i = 0; i < 8; i++) hipEventDestroy(e[i]);
-        }
-    } eg{ev};
-
-    // the direct pass (bake.rs:149-151)
-    TRYH(hipEventRecord(ev[0], st));
-    tp_collect(V.texels, (const uint32_t *)d_cov.ptr, (uint32_t)ncov, (lb_texel *)d_items.ptr, st);
-    TRY(run_pass(gpu, LB_PASS_DIRECT, opt.form, scene, &P, (const lb_texel *)d_items.ptr, ncov, 0,
-                 nullptr, 0, (float *)d_res.ptr, false));
-    TRYH(hipEventRecord(ev[1], st));
-
-    // decode, smooth, field, direct light, fill (bake.rs:196-272)
-    DecodeArgs da{sc.inside_share, sc.ao_strength, sc.ao_contrast, sc.ao_floor};
-    tp_decode(V, (const float4 *)d_res.ptr, (const uint32_t *)d_cov.ptr, (uint32_t)ncov, da, st);
-    // ao and soft smoothed over the valid texels, radius 1
-    tp_smooth(V, 1, V.ao, (float *)d_f1.ptr, V.valid, 1, (const float *)d_w1.ptr, st);
-    tp_copy_f32((const float *)d_f1.ptr, V.ao, (uint32_t)T, st);
-    tp_smooth(V, 1, V.soft, (float *)d_f1.ptr, V.valid, 1, (const float *)d_w1.ptr, st);
-    tp_sun_field(V, V.valid, g, sc.sdf_radius, st);
-    tp_direct_light(V, (const float *)d_f1.ptr, make_float3(sc.sun_irradiance[0], sc.sun_irradiance[1], sc.sun_irradiance[2]),
-                    make_float3(sun.x, sun.y, sun.z), st);
-    // fill ao, field, direct from valid, padding + 4 rings (bake.rs:243,253-256)
-    uint32_t rings = L.padding + 4;
-    auto fill = [&](int channels, float *v, float *tmp, uint32_t nrings, const uint8_t *valid0,
-                    uint8_t *final_mask) -> int {
-        uint8_t *ma = (uint8_t *)d_m1.ptr, *mb = (uint8_t *)d_m2.ptr;
-        tp_copy_u8(valid0, ma, (uint32_t)T, st);
-        float *a = v, *b = tmp;
-        for (uint32_t r = 0; r < nrings; r++) {
-            tp_fill_ring(V, channels, a, b, ma, mb, st);
-            std::swap(a, b);
-            std::swap(ma, mb);
-        }
-        if (a != v) tp_copy_f32(a, v, (uint32_t)(T * (channels == 1 ? 1 : 4)), st);
-        if (final_mask) tp_copy_u8(ma, final_mask, (uint32_t)T, st);
-        return 0;
-    };
-    TRY(fill(1, V.ao, (float *)d_f1.ptr, rings, V.valid, nullptr));
-    TRY(fill(1, V.field, (float *)d_f1.ptr, rings, V.valid, nullptr));
-    DevBuf &d_filled = *D.make();
-    TRY(d_filled.reserve(Tn));
-    uint8_t *filled = (uint8_t *)d_filled.ptr;
-    TRY(fill(3, (float *)V.direct, (float *)d_f4.ptr, rings, V.valid, filled));
-    tp_known(V, filled, st);
-    // the light atlas every tile left (bake.rs:277-281)
-    tp_write_light(V, (float4 *)d_light.ptr, aw, st);
-    TRYH(hipEventRecord(ev[2], st));
-
-    // the gather set (bake.rs:282-299), on the host from the valid texels
-    std::vector<uint8_t> valid(T);
-    TRYH(hipMemcpyAsync(valid.data(), d_valid.ptr, T, hipMemcpyDeviceToHost, st));
-    TRYH(hipStreamSynchronize(st));
-    t0 = clk::now();
-    uint32_t ds = sc.indirect_downscale;
-    std::vector<uint8_t> gathered(T, 0);
-    std::vector<uint32_t> gat;
-    for (size_t t = 0; t < nt; t++) {
-        uint32_t tw = L.tiles[t].w, size = L.tiles[t].w * L.tiles[t].h, o = toff[t];
-        uint32_t bw = (tw + ds - 1) / ds;
-        auto block = [&](uint32_t i) { return (uint64_t)((i / tw) / ds) * bw + (i % tw) / ds; };
-        std::set<std::pair<uint64_t, uint32_t>> seen;
-        for (uint32_t i = 0; i < size; i++) {
-            bool gt = valid[o + i] && (i % tw) % ds == 0 && (i / tw) % ds == 0;
-            gathered[o + i] = gt;
-            if (gt) seen.insert({block(i), chart[o + i]});
-        }
-        for (uint32_t i = 0; i < size; i++)
-            if (valid[o + i] && seen.insert({block(i), chart[o + i]}).second) gathered[o + i] = 1;
-        for (uint32_t i = 0; i < size; i++)
-            if (gathered[o + i]) gat.push_back(o + i);
-    }
-    I.gather_items = gat.size();
-    I.s_gather_set = secs(t0);
-    size_t ng = gat.size();
-    TRY(put(d_gat, gat, st));
-    DevBuf &d_gmask = *D.make();
-    TRY(put(d_gmask, gathered, st));
-    TRY(d_gitems.reserve(std::max<size_t>(ng, 1) * sizeof(lb_texel)));
-    TRY(d_gres.reserve(std::max<size_t>(ng, 1) * 16));
-    tp_collect(V.texels, (const uint32_t *)d_gat.ptr, (uint32_t)ng, (lb_texel *)d_gitems.ptr, st);
-
-    // the bounces (bake.rs:300-330)
-    uint32_t grings = L.padding + 2 * ds + 2;
-    auto bounce = [&]() -> int {
-        TRY(run_pass(gpu, LB_PASS_GATHER, opt.form, scene, &P, (const lb_texel *)d_gitems.ptr, ng,
-                     0, (const float *)d_light.ptr, (size_t)aw * ah, (float *)d_gres.ptr, false));
-        // e: the gathered items' light at their texels, 0 elsewhere (into `indirect`)
-        tp_fill_f32((float *)V.indirect, 0.0f, (uint32_t)(T * 4), st);
-        tp_scatter((const float4 *)d_gres.ptr, (const uint32_t *)d_gat.ptr, (uint32_t)ng, V.indirect, st);
-        tp_smooth(V, 4, (const float *)V.indirect, (float *)d_f4.ptr, (const uint8_t *)d_gmask.ptr, rg,
-                  (const float *)d_wg.ptr, st);
-        float *e = (float *)d_f4.ptr, *tmp = (float *)V.indirect;
-        // fill into d_f4, ping-ponging through `indirect`, then the result to `indirect`
-        uint8_t *ma = (uint8_t *)d_m1.ptr, *mb = (uint8_t *)d_m2.ptr;
-        tp_copy_u8((const uint8_t *)d_gmask.ptr, ma, (uint32_t)T, st);
-        float *a = e, *b = tmp;
-        for (uint32_t r = 0; r < grings; r++) {
-            tp_fill_ring(V, 3, a, b, ma, mb, st);
-            std::swap(a, b);
-            std::swap(ma, mb);
-        }
-        if (a != (float *)V.indirect) tp_copy_f32(a, (float *)V.indirect, (uint32_t)(T * 4), st);
-        tp_write_light(V, (float4 *)d_light.ptr, aw, st);
-        return hipGetLastError() == hipSuccess ? 0 : fail("GPU: bounce launch failed");
-    };
-    TRYH(hipEventRecord(ev[3], st));
-    if (sc.gi_bounces > 0 && ng > 0) {
-        if (opt.graph) {
-            // one bounce captured once, replayed gi_bounces times: the same buffers each time
-            if (opt.form == LB_FORM_TUNED) {
-                // size the tuned form's spill area outside the capture (no allocation inside)
-                if (tuned_prepare(gpu, LB_PASS_GATHER, P, (uint32_t)ng) != hipSuccess)
-                    return fail("GPU: tuned spill area");
-            }
-            hipGraph_t graph;
-            hipGraphExec_t exec;
-            TRYH(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-            int rc = bounce();
-            hipError_t ce = hipStreamEndCapture(st, &graph);
-            if (rc) return 1;
-            if (ce != hipSuccess) return hip_fail(ce, "graph capture");
-            TRYH(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            for (uint32_t b = 0; b < sc.gi_bounces; b++) TRYH(hipGraphLaunch(exec, st));
-            TRYH(hipStreamSynchronize(st));
-            hipGraphExecDestroy(exec);
-            hipGraphDestroy(graph);
-        } else {
-            for (uint32_t b = 0; b < sc.gi_bounces; b++) TRY(bounce());
-        }
-    }
-    TRYH(hipEventRecord(ev[4], st));
-
-    // the volume (volume.rs:34-120)
-    std::vector<float> probes;
-    uint32_t vsz[3] = {0, 0, 0};
+    // the probes (volume.rs:36-75): every probe, in index order (their keys are their indices)
+    std::vector<lb_texel> pitems;
     if (sc.has_volume) {
-        if (!(sc.volume_step > 0.0f)) return fail("volume: bad bounds or step");
-        for (int a = 0; a < 3; a++) {
-            if (sc.volume_max[a] < sc.volume_min[a]) return fail("volume: bad bounds or step");
-            vsz[a] = std::max<uint32_t>((uint32_t)std::round((sc.volume_max[a] - sc.volume_min[a]) / sc.volume_step) + 1, 2);
-        }
-        size_t nx = vsz[0], ny = vsz[1], nz = vsz[2], np = nx * ny * nz;
-        std::vector<lb_texel> pitems(np);
-        for (size_t k = 0; k < np; k++) {
+        for (int a = 0; a < 3; a++)
+            B->vsz[a] = std::max<uint32_t>(
+                (uint32_t)std::round((sc.volume_max[a] - sc.volume_min[a]) / sc.volume_step) + 1, 2);
+        size_t nx = B->vsz[0], ny = B->vsz[1], nz = B->vsz[2];
+        B->np = nx * ny * nz;
+        if (B->np > 0x7fffffffu) return fail("volume: %zu probes exceed 31-bit indices", B->np);
+        pitems.resize(B->np);
+        for (size_t k = 0; k < B->np; k++) {
             float px = sc.volume_min[0] + (float)(k % nx) * sc.volume_step;
             float py = sc.volume_min[1] + (float)(k / (nx * nz)) * sc.volume_step;
             float pz = sc.volume_min[2] + (float)((k / nx) % nz) * sc.volume_step;
@@ -476,41 +334,295 @@ int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, l
             std::memcpy(&kf, &kk, 4);
             pitems[k] = lb_texel{{px, py, pz, 1.0f}, {0.0f, 1.0f, 0.0f, kf}, {0, 0, 0, 0}, {0, 0, 0, 0}};
         }
-        DevBuf &d_pitems = *D.make(), &d_pres = *D.make();
-        TRY(put(d_pitems, pitems, st));
-        TRY(d_pres.reserve(np * 64));
-        lb_params PV = P;
-        PV.counts[2] = sc.volume_rays;  // volume.rs:76
-        if (PV.counts[2] < 1) return fail("volume: rays at least one");
-        TRY(run_pass(gpu, LB_PASS_VOLUME, opt.form, scene, &PV, (const lb_texel *)d_pitems.ptr, np,
-                     0, (const float *)d_light.ptr, (size_t)aw * ah, (float *)d_pres.ptr, false));
-        probes.resize(np * 16);
-        TRYH(hipMemcpyAsync(probes.data(), d_pres.ptr, np * 64, hipMemcpyDeviceToHost, st));
-        I.probes = np;
+        I.probes = B->np;
+        for (int a = 0; a < 3; a++) {
+            I.volume_size[a] = B->vsz[a];
+            I.volume_origin[a] = sc.volume_min[a];
+        }
+        I.volume_step = sc.volume_step;
     }
-    TRYH(hipEventRecord(ev[5], st));
 
-    // read back the tiles' light
+    // device state
+    size_t Tn = std::max<size_t>(T, 1);
+    TRY(put(B->d_tex0, B->texels, st));
+    TRY(B->d_tex.reserve(Tn * sizeof(lb_texel)));
+    TRY(put(B->d_tile_of, tile_of, st));
+    TRY(put(B->d_tiles, tiles, st));
+    TRY(put(B->d_chart, B->chart, st));
+    TRY(put(B->d_normal, B->normal, st));
+    TRY(put(B->d_covered, B->covered, st));
+    TRY(B->d_valid.reserve(Tn));
+    TRY(B->d_known.reserve(Tn));
+    TRY(B->d_state.reserve(Tn));
+    TRY(B->d_filled.reserve(Tn));
+    TRY(B->d_gmask.reserve(Tn));
+    TRY(B->d_hard.reserve(Tn * 4));
+    TRY(B->d_soft.reserve(Tn * 4));
+    TRY(B->d_ao.reserve(Tn * 4));
+    TRY(B->d_field.reserve(Tn * 4));
+    TRY(B->d_direct.reserve(Tn * 16));
+    TRY(B->d_indirect.reserve(Tn * 16));
+    TRY(B->d_f1.reserve(Tn * 4));
+    TRY(B->d_f4.reserve(Tn * 16));
+    TRY(B->d_m1.reserve(Tn));
+    TRY(B->d_m2.reserve(Tn));
+    TRY(put(B->d_cov, B->cov, st));
+    TRY(B->d_items.reserve(std::max<size_t>(B->ncov, 1) * sizeof(lb_texel)));
+    TRY(B->d_res.reserve(std::max<size_t>(B->ncov, 1) * 16));
+    TRY(B->d_light.reserve((size_t)aw * ah * 16));
+    TRY(B->d_gat.reserve(Tn * 4));
+    TRY(B->d_gitems.reserve(Tn * sizeof(lb_texel)));
+    TRY(B->d_gres.reserve(Tn * 16));
+    TRY(put(B->d_w1, gauss(1), st));
+    B->ds = sc.indirect_downscale;
+    B->rg = sc.gi_filter * sc.indirect_downscale;
+    TRY(put(B->d_wg, gauss((int)B->rg), st));
+    B->rings = L.padding + 4;                  // bake.rs:243
+    B->grings = L.padding + 2 * B->ds + 2;     // bake.rs:320
+    if (sc.has_volume) {
+        TRY(put(B->d_pitems, pitems, st));
+        TRY(B->d_pres.reserve(B->np * 64));
+    }
+    TexelView &V = B->V;
+    V.n = (uint32_t)T;
+    V.tile_of = (const uint32_t *)B->d_tile_of.ptr;
+    V.tiles = (const uint4 *)B->d_tiles.ptr;
+    V.texels = (lb_texel *)B->d_tex.ptr;
+    V.chart = (const uint32_t *)B->d_chart.ptr;
+    V.normal = (const float4 *)B->d_normal.ptr;
+    V.valid = (uint8_t *)B->d_valid.ptr;
+    V.known = (uint8_t *)B->d_known.ptr;
+    V.state = (uint8_t *)B->d_state.ptr;
+    V.hard = (uint32_t *)B->d_hard.ptr;
+    V.soft = (float *)B->d_soft.ptr;
+    V.ao = (float *)B->d_ao.ptr;
+    V.field = (float *)B->d_field.ptr;
+    V.direct = (float4 *)B->d_direct.ptr;
+    V.indirect = (float4 *)B->d_indirect.ptr;
+    for (auto &e : B->ev) TRYH(hipEventCreate(&e));
+    if (opt.form == LB_FORM_TUNED) {
+        // the tuned form's spill area, sized for the largest launch, before any capture
+        uint32_t most = (uint32_t)std::max<size_t>({B->ncov, T, B->np, 1});
+        for (int pass = 0; pass < 3; pass++)
+            if (tuned_prepare(gpu, pass, pass == LB_PASS_VOLUME ? B->PV : P, most) != hipSuccess)
+                return fail("GPU: tuned spill area");
+    }
+    TRYH(hipStreamSynchronize(st));
+    I.s_total = secs(t_start);
+    return 0;
+}
+
+namespace {
+
+// filter.rs:58-94 over `rings` rings into `v` (C channels, stride 1 or 4) from `valid0`
+void fill(lb_bake *B, int channels, float *v, float *tmp, uint32_t rings, const uint8_t *valid0,
+          uint8_t *final_mask) {
+    hipStream_t st = B->gpu->stream;
+    uint32_t T = (uint32_t)B->T;
+    uint8_t *ma = (uint8_t *)B->d_m1.ptr, *mb = (uint8_t *)B->d_m2.ptr;
+    tp_copy_u8(valid0, ma, T, st);
+    float *a = v, *b = tmp;
+    for (uint32_t r = 0; r < rings; r++) {
+        tp_fill_ring(B->V, channels, a, b, ma, mb, st);
+        std::swap(a, b);
+        std::swap(ma, mb);
+    }
+    if (a != v) tp_copy_f32(a, v, T * (channels == 1 ? 1u : 4u), st);
+    if (final_mask) tp_copy_u8(ma, final_mask, T, st);
+}
+
+int pass_stats(lb_bake *B, int pass, const lb_params *P, const lb_texel *items, size_t n,
+               const float *light, size_t n_light, float *out, lb_stats *acc) {
+    if (!acc) return run_pass(B->gpu, pass, B->opt.form, B->scene, P, items, n, 0, light, n_light,
+                              out, false);
+    lb_stats s{};
+    if (lb_run_stats(B->gpu, pass, B->opt.form, B->scene, P, items, n, 0, light, n_light, out, &s))
+        return 1;
+    acc->rays += s.rays;
+    acc->inner_visits += s.inner_visits;
+    acc->tri_tests += s.tri_tests;
+    acc->material_loads += s.material_loads;
+    acc->light_gathers += s.light_gathers;
+    acc->leaf_visits += s.leaf_visits;
+    return 0;
+}
+
+// one bounce (bake.rs:301-328): gather, the light at its texels, smooth, fill, the atlas
+int bounce(lb_bake *B, lb_stats *acc) {
+    hipStream_t st = B->gpu->stream;
+    TexelView &V = B->V;
+    uint32_t T = (uint32_t)B->T;
+    TRY(pass_stats(B, LB_PASS_GATHER, &B->P, (const lb_texel *)B->d_gitems.ptr, B->ng,
+                   (const float *)B->d_light.ptr, (size_t)B->aw * B->ah, (float *)B->d_gres.ptr,
+                   acc));
+    tp_fill_f32((float *)V.indirect, 0.0f, T * 4u, st);
+    tp_scatter((const float4 *)B->d_gres.ptr, (const uint32_t *)B->d_gat.ptr, (uint32_t)B->ng,
+               V.indirect, st);
+    tp_smooth(V, 4, (const float *)V.indirect, (float *)B->d_f4.ptr,
+              (const uint8_t *)B->d_gmask.ptr, (int)B->rg, (const float *)B->d_wg.ptr, st);
+    tp_copy_f32((const float *)B->d_f4.ptr, (float *)V.indirect, T * 4u, st);
+    fill(B, 3, (float *)V.indirect, (float *)B->d_f4.ptr, B->grings,
+         (const uint8_t *)B->d_gmask.ptr, nullptr);
+    tp_write_light(V, (float4 *)B->d_light.ptr, B->aw, st);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(e, "bounce");
+}
+
+}  // namespace
+
+int bake_execute(lb_bake *B, lb_stats *per_pass) {
+    lb_gpu *gpu = B->gpu;
+    hipStream_t st = gpu->stream;
+    hipSetDevice(gpu->device);
+    const lb_scene_desc &sc = B->sc;
+    lb_bake_info &I = B->info;
+    TexelView &V = B->V;
+    uint32_t T = (uint32_t)B->T;
+    size_t Tn = std::max<size_t>(B->T, 1);
+    lb_stats *sd = per_pass ? &per_pass[0] : nullptr;
+    lb_stats *sg = per_pass ? &per_pass[1] : nullptr;
+    lb_stats *sv = per_pass ? &per_pass[2] : nullptr;
+    auto t_exec = clk::now();
+
+    // the prepared state (bake.rs:204-205: soft 0, ao 1, hard 0, no state; valid = covered)
+    TRYH(hipEventRecord(B->ev[0], st));
+    TRYH(hipMemcpyAsync(V.texels, B->d_tex0.ptr, B->T * sizeof(lb_texel), hipMemcpyDeviceToDevice, st));
+    TRYH(hipMemsetAsync(B->d_light.ptr, 0, (size_t)B->aw * B->ah * 16, st));
+    TRYH(hipMemcpyAsync(V.valid, B->d_covered.ptr, B->T, hipMemcpyDeviceToDevice, st));
+    TRYH(hipMemsetAsync(V.state, 0, Tn, st));
+    TRYH(hipMemsetAsync(V.hard, 0, Tn * 4, st));
+    TRYH(hipMemsetAsync(V.soft, 0, Tn * 4, st));
+    TRYH(hipMemsetAsync(V.indirect, 0, Tn * 16, st));
+    tp_fill_f32(V.ao, 1.0f, T, st);
+
+    // the direct pass (bake.rs:149-151)
+    tp_collect(V.texels, (const uint32_t *)B->d_cov.ptr, (uint32_t)B->ncov,
+               (lb_texel *)B->d_items.ptr, st);
+    TRY(pass_stats(B, LB_PASS_DIRECT, &B->P, (const lb_texel *)B->d_items.ptr, B->ncov, nullptr, 0,
+                   (float *)B->d_res.ptr, sd));
+    TRYH(hipEventRecord(B->ev[1], st));
+
+    // decode, smooth, field, direct light, fill (bake.rs:196-272)
+    DecodeArgs da{sc.inside_share, sc.ao_strength, sc.ao_contrast, sc.ao_floor};
+    tp_decode(V, (const float4 *)B->d_res.ptr, (const uint32_t *)B->d_cov.ptr, (uint32_t)B->ncov,
+              da, st);
+    tp_smooth(V, 1, V.ao, (float *)B->d_f1.ptr, V.valid, 1, (const float *)B->d_w1.ptr, st);
+    tp_copy_f32((const float *)B->d_f1.ptr, V.ao, T, st);
+    tp_smooth(V, 1, V.soft, (float *)B->d_f1.ptr, V.valid, 1, (const float *)B->d_w1.ptr, st);
+    tp_sun_field(V, V.valid, B->g, sc.sdf_radius, st);
+    tp_direct_light(V, (const float *)B->d_f1.ptr,
+                    make_float3(sc.sun_irradiance[0], sc.sun_irradiance[1], sc.sun_irradiance[2]),
+                    make_float3(B->sun.x, B->sun.y, B->sun.z), st);
+    fill(B, 1, V.ao, (float *)B->d_f1.ptr, B->rings, V.valid, nullptr);
+    fill(B, 1, V.field, (float *)B->d_f1.ptr, B->rings, V.valid, nullptr);
+    fill(B, 3, (float *)V.direct, (float *)B->d_f4.ptr, B->rings, V.valid,
+         (uint8_t *)B->d_filled.ptr);
+    tp_known(V, (const uint8_t *)B->d_filled.ptr, st);
+    tp_write_light(V, (float4 *)B->d_light.ptr, B->aw, st);  // bake.rs:277-281
+    TRYH(hipGetLastError());
+    TRYH(hipEventRecord(B->ev[2], st));
+
+    // the gather set (bake.rs:282-299), on the host from the valid texels
+    std::vector<uint8_t> valid(B->T);
+    TRYH(hipMemcpyAsync(valid.data(), V.valid, B->T, hipMemcpyDeviceToHost, st));
+    TRYH(hipStreamSynchronize(st));
+    auto t0 = clk::now();
+    const Layout &L = B->layout;
+    uint32_t ds = B->ds;
+    std::vector<uint8_t> gathered(B->T, 0);
+    B->gat.clear();
+    for (size_t t = 0; t < L.tiles.size(); t++) {
+        uint32_t tw = L.tiles[t].w, size = L.tiles[t].w * L.tiles[t].h, o = B->toff[t];
+        uint32_t bw = (tw + ds - 1) / ds;
+        auto block = [&](uint32_t i) { return (uint64_t)((i / tw) / ds) * bw + (i % tw) / ds; };
+        std::set<std::pair<uint64_t, uint32_t>> seen;
+        for (uint32_t i = 0; i < size; i++) {
+            bool gt = valid[o + i] && (i % tw) % ds == 0 && (i / tw) % ds == 0;
+            gathered[o + i] = gt;
+            if (gt) seen.insert({block(i), B->chart[o + i]});
+        }
+        for (uint32_t i = 0; i < size; i++)
+            if (valid[o + i] && seen.insert({block(i), B->chart[o + i]}).second) gathered[o + i] = 1;
+        for (uint32_t i = 0; i < size; i++)
+            if (gathered[o + i]) B->gat.push_back(o + i);
+    }
+    B->ng = B->gat.size();
+    I.gather_items = B->ng;
+    I.s_gather_set = secs(t0);
+    if (B->ng) TRYH(hipMemcpyAsync(B->d_gat.ptr, B->gat.data(), B->ng * 4, hipMemcpyHostToDevice, st));
+    TRYH(hipMemcpyAsync(B->d_gmask.ptr, gathered.data(), B->T, hipMemcpyHostToDevice, st));
+    tp_collect(V.texels, (const uint32_t *)B->d_gat.ptr, (uint32_t)B->ng,
+               (lb_texel *)B->d_gitems.ptr, st);
+
+    // the bounces (bake.rs:300-330)
+    TRYH(hipEventRecord(B->ev[3], st));
+    if (sc.gi_bounces > 0 && B->ng > 0) {
+        if (B->opt.graph && !per_pass) {
+            // one bounce captured, replayed gi_bounces times (same buffers each time); the item
+            // count is part of the capture, so a new count captures again
+            if (!B->graph || B->graph_ng != B->ng) {
+                if (B->graph) hipGraphExecDestroy(B->graph);
+                B->graph = nullptr;
+                hipGraph_t gr;
+                TRYH(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+                int rc = bounce(B, nullptr);
+                hipError_t ce = hipStreamEndCapture(st, &gr);
+                if (rc) return 1;
+                if (ce != hipSuccess) return hip_fail(ce, "graph capture");
+                hipError_t ie = hipGraphInstantiate(&B->graph, gr, nullptr, nullptr, 0);
+                hipGraphDestroy(gr);
+                if (ie != hipSuccess) return hip_fail(ie, "graph instantiate");
+                B->graph_ng = B->ng;
+            }
+            for (uint32_t b = 0; b < sc.gi_bounces; b++) TRYH(hipGraphLaunch(B->graph, st));
+        } else {
+            for (uint32_t b = 0; b < sc.gi_bounces; b++) TRY(bounce(B, sg));
+        }
+    }
+    TRYH(hipEventRecord(B->ev[4], st));
+
+    // the volume (volume.rs:77)
+    if (sc.has_volume)
+        TRY(pass_stats(B, LB_PASS_VOLUME, &B->PV, (const lb_texel *)B->d_pitems.ptr, B->np,
+                       (const float *)B->d_light.ptr, (size_t)B->aw * B->ah,
+                       (float *)B->d_pres.ptr, sv));
+    TRYH(hipEventRecord(B->ev[5], st));
+    TRYH(hipStreamSynchronize(st));
+    TRYH(hipEventElapsedTime(&I.ms_direct, B->ev[0], B->ev[1]));
+    TRYH(hipEventElapsedTime(&I.ms_direct_texel, B->ev[1], B->ev[2]));
+    TRYH(hipEventElapsedTime(&I.ms_bounces, B->ev[3], B->ev[4]));
+    TRYH(hipEventElapsedTime(&I.ms_volume, B->ev[4], B->ev[5]));
+    I.s_execute = secs(t_exec);
+    return 0;
+}
+
+int bake_finish(lb_bake *B) {
+    lb_gpu *gpu = B->gpu;
+    hipStream_t st = gpu->stream;
+    const lb_scene_desc &sc = B->sc;
+    lb_bake_info &I = B->info;
+    auto t0 = clk::now();
+    size_t T = B->T;
+    uint32_t aw = B->aw, ah = B->ah;
     std::vector<uint8_t> known(T), state(T);
     std::vector<float> field(T), ao(T), dir4(T * 4), ind4(T * 4);
     B->light.resize((size_t)aw * ah * 4);
-    TRYH(hipMemcpyAsync(known.data(), d_known.ptr, T, hipMemcpyDeviceToHost, st));
-    TRYH(hipMemcpyAsync(state.data(), d_state.ptr, T, hipMemcpyDeviceToHost, st));
-    TRYH(hipMemcpyAsync(field.data(), d_field.ptr, T * 4, hipMemcpyDeviceToHost, st));
-    TRYH(hipMemcpyAsync(ao.data(), d_ao.ptr, T * 4, hipMemcpyDeviceToHost, st));
-    TRYH(hipMemcpyAsync(dir4.data(), d_direct.ptr, T * 16, hipMemcpyDeviceToHost, st));
-    TRYH(hipMemcpyAsync(ind4.data(), d_indirect.ptr, T * 16, hipMemcpyDeviceToHost, st));
-    TRYH(hipMemcpyAsync(B->light.data(), d_light.ptr, (size_t)aw * ah * 16, hipMemcpyDeviceToHost, st));
+    std::vector<float> probes(B->np * 16);
+    if (T) {
+        TRYH(hipMemcpyAsync(known.data(), B->d_known.ptr, T, hipMemcpyDeviceToHost, st));
+        TRYH(hipMemcpyAsync(state.data(), B->d_state.ptr, T, hipMemcpyDeviceToHost, st));
+        TRYH(hipMemcpyAsync(field.data(), B->d_field.ptr, T * 4, hipMemcpyDeviceToHost, st));
+        TRYH(hipMemcpyAsync(ao.data(), B->d_ao.ptr, T * 4, hipMemcpyDeviceToHost, st));
+        TRYH(hipMemcpyAsync(dir4.data(), B->d_direct.ptr, T * 16, hipMemcpyDeviceToHost, st));
+        TRYH(hipMemcpyAsync(ind4.data(), B->d_indirect.ptr, T * 16, hipMemcpyDeviceToHost, st));
+    }
+    TRYH(hipMemcpyAsync(B->light.data(), B->d_light.ptr, (size_t)aw * ah * 16, hipMemcpyDeviceToHost, st));
+    if (B->np) TRYH(hipMemcpyAsync(probes.data(), B->d_pres.ptr, B->np * 64, hipMemcpyDeviceToHost, st));
     TRYH(hipStreamSynchronize(st));
-    TRYH(hipEventElapsedTime(&I.ms_direct, ev[0], ev[1]));
-    TRYH(hipEventElapsedTime(&I.ms_direct_texel, ev[1], ev[2]));
-    TRYH(hipEventElapsedTime(&I.ms_bounces, ev[3], ev[4]));
-    TRYH(hipEventElapsedTime(&I.ms_volume, ev[4], ev[5]));
-    t0 = clk::now();
 
     // probes inside solids from their neighbours, 8 rings (volume.rs:86-117)
-    if (sc.has_volume) {
-        size_t nx = vsz[0], ny = vsz[1], nz = vsz[2], np = nx * ny * nz;
+    size_t nx = B->vsz[0], ny = B->vsz[1], nz = B->vsz[2], np = B->np;
+    if (np) {
         std::vector<uint8_t> pk(np);
         for (size_t k = 0; k < np; k++) pk[k] = probes[k * 16 + 7] <= VOLUME_INSIDE;
         for (int ring = 0; ring < 8; ring++) {
@@ -524,7 +636,8 @@ int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, l
                 const long d6[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
                 for (auto &d : d6) {
                     long xx = x + d[0], yy = y + d[1], zz = z + d[2];
-                    if (xx < 0 || yy < 0 || zz < 0 || xx >= (long)nx || yy >= (long)ny || zz >= (long)nz) continue;
+                    if (xx < 0 || yy < 0 || zz < 0 || xx >= (long)nx || yy >= (long)ny || zz >= (long)nz)
+                        continue;
                     size_t j = ((size_t)yy * nz + (size_t)zz) * nx + (size_t)xx;
                     if (!was[j]) continue;
                     for (int q = 0; q < 16; q++) sum[q] += src[j * 16 + q];
@@ -536,23 +649,18 @@ int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, l
                 }
             }
         }
-        for (int a = 0; a < 3; a++) {
-            I.volume_size[a] = vsz[a];
-            I.volume_origin[a] = sc.volume_min[a];
-        }
-        I.volume_step = sc.volume_step;
     }
 
     // texels with no light of their own take the volume's (bake.rs:337-359)
     std::vector<uint8_t> validity_t(state);
-    if (sc.has_volume) {
+    if (np) {
         for (size_t gi = 0; gi < T; gi++) {
-            if (!covered[gi] || known[gi] != 0) continue;
-            V3 n{normal[gi * 4], normal[gi * 4 + 1], normal[gi * 4 + 2]};
-            V3 p{texels[gi].pos[0], texels[gi].pos[1], texels[gi].pos[2]};
+            if (!B->covered[gi] || known[gi] != 0) continue;
+            V3 n{B->normal[gi * 4], B->normal[gi * 4 + 1], B->normal[gi * 4 + 2]};
+            V3 p{B->texels[gi].pos[0], B->texels[gi].pos[1], B->texels[gi].pos[2]};
             float e[3], s;
-            volume_irradiance(probes, vsz, sc.volume_min, sc.volume_step, p + n * 0.15f, n, e, &s);
-            float c = std::fmax(dot(n, sun), 0.0f);
+            volume_irradiance(probes, B->vsz, sc.volume_min, sc.volume_step, p + n * 0.15f, n, e, &s);
+            float c = std::fmax(dot(n, B->sun), 0.0f);
             for (int k = 0; k < 3; k++) {
                 ind4[gi * 4 + k] = e[k];
                 dir4[gi * 4 + k] = sc.sun_irradiance[k] * c * s;
@@ -565,15 +673,17 @@ int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, l
     }
 
     // the atlas from every tile (bake.rs:364-421)
+    const Layout &L = B->layout;
+    uint32_t ds = B->ds;
     uint32_t iw = (aw + ds - 1) / ds, ih = (ah + ds - 1) / ds;
     B->direct.assign((size_t)aw * ah * 2, 1.0f);
     B->validity.assign((size_t)aw * ah, VALIDITY_NONE);
     std::vector<float> sum((size_t)iw * ih * 4, 0.0f);
-    for (size_t t = 0; t < nt; t++) {
+    for (size_t t = 0; t < L.tiles.size(); t++) {
         const Tile &tl = L.tiles[t];
         for (uint32_t y = 0; y < tl.h; y++)
             for (uint32_t x = 0; x < tl.w; x++) {
-                size_t gi = toff[t] + (size_t)y * tl.w + x;
+                size_t gi = B->toff[t] + (size_t)y * tl.w + x;
                 if (known[gi] == 0) continue;
                 size_t ax = tl.x + x, ay = tl.y + y;
                 B->direct[(ay * aw + ax) * 2] = field[gi];
@@ -615,19 +725,48 @@ int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, l
             }
     }
     B->volume = std::move(probes);
-    B->chart_tiles = L.chart_tile;
-    B->tiles.resize(nt * 6);
-    for (size_t t = 0; t < nt; t++) {
-        const Tile &tl = L.tiles[t];
-        int32_t v[6] = {tl.kx, tl.kz, (int32_t)tl.w, (int32_t)tl.h, (int32_t)tl.x, (int32_t)tl.y};
-        std::memcpy(&B->tiles[t * 6], v, sizeof v);
-    }
     I.atlas_width = aw;
     I.atlas_height = ah;
     I.indirect_width = iw;
     I.indirect_height = ih;
     I.s_finish = secs(t0);
-    I.s_total = secs(t_start);
+    return 0;
+}
+
+// Times `reps` launches of one pass over the bake's own items (resident), with HIP events.
+int bake_time_pass(lb_bake *B, int pass, int reps, float *ms_avg) {
+    lb_gpu *gpu = B->gpu;
+    hipStream_t st = gpu->stream;
+    const lb_texel *items;
+    size_t n;
+    const lb_params *P = &B->P;
+    float *out;
+    if (pass == LB_PASS_DIRECT) {
+        items = (const lb_texel *)B->d_items.ptr;
+        n = B->ncov;
+        out = (float *)B->d_res.ptr;
+    } else if (pass == LB_PASS_GATHER) {
+        items = (const lb_texel *)B->d_gitems.ptr;
+        n = B->ng;
+        out = (float *)B->d_gres.ptr;
+    } else {
+        items = (const lb_texel *)B->d_pitems.ptr;
+        n = B->np;
+        P = &B->PV;
+        out = (float *)B->d_pres.ptr;
+    }
+    if (!n) return fail("bake: pass %d has no items (execute first)", pass);
+    if (reps < 1) return fail("bake: reps %d", reps);
+    TRYH(hipEventRecord(B->ev[6], st));
+    for (int r = 0; r < reps; r++)
+        TRY(run_pass(gpu, pass, B->opt.form, B->scene, P, items, n, 0,
+                     pass == LB_PASS_DIRECT ? nullptr : (const float *)B->d_light.ptr,
+                     (size_t)B->aw * B->ah, out, false));
+    TRYH(hipEventRecord(B->ev[7], st));
+    TRYH(hipEventSynchronize(B->ev[7]));
+    float ms = 0.0f;
+    TRYH(hipEventElapsedTime(&ms, B->ev[6], B->ev[7]));
+    *ms_avg = ms / (float)reps;
     return 0;
 }
 
@@ -635,20 +774,49 @@ int bake_run(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &opt, l
 
 extern "C" {
 
-int lb_bake_run(lb_gpu *gpu, const lb_scene_desc *scene, const lb_bake_options *options,
-                lb_bake **out) {
-    if (!gpu || !scene || !out) return fail("lb_bake_run: null handle");
+int lb_bake_prepare(lb_gpu *gpu, const lb_scene_desc *scene, const lb_bake_options *options,
+                    lb_bake **out) {
+    if (!gpu || !scene || !out) return fail("lb_bake_prepare: null handle");
     *out = nullptr;
     lb_bake_options opt{LB_FORM_PLAIN, 0, 0};
     if (options) opt = *options;
-    if (opt.form != LB_FORM_PLAIN && opt.form != LB_FORM_TUNED) return fail("lb_bake_run: no form %d", opt.form);
+    if (opt.form != LB_FORM_PLAIN && opt.form != LB_FORM_TUNED)
+        return fail("lb_bake_prepare: no form %d", opt.form);
     auto *b = new lb_bake();
-    if (bake_run(gpu, *scene, opt, b)) {
-        hipStreamSynchronize(gpu->stream);
+    if (bake_prepare(gpu, *scene, opt, b)) {
         delete b;
         return 1;
     }
     *out = b;
+    return 0;
+}
+
+int lb_bake_execute(lb_bake *bake, lb_stats *per_pass) {
+    if (!bake) return fail("lb_bake_execute: null handle");
+    if (per_pass) std::memset(per_pass, 0, 3 * sizeof(lb_stats));
+    return bake_execute(bake, per_pass);
+}
+
+int lb_bake_finish(lb_bake *bake) {
+    if (!bake) return fail("lb_bake_finish: null handle");
+    return bake_finish(bake);
+}
+
+int lb_bake_time_pass(lb_bake *bake, int pass, int reps, float *ms_avg) {
+    if (!bake || !ms_avg) return fail("lb_bake_time_pass: null handle");
+    if (pass < LB_PASS_DIRECT || pass > LB_PASS_VOLUME) return fail("lb_bake_time_pass: no pass %d", pass);
+    return bake_time_pass(bake, pass, reps, ms_avg);
+}
+
+int lb_bake_run(lb_gpu *gpu, const lb_scene_desc *scene, const lb_bake_options *options,
+                lb_bake **out) {
+    if (lb_bake_prepare(gpu, scene, options, out)) return 1;
+    if (bake_execute(*out, nullptr) || bake_finish(*out)) {
+        lb_bake_destroy(*out);
+        *out = nullptr;
+        return 1;
+    }
+    (*out)->info.s_total += (*out)->info.s_execute + (*out)->info.s_finish;
     return 0;
 }
 
@@ -674,6 +842,12 @@ size_t lb_bake_array(const lb_bake *b, int which, const void **data) {
     case LB_BAKE_LIGHT: p = b->light.data(); n = b->light.size() * 4; break;
     case LB_BAKE_CHART_TILES: p = b->chart_tiles.data(); n = b->chart_tiles.size() * 4; break;
     case LB_BAKE_TILES: p = b->tiles.data(); n = b->tiles.size() * 4; break;
+    case LB_BAKE_NODES: p = b->nodes.data(); n = b->nodes.size() * sizeof(lb_node); break;
+    case LB_BAKE_TRIS: p = b->tris.data(); n = b->tris.size() * sizeof(lb_tri); break;
+    case LB_BAKE_MATERIALS: p = b->mats.data(); n = b->mats.size() * 4; break;
+    case LB_BAKE_PARAMS: p = &b->P; n = sizeof(lb_params); break;
+    case LB_BAKE_TEXELS: p = b->texels.data(); n = b->texels.size() * sizeof(lb_texel); break;
+    case LB_BAKE_GATHER_INDEX: p = b->gat.data(); n = b->gat.size() * 4; break;
     default: break;
     }
     if (data) *data = p;

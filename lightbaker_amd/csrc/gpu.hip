@@ -216,6 +216,7 @@ int lb_run_stats(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const l
     stats->tri_tests = c[2];
     stats->material_loads = c[3];
     stats->light_gathers = c[4];
+    stats->leaf_visits = c[5];
     return 0;
 }
 

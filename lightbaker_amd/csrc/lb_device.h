@@ -218,7 +218,7 @@ __device__ __forceinline__ void offset(uint32_t q, float *cx, float *cy) {
 
 // Per-thread traversal counters, flushed to the lb_stats block at the end of a kernel.
 struct Counts {
-    uint64_t rays, inner, tris, mats, gathers;
+    uint64_t rays, inner, tris, mats, gathers, leaves;
 };
 
 __device__ __forceinline__ void flush_counts(const Counts &c, unsigned long long *dst) {
@@ -227,6 +227,7 @@ __device__ __forceinline__ void flush_counts(const Counts &c, unsigned long long
     atomicAdd(dst + 2, (unsigned long long)c.tris);
     atomicAdd(dst + 3, (unsigned long long)c.mats);
     atomicAdd(dst + 4, (unsigned long long)c.gathers);
+    atomicAdd(dst + 5, (unsigned long long)c.leaves);
 }
 
 }  // namespace lbk

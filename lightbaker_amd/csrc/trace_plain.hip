@@ -21,6 +21,7 @@ __device__ Hit trace_plain(const SceneView &S, const lb_params &P, f3 o, f3 d, f
         float4 nb = S.nodes[n * 2u + 1u];
         uint32_t first = fbits(na.w), count = fbits(nb.w);
         if (count > 0u) {
+            if (STATS) cnt.leaves++;
             for (uint32_t i = first; i < first + count; i++) {
                 float4 v0 = S.tris[i * 3u], e1 = S.tris[i * 3u + 1u], e2 = S.tris[i * 3u + 2u];
                 if (STATS) cnt.tris++;
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(64) void k_direct_plain(SceneView S, lb_params P,
                                                      unsigned long long *stats) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_items) return;
-    Counts cnt{0, 0, 0, 0, 0};
+    Counts cnt{0, 0, 0, 0, 0, 0};
     const float4 *tp = reinterpret_cast<const float4 *>(items + i);
     float4 tpos = tp[0], tnrm = tp[1], tdx = tp[2], tdy = tp[3];
     f3 n = xyz(tnrm);
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(64) void k_gather_plain(SceneView S, lb_params P,
                                                      unsigned long long *stats) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_items) return;
-    Counts cnt{0, 0, 0, 0, 0};
+    Counts cnt{0, 0, 0, 0, 0, 0};
     const float4 *tp = reinterpret_cast<const float4 *>(items + i);
     float4 tpos = tp[0], tnrm = tp[1], tdx = tp[2], tdy = tp[3];
     f3 n = xyz(tnrm);
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(64) void k_volume_plain(SceneView S, lb_params P,
                                                      unsigned long long *stats) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_items) return;
-    Counts cnt{0, 0, 0, 0, 0};
+    Counts cnt{0, 0, 0, 0, 0, 0};
     const float4 *tp = reinterpret_cast<const float4 *>(items + i);
     float4 tpos = tp[0], tnrm = tp[1];
     f3 o = xyz(tpos);

@@ -75,6 +75,7 @@ __device__ Hit trace_t(const SceneView &S, const lb_params &P, uint32_t root, f3
     for (;;) {
         if (n & LEAF_BIT) {
             uint2 li = S.leaves[n & ~LEAF_BIT];
+            if (STATS) cnt.leaves++;
             for (uint32_t i = li.x; i < li.x + li.y; i++) {
                 float4 v0 = S.tris[i * 3u], e1 = S.tris[i * 3u + 1u], e2 = S.tris[i * 3u + 2u];
                 if (STATS) cnt.tris++;
@@ -265,7 +266,7 @@ __global__ void k_direct_tuned(TunedArgs A) {
     uint32_t g = P.counts[0], gg = g * g;
     f3 sun = mk(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]);
     frame_t sf = frame(sun);
-    Counts cnt{0, 0, 0, 0, 0};
+    Counts cnt{0, 0, 0, 0, 0, 0};
     for (;;) {
         uint32_t unit = next_unit(A.queue, n_units);
         if (unit == 0xffffffffu) break;
@@ -371,7 +372,7 @@ __global__ void k_gather_tuned(TunedArgs A) {
     uint32_t n_units = (A.n_items + per_wave - 1u) / per_wave;
     uint32_t seed = fbits(P.reach[3]);
     uint32_t rays = P.counts[2];
-    Counts cnt{0, 0, 0, 0, 0};
+    Counts cnt{0, 0, 0, 0, 0, 0};
     for (;;) {
         uint32_t unit = next_unit(A.queue, n_units);
         if (unit == 0xffffffffu) break;
@@ -434,7 +435,7 @@ __global__ void k_volume_tuned(TunedArgs A) {
     uint32_t g = P.counts[0], gg = g * g;
     f3 sund = mk(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]);
     frame_t sf = frame(sund);
-    Counts cnt{0, 0, 0, 0, 0};
+    Counts cnt{0, 0, 0, 0, 0, 0};
     for (;;) {
         uint32_t unit = next_unit(A.queue, n_units);
         if (unit == 0xffffffffu) break;

@@ -17,7 +17,8 @@ LIB_PATH = HERE / "liboracle.so"
 
 class Counters(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("inner_visits", C.c_uint64), ("tri_tests", C.c_uint64),
-                ("material_loads", C.c_uint64), ("light_gathers", C.c_uint64)]
+                ("material_loads", C.c_uint64), ("light_gathers", C.c_uint64),
+                ("leaf_visits", C.c_uint64)]
 
 
 _lib = None

@@ -1,0 +1,199 @@
+"""ORACLE — test infrastructure, not product code.
+
+The device half of the reference's bake (crates/lightbaker/src/bake.rs:149-330, volume.rs:69-117)
+and its host finish (bake.rs:337-421) restated over the same inputs the package's bake prepares
+(its layout, texels, BVH and Params, read back from a prepared bake), with the ray passes and
+the texel passes of the oracle's C restatement (trace_ref.c, filter_ref.c). A plain-form bake of
+the package must equal it bit for bit.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib, run
+
+VALIDITY_NONE, VALIDITY_CENTRE, VALIDITY_OFFSET = 0, 1, 2
+VALIDITY_PUSHED, VALIDITY_FILLED, VALIDITY_VOLUME = 3, 4, 5
+PUSH = np.float32(0.03)
+F = np.float32
+
+
+def _filters():
+    L = lib()
+    if not hasattr(L, "_filters_bound"):
+        VP, I = C.c_void_p, C.c_int
+        L.orc_smooth.argtypes = [I, I, VP, VP, VP, I, VP, I, VP]
+        L.orc_smooth.restype = None
+        L.orc_fill.argtypes = [I, I, VP, VP, I, VP, I]
+        L.orc_fill.restype = None
+        L.orc_sun_field.argtypes = [I, I, VP, VP, VP, VP, I, C.c_float, VP]
+        L.orc_sun_field.restype = None
+        L._filters_bound = True
+    return L
+
+
+def smooth(w, h, chart, normal, v, valid, r):
+    L = _filters()
+    v = np.ascontiguousarray(v, dtype=np.float32)
+    C_ = v.shape[1] if v.ndim == 2 else 1
+    out = np.empty_like(v)
+    L.orc_smooth(w, h, chart.ctypes.data, normal.ctypes.data, v.ctypes.data, C_,
+                 np.ascontiguousarray(valid, np.uint8).ctypes.data, r, out.ctypes.data)
+    return out
+
+
+def fill(w, h, chart, v, valid, rings):
+    """In place on v and valid."""
+    L = _filters()
+    C_ = v.shape[1] if v.ndim == 2 else 1
+    L.orc_fill(w, h, chart.ctypes.data, v.ctypes.data, C_, valid.ctypes.data, rings)
+
+
+def sun_field(w, h, chart, normal, hard, covered, g, radius):
+    L = _filters()
+    out = np.empty(w * h, np.float32)
+    L.orc_sun_field(w, h, chart.ctypes.data, normal.ctypes.data, hard.ctypes.data,
+                    np.ascontiguousarray(covered, np.uint8).ctypes.data, g, radius,
+                    out.ctypes.data)
+    return out
+
+
+def offset(q):
+    """bake.rs:495-503"""
+    if 1 <= q <= 4:
+        return (F(0.3) if q & 1 else F(-0.3), F(0.3) if q > 2 else F(-0.3))
+    return {5: (F(-0.45), F(0)), 6: (F(0.45), F(0)), 7: (F(0), F(-0.45))}.get(q, (F(0), F(0.45)))
+
+
+def bake(inp: dict) -> dict:
+    """inp: texels (T,) TEXEL tile after tile; tiles (nt, 6); aw, ah; nodes, tris, mats, charts,
+    params; settings (lightmap/bake fields); sun (3,), irr (3,); probes (np,) TEXEL or None,
+    volume_params; volume (min, step, size)."""
+    st = inp["settings"]
+    tex = inp["texels"].copy()
+    tiles = inp["tiles"]
+    aw, ah = inp["aw"], inp["ah"]
+    P = inp["params"]
+    nodes, tris, mats, charts = inp["nodes"], inp["tris"], inp["mats"], inp["charts"]
+    g = int(P["counts"][0][0])
+    sun = inp["sun"].astype(np.float32)
+    irr = np.asarray(inp["irr"], np.float32)
+    offs = np.concatenate([[0], np.cumsum(tiles[:, 2] * tiles[:, 3])]).astype(np.int64)
+    T = int(offs[-1])
+    covered = tex["pos"][:, 3] > 0.5
+    chart = np.where(covered, tex["normal"][:, 3].view(np.uint32), np.uint32(0xFFFFFFFF)).astype(np.uint32)
+    normal = np.ascontiguousarray(tex["normal"][:, :3], np.float32)
+
+    # the direct pass over the covered texels (bake.rs:149-151)
+    cov = np.nonzero(covered)[0]
+    res = run(0, nodes, tris, mats, charts, P, tex[cov], 0, None)
+    valid = covered.copy()
+    state = np.zeros(T, np.uint8)
+    soft = np.zeros(T, np.float32)
+    ao = np.ones(T, np.float32)
+    hard = np.zeros(T, np.uint32)
+    for k, gi in enumerate(cov):
+        r = res[k]
+        if r[3] > F(st.inside_share):
+            valid[gi] = False
+            continue
+        soft[gi] = r[0]
+        bits = int(r[1:2].view(np.uint32)[0])
+        hard[gi] = bits & 0x00FFFFFF
+        pick = bits >> 24
+        state[gi] = VALIDITY_CENTRE if pick == 0 else (VALIDITY_PUSHED if pick == 9 else VALIDITY_OFFSET)
+        if pick == 9:
+            tex["pos"][gi, :3] = tex["pos"][gi, :3] + tex["normal"][gi, :3] * PUSH
+        elif pick > 0:
+            cx, cy = offset(pick)
+            tex["pos"][gi, :3] = tex["pos"][gi, :3] + (tex["dx"][gi, :3] * cx + tex["dy"][gi, :3] * cy)
+        v = max(F(1.0) - F(st.ao_strength) * r[2], F(0.0))
+        if st.ao_contrast != 1.0:
+            v = F(v ** F(st.ao_contrast))
+        ao[gi] = max(v, F(st.ao_floor))
+    known = np.zeros(T, np.uint8)
+    field = np.zeros(T, np.float32)
+    direct = np.zeros((T, 3), np.float32)
+    indirect = np.zeros((T, 3), np.float32)
+    rings = st.padding + 4
+    for t in range(len(tiles)):
+        w, h = int(tiles[t, 2]), int(tiles[t, 3])
+        sl = slice(offs[t], offs[t + 1])
+        ch, nr, vl = chart[sl].copy(), normal[sl].copy(), valid[sl].astype(np.uint8)
+        a = smooth(w, h, ch, nr, ao[sl], vl, 1)
+        s = smooth(w, h, ch, nr, soft[sl], vl, 1)
+        f = sun_field(w, h, ch, nr, hard[sl].copy(), vl, g, F(st.sdf_radius))
+        c = np.maximum(nr[:, 0] * sun[0] + nr[:, 1] * sun[1] + nr[:, 2] * sun[2], F(0))
+        d = np.stack([irr[0] * c * s, irr[1] * c * s, irr[2] * c * s], axis=1).astype(np.float32)
+        for arr in (a, f):
+            kk = vl.copy()
+            fill(w, h, ch, arr, kk, rings)
+        kn = vl.copy()
+        d = np.ascontiguousarray(d)
+        fill(w, h, ch, d, kn, rings)
+        ao[sl], field[sl], direct[sl] = a, f, d
+        stt = state[sl]
+        stt[covered[sl] & (stt == VALIDITY_NONE) & (kn > 0)] = VALIDITY_FILLED
+        state[sl] = stt
+        known[sl] = np.where(vl > 0, 2, kn)
+
+    def write_light(light):
+        for t in range(len(tiles)):
+            w, h, x0, y0 = (int(v) for v in tiles[t, 2:6])
+            sl = slice(offs[t], offs[t + 1])
+            kn = known[sl].reshape(h, w)
+            val = np.zeros((h, w, 4), np.float32)
+            dd = (direct[sl] + indirect[sl]).reshape(h, w, 3)
+            val[..., :3] = np.where(kn[..., None] >= 1, dd, 0)
+            val[..., 3] = np.where(kn >= 1, 1.0, 0.0)
+            light[y0:y0 + h, x0:x0 + w] = val
+
+    light = np.zeros((ah, aw, 4), np.float32)
+    write_light(light)
+
+    # the gather set (bake.rs:282-299)
+    ds = st.indirect_downscale
+    gathered = np.zeros(T, bool)
+    for t in range(len(tiles)):
+        tw = int(tiles[t, 2])
+        o = int(offs[t])
+        size = int(offs[t + 1] - offs[t])
+        bw = -(-tw // ds)
+        seen = set()
+        for i in range(size):
+            if valid[o + i] and (i % tw) % ds == 0 and (i // tw) % ds == 0:
+                gathered[o + i] = True
+                seen.add((((i // tw) // ds) * bw + (i % tw) // ds, int(chart[o + i])))
+        for i in range(size):
+            key = (((i // tw) // ds) * bw + (i % tw) // ds, int(chart[o + i]))
+            if valid[o + i] and key not in seen:
+                seen.add(key)
+                gathered[o + i] = True
+    gat = np.nonzero(gathered)[0]
+    grings = st.padding + 2 * ds + 2
+    for _ in range(st.gi_bounces):
+        if len(gat) == 0:
+            break
+        gres = run(1, nodes, tris, mats, charts, P, tex[gat], 0, light.reshape(-1, 4))
+        e = np.zeros((T, 3), np.float32)
+        e[gat] = gres[:, :3]
+        for t in range(len(tiles)):
+            w, h = int(tiles[t, 2]), int(tiles[t, 3])
+            sl = slice(offs[t], offs[t + 1])
+            gm = gathered[sl].astype(np.uint8)
+            ee = smooth(w, h, chart[sl].copy(), normal[sl].copy(), e[sl], gm,
+                        st.gi_filter * ds)
+            gk = gm.copy()
+            fill(w, h, chart[sl].copy(), ee, gk, grings)
+            indirect[sl] = ee
+        write_light(light)
+
+    probes = None
+    if inp.get("probes") is not None:
+        probes = run(2, nodes, tris, mats, charts, inp["volume_params"], inp["probes"], 0,
+                     light.reshape(-1, 4)).reshape(-1, 16)
+    return {"light": light, "known": known, "state": state, "field": field, "ao": ao,
+            "direct": direct, "indirect": indirect, "probes": probes, "gathered": gat,
+            "texels": tex}

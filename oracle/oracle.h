@@ -20,6 +20,7 @@ typedef struct orc_counters {
     uint64_t tri_tests;
     uint64_t material_loads;
     uint64_t light_gathers;
+    uint64_t leaf_visits;
 } orc_counters;
 
 void lb_sincos(float x, float *s, float *c);

@@ -183,6 +183,7 @@ static hit_t trace(const ctx_t *c, v3 o, v3 d, float tmax, int any, uint32_t ray
     for (;;) {
         const lb_node *node = &c->nodes[n];
         if (node->count > 0u) {
+            if (c->cnt) c->cnt->leaf_visits++;
             for (uint32_t i = node->first; i < node->first + node->count; i++) {
                 const lb_tri *tr = &c->tris[i];
                 if (c->cnt) c->cnt->tri_tests++;
