@@ -86,6 +86,7 @@ def main() -> int:
     ap.add_argument("--nodes", choices=["wide", "quant"], default="wide")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
@@ -103,12 +104,22 @@ def main() -> int:
     scene = map_scene(2)
     t_scene = time.perf_counter() - t_scene
     gpu = Gpu()
-    gpu.set_tuning(256, 4, 16, layout)
+    gpu.set_tuning(256, 4, 16, layout, 48, 0)
     print(f"# {gpu.adapter}; scene {scene.triangle_count()} triangles ({t_scene:.1f} s)",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     sess = BakeSession(scene, gpu, form=form, graph=True)
     prep_s = time.perf_counter() - t0
+    tuned = None
+    if form == N.FORM_TUNED and not args.no_autotune:
+        from lightbaker_amd.autotune import autotune
+        t0 = time.perf_counter()
+        sess.execute()
+        tuned = autotune(sess, scene.settings.gi_bounces,
+                         log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+        tuned["seconds"] = time.perf_counter() - t0
+        print(f"# autotune: {tuned}", file=sys.stderr, flush=True)
+        layout = tuned["config"]["nodes"]
     stats = sess.execute(stats=True)
     rays = sum(s["rays"] for s in stats)
     info = sess.info()
@@ -166,7 +177,8 @@ def main() -> int:
         "data": "synthetic: seeded map generator (fBm terrain + boxes + cylinders), no dataset",
         "config": {
             "workload": "map_scene(2): 100x100 m, 8 texels/m, 256 GI rays, 3 bounces, 16 tiles",
-            "form": args.form, "nodes": args.nodes, "graph": True,
+            "form": args.form, "nodes": "wide" if layout == N.NODES_WIDE else "quant",
+            "graph": True, "tuning": tuned,
             "triangles": scene.triangle_count(), "bvh_nodes": info["bvh_nodes"],
             "texels": info["texels_baked"], "gather_items": info["gather_items"],
             "probes": info["probes"], "rays_per_step": rays,

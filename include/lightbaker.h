@@ -126,11 +126,18 @@ int lb_run_device(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const 
                   const lb_texel *items_dev, size_t n, uint32_t key_base, const float *light_dev,
                   size_t n_light, float *out_dev);
 
-/* Tuning of the TUNED form: workgroup size (64..1024, multiple of 64), rays per lane (1..8), LDS
- * stack depth (8..48 entries; deeper entries spill to a per-lane scratch area) and node layout. */
-int lb_set_tuning(lb_gpu *gpu, int workgroup, int rays_per_lane, int lds_stack, int nodes);
-int lb_get_tuning(const lb_gpu *gpu, int *workgroup, int *rays_per_lane, int *lds_stack,
-                  int *nodes);
+/* Tuning of the TUNED form. */
+typedef struct lb_tuning {
+    int workgroup;      /* threads a workgroup: 64..1024, a multiple of 64 */
+    int rays_per_lane;  /* rays a lane takes of one item (1..8): sets the lanes sharing an item */
+    int lds_stack;      /* traversal stack entries in LDS (8..48); deeper ones spill to memory */
+    int nodes;          /* LB_NODES_WIDE or LB_NODES_QUANT */
+    int refill;         /* flattened schedule: idle lanes (1..64) a wave waits for to restart */
+    int schedule;       /* 0: a lane traces each ray to the end; 1: flattened, batched refills */
+} lb_tuning;
+
+int lb_set_tuning(lb_gpu *gpu, const lb_tuning *tuning);
+int lb_get_tuning(const lb_gpu *gpu, lb_tuning *tuning);
 
 /* Traversal counters of the last lb_run_stats call: rays traced, inner-node visits (child-box
  * pairs tested), triangle tests, material loads, bounced() light gathers and leaf visits. */

@@ -36,6 +36,11 @@ class Stats(C.Structure):
                 ("leaf_visits", C.c_uint64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("workgroup", C.c_int), ("rays_per_lane", C.c_int), ("lds_stack", C.c_int),
+                ("nodes", C.c_int), ("refill", C.c_int), ("schedule", C.c_int)]
+
+
 class MaterialDesc(C.Structure):
     _fields_ = [("albedo", C.c_float * 3), ("opacity", C.c_float), ("receives", C.c_uint32),
                 ("casts", C.c_uint32)]
@@ -102,8 +107,8 @@ SIGNATURES = {
     "lb_download": (_I, [_VP, _VP, _VP, _SZ]),
     "lb_sync": (_I, [_VP]),
     "lb_run_device": (_I, [_VP, _I, _I, _VP, _VP, _VP, _SZ, _U32, _VP, _SZ, _VP]),
-    "lb_set_tuning": (_I, [_VP, _I, _I, _I, _I]),
-    "lb_get_tuning": (_I, [_VP, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "lb_set_tuning": (_I, [_VP, C.POINTER(Tuning)]),
+    "lb_get_tuning": (_I, [_VP, C.POINTER(Tuning)]),
     "lb_run_stats": (_I, [_VP, _I, _I, _VP, _VP, _VP, _SZ, _U32, _VP, _SZ, _VP, C.POINTER(Stats)]),
     "lb_timer_begin": (_I, [_VP]),
     "lb_timer_end": (_I, [_VP, C.POINTER(C.c_float)]),

@@ -117,11 +117,12 @@ struct lb_bake {
     DevBuf d_tex0, d_tex, d_tile_of, d_tiles, d_chart, d_normal, d_valid, d_known, d_state,
         d_hard, d_soft, d_ao, d_field, d_direct, d_indirect, d_cov, d_items, d_res, d_light, d_w1,
         d_wg, d_f1, d_f4, d_m1, d_m2, d_covered, d_gat, d_gmask, d_gitems, d_gres, d_filled,
-        d_pitems, d_pres;
+        d_pitems, d_pres, d_counts, d_total;
     TexelView V{};
     hipEvent_t ev[8] = {};
     hipGraphExec_t graph = nullptr;
     size_t graph_ng = 0;  // the gather item count the graph was captured with
+    Tuning graph_tuning;  // and the tuning
     // results
     std::vector<float> direct, indirect, volume, charts, light;
     std::vector<uint8_t> validity;
@@ -373,6 +374,8 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     TRY(B->d_gat.reserve(Tn * 4));
     TRY(B->d_gitems.reserve(Tn * sizeof(lb_texel)));
     TRY(B->d_gres.reserve(Tn * 16));
+    TRY(B->d_counts.reserve((size_t)tp_gather_blocks((uint32_t)Tn) * 4 + 16));
+    TRY(B->d_total.reserve(16));
     TRY(put(B->d_w1, gauss(1), st));
     B->ds = sc.indirect_downscale;
     B->rg = sc.gi_filter * sc.indirect_downscale;
@@ -522,35 +525,16 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
     TRYH(hipGetLastError());
     TRYH(hipEventRecord(B->ev[2], st));
 
-    // the gather set (bake.rs:282-299), on the host from the valid texels
-    std::vector<uint8_t> valid(B->T);
-    TRYH(hipMemcpyAsync(valid.data(), V.valid, B->T, hipMemcpyDeviceToHost, st));
-    TRYH(hipStreamSynchronize(st));
+    // the gather set (bake.rs:282-299) on the device; only its count comes back
     auto t0 = clk::now();
-    const Layout &L = B->layout;
-    uint32_t ds = B->ds;
-    std::vector<uint8_t> gathered(B->T, 0);
-    B->gat.clear();
-    for (size_t t = 0; t < L.tiles.size(); t++) {
-        uint32_t tw = L.tiles[t].w, size = L.tiles[t].w * L.tiles[t].h, o = B->toff[t];
-        uint32_t bw = (tw + ds - 1) / ds;
-        auto block = [&](uint32_t i) { return (uint64_t)((i / tw) / ds) * bw + (i % tw) / ds; };
-        std::set<std::pair<uint64_t, uint32_t>> seen;
-        for (uint32_t i = 0; i < size; i++) {
-            bool gt = valid[o + i] && (i % tw) % ds == 0 && (i / tw) % ds == 0;
-            gathered[o + i] = gt;
-            if (gt) seen.insert({block(i), B->chart[o + i]});
-        }
-        for (uint32_t i = 0; i < size; i++)
-            if (valid[o + i] && seen.insert({block(i), B->chart[o + i]}).second) gathered[o + i] = 1;
-        for (uint32_t i = 0; i < size; i++)
-            if (gathered[o + i]) B->gat.push_back(o + i);
-    }
-    B->ng = B->gat.size();
+    tp_gather_set(V, B->ds, (uint8_t *)B->d_gmask.ptr, (uint32_t *)B->d_counts.ptr,
+                  (uint32_t *)B->d_total.ptr, (uint32_t *)B->d_gat.ptr, st);
+    uint32_t ng32 = 0;
+    TRYH(hipMemcpyAsync(&ng32, B->d_total.ptr, 4, hipMemcpyDeviceToHost, st));
+    TRYH(hipStreamSynchronize(st));
+    B->ng = ng32;
     I.gather_items = B->ng;
     I.s_gather_set = secs(t0);
-    if (B->ng) TRYH(hipMemcpyAsync(B->d_gat.ptr, B->gat.data(), B->ng * 4, hipMemcpyHostToDevice, st));
-    TRYH(hipMemcpyAsync(B->d_gmask.ptr, gathered.data(), B->T, hipMemcpyHostToDevice, st));
     tp_collect(V.texels, (const uint32_t *)B->d_gat.ptr, (uint32_t)B->ng,
                (lb_texel *)B->d_gitems.ptr, st);
 
@@ -560,7 +544,15 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
         if (B->opt.graph && !per_pass) {
             // one bounce captured, replayed gi_bounces times (same buffers each time); the item
             // count is part of the capture, so a new count captures again
-            if (!B->graph || B->graph_ng != B->ng) {
+            const Tuning &tn = gpu->tuning;
+            const Tuning &tg = B->graph_tuning;
+            bool same = tn.workgroup == tg.workgroup && tn.rays_per_lane == tg.rays_per_lane &&
+                        tn.lds_stack == tg.lds_stack && tn.nodes == tg.nodes &&
+                        tn.refill == tg.refill && tn.schedule == tg.schedule;
+            if (!B->graph || B->graph_ng != B->ng || !same) {
+                if (B->opt.form == LB_FORM_TUNED &&
+                    tuned_prepare(gpu, LB_PASS_GATHER, B->P, (uint32_t)B->ng) != hipSuccess)
+                    return fail("GPU: tuned spill area");
                 if (B->graph) hipGraphExecDestroy(B->graph);
                 B->graph = nullptr;
                 hipGraph_t gr;
@@ -573,6 +565,7 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
                 hipGraphDestroy(gr);
                 if (ie != hipSuccess) return hip_fail(ie, "graph instantiate");
                 B->graph_ng = B->ng;
+                B->graph_tuning = tn;
             }
             for (uint32_t b = 0; b < sc.gi_bounces; b++) TRYH(hipGraphLaunch(B->graph, st));
         } else {
@@ -847,7 +840,16 @@ size_t lb_bake_array(const lb_bake *b, int which, const void **data) {
     case LB_BAKE_MATERIALS: p = b->mats.data(); n = b->mats.size() * 4; break;
     case LB_BAKE_PARAMS: p = &b->P; n = sizeof(lb_params); break;
     case LB_BAKE_TEXELS: p = b->texels.data(); n = b->texels.size() * sizeof(lb_texel); break;
-    case LB_BAKE_GATHER_INDEX: p = b->gat.data(); n = b->gat.size() * 4; break;
+    case LB_BAKE_GATHER_INDEX:
+        if (b->gat.size() != b->ng) {
+            auto *m = const_cast<lb_bake *>(b);
+            m->gat.resize(b->ng);
+            if (b->ng)
+                hipMemcpy(m->gat.data(), b->d_gat.ptr, b->ng * 4, hipMemcpyDeviceToHost);
+        }
+        p = b->gat.data();
+        n = b->gat.size() * 4;
+        break;
     default: break;
     }
     if (data) *data = p;

@@ -261,30 +261,38 @@ int lb_set_form(lb_gpu *gpu, int form) {
     return 0;
 }
 
-int lb_set_tuning(lb_gpu *gpu, int workgroup, int rays_per_lane, int lds_stack, int nodes) {
-    if (!gpu) return fail("lb_set_tuning: null handle");
-    if (workgroup < 64 || workgroup > 1024 || workgroup % 64)
-        return fail("lb_set_tuning: workgroup %d is not a multiple of 64 in 64..1024", workgroup);
-    if (rays_per_lane < 1 || rays_per_lane > 8)
-        return fail("lb_set_tuning: rays per lane %d outside 1..8", rays_per_lane);
-    if (lds_stack < 8 || lds_stack > 48)
-        return fail("lb_set_tuning: LDS stack %d outside 8..48", lds_stack);
-    if (nodes != LB_NODES_WIDE && nodes != LB_NODES_QUANT)
-        return fail("lb_set_tuning: no node layout %d", nodes);
-    gpu->tuning.workgroup = workgroup;
-    gpu->tuning.rays_per_lane = rays_per_lane;
-    gpu->tuning.lds_stack = lds_stack;
-    gpu->tuning.nodes = nodes;
+int lb_set_tuning(lb_gpu *gpu, const lb_tuning *t) {
+    if (!gpu || !t) return fail("lb_set_tuning: null handle");
+    if (t->workgroup < 64 || t->workgroup > 1024 || t->workgroup % 64)
+        return fail("lb_set_tuning: workgroup %d is not a multiple of 64 in 64..1024", t->workgroup);
+    if (t->rays_per_lane < 1 || t->rays_per_lane > 8)
+        return fail("lb_set_tuning: rays per lane %d outside 1..8", t->rays_per_lane);
+    if (t->lds_stack < 8 || t->lds_stack > 48)
+        return fail("lb_set_tuning: LDS stack %d outside 8..48", t->lds_stack);
+    if (t->nodes != LB_NODES_WIDE && t->nodes != LB_NODES_QUANT)
+        return fail("lb_set_tuning: no node layout %d", t->nodes);
+    if (t->refill < 1 || t->refill > 64) return fail("lb_set_tuning: refill %d outside 1..64", t->refill);
+    if (t->schedule != 0 && t->schedule != 1) return fail("lb_set_tuning: no schedule %d", t->schedule);
+    if ((size_t)t->lds_stack * t->workgroup * 4u > gpu->lds_per_block)
+        return fail("lb_set_tuning: an LDS stack of %d entries for %d lanes exceeds %zu bytes",
+                    t->lds_stack, t->workgroup, gpu->lds_per_block);
+    gpu->tuning.workgroup = t->workgroup;
+    gpu->tuning.rays_per_lane = t->rays_per_lane;
+    gpu->tuning.lds_stack = t->lds_stack;
+    gpu->tuning.nodes = t->nodes;
+    gpu->tuning.refill = t->refill;
+    gpu->tuning.schedule = t->schedule;
     return 0;
 }
 
-int lb_get_tuning(const lb_gpu *gpu, int *workgroup, int *rays_per_lane, int *lds_stack,
-                  int *nodes) {
-    if (!gpu) return fail("lb_get_tuning: null handle");
-    if (workgroup) *workgroup = gpu->tuning.workgroup;
-    if (rays_per_lane) *rays_per_lane = gpu->tuning.rays_per_lane;
-    if (lds_stack) *lds_stack = gpu->tuning.lds_stack;
-    if (nodes) *nodes = gpu->tuning.nodes;
+int lb_get_tuning(const lb_gpu *gpu, lb_tuning *t) {
+    if (!gpu || !t) return fail("lb_get_tuning: null handle");
+    t->workgroup = gpu->tuning.workgroup;
+    t->rays_per_lane = gpu->tuning.rays_per_lane;
+    t->lds_stack = gpu->tuning.lds_stack;
+    t->nodes = gpu->tuning.nodes;
+    t->refill = gpu->tuning.refill;
+    t->schedule = gpu->tuning.schedule;
     return 0;
 }
 

@@ -39,8 +39,11 @@ __device__ __forceinline__ f3 normalize3(f3 v) {
     float l = sqrtf(dot3(v, v));
     return mk(v.x / l, v.y / l, v.z / l);
 }
-__device__ __forceinline__ float vmin(float a, float b) { return b < a ? b : a; }
-__device__ __forceinline__ float vmax(float a, float b) { return b > a ? b : a; }
+// WGSL's min/max. No NaN reaches them here (finite scene coordinates, |1/d| <= 1e12), and the
+// sign of a zero they might pick never changes a comparison or a stored value, so the hardware's
+// v_min/v_max (fused into v_min3/v_max3) give the reference's results.
+__device__ __forceinline__ float vmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }

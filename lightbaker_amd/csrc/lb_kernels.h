@@ -12,6 +12,8 @@ struct Tuning {
     int rays_per_lane = 2;   // rays one lane traces of one item before the wave reduces
     int lds_stack = 16;      // traversal stack entries a lane keeps in LDS
     int nodes = LB_NODES_WIDE;
+    int refill = 16;         // idle lanes (of 64) a wave waits for before it starts new rays
+    int schedule = 0;        // 0: lock-step slots, 1: flattened with batched refills
 };
 
 hipError_t launch_plain(int pass, bool stats, const SceneView &S, const lb_params &P,

@@ -56,6 +56,11 @@ void tp_write_light(const TexelView &V, float4 *light, uint32_t atlas_w, hipStre
 void tp_collect(const lb_texel *texels, const uint32_t *idx, uint32_t n, lb_texel *items,
                 hipStream_t s);
 void tp_fill_f32(float *p, float v, uint32_t n, hipStream_t s);
+// bake.rs:286-299 on the device: flags (u8 per texel), then the flagged texels' indices in order
+// into `out` and their count into `total`; `counts` holds tp_gather_blocks(n) words
+void tp_gather_set(const TexelView &V, uint32_t ds, uint8_t *flag, uint32_t *counts,
+                   uint32_t *total, uint32_t *out, hipStream_t s);
+uint32_t tp_gather_blocks(uint32_t n);
 void tp_copy_f32(const float *src, float *dst, uint32_t n, hipStream_t s);
 void tp_copy_u8(const uint8_t *src, uint8_t *dst, uint32_t n, hipStream_t s);
 

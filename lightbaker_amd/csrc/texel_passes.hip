@@ -252,6 +252,98 @@ __global__ void k_copy_u8(const uint8_t *s, uint8_t *d, uint32_t n) {
     if (g < n) d[g] = s[g];
 }
 
+// bake.rs:286-297, per texel: valid texels on the downscale grid (a block's top-left texel), and
+// the first valid texel (row-major) of each chart in a ds x ds block whose grid texel is not of
+// that chart. A block's texels are all in its tile, so the test is local.
+__global__ void k_gather_flags(TexelView V, uint32_t ds, uint8_t *__restrict__ flag, uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    uint8_t f = 0;
+    if (V.valid[g]) {
+        Where p = where(V, g);
+        uint32_t x0 = p.x - p.x % ds, y0 = p.y - p.y % ds;
+        if (p.x == x0 && p.y == y0) {
+            f = 1;
+        } else {
+            uint32_t c = V.chart[g];
+            uint32_t grid = p.off + y0 * p.w + x0;
+            bool taken = V.valid[grid] && V.chart[grid] == c;
+            uint32_t y1 = min(y0 + ds, p.h), x1 = min(x0 + ds, p.w);
+            for (uint32_t yy = y0; yy < y1 && !taken; yy++)
+                for (uint32_t xx = x0; xx < x1; xx++) {
+                    if (yy > p.y || (yy == p.y && xx >= p.x)) break;
+                    uint32_t j = p.off + yy * p.w + xx;
+                    if (j != grid && V.valid[j] && V.chart[j] == c) {
+                        taken = true;
+                        break;
+                    }
+                }
+            f = taken ? 0 : 1;
+        }
+    }
+    flag[g] = f;
+}
+
+// Stream compaction of the flagged texels' indices, in order: per-block counts, one block's scan
+// of them, then each block writes its texels at its offset.
+constexpr uint32_t CBLOCK = 1024;
+
+__global__ __launch_bounds__(1024) void k_count(const uint8_t *__restrict__ flag, uint32_t n,
+                                                uint32_t *__restrict__ counts) {
+    __shared__ uint32_t part[16];
+    uint32_t g = blockIdx.x * CBLOCK + threadIdx.x;
+    uint32_t v = (g < n && flag[g]) ? 1u : 0u;
+    uint64_t ball = __ballot(v);
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = (uint32_t)__popcll(ball);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (uint32_t w = 0; w < CBLOCK / 64; w++) s += part[w];
+        counts[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_counts(uint32_t *__restrict__ counts, uint32_t nb,
+                                                      uint32_t *__restrict__ total) {
+    // exclusive scan of nb counts by one block, 1024 at a time
+    __shared__ uint32_t buf[CBLOCK];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += CBLOCK) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < nb ? counts[i] : 0u;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t off = 1; off < CBLOCK; off <<= 1) {
+            uint32_t t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0u;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) counts[i] = carry + buf[threadIdx.x] - v;
+        uint32_t last = buf[CBLOCK - 1];
+        __syncthreads();
+        carry += last;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(1024) void k_compact(const uint8_t *__restrict__ flag, uint32_t n,
+                                                  const uint32_t *__restrict__ offsets,
+                                                  uint32_t *__restrict__ out) {
+    __shared__ uint32_t part[16];
+    uint32_t g = blockIdx.x * CBLOCK + threadIdx.x;
+    uint32_t v = (g < n && flag[g]) ? 1u : 0u;
+    uint64_t ball = __ballot(v);
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = (uint32_t)__popcll(ball);
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; w++) before += part[w];
+    uint64_t below = lane ? (ball & ((~0ull) >> (64u - lane))) : 0ull;
+    if (v) out[offsets[blockIdx.x] + before + (uint32_t)__popcll(below)] = g;
+}
+
 // ---- launchers ---------------------------------------------------------------------------------
 
 static inline dim3 grid_for(uint32_t n) { return dim3((n + 255u) / 256u); }
@@ -299,6 +391,20 @@ void tp_collect(const lb_texel *texels, const uint32_t *idx, uint32_t n, lb_texe
 void tp_fill_f32(float *p, float v, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_fill_f32, grid_for(n), dim3(256), 0, s, p, v, n);
 }
+void tp_gather_set(const TexelView &V, uint32_t ds, uint8_t *flag, uint32_t *counts,
+                   uint32_t *total, uint32_t *out, hipStream_t s) {
+    uint32_t n = V.n;
+    if (!n) {
+        hipMemsetAsync(total, 0, 4, s);
+        return;
+    }
+    uint32_t nb = (n + CBLOCK - 1) / CBLOCK;
+    hipLaunchKernelGGL(k_gather_flags, grid_for(n), dim3(256), 0, s, V, ds, flag, n);
+    hipLaunchKernelGGL(k_count, dim3(nb), dim3(CBLOCK), 0, s, flag, n, counts);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(CBLOCK), 0, s, counts, nb, total);
+    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(CBLOCK), 0, s, flag, n, counts, out);
+}
+uint32_t tp_gather_blocks(uint32_t n) { return (n + CBLOCK - 1) / CBLOCK; }
 void tp_copy_f32(const float *src, float *dst, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_copy_f32, grid_for(n), dim3(256), 0, s, src, dst, n);
 }

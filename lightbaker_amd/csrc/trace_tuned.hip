@@ -64,59 +64,91 @@ __device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn
     }
 }
 
-// trace.wgsl:83-130 over the tuned node records
+// A ray in flight: trace.wgsl:83-130's loop state, so a lane can advance its ray one node visit
+// at a time and start its next ray as soon as this one ends.
+struct Ray {
+    f3 o, d, inv;
+    float t;       // the nearest accepted hit so far (tmax until one is found)
+    uint32_t tri;  // NONE until a hit
+    uint32_t n;    // the node to visit next (bit 31: a leaf)
+    uint32_t sp;   // stack entries
+    uint32_t key;  // stops()'s ray key
+    bool back, any;
+};
+
+__device__ __forceinline__ void ray_start(Ray &R, uint32_t root, f3 o, f3 d, float tmax, bool any,
+                                          uint32_t key) {
+    R.o = o;
+    R.d = d;
+    R.inv = inverse_dir(d);
+    R.t = tmax;
+    R.tri = NONE;
+    R.back = false;
+    R.n = root;
+    R.sp = 0;
+    R.key = key;
+    R.any = any;
+}
+
+// One node visit in the reference's order; true when the ray is finished (trace.wgsl:105-128).
+template <int LAYOUT, bool STATS>
+__device__ __forceinline__ bool ray_step(const SceneView &S, const lb_params &P, Ray &R,
+                                         const StackView &st, Counts &cnt) {
+    if (R.n & LEAF_BIT) {
+        uint2 li = S.leaves[R.n & ~LEAF_BIT];
+        if (STATS) cnt.leaves++;
+        for (uint32_t i = li.x; i < li.x + li.y; i++) {
+            float4 v0 = S.tris[i * 3u], e1 = S.tris[i * 3u + 1u], e2 = S.tris[i * 3u + 2u];
+            if (STATS) cnt.tris++;
+            float det;
+            float t = mt(R.o, R.d, v0, e1, e2, &det);
+            if (t > 1e-4f && t < R.t) {
+                if (STATS) cnt.mats++;
+                if (stops(S, P, i, v0, R.key)) {
+                    R.t = t;
+                    R.tri = i;
+                    R.back = det < 0.0f;
+                    if (R.any) return true;
+                }
+            }
+        }
+    } else {
+        f3 amn, amx, bmn, bmx;
+        uint32_t first, flags;
+        children<LAYOUT>(S, R.n, amn, amx, bmn, bmx, first, flags);
+        if (STATS) cnt.inner++;
+        float ta = slab(R.o, R.inv, amn, amx, R.t);
+        float tb = slab(R.o, R.inv, bmn, bmx, R.t);
+        uint32_t ra = first | ((flags & 1u) ? LEAF_BIT : 0u);
+        uint32_t rb = (first + 1u) | ((flags & 2u) ? LEAF_BIT : 0u);
+        bool an = ta <= tb;
+        uint32_t near = an ? ra : rb, far = an ? rb : ra;
+        float tn = vmin(ta, tb), tf = vmax(ta, tb);
+        if (tn < FAR) {
+            if (tf < FAR && R.sp < (uint32_t)STACK) {
+                st.put(R.sp, far);
+                R.sp++;
+            }
+            R.n = near;
+            return false;
+        }
+    }
+    if (R.sp == 0u) return true;
+    R.sp--;
+    R.n = st.get(R.sp);
+    return false;
+}
+
+// trace.wgsl:83-130 over the tuned node records, to the end
 template <int LAYOUT, bool STATS>
 __device__ Hit trace_t(const SceneView &S, const lb_params &P, uint32_t root, f3 o, f3 d,
                        float tmax, bool any, uint32_t ray, const StackView &st, Counts &cnt) {
-    Hit hit{tmax, NONE, false};
-    f3 inv = inverse_dir(d);
-    uint32_t sp = 0, n = root;
+    Ray R;
+    ray_start(R, root, o, d, tmax, any, ray);
     if (STATS) cnt.rays++;
-    for (;;) {
-        if (n & LEAF_BIT) {
-            uint2 li = S.leaves[n & ~LEAF_BIT];
-            if (STATS) cnt.leaves++;
-            for (uint32_t i = li.x; i < li.x + li.y; i++) {
-                float4 v0 = S.tris[i * 3u], e1 = S.tris[i * 3u + 1u], e2 = S.tris[i * 3u + 2u];
-                if (STATS) cnt.tris++;
-                float det;
-                float t = mt(o, d, v0, e1, e2, &det);
-                if (t > 1e-4f && t < hit.t) {
-                    if (STATS) cnt.mats++;
-                    if (stops(S, P, i, v0, ray)) {
-                        hit.t = t;
-                        hit.tri = i;
-                        hit.back = det < 0.0f;
-                        if (any) return hit;
-                    }
-                }
-            }
-        } else {
-            f3 amn, amx, bmn, bmx;
-            uint32_t first, flags;
-            children<LAYOUT>(S, n, amn, amx, bmn, bmx, first, flags);
-            if (STATS) cnt.inner++;
-            float ta = slab(o, inv, amn, amx, hit.t);
-            float tb = slab(o, inv, bmn, bmx, hit.t);
-            uint32_t ra = first | ((flags & 1u) ? LEAF_BIT : 0u);
-            uint32_t rb = (first + 1u) | ((flags & 2u) ? LEAF_BIT : 0u);
-            bool an = ta <= tb;
-            uint32_t near = an ? ra : rb, far = an ? rb : ra;
-            float tn = vmin(ta, tb), tf = vmax(ta, tb);
-            if (tn < FAR) {
-                if (tf < FAR && sp < (uint32_t)STACK) {
-                    st.put(sp, far);
-                    sp++;
-                }
-                n = near;
-                continue;
-            }
-        }
-        if (sp == 0u) break;
-        sp--;
-        n = st.get(sp);
+    while (!ray_step<LAYOUT, STATS>(S, P, R, st, cnt)) {
     }
-    return hit;
+    return Hit{R.t, R.tri, R.back};
 }
 
 // ---- group reductions -------------------------------------------------------------------------
@@ -204,6 +236,7 @@ struct TunedArgs {
     uint32_t *queue;
     uint32_t *spill;
     uint32_t lanes_per_item, depth;
+    uint32_t refill;  // idle lanes a wave waits for before it starts new rays
 };
 
 __device__ __forceinline__ StackView make_stack(const TunedArgs &A) {
@@ -252,8 +285,55 @@ __device__ __forceinline__ bool inside_g(const TunedArgs &A, const StackView &st
     return back / 16.0f > bitsf(A.P.atlas[3]);
 }
 
-// trace.wgsl:170-243
-template <int LAYOUT, bool STATS>
+// The slot loop of one item's rays, in one of two schedules:
+//   SCHED 0  lock-step: each lane traces its slot to the end, then the next (slot += L);
+//   SCHED 1  flattened: each lane advances its ray one node visit an iteration; finished lanes
+//            wait until `refill` of the wave's 64 lanes are idle, then shade and start together
+//            (a ray start costs a few traversal steps, which the whole wave pays).
+// start(slot, R) sets up slot's ray; shade(slot, R) folds its result into the lane's sums.
+template <int LAYOUT, bool STATS, int SCHED, typename Start, typename Shade>
+__device__ __forceinline__ void run_slots(const TunedArgs &A, const StackView &st, uint32_t first,
+                                          uint32_t total, uint32_t L, Counts &cnt, Start start,
+                                          Shade shade) {
+    Ray R;
+    if (SCHED == 0) {
+        for (uint32_t slot = first; slot < total; slot += L) {
+            start(slot, R);
+            if (STATS) cnt.rays++;
+            while (!ray_step<LAYOUT, STATS>(A.S, A.P, R, st, cnt)) {
+            }
+            shade(slot, R);
+        }
+        return;
+    }
+    uint32_t slot = first;
+    bool active = false, pending = false;
+    for (;;) {
+        bool idle = !active && (pending || slot < total);
+        uint64_t idlem = __ballot(idle), actm = __ballot(active);
+        if (actm == 0ull && idlem == 0ull) break;
+        if (idlem != 0ull && (actm == 0ull || __popcll(idlem) >= (int)A.refill)) {
+            if (idle && pending) {
+                pending = false;
+                shade(slot, R);
+                slot += L;
+            }
+            if (idle && slot < total) {
+                start(slot, R);
+                if (STATS) cnt.rays++;
+                active = true;
+            }
+        }
+        if (active && ray_step<LAYOUT, STATS>(A.S, A.P, R, st, cnt)) {
+            active = false;
+            pending = true;
+        }
+    }
+}
+
+// trace.wgsl:170-243. The inside tests (16 rays, and up to 9 more points) keep the group in
+// step; the sun and occlusion rays (2 g^2 + aoRays slots) run in the chosen schedule.
+template <int LAYOUT, bool STATS, int SCHED>
 __global__ void k_direct_tuned(TunedArgs A) {
     StackView st = make_stack(A);
     const lb_params &P = A.P;
@@ -271,8 +351,8 @@ __global__ void k_direct_tuned(TunedArgs A) {
         uint32_t unit = next_unit(A.queue, n_units);
         if (unit == 0xffffffffu) break;
         uint32_t i = unit * per_wave + grp;
-        if (i >= A.n_items) continue;
-        ItemIn tx = load_item(A.items, i);
+        bool valid = i < A.n_items;
+        ItemIn tx = load_item(A.items, valid ? i : 0u);
         f3 n = tx.n;
         uint32_t key = (A.key_base + i) * 7919u;
         f3 lift = scl(n, P.reach[2]);
@@ -280,7 +360,7 @@ __global__ void k_direct_tuned(TunedArgs A) {
         f3 at = tx.pos;
         uint32_t pick = 0u;
         float span = 1.0f;
-        if (inside_g<LAYOUT, STATS>(A, st, add(at, lift), f, key, lig, L, cnt)) {
+        if (valid && inside_g<LAYOUT, STATS>(A, st, add(at, lift), f, key, lig, L, cnt)) {
             for (uint32_t q = 1u; q < 10u; q++) {
                 f3 p;
                 uint32_t kq;
@@ -304,24 +384,21 @@ __global__ void k_direct_tuned(TunedArgs A) {
         bool lit = dot3(sun, n) > 0.0f;
         uint32_t n_sun = lit ? 2u * gg : 0u;
         uint32_t rays = P.counts[1];
-        uint32_t total = n_sun + rays;
+        uint32_t total = valid ? n_sun + rays : 0u;
         float s1 = rnd(key, 3u, seed), s2 = rnd(key, 5u, seed);
         float kr = rnd(key, 17u, seed);
         f3 o_ao = add(at, lift);
         uint32_t hard = 0u;
         float soft = 0.0f, back = 0.0f;
         KSum occ{0.0f, 0.0f};
-        for (uint32_t slot = lig; slot < total; slot += (uint32_t)L) {
+        auto start = [&](uint32_t slot, Ray &R) {
             if (slot < n_sun) {
                 uint32_t s = slot < gg ? slot : slot - gg;
                 float cx = (((float)(s % g) + 0.5f) / (float)g - 0.5f) * span;
                 float cy = (((float)(s / g) + 0.5f) / (float)g - 0.5f) * span;
                 f3 o = add(add(add(at, scl(tx.dx, cx)), scl(tx.dy, cy)), lift);
                 if (slot < gg) {
-                    if (trace_t<LAYOUT, STATS>(A.S, P, A.S.root, o, sun, FAR, true, key + s,
-                                               st, cnt)
-                            .tri == NONE)
-                        hard |= 1u << s;
+                    ray_start(R, A.S.root, o, sun, FAR, true, key + s);
                 } else {
                     uint32_t k = (s + (uint32_t)(kr * (float)gg)) % gg;
                     float a = (float)k * 2.399963f;
@@ -329,29 +406,34 @@ __global__ void k_direct_tuned(TunedArgs A) {
                     float sa, ca;
                     sincos_lb(a, &sa, &ca);
                     f3 d = normalize3(add(add(sun, scl(scl(sf.t, ca), r)), scl(scl(sf.b, sa), r)));
-                    if (trace_t<LAYOUT, STATS>(A.S, P, A.S.root, o, d, FAR, true,
-                                               key + s + 1000u, st, cnt)
-                            .tri == NONE)
-                        soft += 1.0f;
+                    ray_start(R, A.S.root, o, d, FAR, true, key + s + 1000u);
                 }
             } else {
                 uint32_t r = slot - n_sun;
-                f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2),
-                              f);
-                Hit h = trace_t<LAYOUT, STATS>(A.S, P, A.S.root, o_ao, d, P.reach[0], false,
-                                               key + r + 2000u, st, cnt);
-                if (h.tri != NONE) {
-                    float k = h.t / P.reach[0];
-                    occ.add(1.0f - k * k);
-                    if (h.back && h.t < P.ground[3]) back += 1.0f;
-                }
+                f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
+                ray_start(R, A.S.root, o_ao, d, P.reach[0], false, key + r + 2000u);
             }
-        }
+        };
+        auto shade = [&](uint32_t slot, const Ray &R) {
+            if (slot < n_sun) {
+                if (R.tri == NONE) {
+                    if (slot < gg)
+                        hard |= 1u << slot;
+                    else
+                        soft += 1.0f;
+                }
+            } else if (R.tri != NONE) {
+                float k = R.t / P.reach[0];
+                occ.add(1.0f - k * k);
+                if (R.back && R.t < P.ground[3]) back += 1.0f;
+            }
+        };
+        run_slots<LAYOUT, STATS, SCHED>(A, st, lig, total, (uint32_t)L, cnt, start, shade);
         hard = group_or(hard, L);
         soft = group_sum(soft, L);
         back = group_sum(back, L);
         group_ksum(occ, L);
-        if (lig == 0u) {
+        if (valid && lig == 0u) {
             float sv = lit ? soft / (float)gg : 0.0f;
             A.out[i] = make_float4(sv, bitsf(hard | (pick << 24u)), occ.value() / (float)rays,
                                    back / (float)rays);
@@ -360,8 +442,8 @@ __global__ void k_direct_tuned(TunedArgs A) {
     if (STATS) flush_counts(cnt, A.stats);
 }
 
-// trace.wgsl:245-270
-template <int LAYOUT, bool STATS>
+// trace.wgsl:245-270: lane lig of an item's group takes its rays r = lig, lig + L, ...
+template <int LAYOUT, bool STATS, int SCHED>
 __global__ void k_gather_tuned(TunedArgs A) {
     StackView st = make_stack(A);
     const lb_params &P = A.P;
@@ -377,51 +459,55 @@ __global__ void k_gather_tuned(TunedArgs A) {
         uint32_t unit = next_unit(A.queue, n_units);
         if (unit == 0xffffffffu) break;
         uint32_t i = unit * per_wave + grp;
-        if (i >= A.n_items) continue;
-        ItemIn tx = load_item(A.items, i);
+        bool valid = i < A.n_items;
+        ItemIn tx = load_item(A.items, valid ? i : 0u);
         f3 n = tx.n;
         frame_t f = frame(n);
         uint32_t key = (A.key_base + i) * 7919u;
         float s1 = rnd(key, 11u, seed), s2 = rnd(key, 13u, seed);
         KSum sr{0, 0}, sg{0, 0}, sb{0, 0};
         float back = 0.0f;
-        for (uint32_t r = lig; r < rays; r += (uint32_t)L) {
+        auto start = [&](uint32_t r, Ray &R) {
             float jx = rnd(key + r, 19u, seed) - 0.5f;
             float jy = rnd(key + r, 23u, seed) - 0.5f;
             f3 o = add(add(add(tx.pos, scl(tx.dx, jx)), scl(tx.dy, jy)), scl(n, P.reach[2]));
             f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
-            Hit h = trace_t<LAYOUT, STATS>(A.S, P, A.S.root, o, d, P.reach[1], false,
-                                           key + r + 4000u, st, cnt);
-            f3 l;
-            if (h.tri == NONE) {
-                l = sky(P, d);
-            } else if (h.back) {
+            ray_start(R, A.S.root, o, d, P.reach[1], false, key + r + 4000u);
+        };
+        auto shade = [&](uint32_t, const Ray &R) {
+            if (R.tri == NONE) {
+                f3 l = sky(P, R.d);
+                sr.add(l.x);
+                sg.add(l.y);
+                sb.add(l.z);
+            } else if (R.back) {
                 back += 1.0f;
-                continue;
             } else {
                 if (STATS) {
                     cnt.mats++;
                     cnt.gathers++;
                 }
-                l = bounced(A.S, P, o, d, h);
+                f3 l = bounced(A.S, P, R.o, R.d, Hit{R.t, R.tri, R.back});
+                sr.add(l.x);
+                sg.add(l.y);
+                sb.add(l.z);
             }
-            sr.add(l.x);
-            sg.add(l.y);
-            sb.add(l.z);
-        }
+        };
+        run_slots<LAYOUT, STATS, SCHED>(A, st, lig, valid ? rays : 0u, (uint32_t)L, cnt, start,
+                                        shade);
         group_ksum(sr, L);
         group_ksum(sg, L);
         group_ksum(sb, L);
         back = group_sum(back, L);
-        if (lig == 0u)
+        if (valid && lig == 0u)
             A.out[i] = make_float4(sr.value() * PI / (float)rays, sg.value() * PI / (float)rays,
                                    sb.value() * PI / (float)rays, back / (float)rays);
     }
     if (STATS) flush_counts(cnt, A.stats);
 }
 
-// trace.wgsl:272-324
-template <int LAYOUT, bool STATS>
+// trace.wgsl:272-324 over rays + g^2 sun slots
+template <int LAYOUT, bool STATS, int SCHED>
 __global__ void k_volume_tuned(TunedArgs A) {
     StackView st = make_stack(A);
     const lb_params &P = A.P;
@@ -440,8 +526,8 @@ __global__ void k_volume_tuned(TunedArgs A) {
         uint32_t unit = next_unit(A.queue, n_units);
         if (unit == 0xffffffffu) break;
         uint32_t i = unit * per_wave + grp;
-        if (i >= A.n_items) continue;
-        ItemIn pr = load_item(A.items, i);
+        bool valid = i < A.n_items;
+        ItemIn pr = load_item(A.items, valid ? i : 0u);
         f3 o = pr.pos;
         uint32_t key = pr.w * 7919u + 77u;
         float s1 = rnd(key, 29u, seed), s2 = rnd(key, 31u, seed);
@@ -449,23 +535,32 @@ __global__ void k_volume_tuned(TunedArgs A) {
 #pragma unroll
         for (int k = 0; k < 12; k++) c[k] = KSum{0.0f, 0.0f};
         float back = 0.0f, sun = 0.0f;
-        uint32_t total = rays + gg;
-        for (uint32_t slot = lig; slot < total; slot += (uint32_t)L) {
+        auto start = [&](uint32_t slot, Ray &R) {
             if (slot < rays) {
-                uint32_t r = slot;
-                float u = fract(((float)r + 0.5f) / (float)rays + s1);
-                float v = fract(radical(r) + s2);
+                float u = fract(((float)slot + 0.5f) / (float)rays + s1);
+                float v = fract(radical(slot) + s2);
                 float z = 1.0f - 2.0f * u;
                 float rr = sqrtf(vmax(0.0f, 1.0f - z * z));
                 float a = v * 2.0f * PI;
                 float sa, ca;
                 sincos_lb(a, &sa, &ca);
-                f3 d = mk(rr * ca, z, rr * sa);
-                Hit h = trace_t<LAYOUT, STATS>(A.S, P, A.S.root, o, d, P.reach[1], false,
-                                               key + r, st, cnt);
+                ray_start(R, A.S.root, o, mk(rr * ca, z, rr * sa), P.reach[1], false, key + slot);
+            } else {
+                uint32_t s = slot - rays;
+                float a = (float)s * 2.399963f;
+                float r = P.sun_dir[3] * sqrtf(((float)s + 0.5f) / (float)gg);
+                float sa, ca;
+                sincos_lb(a, &sa, &ca);
+                f3 d = normalize3(add(add(sund, scl(scl(sf.t, ca), r)), scl(scl(sf.b, sa), r)));
+                ray_start(R, A.S.root, o, d, FAR, true, key + s + 9000u);
+            }
+        };
+        auto shade = [&](uint32_t slot, const Ray &R) {
+            if (slot < rays) {
+                f3 d = R.d;
                 f3 l = sky(P, d);
-                if (h.tri != NONE) {
-                    if (h.back) {
+                if (R.tri != NONE) {
+                    if (R.back) {
                         back += 1.0f;
                         l = mk(0.0f, 0.0f, 0.0f);
                     } else {
@@ -473,7 +568,7 @@ __global__ void k_volume_tuned(TunedArgs A) {
                             cnt.mats++;
                             cnt.gathers++;
                         }
-                        l = bounced(A.S, P, o, d, h);
+                        l = bounced(A.S, P, R.o, d, Hit{R.t, R.tri, R.back});
                     }
                 }
                 f3 l0 = scl(l, 0.282095f);
@@ -484,24 +579,17 @@ __global__ void k_volume_tuned(TunedArgs A) {
                 c[3].add(lx.x); c[4].add(lx.y); c[5].add(lx.z);
                 c[6].add(ly.x); c[7].add(ly.y); c[8].add(ly.z);
                 c[9].add(lz.x); c[10].add(lz.y); c[11].add(lz.z);
-            } else {
-                uint32_t s = slot - rays;
-                float a = (float)s * 2.399963f;
-                float r = P.sun_dir[3] * sqrtf(((float)s + 0.5f) / (float)gg);
-                float sa, ca;
-                sincos_lb(a, &sa, &ca);
-                f3 d = normalize3(add(add(sund, scl(scl(sf.t, ca), r)), scl(scl(sf.b, sa), r)));
-                if (trace_t<LAYOUT, STATS>(A.S, P, A.S.root, o, d, FAR, true, key + s + 9000u,
-                                           st, cnt)
-                        .tri == NONE)
-                    sun += 1.0f;
+            } else if (R.tri == NONE) {
+                sun += 1.0f;
             }
-        }
+        };
+        run_slots<LAYOUT, STATS, SCHED>(A, st, lig, valid ? rays + gg : 0u, (uint32_t)L, cnt,
+                                        start, shade);
 #pragma unroll
         for (int k = 0; k < 12; k++) group_ksum(c[k], L);
         back = group_sum(back, L);
         sun = group_sum(sun, L);
-        if (lig == 0u) {
+        if (valid && lig == 0u) {
             float w = 4.0f * PI / (float)rays;
             A.out[i * 4u] = make_float4(c[0].value() * w, c[1].value() * w, c[2].value() * w,
                                         sun / (float)gg);
@@ -520,19 +608,24 @@ __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
 
 using KernelFn = void (*)(TunedArgs);
 
-static KernelFn pick_kernel(int pass, int layout, bool stats) {
-#define LB_PICK(K)                                                                  \
-    if (layout == LB_NODES_WIDE)                                                    \
-        return stats ? K<LB_NODES_WIDE, true> : K<LB_NODES_WIDE, false>;            \
-    return stats ? K<LB_NODES_QUANT, true> : K<LB_NODES_QUANT, false>;
-    if (pass == LB_PASS_DIRECT) {
-        LB_PICK(k_direct_tuned)
-    } else if (pass == LB_PASS_GATHER) {
-        LB_PICK(k_gather_tuned)
-    } else {
-        LB_PICK(k_volume_tuned)
+#define LB_KERNEL_FAMILY(NAME)                                                        \
+    static KernelFn pick_##NAME(int layout, bool stats, int sched) {                  \
+        if (layout == LB_NODES_WIDE) {                                                \
+            if (sched == 0) return stats ? NAME<0, true, 0> : NAME<0, false, 0>;      \
+            return stats ? NAME<0, true, 1> : NAME<0, false, 1>;                      \
+        }                                                                             \
+        if (sched == 0) return stats ? NAME<1, true, 0> : NAME<1, false, 0>;          \
+        return stats ? NAME<1, true, 1> : NAME<1, false, 1>;                          \
     }
-#undef LB_PICK
+LB_KERNEL_FAMILY(k_direct_tuned)
+LB_KERNEL_FAMILY(k_gather_tuned)
+LB_KERNEL_FAMILY(k_volume_tuned)
+#undef LB_KERNEL_FAMILY
+
+static KernelFn pick_kernel(int pass, int layout, bool stats, int sched) {
+    if (pass == LB_PASS_DIRECT) return pick_k_direct_tuned(layout, stats, sched);
+    if (pass == LB_PASS_GATHER) return pick_k_gather_tuned(layout, stats, sched);
+    return pick_k_volume_tuned(layout, stats, sched);
 }
 
 // Lanes sharing one item: the rays an item casts over the rays a lane should take, as a power
@@ -557,7 +650,7 @@ struct TunedGrid {
 static hipError_t tuned_grid(lb_gpu *gpu, int pass, bool stats, const lb_params &P, uint32_t n,
                              TunedGrid *out) {
     const Tuning &T = gpu->tuning;
-    out->fn = pick_kernel(pass, T.nodes, stats);
+    out->fn = pick_kernel(pass, T.nodes, stats, T.schedule);
     out->block = (uint32_t)T.workgroup;
     out->lds = (size_t)T.lds_stack * out->block * 4u;
     int per_cu = 0;
@@ -603,6 +696,7 @@ hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
     A.spill = static_cast<uint32_t *>(gpu->spill.ptr);
     A.lanes_per_item = tg.lanes;
     A.depth = (uint32_t)gpu->tuning.lds_stack;
+    A.refill = (uint32_t)gpu->tuning.refill;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, gpu->stream, gpu->queue);
     hipLaunchKernelGGL(tg.fn, dim3(tg.grid), dim3(tg.block), tg.lds, gpu->stream, A);
     return hipGetLastError();

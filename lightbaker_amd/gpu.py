@@ -141,13 +141,22 @@ class Gpu:
     def set_form(self, form: int) -> None:
         N.check(self._lib.lb_set_form(self.handle, int(form)))
 
-    def set_tuning(self, workgroup: int, rays_per_lane: int, lds_stack: int, nodes: int) -> None:
-        N.check(self._lib.lb_set_tuning(self.handle, workgroup, rays_per_lane, lds_stack, nodes))
+    TUNING_KEYS = ("workgroup", "rays_per_lane", "lds_stack", "nodes", "refill", "schedule")
 
-    def tuning(self) -> tuple[int, int, int, int]:
-        v = [C.c_int() for _ in range(4)]
-        N.check(self._lib.lb_get_tuning(self.handle, *[C.byref(x) for x in v]))
-        return tuple(x.value for x in v)
+    def set_tuning(self, workgroup: int = None, rays_per_lane: int = None, lds_stack: int = None,
+                   nodes: int = None, refill: int = None, schedule: int = None) -> None:
+        """Sets the TUNED form's knobs; the ones not given keep their value."""
+        cur = self.tuning()
+        given = dict(workgroup=workgroup, rays_per_lane=rays_per_lane, lds_stack=lds_stack,
+                     nodes=nodes, refill=refill, schedule=schedule)
+        cur.update({k: v for k, v in given.items() if v is not None})
+        t = N.Tuning(**{k: int(cur[k]) for k in self.TUNING_KEYS})
+        N.check(self._lib.lb_set_tuning(self.handle, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = N.Tuning()
+        N.check(self._lib.lb_get_tuning(self.handle, C.byref(t)))
+        return {k: getattr(t, k) for k in self.TUNING_KEYS}
 
     def run(self, pass_: Pass, scene: SceneBuffers, params: np.ndarray, texels: np.ndarray,
             light: np.ndarray | None) -> np.ndarray:

@@ -50,6 +50,8 @@ def lib() -> C.CDLL:
         L.orc_trace_brute.argtypes = [VP, SZ, VP, VP, VP, VP, F, U32, C.POINTER(F),
                                       C.POINTER(U32), C.POINTER(I)]
         L.orc_trace_brute.restype = None
+        L.orc64_run.argtypes = [I, VP, SZ, VP, VP, VP, VP, VP, SZ, U32, VP, VP]
+        L.orc64_run.restype = I
         _lib = L
     return _lib
 
@@ -80,6 +82,25 @@ def run(pass_: int, nodes, tris, materials, charts, params, items, key_base: int
     assert rc == 0
     if counters:
         return out, {k: getattr(cnt, k) for k, _ in Counters._fields_}
+    return out
+
+
+def run64(pass_: int, nodes, tris, materials, charts, params, items, key_base: int = 0,
+          light=None):
+    """orc64_run: the FP64 tracer's results over the same rays, (n * width, 4) float64."""
+    L = lib()
+    items = np.ascontiguousarray(items)
+    nodes, tris = np.ascontiguousarray(nodes), np.ascontiguousarray(tris)
+    params = np.ascontiguousarray(params)
+    n = len(items)
+    out = np.zeros((n * (4 if pass_ == 2 else 1), 4), dtype=np.float64)
+    mats = np.ascontiguousarray(materials, dtype=np.float32)
+    ch = None if charts is None or len(charts) == 0 else np.ascontiguousarray(charts,
+                                                                               dtype=np.float32)
+    lt = None if light is None or len(light) == 0 else np.ascontiguousarray(light,
+                                                                             dtype=np.float32)
+    assert L.orc64_run(pass_, _p(nodes), len(nodes), _p(tris), _p(mats), _p(ch), _p(params),
+                       _p(items), n, key_base, _p(lt), _p(out)) == 0
     return out
 
 

@@ -37,6 +37,9 @@ int orc_run(int pass, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
 void orc_trace(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, const float *materials,
                const lb_params *params, const float o[3], const float d[3], float tmax, int any,
                uint32_t ray, float *t_out, uint32_t *tri_out, int *back_out, orc_counters *cnt);
+int orc64_run(int pass, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+              const float *materials, const float *charts, const lb_params *params,
+              const lb_texel *items, size_t n, uint32_t key_base, const float *light, double *out);
 void orc_trace_brute(const lb_tri *tris, size_t n_tris, const float *materials,
                      const lb_params *params, const float o[3], const float d[3], float tmax,
                      uint32_t ray, float *t_out, uint32_t *tri_out, int *back_out);

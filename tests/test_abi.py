@@ -35,7 +35,8 @@ def test_null_handles_are_errors_not_crashes():
     lib = N.lib()
     assert lib.lb_run(None, 0, None, None, None, 0, None, 0, None, 0) != 0
     assert b"null" in lib.lb_last_error()
-    assert lib.lb_set_tuning(None, 256, 2, 16, 0) != 0
+    t = N.Tuning(256, 2, 16, 0, 16, 0)
+    assert lib.lb_set_tuning(None, C.byref(t)) != 0
     st = N.Stats()
     assert lib.lb_run_stats(None, 0, 0, None, None, None, 0, 0, None, 0, None, C.byref(st)) != 0
 

@@ -88,7 +88,7 @@ def test_tuned_matches(gpu, setup, pass_, nodes):
     items = setup["probes"] if pass_ == Pass.VOLUME else setup["items"]
     print(f"tuned {pass_.name} nodes={nodes}: {len(items)} items", flush=True)
     gpu.set_form(N.FORM_TUNED)
-    gpu.set_tuning(256, 2, 16, nodes)
+    gpu.set_tuning(256, 2, 16, nodes, 16, 0)
     try:
         got = gpu.run(pass_, setup["sb"], setup["params"], items, setup["light"])
     finally:
@@ -97,18 +97,19 @@ def test_tuned_matches(gpu, setup, pass_, nodes):
     _compare_tuned(pass_, got, want, nodes == N.NODES_QUANT)
 
 
-@pytest.mark.parametrize("cfg", [(64, 1, 8), (128, 4, 24), (512, 8, 48), (256, 2, 32)])
+@pytest.mark.parametrize("cfg", [(64, 1, 8, 1, 1), (128, 4, 24, 64, 1), (512, 8, 24, 16, 1),
+                                 (256, 2, 32, 32, 0), (256, 8, 16, 48, 0)])
 def test_tuned_configs(gpu, setup, cfg):
     """Every autotuner setting gives the same results (workgroup, rays/lane, LDS stack)."""
-    wg, rpl, depth = cfg
+    wg, rpl, depth, refill, sched = cfg
     print(f"tuned gather cfg={cfg}", flush=True)
     gpu.set_form(N.FORM_TUNED)
-    gpu.set_tuning(wg, rpl, depth, N.NODES_WIDE)
+    gpu.set_tuning(wg, rpl, depth, N.NODES_WIDE, refill, sched)
     try:
         got = gpu.run(Pass.GATHER, setup["sb"], setup["params"], setup["items"], setup["light"])
     finally:
         gpu.set_form(N.FORM_PLAIN)
-        gpu.set_tuning(256, 2, 16, N.NODES_WIDE)
+        gpu.set_tuning(256, 2, 16, N.NODES_WIDE, 16, 0)
     want = _oracle(setup, Pass.GATHER, setup["items"])
     _compare_tuned(Pass.GATHER, got, want, False)
 
