@@ -30,13 +30,15 @@ def node_bytes(stats: dict, form: int, nodes_layout: int) -> int:
     """Algorithmic bytes of one pass from its traversal counters, with each form's record sizes:
     PLAIN reads the node (32 B) at every visit and both children (64 B) at an inner visit
     (trace.wgsl:90,111-112); TUNED reads one record with both children's boxes (WIDE 64 B, QUANT
-    32 B) at an inner visit and a leaf's (first, count) (8 B) at a leaf; every triangle test 48 B,
+    32 B) at an inner visit and, for QUANT, a leaf's (first, count) (8 B); every triangle test 48 B,
     every material load 16 B, every bounced() 80 B (its triangle's v0 and e1, the chart's two rows,
     the light texel)."""
     if form == 0:
         nb = (stats["inner_visits"] + stats["leaf_visits"]) * 32 + stats["inner_visits"] * 64
     else:
-        nb = stats["inner_visits"] * (64 if nodes_layout == 0 else 32) + stats["leaf_visits"] * 8
+        # WIDE carries a leaf child's (first, count) in the parent's record; QUANT reads 8 B
+        nb = stats["inner_visits"] * (64 if nodes_layout == 0 else 32)
+        nb += 0 if nodes_layout == 0 else stats["leaf_visits"] * 8
     return nb + stats["tri_tests"] * 48 + stats["material_loads"] * 16 + stats["light_gathers"] * 80
 
 
@@ -87,11 +89,26 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--profile-gather", type=int, default=0, metavar="N",
+                    help="prepare, execute once, then N gather launches only (for counters)")
+    ap.add_argument("--copy-kernel", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device with torch (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
               f"{args.gpus}); run with --gpus 1", file=sys.stderr)
         return 2
+
+    if args.copy_kernel:
+        import torch
+        n = args.copy_kernel * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copied {args.copy_kernel} MiB three times", file=sys.stderr)
+        return 0
 
     from lightbaker_amd import _native as N
     from lightbaker_amd.bake import BakeSession
@@ -110,6 +127,11 @@ def main() -> int:
     t0 = time.perf_counter()
     sess = BakeSession(scene, gpu, form=form, graph=True)
     prep_s = time.perf_counter() - t0
+    if args.profile_gather:
+        sess.execute()
+        ms = sess.time_pass(Pass.GATHER, reps=args.profile_gather)
+        print(f"# {args.profile_gather} gather launches, {ms:.2f} ms each", file=sys.stderr)
+        return 0
     tuned = None
     if form == N.FORM_TUNED and not args.no_autotune:
         from lightbaker_amd.autotune import autotune

@@ -134,7 +134,7 @@ int lb_scene_create(lb_gpu *gpu, const lb_node *nodes, size_t n_nodes, const lb_
     rc |= s->tris.upload(tris, n_tris * sizeof(lb_tri), gpu->stream);
     rc |= s->materials.upload(materials, n_materials * 16, gpu->stream);
     rc |= s->charts.upload(ch, nch * 32, gpu->stream);
-    if (rc == 0) rc = build_tuned_nodes(s, nodes, n_nodes, tris, n_tris);
+    if (rc == 0) rc = build_tuned_nodes(s, nodes, n_nodes, tris, n_tris, materials);
     if (rc == 0 && hipStreamSynchronize(gpu->stream) != hipSuccess)
         rc = fail("GPU: scene upload failed");
     if (rc) {
@@ -353,6 +353,7 @@ SceneView scene_view(const lb_scene *s, const float *light) {
     v.wide = static_cast<const float4 *>(s->wide.ptr);
     v.quant = static_cast<const uint4 *>(s->quant.ptr);
     v.leaves = static_cast<const uint2 *>(s->leaves.ptr);
+    v.tris_t = static_cast<const float4 *>(s->tris_t.ptr);
     v.n_nodes = s->n_nodes;
     v.root = s->root_ref;
     v.n_tris = s->n_tris;

@@ -156,6 +156,7 @@ struct SceneView {
     const float4 *wide;       // TUNED: 4 float4 per inner node (both children's boxes, refs)
     const uint4 *quant;       // TUNED: 2 uint4 per inner node (quantized children, refs)
     const uint2 *leaves;      // TUNED: (first triangle, count) at a leaf's node index
+    const float4 *tris_t;     // TUNED: the triangles with their material's opacity in e2.w
     uint32_t n_nodes, n_tris;
     uint32_t root;            // TUNED: the root as a child reference (bit 31: a leaf)
 };
@@ -183,6 +184,29 @@ __device__ __forceinline__ float mt(f3 o, f3 d, float4 v0, float4 e1v, float4 e2
     float v = dot3(d, q) * id;
     if (v < 0.0f || u + v > 1.0f) return -1.0f;
     return dot3(e2, q) * id;
+}
+
+// Moller-Trumbore with every product of the three edge vectors taken before any test, so the
+// triangle's three loads are consumed together (one memory round trip); the same values and
+// rejections as mt().
+__device__ __forceinline__ float mt_all(f3 o, f3 d, float4 v0, float4 e1v, float4 e2v,
+                                        float *det_out) {
+    f3 e1 = xyz(e1v), e2 = xyz(e2v);
+    f3 p = cross3(d, e2);
+    float det = dot3(e1, p);
+    f3 s = sub(o, xyz(v0));
+    float du = dot3(s, p);
+    f3 q = cross3(s, e1);
+    float dv = dot3(d, q);
+    float dt = dot3(e2, q);
+    *det_out = det;
+    if (fabsf(det) < 1e-12f) return -1.0f;
+    float id = 1.0f / det;
+    float u = du * id;
+    if (u < 0.0f || u > 1.0f) return -1.0f;
+    float v = dv * id;
+    if (v < 0.0f || u + v > 1.0f) return -1.0f;
+    return dt * id;
 }
 
 // trace.wgsl:133-148

@@ -48,6 +48,7 @@ struct lb_scene {
     int device = 0;
     lbk::DevBuf nodes, tris, materials, charts;
     lbk::DevBuf wide, quant, leaves;  // TUNED node layouts
+    lbk::DevBuf tris_t;               // TUNED triangles: opacity in e2.w
     uint32_t n_nodes = 0, n_tris = 0, n_materials = 0, n_charts = 0, n_inner = 0;
     uint32_t root_ref = 0;  // the root as a TUNED child reference
 };
@@ -59,7 +60,7 @@ namespace lbk {
 bool validate_bvh(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, size_t n_tris,
                   size_t n_materials, size_t n_charts, std::string *err);
 int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
-                      size_t n_tris);
+                      size_t n_tris, const float *materials);
 SceneView scene_view(const lb_scene *s, const float *light);
 int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_params *params,
              const lb_texel *items, size_t n, uint32_t key_base, const float *light,

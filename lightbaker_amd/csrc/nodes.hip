@@ -107,10 +107,17 @@ static bool quantize_axis(float origin, uint32_t ebits, float lo, float hi, uint
     return true;
 }
 
+// A TUNED leaf reference: bit 31 set; bits 26..30 the triangle count and bits 0..25 the first
+// triangle, or count field 31 and the node index when those do not fit (the kernel then reads
+// the node's (first, count) from `leaves`).
+static uint32_t leaf_ref(const lb_node &leaf, size_t index) {
+    if (leaf.count <= 30 && leaf.first < (1u << 26)) return 0x80000000u | (leaf.count << 26) | leaf.first;
+    return 0x80000000u | (31u << 26) | (uint32_t)index;
+}
+
 int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
-                      size_t n_tris) {
-    (void)tris;
-    (void)n_tris;
+                      size_t n_tris, const float *materials) {
+    if (n_nodes >= (1u << 26)) return fail("scene: %zu nodes exceed 26-bit node indices", n_nodes);
     std::vector<float> wide(n_nodes * 16, 0.0f);
     std::vector<uint32_t> quant(n_nodes * 8, 0u);
     std::vector<uint32_t> leafinfo(n_nodes * 2, 0u);
@@ -126,6 +133,9 @@ int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const l
         std::memcpy(w, vals, sizeof vals);
         w[12] = fl(n.first);
         w[13] = fl(flags);
+        // the leaf children's (first, count): a leaf visit then reads no node record
+        w[14] = fl(a.count > 0 ? leaf_ref(a, n.first) : 0u);
+        w[15] = fl(b.count > 0 ? leaf_ref(b, n.first + 1) : 0u);
         // quantized: origin at the union's min corner, one exponent an axis
         uint32_t *q = &quant[i * 8];
         uint8_t qb[12];
@@ -164,8 +174,16 @@ int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const l
         leafinfo[0] = 0;
         leafinfo[1] = 1;
     }
-    s->root_ref = (empty || nodes[0].count > 0) ? 0x80000000u : 0u;
+    s->root_ref = (empty || nodes[0].count > 0) ? (0x80000000u | (31u << 26)) : 0u;
+    // the tuned form's triangles carry their material's opacity in e2.w (0 in the reference's
+    // record), so stops() reads no material
+    std::vector<lb_tri> tt(tris, tris + n_tris);
+    for (size_t t = 0; t < n_tris; t++) {
+        uint32_t m = bits(tt[t].v0[3]);
+        tt[t].e2[3] = materials[(size_t)m * 4 + 3];
+    }
     hipStream_t st = s->gpu->stream;
+    if (s->tris_t.upload(tt.data(), tt.size() * sizeof(lb_tri), st)) return 1;
     if (s->wide.upload(wide.data(), wide.size() * 4, st)) return 1;
     if (s->quant.upload(quant.data(), quant.size() * 4, st)) return 1;
     if (s->leaves.upload(leafinfo.data(), leafinfo.size() * 4, st)) return 1;

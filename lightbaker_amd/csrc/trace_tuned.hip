@@ -38,7 +38,8 @@ struct StackView {
 // The children of inner node n: boxes, first child's index and which of the two are leaves.
 template <int LAYOUT>
 __device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn, f3 &amx,
-                                         f3 &bmn, f3 &bmx, uint32_t &first, uint32_t &flags) {
+                                         f3 &bmn, f3 &bmx, uint32_t &ra, uint32_t &rb) {
+    uint32_t first, flags;
     if (LAYOUT == LB_NODES_WIDE) {
         const float4 *w = S.wide + n * 4u;
         float4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
@@ -48,6 +49,8 @@ __device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn
         bmx = mk(w2.y, w2.z, w2.w);
         first = fbits(w3.x);
         flags = fbits(w3.y);
+        ra = flags & 1u ? fbits(w3.z) : first;
+        rb = flags & 2u ? fbits(w3.w) : first + 1u;
     } else {
         const uint4 *q = S.quant + n * 2u;
         uint4 q0 = q[0], q1 = q[1];
@@ -61,6 +64,8 @@ __device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn
         bmx = mk(ox + b(q1.z, 1) * sx, oy + b(q1.z, 2) * sy, oz + b(q1.z, 3) * sz);
         first = q1.w & 0x3fffffffu;
         flags = q1.w >> 30;
+        ra = flags & 1u ? (LEAF_BIT | (31u << 26) | first) : first;
+        rb = flags & 2u ? (LEAF_BIT | (31u << 26) | (first + 1u)) : first + 1u;
     }
 }
 
@@ -95,16 +100,26 @@ template <int LAYOUT, bool STATS>
 __device__ __forceinline__ bool ray_step(const SceneView &S, const lb_params &P, Ray &R,
                                          const StackView &st, Counts &cnt) {
     if (R.n & LEAF_BIT) {
-        uint2 li = S.leaves[R.n & ~LEAF_BIT];
+        uint32_t first = R.n & 0x03ffffffu, count = (R.n >> 26) & 31u;
+        if (count == 31u) {
+            uint2 li = S.leaves[first];
+            first = li.x;
+            count = li.y;
+        }
         if (STATS) cnt.leaves++;
-        for (uint32_t i = li.x; i < li.x + li.y; i++) {
-            float4 v0 = S.tris[i * 3u], e1 = S.tris[i * 3u + 1u], e2 = S.tris[i * 3u + 2u];
+        for (uint32_t i = first; i < first + count; i++) {
+            float4 v0 = S.tris_t[i * 3u], e1 = S.tris_t[i * 3u + 1u], e2 = S.tris_t[i * 3u + 2u];
+            // the three loads are issued together and waited for once: without this the
+            // compiler sinks v0's load past the determinant test, a second memory round trip
+            asm volatile("" ::"v"(v0.x), "v"(v0.y), "v"(v0.z), "v"(e2.w));
             if (STATS) cnt.tris++;
             float det;
-            float t = mt(R.o, R.d, v0, e1, e2, &det);
+            float t = mt_all(R.o, R.d, v0, e1, e2, &det);
             if (t > 1e-4f && t < R.t) {
                 if (STATS) cnt.mats++;
-                if (stops(S, P, i, v0, R.key)) {
+                // stops() (trace.wgsl:69-72) with the opacity carried in e2.w
+                float op = e2.w;
+                if (op >= 1.0f || rnd(i * 9781u, R.key, fbits(P.reach[3])) < op) {
                     R.t = t;
                     R.tri = i;
                     R.back = det < 0.0f;
@@ -114,13 +129,11 @@ __device__ __forceinline__ bool ray_step(const SceneView &S, const lb_params &P,
         }
     } else {
         f3 amn, amx, bmn, bmx;
-        uint32_t first, flags;
-        children<LAYOUT>(S, R.n, amn, amx, bmn, bmx, first, flags);
+        uint32_t ra, rb;
+        children<LAYOUT>(S, R.n, amn, amx, bmn, bmx, ra, rb);
         if (STATS) cnt.inner++;
         float ta = slab(R.o, R.inv, amn, amx, R.t);
         float tb = slab(R.o, R.inv, bmn, bmx, R.t);
-        uint32_t ra = first | ((flags & 1u) ? LEAF_BIT : 0u);
-        uint32_t rb = (first + 1u) | ((flags & 2u) ? LEAF_BIT : 0u);
         bool an = ta <= tb;
         uint32_t near = an ? ra : rb, far = an ? rb : ra;
         float tn = vmin(ta, tb), tf = vmax(ta, tb);
