@@ -86,6 +86,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--form", choices=["tuned", "plain"], default="tuned")
     ap.add_argument("--nodes", choices=["wide", "quant"], default="wide")
+    ap.add_argument("--schedule", type=int, choices=[0, 1, 2, 3, 4, 5], default=0,
+                    help="TUNED schedule before autotuning: 0 lock-step, 1 flattened, 2 streamed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -121,7 +123,7 @@ def main() -> int:
     scene = map_scene(2)
     t_scene = time.perf_counter() - t_scene
     gpu = Gpu()
-    gpu.set_tuning(256, 4, 16, layout, 48, 0)
+    gpu.set_tuning(256, 4, 16, layout, 48 if args.schedule == 1 else 16, args.schedule)
     print(f"# {gpu.adapter}; scene {scene.triangle_count()} triangles ({t_scene:.1f} s)",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()

@@ -3,9 +3,9 @@ threshold and node layout with the best time on a prepared bake's own passes, ch
 every candidate gives the same results as the starting configuration.
 
 Search: coordinate descent from the current setting, one dimension at a time over its range
-(schedule lock-step/flattened, workgroup 64..512, rays/lane 1..8, LDS stack 8..48, refill
-1..64 for the flattened schedule, WIDE/QUANT nodes), twice
-round; each candidate timed on the gather pass (weighted by the bounces) and the direct pass,
+(schedule lock-step/flattened/streamed, workgroup 64..512, rays/lane 1..8 for the first two,
+LDS stack 8..48, refill 1..64 for the last two, WIDE/QUANT nodes), at most two rounds; each
+candidate timed on the gather pass (weighted by the bounces) and the direct pass,
 the median of `reps` launches. A candidate the device cannot launch (an LDS stack larger than a
 workgroup's LDS) is skipped.
 """
@@ -22,7 +22,7 @@ RANGES = {
     "lds_stack": [8, 16, 24, 32, 48],
     "refill": [1, 8, 16, 24, 32, 48],
     "nodes": [N.NODES_WIDE, N.NODES_QUANT],
-    "schedule": [0, 1],
+    "schedule": [0, 1, 2],
 }
 ORDER = ["schedule", "rays_per_lane", "refill", "workgroup", "lds_stack", "nodes"]
 
@@ -59,6 +59,8 @@ def autotune(sess, bounces: int, reps: int = 3, rounds: int = 2, log=None) -> di
         for dim in ORDER:
             for v in RANGES[dim]:
                 if v == cur[dim] or (dim == "refill" and cur["schedule"] == 0):
+                    continue
+                if dim == "rays_per_lane" and cur["schedule"] == 2:
                     continue
                 cand = dict(cur, **{dim: v})
                 t = measure(cand)

@@ -87,6 +87,9 @@ void lb_close(lb_gpu *gpu) {
     gpu->run_out.release();
     gpu->zero.release();
     gpu->spill.release();
+    gpu->stream_rays.release();
+    gpu->stream_hits.release();
+    gpu->stream_aux.release();
     if (gpu->counters) hipFree(gpu->counters);
     if (gpu->queue) hipFree(gpu->queue);
     hipEventDestroy(gpu->ev0);
@@ -272,7 +275,8 @@ int lb_set_tuning(lb_gpu *gpu, const lb_tuning *t) {
     if (t->nodes != LB_NODES_WIDE && t->nodes != LB_NODES_QUANT)
         return fail("lb_set_tuning: no node layout %d", t->nodes);
     if (t->refill < 1 || t->refill > 64) return fail("lb_set_tuning: refill %d outside 1..64", t->refill);
-    if (t->schedule != 0 && t->schedule != 1) return fail("lb_set_tuning: no schedule %d", t->schedule);
+    if (t->schedule < 0 || t->schedule > 5) return fail("lb_set_tuning: no schedule %d", t->schedule);
+    if (t->chunk < 0) return fail("lb_set_tuning: chunk %d is negative", t->chunk);
     if ((size_t)t->lds_stack * t->workgroup * 4u > gpu->lds_per_block)
         return fail("lb_set_tuning: an LDS stack of %d entries for %d lanes exceeds %zu bytes",
                     t->lds_stack, t->workgroup, gpu->lds_per_block);
@@ -282,6 +286,7 @@ int lb_set_tuning(lb_gpu *gpu, const lb_tuning *t) {
     gpu->tuning.nodes = t->nodes;
     gpu->tuning.refill = t->refill;
     gpu->tuning.schedule = t->schedule;
+    gpu->tuning.chunk = t->chunk;
     return 0;
 }
 
@@ -293,6 +298,7 @@ int lb_get_tuning(const lb_gpu *gpu, lb_tuning *t) {
     t->nodes = gpu->tuning.nodes;
     t->refill = gpu->tuning.refill;
     t->schedule = gpu->tuning.schedule;
+    t->chunk = gpu->tuning.chunk;
     return 0;
 }
 

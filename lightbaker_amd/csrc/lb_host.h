@@ -39,6 +39,8 @@ struct lb_gpu {
     lbk::DevBuf run_items, run_light, run_out;  // lb_run's staging
     lbk::DevBuf zero;                           // one zero vec4 (no light)
     lbk::DevBuf spill;                          // TUNED: stack entries beyond the LDS part
+    lbk::DevBuf stream_rays, stream_hits;       // TUNED, streamed: a chunk's rays and hits
+    lbk::DevBuf stream_aux;                     // TUNED, streamed: picks, item lists, counts
     int form = LB_FORM_PLAIN;
     lbk::Tuning tuning;
 };
@@ -68,6 +70,10 @@ int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_pa
 bool is_empty_bvh(const lb_node *nodes, size_t n_nodes);
 // sizes the tuned form's spill area for a launch over n items (before a graph capture)
 hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n);
+hipError_t stream_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n);
+hipError_t launch_stream(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
+                         const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
+                         float4 *out);
 hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                         const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
                         float4 *out);

@@ -13,7 +13,9 @@ struct Tuning {
     int lds_stack = 16;      // traversal stack entries a lane keeps in LDS
     int nodes = LB_NODES_WIDE;
     int refill = 16;         // idle lanes (of 64) a wave waits for before it starts new rays
-    int schedule = 0;        // 0: lock-step slots, 1: flattened with batched refills
+    int schedule = 0;        // 0: lock-step slots, 1: flattened with batched refills,
+                             // 2: streamed (generate, trace, shade kernels)
+    int chunk = 0;           // streamed: rays a chunk holds (0: the default)
 };
 
 hipError_t launch_plain(int pass, bool stats, const SceneView &S, const lb_params &P,

@@ -1,0 +1,189 @@
+// The traversal of trace.wgsl:83-130 over the TUNED node records, as steps a kernel can schedule:
+// a ray's state lives in `Ray`, `inner_visit` and `leaf_tests` take one node visit, `pop` takes
+// the next node off the ray's stack. Any sequence of these steps that follows the reference's
+// loop visits the same nodes in the same order and accepts the same hit.
+//
+// Node references: bit 31 marks a leaf; bits 26-30 hold its triangle count (31: read the
+// (first, count) pair from `leaves` at the node's index), bits 0-25 its first triangle or, for an
+// inner node, its index.
+#pragma once
+
+#include "lb_device.h"
+
+namespace lbk {
+
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+
+// The two halves of a lane's stack carry their address spaces in their types, so an entry moves
+// with ds_read / ds_write (LDS) or global_load / global_store, never with a flat access that
+// waits on both counters.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(1))) uint32_t glb_u32;
+
+struct StackView {
+    lds_u32 *lds;    // this lane's entry 0; entry k at lds[k * lds_stride]
+    glb_u32 *spill;  // this lane's spilled entry D; entry k at spill[(k - D) * spill_stride]
+    uint32_t lds_stride, spill_stride, depth;
+    __device__ __forceinline__ void put(uint32_t k, uint32_t v) const {
+        if (k < depth)
+            lds[k * lds_stride] = v;
+        else
+            spill[(k - depth) * spill_stride] = v;
+    }
+    __device__ __forceinline__ uint32_t get(uint32_t k) const {
+        return k < depth ? lds[k * lds_stride] : spill[(k - depth) * spill_stride];
+    }
+};
+
+// A workgroup's lanes: entry-major LDS stacks of `depth` entries (the kernel's dynamic shared
+// memory, depth * blockDim.x words) and a spill area of (STACK - depth) entries per lane of the
+// grid.
+__device__ __forceinline__ StackView make_stack(uint32_t *spill, uint32_t depth) {
+    extern __shared__ uint32_t lds_stack[];
+    StackView st;
+    st.lds = (lds_u32 *)lds_stack + threadIdx.x;
+    st.lds_stride = blockDim.x;
+    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    st.spill = (glb_u32 *)spill + gtid;
+    st.spill_stride = gridDim.x * blockDim.x;
+    st.depth = depth;
+    return st;
+}
+
+// The children of inner node n: boxes, and each child's reference.
+template <int LAYOUT>
+__device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn, f3 &amx,
+                                         f3 &bmn, f3 &bmx, uint32_t &ra, uint32_t &rb) {
+    uint32_t first, flags;
+    if (LAYOUT == LB_NODES_WIDE) {
+        const float4 *w = S.wide + n * 4u;
+        float4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+        amn = mk(w0.x, w0.y, w0.z);
+        amx = mk(w0.w, w1.x, w1.y);
+        bmn = mk(w1.z, w1.w, w2.x);
+        bmx = mk(w2.y, w2.z, w2.w);
+        first = fbits(w3.x);
+        flags = fbits(w3.y);
+        ra = flags & 1u ? fbits(w3.z) : first;
+        rb = flags & 2u ? fbits(w3.w) : first + 1u;
+    } else {
+        const uint4 *q = S.quant + n * 2u;
+        uint4 q0 = q[0], q1 = q[1];
+        float ox = bitsf(q0.x), oy = bitsf(q0.y), oz = bitsf(q0.z);
+        float sx = bitsf((q0.w & 0xffu) << 23), sy = bitsf(((q0.w >> 8) & 0xffu) << 23),
+              sz = bitsf(((q0.w >> 16) & 0xffu) << 23);
+        auto b = [](uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); };
+        amn = mk(ox + b(q1.x, 0) * sx, oy + b(q1.x, 1) * sy, oz + b(q1.x, 2) * sz);
+        amx = mk(ox + b(q1.x, 3) * sx, oy + b(q1.y, 0) * sy, oz + b(q1.y, 1) * sz);
+        bmn = mk(ox + b(q1.y, 2) * sx, oy + b(q1.y, 3) * sy, oz + b(q1.z, 0) * sz);
+        bmx = mk(ox + b(q1.z, 1) * sx, oy + b(q1.z, 2) * sy, oz + b(q1.z, 3) * sz);
+        first = q1.w & 0x3fffffffu;
+        flags = q1.w >> 30;
+        ra = flags & 1u ? (LEAF_BIT | (31u << 26) | first) : first;
+        rb = flags & 2u ? (LEAF_BIT | (31u << 26) | (first + 1u)) : first + 1u;
+    }
+}
+
+// A ray in flight: trace.wgsl:83-130's loop state.
+struct Ray {
+    f3 o, d, inv;
+    float t;       // the nearest accepted hit so far (tmax until one is found)
+    uint32_t tri;  // NONE until a hit
+    uint32_t n;    // the node to visit next
+    uint32_t sp;   // stack entries
+    uint32_t key;  // stops()'s ray key
+    bool back, any;
+};
+
+__device__ __forceinline__ void ray_start(Ray &R, uint32_t root, f3 o, f3 d, float tmax, bool any,
+                                          uint32_t key) {
+    R.o = o;
+    R.d = d;
+    R.inv = inverse_dir(d);
+    R.t = tmax;
+    R.tri = NONE;
+    R.back = false;
+    R.n = root;
+    R.sp = 0;
+    R.key = key;
+    R.any = any;
+}
+
+// The triangles of leaf R.n (trace.wgsl:91-104); true when an any-hit ray accepted one.
+template <bool STATS>
+__device__ __forceinline__ bool leaf_tests(const SceneView &S, const lb_params &P, Ray &R,
+                                           Counts &cnt) {
+    uint32_t first = R.n & 0x03ffffffu, count = (R.n >> 26) & 31u;
+    if (count == 31u) {
+        uint2 li = S.leaves[first];
+        first = li.x;
+        count = li.y;
+    }
+    if (STATS) cnt.leaves++;
+    for (uint32_t i = first; i < first + count; i++) {
+        float4 v0 = S.tris_t[i * 3u], e1 = S.tris_t[i * 3u + 1u], e2 = S.tris_t[i * 3u + 2u];
+        // the three loads are issued together and waited for once: without this the compiler
+        // sinks v0's load past the determinant test, a second memory round trip
+        asm volatile("" ::"v"(v0.x), "v"(v0.y), "v"(v0.z), "v"(e2.w));
+        if (STATS) cnt.tris++;
+        float det;
+        float t = mt_all(R.o, R.d, v0, e1, e2, &det);
+        if (t > 1e-4f && t < R.t) {
+            if (STATS) cnt.mats++;
+            // stops() (trace.wgsl:69-72) with the opacity carried in e2.w
+            float op = e2.w;
+            if (op >= 1.0f || rnd(i * 9781u, R.key, fbits(P.reach[3])) < op) {
+                R.t = t;
+                R.tri = i;
+                R.back = det < 0.0f;
+                if (R.any) return true;
+            }
+        }
+    }
+    return false;
+}
+
+// Inner node R.n (trace.wgsl:105-123): true when the ray moved on to its near child (the far one
+// pushed if it was hit too), false when it missed both.
+template <int LAYOUT, bool STATS>
+__device__ __forceinline__ bool inner_visit(const SceneView &S, Ray &R, const StackView &st,
+                                            Counts &cnt) {
+    f3 amn, amx, bmn, bmx;
+    uint32_t ra, rb;
+    children<LAYOUT>(S, R.n, amn, amx, bmn, bmx, ra, rb);
+    if (STATS) cnt.inner++;
+    float ta = slab(R.o, R.inv, amn, amx, R.t);
+    float tb = slab(R.o, R.inv, bmn, bmx, R.t);
+    bool an = ta <= tb;
+    uint32_t near = an ? ra : rb, far = an ? rb : ra;
+    float tn = vmin(ta, tb), tf = vmax(ta, tb);
+    if (!(tn < FAR)) return false;
+    if (tf < FAR && R.sp < (uint32_t)STACK) {
+        st.put(R.sp, far);
+        R.sp++;
+    }
+    R.n = near;
+    return true;
+}
+
+// trace.wgsl:124-127: the next node off the stack; true when it was empty (the ray ends).
+__device__ __forceinline__ bool pop(Ray &R, const StackView &st) {
+    if (R.sp == 0u) return true;
+    R.sp--;
+    R.n = st.get(R.sp);
+    return false;
+}
+
+// One node visit in the reference's order; true when the ray is finished.
+template <int LAYOUT, bool STATS>
+__device__ __forceinline__ bool ray_step(const SceneView &S, const lb_params &P, Ray &R,
+                                         const StackView &st, Counts &cnt) {
+    if (R.n & LEAF_BIT) {
+        if (leaf_tests<STATS>(S, P, R, cnt)) return true;
+    } else if (inner_visit<LAYOUT, STATS>(S, R, st, cnt)) {
+        return false;
+    }
+    return pop(R, st);
+}
+
+}  // namespace lbk

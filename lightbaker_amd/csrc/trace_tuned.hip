@@ -13,144 +13,10 @@
 // FP32 (WIDE, 64 B a visit) or 8-bit outward-rounded (QUANT, 32 B a visit). The stack's first
 // `lds_stack` entries live in LDS, laid out entry-major so a wave's lanes hit distinct banks;
 // deeper entries spill to a per-lane global area.
-#include "lb_device.h"
 #include "lb_host.h"
+#include "lb_traverse.h"
 
 namespace lbk {
-
-constexpr uint32_t LEAF_BIT = 0x80000000u;
-
-struct StackView {
-    uint32_t *lds;    // this lane's entry 0; entry k at lds[k * lds_stride]
-    uint32_t *spill;  // this lane's spilled entry D; entry k at spill[(k - D) * spill_stride]
-    uint32_t lds_stride, spill_stride, depth;
-    __device__ __forceinline__ void put(uint32_t k, uint32_t v) const {
-        if (k < depth)
-            lds[k * lds_stride] = v;
-        else
-            spill[(k - depth) * spill_stride] = v;
-    }
-    __device__ __forceinline__ uint32_t get(uint32_t k) const {
-        return k < depth ? lds[k * lds_stride] : spill[(k - depth) * spill_stride];
-    }
-};
-
-// The children of inner node n: boxes, first child's index and which of the two are leaves.
-template <int LAYOUT>
-__device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn, f3 &amx,
-                                         f3 &bmn, f3 &bmx, uint32_t &ra, uint32_t &rb) {
-    uint32_t first, flags;
-    if (LAYOUT == LB_NODES_WIDE) {
-        const float4 *w = S.wide + n * 4u;
-        float4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
-        amn = mk(w0.x, w0.y, w0.z);
-        amx = mk(w0.w, w1.x, w1.y);
-        bmn = mk(w1.z, w1.w, w2.x);
-        bmx = mk(w2.y, w2.z, w2.w);
-        first = fbits(w3.x);
-        flags = fbits(w3.y);
-        ra = flags & 1u ? fbits(w3.z) : first;
-        rb = flags & 2u ? fbits(w3.w) : first + 1u;
-    } else {
-        const uint4 *q = S.quant + n * 2u;
-        uint4 q0 = q[0], q1 = q[1];
-        float ox = bitsf(q0.x), oy = bitsf(q0.y), oz = bitsf(q0.z);
-        float sx = bitsf((q0.w & 0xffu) << 23), sy = bitsf(((q0.w >> 8) & 0xffu) << 23),
-              sz = bitsf(((q0.w >> 16) & 0xffu) << 23);
-        auto b = [](uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); };
-        amn = mk(ox + b(q1.x, 0) * sx, oy + b(q1.x, 1) * sy, oz + b(q1.x, 2) * sz);
-        amx = mk(ox + b(q1.x, 3) * sx, oy + b(q1.y, 0) * sy, oz + b(q1.y, 1) * sz);
-        bmn = mk(ox + b(q1.y, 2) * sx, oy + b(q1.y, 3) * sy, oz + b(q1.z, 0) * sz);
-        bmx = mk(ox + b(q1.z, 1) * sx, oy + b(q1.z, 2) * sy, oz + b(q1.z, 3) * sz);
-        first = q1.w & 0x3fffffffu;
-        flags = q1.w >> 30;
-        ra = flags & 1u ? (LEAF_BIT | (31u << 26) | first) : first;
-        rb = flags & 2u ? (LEAF_BIT | (31u << 26) | (first + 1u)) : first + 1u;
-    }
-}
-
-// A ray in flight: trace.wgsl:83-130's loop state, so a lane can advance its ray one node visit
-// at a time and start its next ray as soon as this one ends.
-struct Ray {
-    f3 o, d, inv;
-    float t;       // the nearest accepted hit so far (tmax until one is found)
-    uint32_t tri;  // NONE until a hit
-    uint32_t n;    // the node to visit next (bit 31: a leaf)
-    uint32_t sp;   // stack entries
-    uint32_t key;  // stops()'s ray key
-    bool back, any;
-};
-
-__device__ __forceinline__ void ray_start(Ray &R, uint32_t root, f3 o, f3 d, float tmax, bool any,
-                                          uint32_t key) {
-    R.o = o;
-    R.d = d;
-    R.inv = inverse_dir(d);
-    R.t = tmax;
-    R.tri = NONE;
-    R.back = false;
-    R.n = root;
-    R.sp = 0;
-    R.key = key;
-    R.any = any;
-}
-
-// One node visit in the reference's order; true when the ray is finished (trace.wgsl:105-128).
-template <int LAYOUT, bool STATS>
-__device__ __forceinline__ bool ray_step(const SceneView &S, const lb_params &P, Ray &R,
-                                         const StackView &st, Counts &cnt) {
-    if (R.n & LEAF_BIT) {
-        uint32_t first = R.n & 0x03ffffffu, count = (R.n >> 26) & 31u;
-        if (count == 31u) {
-            uint2 li = S.leaves[first];
-            first = li.x;
-            count = li.y;
-        }
-        if (STATS) cnt.leaves++;
-        for (uint32_t i = first; i < first + count; i++) {
-            float4 v0 = S.tris_t[i * 3u], e1 = S.tris_t[i * 3u + 1u], e2 = S.tris_t[i * 3u + 2u];
-            // the three loads are issued together and waited for once: without this the
-            // compiler sinks v0's load past the determinant test, a second memory round trip
-            asm volatile("" ::"v"(v0.x), "v"(v0.y), "v"(v0.z), "v"(e2.w));
-            if (STATS) cnt.tris++;
-            float det;
-            float t = mt_all(R.o, R.d, v0, e1, e2, &det);
-            if (t > 1e-4f && t < R.t) {
-                if (STATS) cnt.mats++;
-                // stops() (trace.wgsl:69-72) with the opacity carried in e2.w
-                float op = e2.w;
-                if (op >= 1.0f || rnd(i * 9781u, R.key, fbits(P.reach[3])) < op) {
-                    R.t = t;
-                    R.tri = i;
-                    R.back = det < 0.0f;
-                    if (R.any) return true;
-                }
-            }
-        }
-    } else {
-        f3 amn, amx, bmn, bmx;
-        uint32_t ra, rb;
-        children<LAYOUT>(S, R.n, amn, amx, bmn, bmx, ra, rb);
-        if (STATS) cnt.inner++;
-        float ta = slab(R.o, R.inv, amn, amx, R.t);
-        float tb = slab(R.o, R.inv, bmn, bmx, R.t);
-        bool an = ta <= tb;
-        uint32_t near = an ? ra : rb, far = an ? rb : ra;
-        float tn = vmin(ta, tb), tf = vmax(ta, tb);
-        if (tn < FAR) {
-            if (tf < FAR && R.sp < (uint32_t)STACK) {
-                st.put(R.sp, far);
-                R.sp++;
-            }
-            R.n = near;
-            return false;
-        }
-    }
-    if (R.sp == 0u) return true;
-    R.sp--;
-    R.n = st.get(R.sp);
-    return false;
-}
 
 // trace.wgsl:83-130 over the tuned node records, to the end
 template <int LAYOUT, bool STATS>
@@ -252,17 +118,6 @@ struct TunedArgs {
     uint32_t refill;  // idle lanes a wave waits for before it starts new rays
 };
 
-__device__ __forceinline__ StackView make_stack(const TunedArgs &A) {
-    extern __shared__ uint32_t lds_stack[];
-    StackView st;
-    st.lds = lds_stack + threadIdx.x;
-    st.lds_stride = blockDim.x;
-    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    st.spill = A.spill + gtid;
-    st.spill_stride = gridDim.x * blockDim.x;
-    st.depth = A.depth;
-    return st;
-}
 
 // Takes the next unit for the whole wave; UINT_MAX when the queue is empty.
 __device__ __forceinline__ uint32_t next_unit(uint32_t *queue, uint32_t n_units) {
@@ -298,22 +153,46 @@ __device__ __forceinline__ bool inside_g(const TunedArgs &A, const StackView &st
     return back / 16.0f > bitsf(A.P.atlas[3]);
 }
 
-// The slot loop of one item's rays, in one of two schedules:
-//   SCHED 0  lock-step: each lane traces its slot to the end, then the next (slot += L);
-//   SCHED 1  flattened: each lane advances its ray one node visit an iteration; finished lanes
-//            wait until `refill` of the wave's 64 lanes are idle, then shade and start together
-//            (a ray start costs a few traversal steps, which the whole wave pays).
+// Slot s of an item's `total` rays in azimuth order: for a power of two, the ray whose radical
+// inverse is s / total (r = s bit-reversed), so consecutive slots sweep the azimuth parameter v
+// and a round of 64 lanes covers one wedge of directions. Any other count keeps r = s.
+__device__ __forceinline__ uint32_t az_slot(uint32_t s, uint32_t total) {
+    if (total < 2u || (total & (total - 1u)) != 0u) return s;
+    uint32_t b = 31u - (uint32_t)__builtin_clz(total);
+    return __builtin_bitreverse32(s) >> (32u - b);
+}
+
+// The slot loop of one item's rays. SCHED bits:
+//   1  flattened: each lane advances its ray one node visit an iteration; finished lanes wait
+//      until `refill` of the wave's 64 lanes are idle, then shade and start together; without
+//      it, lock-step: each lane traces its slot to the end, then the next (slot += L);
+//   2  the callers take slots in azimuth order (az_slot);
+//   4  lock-step traversal as while-while: inner nodes until every lane stands at a leaf or is
+//      done, then the leaves.
 // start(slot, R) sets up slot's ray; shade(slot, R) folds its result into the lane's sums.
 template <int LAYOUT, bool STATS, int SCHED, typename Start, typename Shade>
 __device__ __forceinline__ void run_slots(const TunedArgs &A, const StackView &st, uint32_t first,
                                           uint32_t total, uint32_t L, Counts &cnt, Start start,
                                           Shade shade) {
     Ray R;
-    if (SCHED == 0) {
+    if (!(SCHED & 1)) {
         for (uint32_t slot = first; slot < total; slot += L) {
             start(slot, R);
             if (STATS) cnt.rays++;
-            while (!ray_step<LAYOUT, STATS>(A.S, A.P, R, st, cnt)) {
+            if (SCHED & 4) {
+                for (;;) {
+                    bool done = false;
+                    while (!(R.n & LEAF_BIT)) {
+                        if (!inner_visit<LAYOUT, STATS>(A.S, R, st, cnt) && pop(R, st)) {
+                            done = true;
+                            break;
+                        }
+                    }
+                    if (done || leaf_tests<STATS>(A.S, A.P, R, cnt) || pop(R, st)) break;
+                }
+            } else {
+                while (!ray_step<LAYOUT, STATS>(A.S, A.P, R, st, cnt)) {
+                }
             }
             shade(slot, R);
         }
@@ -348,7 +227,7 @@ __device__ __forceinline__ void run_slots(const TunedArgs &A, const StackView &s
 // step; the sun and occlusion rays (2 g^2 + aoRays slots) run in the chosen schedule.
 template <int LAYOUT, bool STATS, int SCHED>
 __global__ void k_direct_tuned(TunedArgs A) {
-    StackView st = make_stack(A);
+    StackView st = make_stack(A.spill, A.depth);
     const lb_params &P = A.P;
     int L = (int)A.lanes_per_item;
     uint32_t lane = threadIdx.x & 63u;
@@ -423,6 +302,7 @@ __global__ void k_direct_tuned(TunedArgs A) {
                 }
             } else {
                 uint32_t r = slot - n_sun;
+                if (SCHED & 2) r = az_slot(r, rays);
                 f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
                 ray_start(R, A.S.root, o_ao, d, P.reach[0], false, key + r + 2000u);
             }
@@ -458,7 +338,7 @@ __global__ void k_direct_tuned(TunedArgs A) {
 // trace.wgsl:245-270: lane lig of an item's group takes its rays r = lig, lig + L, ...
 template <int LAYOUT, bool STATS, int SCHED>
 __global__ void k_gather_tuned(TunedArgs A) {
-    StackView st = make_stack(A);
+    StackView st = make_stack(A.spill, A.depth);
     const lb_params &P = A.P;
     int L = (int)A.lanes_per_item;
     uint32_t lane = threadIdx.x & 63u;
@@ -480,7 +360,8 @@ __global__ void k_gather_tuned(TunedArgs A) {
         float s1 = rnd(key, 11u, seed), s2 = rnd(key, 13u, seed);
         KSum sr{0, 0}, sg{0, 0}, sb{0, 0};
         float back = 0.0f;
-        auto start = [&](uint32_t r, Ray &R) {
+        auto start = [&](uint32_t slot, Ray &R) {
+            uint32_t r = (SCHED & 2) ? az_slot(slot, rays) : slot;
             float jx = rnd(key + r, 19u, seed) - 0.5f;
             float jy = rnd(key + r, 23u, seed) - 0.5f;
             f3 o = add(add(add(tx.pos, scl(tx.dx, jx)), scl(tx.dy, jy)), scl(n, P.reach[2]));
@@ -522,7 +403,7 @@ __global__ void k_gather_tuned(TunedArgs A) {
 // trace.wgsl:272-324 over rays + g^2 sun slots
 template <int LAYOUT, bool STATS, int SCHED>
 __global__ void k_volume_tuned(TunedArgs A) {
-    StackView st = make_stack(A);
+    StackView st = make_stack(A.spill, A.depth);
     const lb_params &P = A.P;
     int L = (int)A.lanes_per_item;
     uint32_t lane = threadIdx.x & 63u;
@@ -550,14 +431,15 @@ __global__ void k_volume_tuned(TunedArgs A) {
         float back = 0.0f, sun = 0.0f;
         auto start = [&](uint32_t slot, Ray &R) {
             if (slot < rays) {
-                float u = fract(((float)slot + 0.5f) / (float)rays + s1);
-                float v = fract(radical(slot) + s2);
+                uint32_t r = (SCHED & 2) ? az_slot(slot, rays) : slot;
+                float u = fract(((float)r + 0.5f) / (float)rays + s1);
+                float v = fract(radical(r) + s2);
                 float z = 1.0f - 2.0f * u;
                 float rr = sqrtf(vmax(0.0f, 1.0f - z * z));
                 float a = v * 2.0f * PI;
                 float sa, ca;
                 sincos_lb(a, &sa, &ca);
-                ray_start(R, A.S.root, o, mk(rr * ca, z, rr * sa), P.reach[1], false, key + slot);
+                ray_start(R, A.S.root, o, mk(rr * ca, z, rr * sa), P.reach[1], false, key + r);
             } else {
                 uint32_t s = slot - rays;
                 float a = (float)s * 2.399963f;
@@ -621,14 +503,22 @@ __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
 
 using KernelFn = void (*)(TunedArgs);
 
+// tuning schedule -> SCHED bits: 0 lock-step, 1 flattened, 3 lock-step in azimuth order,
+// 4 the same as while-while, 5 flattened in azimuth order
 #define LB_KERNEL_FAMILY(NAME)                                                        \
-    static KernelFn pick_##NAME(int layout, bool stats, int sched) {                  \
-        if (layout == LB_NODES_WIDE) {                                                \
-            if (sched == 0) return stats ? NAME<0, true, 0> : NAME<0, false, 0>;      \
-            return stats ? NAME<0, true, 1> : NAME<0, false, 1>;                      \
+    template <int LAYOUT>                                                             \
+    static KernelFn pick_##NAME##_l(bool stats, int sched) {                          \
+        switch (sched) {                                                              \
+        case 1: return stats ? NAME<LAYOUT, true, 1> : NAME<LAYOUT, false, 1>;        \
+        case 3: return stats ? NAME<LAYOUT, true, 2> : NAME<LAYOUT, false, 2>;        \
+        case 4: return stats ? NAME<LAYOUT, true, 6> : NAME<LAYOUT, false, 6>;        \
+        case 5: return stats ? NAME<LAYOUT, true, 3> : NAME<LAYOUT, false, 3>;        \
+        default: return stats ? NAME<LAYOUT, true, 0> : NAME<LAYOUT, false, 0>;       \
         }                                                                             \
-        if (sched == 0) return stats ? NAME<1, true, 0> : NAME<1, false, 0>;          \
-        return stats ? NAME<1, true, 1> : NAME<1, false, 1>;                          \
+    }                                                                                 \
+    static KernelFn pick_##NAME(int layout, bool stats, int sched) {                  \
+        return layout == LB_NODES_WIDE ? pick_##NAME##_l<0>(stats, sched)             \
+                                       : pick_##NAME##_l<1>(stats, sched);            \
     }
 LB_KERNEL_FAMILY(k_direct_tuned)
 LB_KERNEL_FAMILY(k_gather_tuned)
@@ -685,6 +575,7 @@ static hipError_t tuned_grid(lb_gpu *gpu, int pass, bool stats, const lb_params 
 }
 
 hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n) {
+    if (gpu->tuning.schedule == 2) return stream_prepare(gpu, pass, P, n);
     TunedGrid tg;
     hipError_t e = tuned_grid(gpu, pass, false, P, n, &tg);
     if (e != hipSuccess) return e;
@@ -694,6 +585,7 @@ hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n) 
 hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                         const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
                         float4 *out) {
+    if (gpu->tuning.schedule == 2) return launch_stream(gpu, pass, stats, S, P, items, n, key_base, out);
     TunedGrid tg;
     hipError_t e = tuned_grid(gpu, pass, stats, P, n, &tg);
     if (e != hipSuccess) return e;

@@ -141,14 +141,16 @@ class Gpu:
     def set_form(self, form: int) -> None:
         N.check(self._lib.lb_set_form(self.handle, int(form)))
 
-    TUNING_KEYS = ("workgroup", "rays_per_lane", "lds_stack", "nodes", "refill", "schedule")
+    TUNING_KEYS = ("workgroup", "rays_per_lane", "lds_stack", "nodes", "refill", "schedule",
+                   "chunk")
 
     def set_tuning(self, workgroup: int = None, rays_per_lane: int = None, lds_stack: int = None,
-                   nodes: int = None, refill: int = None, schedule: int = None) -> None:
+                   nodes: int = None, refill: int = None, schedule: int = None,
+                   chunk: int = None) -> None:
         """Sets the TUNED form's knobs; the ones not given keep their value."""
         cur = self.tuning()
         given = dict(workgroup=workgroup, rays_per_lane=rays_per_lane, lds_stack=lds_stack,
-                     nodes=nodes, refill=refill, schedule=schedule)
+                     nodes=nodes, refill=refill, schedule=schedule, chunk=chunk)
         cur.update({k: v for k, v in given.items() if v is not None})
         t = N.Tuning(**{k: int(cur[k]) for k in self.TUNING_KEYS})
         N.check(self._lib.lb_set_tuning(self.handle, C.byref(t)))

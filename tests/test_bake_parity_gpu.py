@@ -1,9 +1,9 @@
 """The whole device half of the bake against the oracle's restatement of it (oracle/bake_ref.py:
 bake.rs:149-330 and volume.rs:69-117 over the oracle's C passes), on the same prepared inputs.
 
-Bar: the PLAIN form gives the final light atlas (direct + last bounce per texel), the field, the
-occlusion and the raw probes bit for bit. The TUNED form (compensated sums in another order)
-within 1e-4 relative on the light and probes, and its sun field exactly (exact shadow bits).
+Bar: the PLAIN form, and the TUNED form's streamed schedule (which shades in the reference's
+order), give the final light atlas (direct + last bounce per texel) bit for bit. The TUNED form's
+lock-step schedule (compensated sums in another order) within 1e-4 relative.
 """
 from __future__ import annotations
 
@@ -53,19 +53,24 @@ def oracle_inputs(scene, sess):
 
 @pytest.mark.parametrize("gen", [lambda: reference_test_scene(8.0), lambda: small_scene(1)],
                          ids=["reference-test-scene", "small-scene"])
-@pytest.mark.parametrize("form", [N.FORM_PLAIN, N.FORM_TUNED], ids=["plain", "tuned"])
-def test_bake_matches_oracle(gpu, gen, form):
+@pytest.mark.parametrize("form,schedule", [(N.FORM_PLAIN, 0), (N.FORM_TUNED, 0),
+                                           (N.FORM_TUNED, 2)], ids=["plain", "tuned", "streamed"])
+def test_bake_matches_oracle(gpu, gen, form, schedule):
     scene = gen()
-    print(f"bake parity form={form}", flush=True)
-    sess = BakeSession(scene, gpu, form=form, graph=form == N.FORM_TUNED)
-    sess.execute()
-    sess.finish()
+    print(f"bake parity form={form} schedule={schedule}", flush=True)
+    gpu.set_tuning(schedule=schedule)
+    try:
+        sess = BakeSession(scene, gpu, form=form, graph=form == N.FORM_TUNED)
+        sess.execute()
+        sess.finish()
+    finally:
+        gpu.set_tuning(schedule=0)
     light = sess.array(N.BAKE_LIGHT, np.float32).reshape(-1, 4)
     want = ref.bake(oracle_inputs(scene, sess))
     wl = want["light"].reshape(-1, 4)
     gat = sess.array(N.BAKE_GATHER_INDEX, np.uint32)
     assert np.array_equal(gat, want["gathered"]), "gather sets differ"
-    if form == N.FORM_PLAIN:
+    if form == N.FORM_PLAIN or schedule == 2:
         bad = np.nonzero((light.view(np.uint32) != wl.view(np.uint32)).any(axis=1))[0]
         assert len(bad) == 0, f"{len(bad)} atlas texels differ: {light[bad[:3]]} vs {wl[bad[:3]]}"
     else:
