@@ -28,18 +28,32 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 def node_bytes(stats: dict, form: int, nodes_layout: int) -> int:
     """Algorithmic bytes of one pass from its traversal counters, with each form's record sizes:
-    PLAIN reads the node (32 B) at every visit and both children (64 B) at an inner visit
-    (trace.wgsl:90,111-112); TUNED reads one record with both children's boxes (WIDE 64 B, QUANT
-    32 B) at an inner visit and, for QUANT, a leaf's (first, count) (8 B); every triangle test 48 B,
-    every material load 16 B, every bounced() 80 B (its triangle's v0 and e1, the chart's two rows,
-    the light texel)."""
+    PLAIN reads the node (32 B) at every visit and both children (2 x 32 B) at an inner visit
+    (trace.wgsl:90,111-112); TUNED reads one record with both children's boxes at an inner visit
+    (WIDE 2 x 32 B, QUANT 32 B) and, for QUANT, a leaf's (first, count) (8 B). Every triangle test
+    48 B, every material load 16 B, every bounced() 32 B of chart rows and 16 B of light texel."""
     if form == 0:
         nb = (stats["inner_visits"] + stats["leaf_visits"]) * 32 + stats["inner_visits"] * 64
     else:
         # WIDE carries a leaf child's (first, count) in the parent's record; QUANT reads 8 B
         nb = stats["inner_visits"] * (64 if nodes_layout == 0 else 32)
         nb += 0 if nodes_layout == 0 else stats["leaf_visits"] * 8
-    return nb + stats["tri_tests"] * 48 + stats["material_loads"] * 16 + stats["light_gathers"] * 80
+    return nb + stats["tri_tests"] * 48 + stats["material_loads"] * 16 + stats["light_gathers"] * 48
+
+
+def measured_traffic(kernel: str) -> dict | None:
+    """HBM bytes a launch of `kernel` moves, as measured by rocprofv3 (FETCH_SIZE and WRITE_SIZE,
+    scaled by what the counters read for a copy of known size) and recorded with the command
+    that measured them in profiles/gather_map_scene_traffic.json; None when the record is for
+    another kernel."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "gather_map_scene_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return rec if rec.get("kernel") == kernel else None
 
 
 def cpu_baseline(sess, target_s: float = 12.0) -> dict:
@@ -86,8 +100,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--form", choices=["tuned", "plain"], default="tuned")
     ap.add_argument("--nodes", choices=["wide", "quant"], default="wide")
-    ap.add_argument("--schedule", type=int, choices=[0, 1, 2, 3, 4, 5], default=0,
-                    help="TUNED schedule before autotuning: 0 lock-step, 1 flattened, 2 streamed")
+    ap.add_argument("--schedule", type=int, choices=[0, 1], default=0,
+                    help="TUNED schedule before autotuning: 0 lock-step, 1 pooled")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -123,7 +137,7 @@ def main() -> int:
     scene = map_scene(2)
     t_scene = time.perf_counter() - t_scene
     gpu = Gpu()
-    gpu.set_tuning(256, 4, 16, layout, 48 if args.schedule == 1 else 16, args.schedule)
+    gpu.set_tuning(256, 4, 16, layout, 32, args.schedule, leaf_batch=24)
     print(f"# {gpu.adapter}; scene {scene.triangle_count()} triangles ({t_scene:.1f} s)",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -171,6 +185,14 @@ def main() -> int:
     per_launch = {k: v / bounces for k, v in g.items()}
     bytes_launch = node_bytes(per_launch, form, layout) + info["gather_items"] * (64 + 16)
     achieved = bytes_launch / (ms_gather / 1e3) / 1e9
+    tuning = gpu.tuning()
+    if form == N.FORM_PLAIN:
+        kernel = "k_gather_plain<false>"
+    elif tuning["schedule"] == 0:
+        kernel = f"k_gather_tuned<{layout}, false>"
+    else:
+        kernel = f"k_pool<0, {layout}, false>"
+    traffic = measured_traffic(kernel)
     print(f"# gather launch {ms_gather:.2f} ms, {bytes_launch / 1e9:.3f} GB algorithmic, "
           f"{per_launch['rays'] / ms_gather / 1e6:.2f} Grays/s", file=sys.stderr, flush=True)
 
@@ -216,9 +238,11 @@ def main() -> int:
             "tuned_vs_plain": (plain_ms / ms_step) if plain_ms else None,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "gather", "ms_per_launch": ms_gather,
-                     "bytes_per_launch": bytes_launch},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["hbm_bytes"] if traffic else None,
+                     "kernel": kernel, "ms_per_launch": ms_gather,
+                     "bytes_per_launch": bytes_launch,
+                     "traffic_source": traffic["source"] if traffic else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -132,14 +132,16 @@ typedef struct lb_tuning {
     int rays_per_lane;  /* rays a lane takes of one item (1..8): sets the lanes sharing an item */
     int lds_stack;      /* traversal stack entries in LDS (8..48); deeper ones spill to memory */
     int nodes;          /* LB_NODES_WIDE or LB_NODES_QUANT */
-    int refill;         /* flattened and streamed schedules: idle lanes (1..64) a wave waits for
-                           before it starts new rays */
-    int schedule;       /* 0: a lane traces each ray to the end; 1: flattened, batched refills;
-                           2: streamed: rays generated to HBM, traced by lanes that take the
-                           next ray as soon as theirs ends, hits shaded in the reference's order */
-    int chunk;          /* streamed schedule: rays one chunk holds in HBM, 0 for 2^26 (2 GiB of
-                           ray records and 512 MiB of hits); a launch runs ceil(rays / chunk)
-                           chunks */
+    int refill;         /* pooled schedule: idle lanes (1..64) a wave waits for before it
+                           starts new rays */
+    int schedule;       /* 0: lock-step: a lane traces each ray to the end, then its next;
+                           1: pooled: lanes draw rays from the pass's pool as theirs end, the
+                           hits are shaded in the reference's order (bit-exact sums) */
+    int chunk;          /* pooled schedule: hits one chunk holds in HBM, 0 for 2^28 (2 GiB); a
+                           launch runs ceil(rays / chunk) chunks */
+    int leaf_batch;     /* pooled schedule: lanes (1..64) standing at a leaf before a wave
+                           tests leaves rather than stepping its other lanes through inner
+                           nodes */
 } lb_tuning;
 
 int lb_set_tuning(lb_gpu *gpu, const lb_tuning *tuning);

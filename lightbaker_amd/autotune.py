@@ -3,8 +3,8 @@ threshold and node layout with the best time on a prepared bake's own passes, ch
 every candidate gives the same results as the starting configuration.
 
 Search: coordinate descent from the current setting, one dimension at a time over its range
-(schedule lock-step/flattened/streamed, workgroup 64..512, rays/lane 1..8 for the first two,
-LDS stack 8..48, refill 1..64 for the last two, WIDE/QUANT nodes), at most two rounds; each
+(schedule lock-step/pooled, workgroup 64..512, rays/lane 1..8 for lock-step, LDS stack 8..48,
+refill and leaf batch for pooled, WIDE/QUANT nodes), at most two rounds; each
 candidate timed on the gather pass (weighted by the bounces) and the direct pass,
 the median of `reps` launches. A candidate the device cannot launch (an LDS stack larger than a
 workgroup's LDS) is skipped.
@@ -20,11 +20,12 @@ RANGES = {
     "workgroup": [64, 128, 256, 512],
     "rays_per_lane": [1, 2, 4, 8],
     "lds_stack": [8, 16, 24, 32, 48],
-    "refill": [1, 8, 16, 24, 32, 48],
+    "refill": [16, 32, 48, 56],
     "nodes": [N.NODES_WIDE, N.NODES_QUANT],
-    "schedule": [0, 1, 2],
+    "schedule": [0, 1],
+    "leaf_batch": [8, 16, 24, 32, 64],
 }
-ORDER = ["schedule", "rays_per_lane", "refill", "workgroup", "lds_stack", "nodes"]
+ORDER = ["schedule", "rays_per_lane", "refill", "leaf_batch", "workgroup", "lds_stack", "nodes"]
 
 
 def _cost(sess, bounces: int, reps: int) -> float:
@@ -58,9 +59,11 @@ def autotune(sess, bounces: int, reps: int = 3, rounds: int = 2, log=None) -> di
         changed = False
         for dim in ORDER:
             for v in RANGES[dim]:
-                if v == cur[dim] or (dim == "refill" and cur["schedule"] == 0):
+                if v == cur[dim]:
                     continue
-                if dim == "rays_per_lane" and cur["schedule"] == 2:
+                if dim in ("refill", "leaf_batch") and cur["schedule"] == 0:
+                    continue
+                if dim == "rays_per_lane" and cur["schedule"] == 1:
                     continue
                 cand = dict(cur, **{dim: v})
                 t = measure(cand)

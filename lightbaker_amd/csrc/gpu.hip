@@ -87,7 +87,6 @@ void lb_close(lb_gpu *gpu) {
     gpu->run_out.release();
     gpu->zero.release();
     gpu->spill.release();
-    gpu->stream_rays.release();
     gpu->stream_hits.release();
     gpu->stream_aux.release();
     if (gpu->counters) hipFree(gpu->counters);
@@ -275,9 +274,11 @@ int lb_set_tuning(lb_gpu *gpu, const lb_tuning *t) {
     if (t->nodes != LB_NODES_WIDE && t->nodes != LB_NODES_QUANT)
         return fail("lb_set_tuning: no node layout %d", t->nodes);
     if (t->refill < 1 || t->refill > 64) return fail("lb_set_tuning: refill %d outside 1..64", t->refill);
-    if (t->schedule < 0 || t->schedule > 5) return fail("lb_set_tuning: no schedule %d", t->schedule);
+    if (t->schedule < 0 || t->schedule > 1) return fail("lb_set_tuning: no schedule %d", t->schedule);
     if (t->chunk < 0) return fail("lb_set_tuning: chunk %d is negative", t->chunk);
-    if ((size_t)t->lds_stack * t->workgroup * 4u > gpu->lds_per_block)
+    if (t->leaf_batch < 1 || t->leaf_batch > 64)
+        return fail("lb_set_tuning: leaf batch %d outside 1..64", t->leaf_batch);
+    if (stack_lds_bytes(t->lds_stack, t->workgroup) > gpu->lds_per_block)
         return fail("lb_set_tuning: an LDS stack of %d entries for %d lanes exceeds %zu bytes",
                     t->lds_stack, t->workgroup, gpu->lds_per_block);
     gpu->tuning.workgroup = t->workgroup;
@@ -287,6 +288,7 @@ int lb_set_tuning(lb_gpu *gpu, const lb_tuning *t) {
     gpu->tuning.refill = t->refill;
     gpu->tuning.schedule = t->schedule;
     gpu->tuning.chunk = t->chunk;
+    gpu->tuning.leaf_batch = t->leaf_batch;
     return 0;
 }
 
@@ -299,6 +301,7 @@ int lb_get_tuning(const lb_gpu *gpu, lb_tuning *t) {
     t->refill = gpu->tuning.refill;
     t->schedule = gpu->tuning.schedule;
     t->chunk = gpu->tuning.chunk;
+    t->leaf_batch = gpu->tuning.leaf_batch;
     return 0;
 }
 
@@ -395,7 +398,7 @@ int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_pa
     if (pass != LB_PASS_DIRECT && !light) return fail("run: the gather and volume read a light");
     if (check_params(scene, params, pass, pass == LB_PASS_DIRECT ? 1 : n_light)) return 1;
     if (form == LB_FORM_TUNED &&
-        (size_t)gpu->tuning.lds_stack * gpu->tuning.workgroup * 4u > gpu->lds_per_block)
+        stack_lds_bytes(gpu->tuning.lds_stack, gpu->tuning.workgroup) > gpu->lds_per_block)
         return fail("run: an LDS stack of %d entries for %d lanes exceeds %zu bytes a workgroup",
                     gpu->tuning.lds_stack, gpu->tuning.workgroup, gpu->lds_per_block);
     hipSetDevice(gpu->device);

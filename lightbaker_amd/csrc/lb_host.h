@@ -39,8 +39,8 @@ struct lb_gpu {
     lbk::DevBuf run_items, run_light, run_out;  // lb_run's staging
     lbk::DevBuf zero;                           // one zero vec4 (no light)
     lbk::DevBuf spill;                          // TUNED: stack entries beyond the LDS part
-    lbk::DevBuf stream_rays, stream_hits;       // TUNED, streamed: a chunk's rays and hits
-    lbk::DevBuf stream_aux;                     // TUNED, streamed: picks, item lists, counts
+    lbk::DevBuf stream_hits;                    // TUNED, pooled: a chunk's hits
+    lbk::DevBuf stream_aux;                     // TUNED, pooled: picks, item lists, counts
     int form = LB_FORM_PLAIN;
     lbk::Tuning tuning;
 };

@@ -12,11 +12,19 @@ struct Tuning {
     int rays_per_lane = 2;   // rays one lane traces of one item before the wave reduces
     int lds_stack = 16;      // traversal stack entries a lane keeps in LDS
     int nodes = LB_NODES_WIDE;
-    int refill = 16;         // idle lanes (of 64) a wave waits for before it starts new rays
-    int schedule = 0;        // 0: lock-step slots, 1: flattened with batched refills,
-                             // 2: streamed (generate, trace, shade kernels)
-    int chunk = 0;           // streamed: rays a chunk holds (0: the default)
+    int refill = 32;         // pooled: idle lanes (of 64) a wave waits for before it refills
+    int schedule = 0;        // 0: lock-step slots, 1: pooled (one ray pool, shaded in order)
+    int chunk = 0;           // pooled: hits a chunk holds (0: the default)
+    int leaf_batch = 24;     // pooled: lanes at a leaf before the wave tests leaves
 };
+
+// LDS words between two lanes' traversal stacks of `depth` entries: odd, so a wave's lanes
+// spread over all banks.
+__host__ __device__ inline uint32_t lane_stride(uint32_t depth) { return (depth + 1u) | 1u; }
+// LDS bytes of a workgroup's traversal stacks
+inline size_t stack_lds_bytes(int depth, int workgroup) {
+    return (size_t)lane_stride((uint32_t)depth) * (size_t)workgroup * 4u;
+}
 
 hipError_t launch_plain(int pass, bool stats, const SceneView &S, const lb_params &P,
                         const lb_texel *items, uint32_t n, uint32_t key_base, float4 *out,

@@ -9,6 +9,7 @@
 #pragma once
 
 #include "lb_device.h"
+#include "lb_kernels.h"
 
 namespace lbk {
 
@@ -20,32 +21,33 @@ constexpr uint32_t LEAF_BIT = 0x80000000u;
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(1))) uint32_t glb_u32;
 
+// A lane's entries are consecutive words (an entry's address is a shift and an add, no multiply)
+// and lanes sit an odd number of words apart, so the 32 lanes of an LDS access hit 32 distinct
+// banks.
 struct StackView {
-    lds_u32 *lds;    // this lane's entry 0; entry k at lds[k * lds_stride]
-    glb_u32 *spill;  // this lane's spilled entry D; entry k at spill[(k - D) * spill_stride]
-    uint32_t lds_stride, spill_stride, depth;
+    lds_u32 *lds;    // this lane's entry 0; entry k at lds[k]
+    glb_u32 *spill;  // this lane's entry `depth`; entry k at spill[k - depth]
+    uint32_t depth;
     __device__ __forceinline__ void put(uint32_t k, uint32_t v) const {
         if (k < depth)
-            lds[k * lds_stride] = v;
+            lds[k] = v;
         else
-            spill[(k - depth) * spill_stride] = v;
+            spill[k - depth] = v;
     }
     __device__ __forceinline__ uint32_t get(uint32_t k) const {
-        return k < depth ? lds[k * lds_stride] : spill[(k - depth) * spill_stride];
+        return k < depth ? lds[k] : spill[k - depth];
     }
 };
 
-// A workgroup's lanes: entry-major LDS stacks of `depth` entries (the kernel's dynamic shared
-// memory, depth * blockDim.x words) and a spill area of (STACK - depth) entries per lane of the
+// A workgroup's lanes: LDS stacks of `depth` entries, lane_stride(depth) words apart (the
+// kernel's dynamic shared memory), and a spill area of STACK - depth entries per lane of the
 // grid.
 __device__ __forceinline__ StackView make_stack(uint32_t *spill, uint32_t depth) {
     extern __shared__ uint32_t lds_stack[];
     StackView st;
-    st.lds = (lds_u32 *)lds_stack + threadIdx.x;
-    st.lds_stride = blockDim.x;
+    st.lds = (lds_u32 *)lds_stack + threadIdx.x * lane_stride(depth);
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    st.spill = (glb_u32 *)spill + gtid;
-    st.spill_stride = gridDim.x * blockDim.x;
+    st.spill = (glb_u32 *)spill + (size_t)gtid * (uint32_t)(STACK - (int)depth);
     st.depth = depth;
     return st;
 }

@@ -1,31 +1,40 @@
-// The TUNED form of the three ray passes (crates/lightbaker/src/gpu/trace.wgsl) for gfx950.
+// The TUNED form of the three ray passes (crates/lightbaker/src/gpu/trace.wgsl) for gfx950, in
+// its lock-step schedule (schedule 0); the pooled schedule (1) is in trace_stream.hip.
 //
 // Mapping: a wave64 takes work units from one global atomic queue until it is empty (a persistent
 // grid sized to the device's residency). A unit is 64 / L items; L lanes of the wave (a group,
-// L a power of two in 8..64) share one item and spread its rays over themselves, ray r on lane
-// r mod L, so a pass over uneven items keeps every lane busy. Each lane sums its rays with
-// Kahan-Neumaier compensation and the group combines the (sum, compensation) pairs with
-// ds_swizzle butterflies (xor 1..16) and one cross-half exchange (xor 32). Counts (shadow bits,
-// back faces, sun hits) are exact integers, so they equal the reference's whatever the order.
+// L a power of two in 8..64) share one item and spread its ray slots over themselves, and the
+// slots run in azimuth order (consecutive slots sweep the azimuth), so the rays a wave traces
+// together point into one wedge. Each lane sums its rays with Kahan-Neumaier compensation and the
+// group combines the (sum, compensation) pairs with ds_swizzle butterflies (xor 1..16) and one
+// cross-half exchange (xor 32). Counts (shadow bits, back faces, sun hits) are exact integers, so
+// they equal the reference's whatever the order.
 //
 // Traversal: the reference's order exactly (near child first on ta <= tb, far child pushed,
 // 48 entries, a push beyond them dropped) over a node record that holds both children's boxes,
 // FP32 (WIDE, 64 B a visit) or 8-bit outward-rounded (QUANT, 32 B a visit). The stack's first
-// `lds_stack` entries live in LDS, laid out entry-major so a wave's lanes hit distinct banks;
-// deeper entries spill to a per-lane global area.
+// `lds_stack` entries live in LDS; deeper entries spill to a per-lane global area.
 #include "lb_host.h"
 #include "lb_traverse.h"
 
 namespace lbk {
 
-// trace.wgsl:83-130 over the tuned node records, to the end
+// trace.wgsl:83-130 over the tuned node records, to the end, as while-while
 template <int LAYOUT, bool STATS>
 __device__ Hit trace_t(const SceneView &S, const lb_params &P, uint32_t root, f3 o, f3 d,
                        float tmax, bool any, uint32_t ray, const StackView &st, Counts &cnt) {
     Ray R;
     ray_start(R, root, o, d, tmax, any, ray);
     if (STATS) cnt.rays++;
-    while (!ray_step<LAYOUT, STATS>(S, P, R, st, cnt)) {
+    for (;;) {
+        bool done = false;
+        while (!(R.n & LEAF_BIT)) {
+            if (!inner_visit<LAYOUT, STATS>(S, R, st, cnt) && pop(R, st)) {
+                done = true;
+                break;
+            }
+        }
+        if (done || leaf_tests<STATS>(S, P, R, cnt) || pop(R, st)) break;
     }
     return Hit{R.t, R.tri, R.back};
 }
@@ -115,7 +124,6 @@ struct TunedArgs {
     uint32_t *queue;
     uint32_t *spill;
     uint32_t lanes_per_item, depth;
-    uint32_t refill;  // idle lanes a wave waits for before it starts new rays
 };
 
 
@@ -162,70 +170,35 @@ __device__ __forceinline__ uint32_t az_slot(uint32_t s, uint32_t total) {
     return __builtin_bitreverse32(s) >> (32u - b);
 }
 
-// The slot loop of one item's rays. SCHED bits:
-//   1  flattened: each lane advances its ray one node visit an iteration; finished lanes wait
-//      until `refill` of the wave's 64 lanes are idle, then shade and start together; without
-//      it, lock-step: each lane traces its slot to the end, then the next (slot += L);
-//   2  the callers take slots in azimuth order (az_slot);
-//   4  lock-step traversal as while-while: inner nodes until every lane stands at a leaf or is
-//      done, then the leaves.
-// start(slot, R) sets up slot's ray; shade(slot, R) folds its result into the lane's sums.
-template <int LAYOUT, bool STATS, int SCHED, typename Start, typename Shade>
+// The slot loop of one item's rays: each lane traces its slot to the end as while-while (inner
+// nodes until every lane of the wave stands at a leaf or is done, then the leaves), then takes
+// slot + L. start(slot, R) sets up slot's ray; shade(slot, R) folds its result into the lane's
+// sums.
+template <int LAYOUT, bool STATS, typename Start, typename Shade>
 __device__ __forceinline__ void run_slots(const TunedArgs &A, const StackView &st, uint32_t first,
                                           uint32_t total, uint32_t L, Counts &cnt, Start start,
                                           Shade shade) {
     Ray R;
-    if (!(SCHED & 1)) {
-        for (uint32_t slot = first; slot < total; slot += L) {
-            start(slot, R);
-            if (STATS) cnt.rays++;
-            if (SCHED & 4) {
-                for (;;) {
-                    bool done = false;
-                    while (!(R.n & LEAF_BIT)) {
-                        if (!inner_visit<LAYOUT, STATS>(A.S, R, st, cnt) && pop(R, st)) {
-                            done = true;
-                            break;
-                        }
-                    }
-                    if (done || leaf_tests<STATS>(A.S, A.P, R, cnt) || pop(R, st)) break;
-                }
-            } else {
-                while (!ray_step<LAYOUT, STATS>(A.S, A.P, R, st, cnt)) {
+    for (uint32_t slot = first; slot < total; slot += L) {
+        start(slot, R);
+        if (STATS) cnt.rays++;
+        for (;;) {
+            bool done = false;
+            while (!(R.n & LEAF_BIT)) {
+                if (!inner_visit<LAYOUT, STATS>(A.S, R, st, cnt) && pop(R, st)) {
+                    done = true;
+                    break;
                 }
             }
-            shade(slot, R);
+            if (done || leaf_tests<STATS>(A.S, A.P, R, cnt) || pop(R, st)) break;
         }
-        return;
-    }
-    uint32_t slot = first;
-    bool active = false, pending = false;
-    for (;;) {
-        bool idle = !active && (pending || slot < total);
-        uint64_t idlem = __ballot(idle), actm = __ballot(active);
-        if (actm == 0ull && idlem == 0ull) break;
-        if (idlem != 0ull && (actm == 0ull || __popcll(idlem) >= (int)A.refill)) {
-            if (idle && pending) {
-                pending = false;
-                shade(slot, R);
-                slot += L;
-            }
-            if (idle && slot < total) {
-                start(slot, R);
-                if (STATS) cnt.rays++;
-                active = true;
-            }
-        }
-        if (active && ray_step<LAYOUT, STATS>(A.S, A.P, R, st, cnt)) {
-            active = false;
-            pending = true;
-        }
+        shade(slot, R);
     }
 }
 
 // trace.wgsl:170-243. The inside tests (16 rays, and up to 9 more points) keep the group in
 // step; the sun and occlusion rays (2 g^2 + aoRays slots) run in the chosen schedule.
-template <int LAYOUT, bool STATS, int SCHED>
+template <int LAYOUT, bool STATS>
 __global__ void k_direct_tuned(TunedArgs A) {
     StackView st = make_stack(A.spill, A.depth);
     const lb_params &P = A.P;
@@ -302,7 +275,7 @@ __global__ void k_direct_tuned(TunedArgs A) {
                 }
             } else {
                 uint32_t r = slot - n_sun;
-                if (SCHED & 2) r = az_slot(r, rays);
+                r = az_slot(r, rays);
                 f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
                 ray_start(R, A.S.root, o_ao, d, P.reach[0], false, key + r + 2000u);
             }
@@ -321,7 +294,7 @@ __global__ void k_direct_tuned(TunedArgs A) {
                 if (R.back && R.t < P.ground[3]) back += 1.0f;
             }
         };
-        run_slots<LAYOUT, STATS, SCHED>(A, st, lig, total, (uint32_t)L, cnt, start, shade);
+        run_slots<LAYOUT, STATS>(A, st, lig, total, (uint32_t)L, cnt, start, shade);
         hard = group_or(hard, L);
         soft = group_sum(soft, L);
         back = group_sum(back, L);
@@ -336,7 +309,7 @@ __global__ void k_direct_tuned(TunedArgs A) {
 }
 
 // trace.wgsl:245-270: lane lig of an item's group takes its rays r = lig, lig + L, ...
-template <int LAYOUT, bool STATS, int SCHED>
+template <int LAYOUT, bool STATS>
 __global__ void k_gather_tuned(TunedArgs A) {
     StackView st = make_stack(A.spill, A.depth);
     const lb_params &P = A.P;
@@ -361,7 +334,7 @@ __global__ void k_gather_tuned(TunedArgs A) {
         KSum sr{0, 0}, sg{0, 0}, sb{0, 0};
         float back = 0.0f;
         auto start = [&](uint32_t slot, Ray &R) {
-            uint32_t r = (SCHED & 2) ? az_slot(slot, rays) : slot;
+            uint32_t r = az_slot(slot, rays);
             float jx = rnd(key + r, 19u, seed) - 0.5f;
             float jy = rnd(key + r, 23u, seed) - 0.5f;
             f3 o = add(add(add(tx.pos, scl(tx.dx, jx)), scl(tx.dy, jy)), scl(n, P.reach[2]));
@@ -387,7 +360,7 @@ __global__ void k_gather_tuned(TunedArgs A) {
                 sb.add(l.z);
             }
         };
-        run_slots<LAYOUT, STATS, SCHED>(A, st, lig, valid ? rays : 0u, (uint32_t)L, cnt, start,
+        run_slots<LAYOUT, STATS>(A, st, lig, valid ? rays : 0u, (uint32_t)L, cnt, start,
                                         shade);
         group_ksum(sr, L);
         group_ksum(sg, L);
@@ -401,7 +374,7 @@ __global__ void k_gather_tuned(TunedArgs A) {
 }
 
 // trace.wgsl:272-324 over rays + g^2 sun slots
-template <int LAYOUT, bool STATS, int SCHED>
+template <int LAYOUT, bool STATS>
 __global__ void k_volume_tuned(TunedArgs A) {
     StackView st = make_stack(A.spill, A.depth);
     const lb_params &P = A.P;
@@ -431,7 +404,7 @@ __global__ void k_volume_tuned(TunedArgs A) {
         float back = 0.0f, sun = 0.0f;
         auto start = [&](uint32_t slot, Ray &R) {
             if (slot < rays) {
-                uint32_t r = (SCHED & 2) ? az_slot(slot, rays) : slot;
+                uint32_t r = az_slot(slot, rays);
                 float u = fract(((float)r + 0.5f) / (float)rays + s1);
                 float v = fract(radical(r) + s2);
                 float z = 1.0f - 2.0f * u;
@@ -478,7 +451,7 @@ __global__ void k_volume_tuned(TunedArgs A) {
                 sun += 1.0f;
             }
         };
-        run_slots<LAYOUT, STATS, SCHED>(A, st, lig, valid ? rays + gg : 0u, (uint32_t)L, cnt,
+        run_slots<LAYOUT, STATS>(A, st, lig, valid ? rays + gg : 0u, (uint32_t)L, cnt,
                                         start, shade);
 #pragma unroll
         for (int k = 0; k < 12; k++) group_ksum(c[k], L);
@@ -503,32 +476,20 @@ __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
 
 using KernelFn = void (*)(TunedArgs);
 
-// tuning schedule -> SCHED bits: 0 lock-step, 1 flattened, 3 lock-step in azimuth order,
-// 4 the same as while-while, 5 flattened in azimuth order
 #define LB_KERNEL_FAMILY(NAME)                                                        \
-    template <int LAYOUT>                                                             \
-    static KernelFn pick_##NAME##_l(bool stats, int sched) {                          \
-        switch (sched) {                                                              \
-        case 1: return stats ? NAME<LAYOUT, true, 1> : NAME<LAYOUT, false, 1>;        \
-        case 3: return stats ? NAME<LAYOUT, true, 2> : NAME<LAYOUT, false, 2>;        \
-        case 4: return stats ? NAME<LAYOUT, true, 6> : NAME<LAYOUT, false, 6>;        \
-        case 5: return stats ? NAME<LAYOUT, true, 3> : NAME<LAYOUT, false, 3>;        \
-        default: return stats ? NAME<LAYOUT, true, 0> : NAME<LAYOUT, false, 0>;       \
-        }                                                                             \
-    }                                                                                 \
-    static KernelFn pick_##NAME(int layout, bool stats, int sched) {                  \
-        return layout == LB_NODES_WIDE ? pick_##NAME##_l<0>(stats, sched)             \
-                                       : pick_##NAME##_l<1>(stats, sched);            \
+    static KernelFn pick_##NAME(int layout, bool stats) {                             \
+        if (layout == LB_NODES_WIDE) return stats ? NAME<0, true> : NAME<0, false>;   \
+        return stats ? NAME<1, true> : NAME<1, false>;                                \
     }
 LB_KERNEL_FAMILY(k_direct_tuned)
 LB_KERNEL_FAMILY(k_gather_tuned)
 LB_KERNEL_FAMILY(k_volume_tuned)
 #undef LB_KERNEL_FAMILY
 
-static KernelFn pick_kernel(int pass, int layout, bool stats, int sched) {
-    if (pass == LB_PASS_DIRECT) return pick_k_direct_tuned(layout, stats, sched);
-    if (pass == LB_PASS_GATHER) return pick_k_gather_tuned(layout, stats, sched);
-    return pick_k_volume_tuned(layout, stats, sched);
+static KernelFn pick_kernel(int pass, int layout, bool stats) {
+    if (pass == LB_PASS_DIRECT) return pick_k_direct_tuned(layout, stats);
+    if (pass == LB_PASS_GATHER) return pick_k_gather_tuned(layout, stats);
+    return pick_k_volume_tuned(layout, stats);
 }
 
 // Lanes sharing one item: the rays an item casts over the rays a lane should take, as a power
@@ -553,9 +514,9 @@ struct TunedGrid {
 static hipError_t tuned_grid(lb_gpu *gpu, int pass, bool stats, const lb_params &P, uint32_t n,
                              TunedGrid *out) {
     const Tuning &T = gpu->tuning;
-    out->fn = pick_kernel(pass, T.nodes, stats, T.schedule);
+    out->fn = pick_kernel(pass, T.nodes, stats);
     out->block = (uint32_t)T.workgroup;
-    out->lds = (size_t)T.lds_stack * out->block * 4u;
+    out->lds = stack_lds_bytes(T.lds_stack, (int)out->block);
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, reinterpret_cast<const void *>(out->fn), (int)out->block, out->lds);
@@ -575,7 +536,7 @@ static hipError_t tuned_grid(lb_gpu *gpu, int pass, bool stats, const lb_params 
 }
 
 hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n) {
-    if (gpu->tuning.schedule == 2) return stream_prepare(gpu, pass, P, n);
+    if (gpu->tuning.schedule == 1) return stream_prepare(gpu, pass, P, n);
     TunedGrid tg;
     hipError_t e = tuned_grid(gpu, pass, false, P, n, &tg);
     if (e != hipSuccess) return e;
@@ -585,7 +546,7 @@ hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n) 
 hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                         const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
                         float4 *out) {
-    if (gpu->tuning.schedule == 2) return launch_stream(gpu, pass, stats, S, P, items, n, key_base, out);
+    if (gpu->tuning.schedule == 1) return launch_stream(gpu, pass, stats, S, P, items, n, key_base, out);
     TunedGrid tg;
     hipError_t e = tuned_grid(gpu, pass, stats, P, n, &tg);
     if (e != hipSuccess) return e;
@@ -601,7 +562,6 @@ hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
     A.spill = static_cast<uint32_t *>(gpu->spill.ptr);
     A.lanes_per_item = tg.lanes;
     A.depth = (uint32_t)gpu->tuning.lds_stack;
-    A.refill = (uint32_t)gpu->tuning.refill;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, gpu->stream, gpu->queue);
     hipLaunchKernelGGL(tg.fn, dim3(tg.grid), dim3(tg.block), tg.lds, gpu->stream, A);
     return hipGetLastError();

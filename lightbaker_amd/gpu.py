@@ -142,15 +142,16 @@ class Gpu:
         N.check(self._lib.lb_set_form(self.handle, int(form)))
 
     TUNING_KEYS = ("workgroup", "rays_per_lane", "lds_stack", "nodes", "refill", "schedule",
-                   "chunk")
+                   "chunk", "leaf_batch")
 
     def set_tuning(self, workgroup: int = None, rays_per_lane: int = None, lds_stack: int = None,
                    nodes: int = None, refill: int = None, schedule: int = None,
-                   chunk: int = None) -> None:
+                   chunk: int = None, leaf_batch: int = None) -> None:
         """Sets the TUNED form's knobs; the ones not given keep their value."""
         cur = self.tuning()
         given = dict(workgroup=workgroup, rays_per_lane=rays_per_lane, lds_stack=lds_stack,
-                     nodes=nodes, refill=refill, schedule=schedule, chunk=chunk)
+                     nodes=nodes, refill=refill, schedule=schedule, chunk=chunk,
+                     leaf_batch=leaf_batch)
         cur.update({k: v for k, v in given.items() if v is not None})
         t = N.Tuning(**{k: int(cur[k]) for k in self.TUNING_KEYS})
         N.check(self._lib.lb_set_tuning(self.handle, C.byref(t)))

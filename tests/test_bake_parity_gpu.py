@@ -1,7 +1,7 @@
 """The whole device half of the bake against the oracle's restatement of it (oracle/bake_ref.py:
 bake.rs:149-330 and volume.rs:69-117 over the oracle's C passes), on the same prepared inputs.
 
-Bar: the PLAIN form, and the TUNED form's streamed schedule (which shades in the reference's
+Bar: the PLAIN form, and the TUNED form's pooled schedule (which shades in the reference's
 order), give the final light atlas (direct + last bounce per texel) bit for bit. The TUNED form's
 lock-step schedule (compensated sums in another order) within 1e-4 relative.
 """
@@ -54,7 +54,7 @@ def oracle_inputs(scene, sess):
 @pytest.mark.parametrize("gen", [lambda: reference_test_scene(8.0), lambda: small_scene(1)],
                          ids=["reference-test-scene", "small-scene"])
 @pytest.mark.parametrize("form,schedule", [(N.FORM_PLAIN, 0), (N.FORM_TUNED, 0),
-                                           (N.FORM_TUNED, 2)], ids=["plain", "tuned", "streamed"])
+                                           (N.FORM_TUNED, 1)], ids=["plain", "tuned", "pooled"])
 def test_bake_matches_oracle(gpu, gen, form, schedule):
     scene = gen()
     print(f"bake parity form={form} schedule={schedule}", flush=True)
@@ -70,7 +70,7 @@ def test_bake_matches_oracle(gpu, gen, form, schedule):
     wl = want["light"].reshape(-1, 4)
     gat = sess.array(N.BAKE_GATHER_INDEX, np.uint32)
     assert np.array_equal(gat, want["gathered"]), "gather sets differ"
-    if form == N.FORM_PLAIN or schedule == 2:
+    if form == N.FORM_PLAIN or schedule == 1:
         bad = np.nonzero((light.view(np.uint32) != wl.view(np.uint32)).any(axis=1))[0]
         assert len(bad) == 0, f"{len(bad)} atlas texels differ: {light[bad[:3]]} vs {wl[bad[:3]]}"
     else:

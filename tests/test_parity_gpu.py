@@ -97,9 +97,8 @@ def test_tuned_matches(gpu, setup, pass_, nodes):
     _compare_tuned(pass_, got, want, nodes == N.NODES_QUANT)
 
 
-@pytest.mark.parametrize("cfg", [(64, 1, 8, 1, 1), (128, 4, 24, 64, 1), (512, 8, 24, 16, 1),
-                                 (256, 2, 32, 32, 0), (256, 8, 16, 48, 0), (256, 4, 16, 16, 3),
-                                 (256, 4, 16, 16, 4), (128, 2, 24, 48, 5)])
+@pytest.mark.parametrize("cfg", [(64, 1, 8, 1, 0), (128, 4, 24, 64, 0), (512, 8, 24, 16, 0),
+                                 (256, 2, 32, 32, 0), (256, 8, 48, 48, 0)])
 def test_tuned_configs(gpu, setup, cfg):
     """Every autotuner setting gives the same results (workgroup, rays/lane, LDS stack)."""
     wg, rpl, depth, refill, sched = cfg
@@ -145,12 +144,12 @@ def test_counters_match_oracle(gpu, setup):
     assert st == want
 
 
-# ---- the streamed schedule -----------------------------------------------------------------
+# ---- the pooled schedule -----------------------------------------------------------------
 
-STREAM = dict(workgroup=256, rays_per_lane=2, lds_stack=16, nodes=N.NODES_WIDE, refill=16,
-              schedule=2, chunk=0)
-DEFAULT = dict(workgroup=256, rays_per_lane=2, lds_stack=16, nodes=N.NODES_WIDE, refill=16,
-               schedule=0, chunk=0)
+STREAM = dict(workgroup=256, rays_per_lane=2, lds_stack=16, nodes=N.NODES_WIDE, refill=32,
+              schedule=1, chunk=0, leaf_batch=24)
+DEFAULT = dict(workgroup=256, rays_per_lane=2, lds_stack=16, nodes=N.NODES_WIDE, refill=32,
+               schedule=0, chunk=0, leaf_batch=24)
 
 
 def _run_tuned(gpu, setup, pass_, **tuning):
@@ -172,43 +171,43 @@ def _assert_bits(got, want):
 
 
 @pytest.mark.parametrize("pass_", [Pass.DIRECT, Pass.GATHER, Pass.VOLUME])
-def test_streamed_bit_exact(gpu, setup, pass_):
-    """The streamed schedule traces each ray in the reference's order and folds an item's hits in
+def test_pooled_bit_exact(gpu, setup, pass_):
+    """The pooled schedule traces each ray in the reference's order and folds an item's hits in
     the reference's ray order with its arithmetic: every output bit equals the oracle's."""
-    print(f"streamed {pass_.name}", flush=True)
+    print(f"pooled {pass_.name}", flush=True)
     items = setup["probes"] if pass_ == Pass.VOLUME else setup["items"]
     _assert_bits(_run_tuned(gpu, setup, pass_), _oracle(setup, pass_, items))
 
 
-@pytest.mark.parametrize("cfg", [(64, 8, 1, 0), (512, 48, 64, 0), (256, 16, 32, 5000),
-                                 (128, 24, 8, 777)])
-def test_streamed_configs(gpu, setup, cfg):
-    """Workgroup, LDS stack, refill threshold and chunking (777 rays: a few items a chunk) leave
-    every output bit as it is."""
-    wg, depth, refill, chunk = cfg
+@pytest.mark.parametrize("cfg", [(64, 8, 1, 0, 1), (512, 48, 64, 0, 64), (256, 16, 32, 5000, 8),
+                                 (128, 24, 8, 777, 32)])
+def test_pooled_configs(gpu, setup, cfg):
+    """Workgroup, LDS stack, refill threshold, leaf batch and chunking (777 rays: a few items a
+    chunk) leave every output bit as it is."""
+    wg, depth, refill, chunk, leaf_batch = cfg
     for pass_ in (Pass.DIRECT, Pass.GATHER, Pass.VOLUME):
-        print(f"streamed {pass_.name} cfg={cfg}", flush=True)
+        print(f"pooled {pass_.name} cfg={cfg}", flush=True)
         items = setup["probes"] if pass_ == Pass.VOLUME else setup["items"]
         got = _run_tuned(gpu, setup, pass_, workgroup=wg, lds_stack=depth, refill=refill,
-                         chunk=chunk)
+                         chunk=chunk, leaf_batch=leaf_batch)
         _assert_bits(got, _oracle(setup, pass_, items))
 
 
 @pytest.mark.parametrize("pass_", [Pass.DIRECT, Pass.GATHER, Pass.VOLUME])
-def test_streamed_quant(gpu, setup, pass_):
+def test_pooled_quant(gpu, setup, pass_):
     items = setup["probes"] if pass_ == Pass.VOLUME else setup["items"]
-    print(f"streamed quant {pass_.name}", flush=True)
+    print(f"pooled quant {pass_.name}", flush=True)
     got = _run_tuned(gpu, setup, pass_, nodes=N.NODES_QUANT)
     _compare_tuned(pass_, got, _oracle(setup, pass_, items), True)
 
 
 @pytest.mark.parametrize("pass_", [Pass.DIRECT, Pass.GATHER, Pass.VOLUME])
-def test_streamed_counters(gpu, setup, pass_):
-    """The streamed kernels trace exactly the rays the reference traces (the inside test's
+def test_pooled_counters(gpu, setup, pass_):
+    """The pooled kernels trace exactly the rays the reference traces (the inside test's
     rounds included) and visit exactly its nodes: their counters equal the oracle's."""
     from lightbaker_amd.gpu import DeviceArray
     items = (setup["probes"] if pass_ == Pass.VOLUME else setup["items"])[:300].copy()
-    print(f"streamed counters {pass_.name}", flush=True)
+    print(f"pooled counters {pass_.name}", flush=True)
     di = DeviceArray.of(gpu, items)
     dl = DeviceArray.of(gpu, setup["light"])
     do = DeviceArray(gpu, len(items) * 16 * pass_.width)
