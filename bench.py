@@ -56,10 +56,14 @@ def measured_traffic(kernel: str) -> dict | None:
     return rec if rec.get("kernel") == kernel else None
 
 
-def cpu_baseline(sess, target_s: float = 12.0) -> dict:
-    """The oracle (oracle/trace_ref.c, FP32, OpenMP over items) on a sample of the gather pass."""
+def cpu_baseline(sess, gpu, target_s: float = 12.0) -> dict:
+    """The CPU leg: the oracle (oracle/trace_ref.c, FP32, OpenMP over items) timed on a sample of
+    the gather pass, and the error of the PLAIN and TUNED forms against the oracle's FP64 tracer
+    on identical rays (gather: every gather item; direct: a sample of texels), with the final
+    light atlas."""
     import oracle
     from lightbaker_amd import _native as N
+    from lightbaker_amd.gpu import Bvh, Pass
     texels = sess.array(N.BAKE_TEXELS, N.TEXEL)
     gat = sess.array(N.BAKE_GATHER_INDEX, np.uint32)
     nodes = sess.array(N.BAKE_NODES, N.NODE)
@@ -87,10 +91,45 @@ def cpu_baseline(sess, target_s: float = 12.0) -> dict:
     n = int(min(len(items), max(256, n * target_s / max(dt, 1e-3))))
     dt = run(n)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": n * rays_per_item / dt / 1e9, "unit": "Grays/s", "cores": threads,
+    base = {"value": n * rays_per_item / dt / 1e9, "unit": "Grays/s", "cores": threads,
             "kind": "port",
-            "sample": f"{n} gather items x {rays_per_item} rays of the map's first bounce "
+            "sample": f"{n} gather items x {rays_per_item} rays of the map's last bounce "
                       f"(random subset, final light atlas), {dt:.1f} s"}
+
+    # error against FP64 on identical rays; the key of an item is its index in the pass
+    sb = gpu.scene(Bvh(nodes, tris), mats, charts)
+
+    def forms(pass_, its, lt):
+        out = {}
+        for name, form in (("plain", N.FORM_PLAIN), ("tuned", N.FORM_TUNED)):
+            gpu.set_form(form)
+            out[name] = gpu.run(pass_, sb, params, its, lt)
+        gpu.set_form(N.FORM_PLAIN)
+        return out
+
+    def errors(got, ref, cols):
+        r = ref[:, cols]
+        scale = float(np.max(np.abs(r))) or 1.0
+        res = {}
+        for name, g in got.items():
+            d = g[:, cols].astype(np.float64) - r
+            res[name] = {"rmse": float(np.sqrt(np.mean(d * d))),
+                         "max_abs_of_full_scale": float(np.max(np.abs(d))) / scale}
+        return res
+
+    acc = {}
+    g_items = np.ascontiguousarray(texels[gat])
+    got = forms(Pass.GATHER, g_items, light)
+    ref = oracle.run64(1, nodes, tris, mats, charts, params, g_items, 0, light)
+    acc["gather_irradiance"] = dict(errors(got, ref, [0, 1, 2]), items=len(g_items))
+    d_items = np.ascontiguousarray(texels[rng.permutation(len(texels))[:200000]])
+    got = forms(Pass.DIRECT, d_items, None)
+    ref = oracle.run64(0, nodes, tris, mats, charts, params, d_items, 0, None)
+    acc["direct_occlusion"] = dict(errors(got, ref, [2]), items=len(d_items))
+    acc["direct_soft_sun"] = dict(errors(got, ref, [0]), items=len(d_items))
+    sb.close()
+    base["error_vs_fp64"] = acc
+    return base
 
 
 def main() -> int:
@@ -153,11 +192,10 @@ def main() -> int:
         from lightbaker_amd.autotune import autotune
         t0 = time.perf_counter()
         sess.execute()
-        tuned = autotune(sess, scene.settings.gi_bounces,
-                         log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+        tuned = autotune(sess, log=lambda m: print("# " + m, file=sys.stderr, flush=True))
         tuned["seconds"] = time.perf_counter() - t0
         print(f"# autotune: {tuned}", file=sys.stderr, flush=True)
-        layout = tuned["config"]["nodes"]
+        layout = gpu.tuning(Pass.GATHER)["nodes"]
     stats = sess.execute(stats=True)
     rays = sum(s["rays"] for s in stats)
     info = sess.info()
@@ -207,7 +245,7 @@ def main() -> int:
         plain_ms = (time.perf_counter() - t0) * 1000.0
         ps.close()
 
-    cpu = None if args.no_cpu else cpu_baseline(sess)
+    cpu = None if args.no_cpu else cpu_baseline(sess, gpu)
     out = {
         "metric": "Grays/s, 100x100 m 3-bounce bake (direct + 3 gather bounces + volume)",
         "value": rays / (ms_step / 1e3) / 1e9,

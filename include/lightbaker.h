@@ -144,8 +144,12 @@ typedef struct lb_tuning {
                            nodes */
 } lb_tuning;
 
+/* Sets the knobs of all three passes; lb_get_tuning reads the gather pass's. */
 int lb_set_tuning(lb_gpu *gpu, const lb_tuning *tuning);
 int lb_get_tuning(const lb_gpu *gpu, lb_tuning *tuning);
+/* The knobs of one pass (LB_PASS_*): each pass is its own kernel with its own best setting. */
+int lb_set_pass_tuning(lb_gpu *gpu, int pass, const lb_tuning *tuning);
+int lb_get_pass_tuning(const lb_gpu *gpu, int pass, lb_tuning *tuning);
 
 /* Traversal counters of the last lb_run_stats call: rays traced, inner-node visits (child-box
  * pairs tested), triangle tests, material loads, bounced() light gathers and leaf visits. */

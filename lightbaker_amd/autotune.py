@@ -1,13 +1,13 @@
-"""The TUNED form's autotuner: picks the workgroup size, rays per lane, LDS stack depth, refill
-threshold and node layout with the best time on a prepared bake's own passes, checking that
-every candidate gives the same results as the starting configuration.
+"""The TUNED form's autotuner: picks, for each pass on its own, the schedule, workgroup size, rays
+per lane, LDS stack depth, refill threshold, leaf batch and node layout with the best time on a
+prepared bake's own launches, then checks that the bake still gives the starting setting's light
+atlas.
 
-Search: coordinate descent from the current setting, one dimension at a time over its range
-(schedule lock-step/pooled, workgroup 64..512, rays/lane 1..8 for lock-step, LDS stack 8..48,
-refill and leaf batch for pooled, WIDE/QUANT nodes), at most two rounds; each
-candidate timed on the gather pass (weighted by the bounces) and the direct pass,
-the median of `reps` launches. A candidate the device cannot launch (an LDS stack larger than a
-workgroup's LDS) is skipped.
+Search: coordinate descent from the current setting, one knob at a time over its range, at most
+`rounds` rounds, each candidate timed as the median of `reps` launches of its pass. A knob that
+the candidate's schedule ignores is skipped (rays per lane for the pooled schedule, refill and
+leaf batch for the lock-step one); a candidate the device cannot launch (an LDS stack larger
+than a workgroup's LDS) is skipped too.
 """
 from __future__ import annotations
 
@@ -17,71 +17,74 @@ from . import _native as N
 from .gpu import Pass
 
 RANGES = {
-    "workgroup": [64, 128, 256, 512],
-    "rays_per_lane": [1, 2, 4, 8],
-    "lds_stack": [8, 16, 24, 32, 48],
-    "refill": [16, 32, 48, 56],
-    "nodes": [N.NODES_WIDE, N.NODES_QUANT],
     "schedule": [0, 1],
+    "rays_per_lane": [1, 2, 4, 8],
+    "refill": [16, 32, 48, 56],
     "leaf_batch": [8, 16, 24, 32, 64],
+    "workgroup": [64, 128, 256, 512],
+    "lds_stack": [8, 16, 24, 32, 48],
+    "nodes": [N.NODES_WIDE, N.NODES_QUANT],
 }
-ORDER = ["schedule", "rays_per_lane", "refill", "leaf_batch", "workgroup", "lds_stack", "nodes"]
 
 
-def _cost(sess, bounces: int, reps: int) -> float:
-    g = sorted(sess.time_pass(Pass.GATHER, 1) for _ in range(reps))[reps // 2]
-    d = sorted(sess.time_pass(Pass.DIRECT, 1) for _ in range(reps))[reps // 2]
-    return g * max(bounces, 1) + d
+def _skip(dim: str, cand: dict) -> bool:
+    if dim in ("refill", "leaf_batch") and cand["schedule"] == 0:
+        return True
+    return dim == "rays_per_lane" and cand["schedule"] == 1
 
 
-def autotune(sess, bounces: int, reps: int = 3, rounds: int = 2, log=None) -> dict:
-    """Tunes the session's GPU in place (the session must be TUNED and executed once)."""
+def _tune_pass(sess, pass_: Pass, reps: int, rounds: int, log) -> tuple[dict, float, int]:
     gpu = sess.gpu
-    cur = gpu.tuning()
-
-    def apply(c):
-        gpu.set_tuning(**c)
 
     def measure(c):
         try:
-            apply(c)
-            return _cost(sess, bounces, reps)
+            gpu.set_tuning(pass_=pass_, **c)
+            return sorted(sess.time_pass(pass_, 1) for _ in range(reps))[reps // 2]
         except N.NativeError:
             return float("inf")
 
-    start = dict(cur)
-    sess.execute()
-    sess.finish()
-    ref_light = sess.array(N.BAKE_LIGHT, np.float32).copy()
+    cur = gpu.tuning(pass_)
     best = measure(cur)
-    history = [(dict(cur), best)]
+    tried = 1
     for _ in range(rounds):
         changed = False
-        for dim in ORDER:
-            for v in RANGES[dim]:
-                if v == cur[dim]:
-                    continue
-                if dim in ("refill", "leaf_batch") and cur["schedule"] == 0:
-                    continue
-                if dim == "rays_per_lane" and cur["schedule"] == 1:
-                    continue
+        for dim, values in RANGES.items():
+            for v in values:
                 cand = dict(cur, **{dim: v})
+                if v == cur[dim] or _skip(dim, cand):
+                    continue
                 t = measure(cand)
-                history.append((cand, t))
+                tried += 1
                 if log:
-                    log(f"autotune {cand}: {t:.2f} ms")
+                    log(f"autotune {pass_.name} {cand}: {t:.2f} ms")
                 if t < best * 0.995:
                     best, cur, changed = t, cand, True
         if not changed:
             break
-    apply(cur)
-    # the pick must bake what the starting setting bakes (ray sums to compensated rounding)
+    gpu.set_tuning(pass_=pass_, **cur)
+    return cur, best, tried
+
+
+def autotune(sess, reps: int = 3, rounds: int = 2, log=None) -> dict:
+    """Tunes the session's GPU in place, pass by pass (the session must be TUNED and executed
+    once). Returns each pass's pick and time; reverts every pass if the picks change the light
+    atlas by more than the compensated sums' rounding (1e-4 relative)."""
+    gpu = sess.gpu
+    start = {p: gpu.tuning(p) for p in Pass}
+    sess.execute()
+    sess.finish()
+    ref_light = sess.array(N.BAKE_LIGHT, np.float32).copy()
+    picks, ms, tried = {}, {}, 0
+    for p in (Pass.GATHER, Pass.DIRECT, Pass.VOLUME):
+        picks[p.name.lower()], ms[p.name.lower()], n = _tune_pass(sess, p, reps, rounds, log)
+        tried += n
     sess.execute()
     sess.finish()
     light = sess.array(N.BAKE_LIGHT, np.float32)
     err = float(np.max(np.abs(light - ref_light) / (np.abs(ref_light) + 1e-3)))
     if not err < 1e-4:
-        apply(start)
-        return {"config": start, "ms": history[0][1], "tried": len(history),
-                "rejected": cur, "max_rel_diff": err}
-    return {"config": cur, "ms": best, "tried": len(history), "max_rel_diff": err}
+        for p, t in start.items():
+            gpu.set_tuning(pass_=p, **t)
+        return {"config": {p.name.lower(): t for p, t in start.items()}, "tried": tried,
+                "rejected": picks, "max_rel_diff": err}
+    return {"config": picks, "ms": ms, "tried": tried, "max_rel_diff": err}

@@ -544,7 +544,7 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
         if (B->opt.graph && !per_pass) {
             // one bounce captured, replayed gi_bounces times (same buffers each time); the item
             // count is part of the capture, so a new count captures again
-            const Tuning &tn = gpu->tuning;
+            const Tuning &tn = gpu->tuning[LB_PASS_GATHER];
             const Tuning &tg = B->graph_tuning;
             bool same = tn.workgroup == tg.workgroup && tn.rays_per_lane == tg.rays_per_lane &&
                         tn.lds_stack == tg.lds_stack && tn.nodes == tg.nodes &&

@@ -263,45 +263,74 @@ int lb_set_form(lb_gpu *gpu, int form) {
     return 0;
 }
 
-int lb_set_tuning(lb_gpu *gpu, const lb_tuning *t) {
-    if (!gpu || !t) return fail("lb_set_tuning: null handle");
+static int check_tuning(const lb_gpu *gpu, const lb_tuning *t, const char *who) {
+    if (!gpu || !t) return fail("%s: null handle", who);
     if (t->workgroup < 64 || t->workgroup > 1024 || t->workgroup % 64)
-        return fail("lb_set_tuning: workgroup %d is not a multiple of 64 in 64..1024", t->workgroup);
+        return fail("%s: workgroup %d is not a multiple of 64 in 64..1024", who, t->workgroup);
     if (t->rays_per_lane < 1 || t->rays_per_lane > 8)
-        return fail("lb_set_tuning: rays per lane %d outside 1..8", t->rays_per_lane);
+        return fail("%s: rays per lane %d outside 1..8", who, t->rays_per_lane);
     if (t->lds_stack < 8 || t->lds_stack > 48)
-        return fail("lb_set_tuning: LDS stack %d outside 8..48", t->lds_stack);
+        return fail("%s: LDS stack %d outside 8..48", who, t->lds_stack);
     if (t->nodes != LB_NODES_WIDE && t->nodes != LB_NODES_QUANT)
-        return fail("lb_set_tuning: no node layout %d", t->nodes);
-    if (t->refill < 1 || t->refill > 64) return fail("lb_set_tuning: refill %d outside 1..64", t->refill);
-    if (t->schedule < 0 || t->schedule > 1) return fail("lb_set_tuning: no schedule %d", t->schedule);
-    if (t->chunk < 0) return fail("lb_set_tuning: chunk %d is negative", t->chunk);
+        return fail("%s: no node layout %d", who, t->nodes);
+    if (t->refill < 1 || t->refill > 64) return fail("%s: refill %d outside 1..64", who, t->refill);
+    if (t->schedule < 0 || t->schedule > 1) return fail("%s: no schedule %d", who, t->schedule);
+    if (t->chunk < 0) return fail("%s: chunk %d is negative", who, t->chunk);
     if (t->leaf_batch < 1 || t->leaf_batch > 64)
-        return fail("lb_set_tuning: leaf batch %d outside 1..64", t->leaf_batch);
+        return fail("%s: leaf batch %d outside 1..64", who, t->leaf_batch);
     if (stack_lds_bytes(t->lds_stack, t->workgroup) > gpu->lds_per_block)
-        return fail("lb_set_tuning: an LDS stack of %d entries for %d lanes exceeds %zu bytes",
+        return fail("%s: an LDS stack of %d entries for %d lanes exceeds %zu bytes", who,
                     t->lds_stack, t->workgroup, gpu->lds_per_block);
-    gpu->tuning.workgroup = t->workgroup;
-    gpu->tuning.rays_per_lane = t->rays_per_lane;
-    gpu->tuning.lds_stack = t->lds_stack;
-    gpu->tuning.nodes = t->nodes;
-    gpu->tuning.refill = t->refill;
-    gpu->tuning.schedule = t->schedule;
-    gpu->tuning.chunk = t->chunk;
-    gpu->tuning.leaf_batch = t->leaf_batch;
+    return 0;
+}
+
+static void put_tuning(Tuning &d, const lb_tuning *t) {
+    d.workgroup = t->workgroup;
+    d.rays_per_lane = t->rays_per_lane;
+    d.lds_stack = t->lds_stack;
+    d.nodes = t->nodes;
+    d.refill = t->refill;
+    d.schedule = t->schedule;
+    d.chunk = t->chunk;
+    d.leaf_batch = t->leaf_batch;
+}
+
+static void get_tuning(const Tuning &s, lb_tuning *t) {
+    t->workgroup = s.workgroup;
+    t->rays_per_lane = s.rays_per_lane;
+    t->lds_stack = s.lds_stack;
+    t->nodes = s.nodes;
+    t->refill = s.refill;
+    t->schedule = s.schedule;
+    t->chunk = s.chunk;
+    t->leaf_batch = s.leaf_batch;
+}
+
+int lb_set_tuning(lb_gpu *gpu, const lb_tuning *t) {
+    if (check_tuning(gpu, t, "lb_set_tuning")) return 1;
+    for (auto &d : gpu->tuning) put_tuning(d, t);
     return 0;
 }
 
 int lb_get_tuning(const lb_gpu *gpu, lb_tuning *t) {
     if (!gpu || !t) return fail("lb_get_tuning: null handle");
-    t->workgroup = gpu->tuning.workgroup;
-    t->rays_per_lane = gpu->tuning.rays_per_lane;
-    t->lds_stack = gpu->tuning.lds_stack;
-    t->nodes = gpu->tuning.nodes;
-    t->refill = gpu->tuning.refill;
-    t->schedule = gpu->tuning.schedule;
-    t->chunk = gpu->tuning.chunk;
-    t->leaf_batch = gpu->tuning.leaf_batch;
+    get_tuning(gpu->tuning[LB_PASS_GATHER], t);
+    return 0;
+}
+
+int lb_set_pass_tuning(lb_gpu *gpu, int pass, const lb_tuning *t) {
+    if (check_tuning(gpu, t, "lb_set_pass_tuning")) return 1;
+    if (pass < LB_PASS_DIRECT || pass > LB_PASS_VOLUME)
+        return fail("lb_set_pass_tuning: no pass %d", pass);
+    put_tuning(gpu->tuning[pass], t);
+    return 0;
+}
+
+int lb_get_pass_tuning(const lb_gpu *gpu, int pass, lb_tuning *t) {
+    if (!gpu || !t) return fail("lb_get_pass_tuning: null handle");
+    if (pass < LB_PASS_DIRECT || pass > LB_PASS_VOLUME)
+        return fail("lb_get_pass_tuning: no pass %d", pass);
+    get_tuning(gpu->tuning[pass], t);
     return 0;
 }
 
@@ -398,9 +427,10 @@ int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_pa
     if (pass != LB_PASS_DIRECT && !light) return fail("run: the gather and volume read a light");
     if (check_params(scene, params, pass, pass == LB_PASS_DIRECT ? 1 : n_light)) return 1;
     if (form == LB_FORM_TUNED &&
-        stack_lds_bytes(gpu->tuning.lds_stack, gpu->tuning.workgroup) > gpu->lds_per_block)
+        stack_lds_bytes(gpu->tuning[pass].lds_stack, gpu->tuning[pass].workgroup) >
+            gpu->lds_per_block)
         return fail("run: an LDS stack of %d entries for %d lanes exceeds %zu bytes a workgroup",
-                    gpu->tuning.lds_stack, gpu->tuning.workgroup, gpu->lds_per_block);
+                    gpu->tuning[pass].lds_stack, gpu->tuning[pass].workgroup, gpu->lds_per_block);
     hipSetDevice(gpu->device);
     lb_params P = *params;
     P.counts[3] = (uint32_t)n;

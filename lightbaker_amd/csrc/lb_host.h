@@ -42,7 +42,7 @@ struct lb_gpu {
     lbk::DevBuf stream_hits;                    // TUNED, pooled: a chunk's hits
     lbk::DevBuf stream_aux;                     // TUNED, pooled: picks, item lists, counts
     int form = LB_FORM_PLAIN;
-    lbk::Tuning tuning;
+    lbk::Tuning tuning[3];  // TUNED: the knobs of each pass (LB_PASS_*)
 };
 
 struct lb_scene {

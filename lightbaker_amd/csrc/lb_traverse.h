@@ -56,7 +56,6 @@ __device__ __forceinline__ StackView make_stack(uint32_t *spill, uint32_t depth)
 template <int LAYOUT>
 __device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn, f3 &amx,
                                          f3 &bmn, f3 &bmx, uint32_t &ra, uint32_t &rb) {
-    uint32_t first, flags;
     if (LAYOUT == LB_NODES_WIDE) {
         const float4 *w = S.wide + n * 4u;
         float4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
@@ -64,10 +63,8 @@ __device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn
         amx = mk(w0.w, w1.x, w1.y);
         bmn = mk(w1.z, w1.w, w2.x);
         bmx = mk(w2.y, w2.z, w2.w);
-        first = fbits(w3.x);
-        flags = fbits(w3.y);
-        ra = flags & 1u ? fbits(w3.z) : first;
-        rb = flags & 2u ? fbits(w3.w) : first + 1u;
+        ra = fbits(w3.x);
+        rb = fbits(w3.y);
     } else {
         const uint4 *q = S.quant + n * 2u;
         uint4 q0 = q[0], q1 = q[1];
@@ -79,8 +76,7 @@ __device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn
         amx = mk(ox + b(q1.x, 3) * sx, oy + b(q1.y, 0) * sy, oz + b(q1.y, 1) * sz);
         bmn = mk(ox + b(q1.y, 2) * sx, oy + b(q1.y, 3) * sy, oz + b(q1.z, 0) * sz);
         bmx = mk(ox + b(q1.z, 1) * sx, oy + b(q1.z, 2) * sy, oz + b(q1.z, 3) * sz);
-        first = q1.w & 0x3fffffffu;
-        flags = q1.w >> 30;
+        uint32_t first = q1.w & 0x3fffffffu, flags = q1.w >> 30;
         ra = flags & 1u ? (LEAF_BIT | (31u << 26) | first) : first;
         rb = flags & 2u ? (LEAF_BIT | (31u << 26) | (first + 1u)) : first + 1u;
     }
@@ -156,9 +152,10 @@ __device__ __forceinline__ bool inner_visit(const SceneView &S, Ray &R, const St
     if (STATS) cnt.inner++;
     float ta = slab(R.o, R.inv, amn, amx, R.t);
     float tb = slab(R.o, R.inv, bmn, bmx, R.t);
+    // tn, tf: min and max of (ta, tb), taken by the same comparison (no NaN reaches here)
     bool an = ta <= tb;
     uint32_t near = an ? ra : rb, far = an ? rb : ra;
-    float tn = vmin(ta, tb), tf = vmax(ta, tb);
+    float tn = an ? ta : tb, tf = an ? tb : ta;
     if (!(tn < FAR)) return false;
     if (tf < FAR && R.sp < (uint32_t)STACK) {
         st.put(R.sp, far);

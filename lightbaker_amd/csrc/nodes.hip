@@ -3,7 +3,7 @@
 // Both layouts are indexed by the reference node index and hold, for an inner node, the boxes of
 // its two children (first, first + 1) so one fetch serves the pair the reference reads as three
 // 32 B records (trace.wgsl:90,111-112):
-//   WIDE   4 x 16 B: a.min xyz a.max.x | a.max.yz b.min.xy | b.min.z b.max xyz | first, leaf flags
+//   WIDE   4 x 16 B: a.min xyz a.max.x | a.max.yz b.min.xy | b.min.z b.max xyz | a ref, b ref
 //   QUANT  2 x 16 B: origin xyz, exponents | 12 bytes of children's boxes, first + leaf flags
 // QUANT boxes are 8-bit multiples of a power of two per axis above the parent's min corner, each
 // rounded outward and checked against the device's own decode (origin + q * 2^e in FP32), so
@@ -131,11 +131,10 @@ int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const l
         const float vals[12] = {a.min[0], a.min[1], a.min[2], a.max[0], a.max[1], a.max[2],
                                 b.min[0], b.min[1], b.min[2], b.max[0], b.max[1], b.max[2]};
         std::memcpy(w, vals, sizeof vals);
-        w[12] = fl(n.first);
-        w[13] = fl(flags);
-        // the leaf children's (first, count): a leaf visit then reads no node record
-        w[14] = fl(a.count > 0 ? leaf_ref(a, n.first) : 0u);
-        w[15] = fl(b.count > 0 ? leaf_ref(b, n.first + 1) : 0u);
+        // each child's reference, a leaf's carrying its (first, count): a leaf visit then reads
+        // no node record
+        w[12] = fl(a.count > 0 ? leaf_ref(a, n.first) : n.first);
+        w[13] = fl(b.count > 0 ? leaf_ref(b, n.first + 1) : n.first + 1);
         // quantized: origin at the union's min corner, one exponent an axis
         uint32_t *q = &quant[i * 8];
         uint8_t qb[12];

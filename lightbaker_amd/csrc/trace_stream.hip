@@ -483,7 +483,7 @@ static hipError_t pool_grid(lb_gpu *gpu, PoolFn fn, uint32_t block, size_t lds, 
 // Grid and chunk of a launch over n items; reserves the hits, item lists and spill area.
 static hipError_t pool_plan(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n,
                             PoolPlan *out) {
-    const Tuning &T = gpu->tuning;
+    const Tuning &T = gpu->tuning[pass];
     out->block = (uint32_t)T.workgroup;
     out->lds = stack_lds_bytes(T.lds_stack, (int)out->block);
     // the spill area covers the largest grid any source's kernel launches with
@@ -516,9 +516,9 @@ hipError_t stream_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n)
     return pool_plan(gpu, pass, P, n, &pp);
 }
 
-static hipError_t pool_launch(lb_gpu *gpu, const PoolPlan &pp, int src, bool stats,
+static hipError_t pool_launch(lb_gpu *gpu, int pass, const PoolPlan &pp, int src, bool stats,
                               PoolArgs A) {
-    const Tuning &T = gpu->tuning;
+    const Tuning &T = gpu->tuning[pass];
     PoolFn fn = pick_pool(src, T.nodes, stats);
     uint32_t grid = 0;
     hipError_t e = pool_grid(gpu, fn, pp.block, pp.lds, &grid);
@@ -558,7 +558,7 @@ hipError_t launch_stream(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
         A.key0 = key_base + c0;
         if (pass == LB_PASS_GATHER) {
             A.slots = P.counts[2];
-            if ((e = pool_launch(gpu, pp, SRC_GATHER, stats, A)) != hipSuccess) return e;
+            if ((e = pool_launch(gpu, pass, pp, SRC_GATHER, stats, A)) != hipSuccess) return e;
             if (stats)
                 hipLaunchKernelGGL(k_shade_gather<true>, items_grid, blk, 0, st, S, P, A.items,
                                    nc, A.key0, hits, out + c0, ctr);
@@ -567,7 +567,7 @@ hipError_t launch_stream(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                                    nc, A.key0, hits, out + c0, ctr);
         } else if (pass == LB_PASS_VOLUME) {
             A.slots = P.counts[2] + P.counts[0] * P.counts[0];
-            if ((e = pool_launch(gpu, pp, SRC_VOLUME, stats, A)) != hipSuccess) return e;
+            if ((e = pool_launch(gpu, pass, pp, SRC_VOLUME, stats, A)) != hipSuccess) return e;
             if (stats)
                 hipLaunchKernelGGL(k_shade_volume<true>, items_grid, blk, 0, st, S, P, A.items,
                                    nc, hits, out + (size_t)c0 * 4u, ctr);
@@ -587,13 +587,13 @@ hipError_t launch_stream(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                 B.q = q;
                 B.list = cur;
                 B.n_dev = count;
-                if ((e = pool_launch(gpu, pp, SRC_INSIDE, stats, B)) != hipSuccess) return e;
+                if ((e = pool_launch(gpu, pass, pp, SRC_INSIDE, stats, B)) != hipSuccess) return e;
                 hipLaunchKernelGGL(k_settle_inside, items_grid, blk, 0, st, P, nc, q, cur, count,
                                    hits, list[(q + 1u) & 1u], counts + q + 1u, pick);
             }
             A.slots = 2u * P.counts[0] * P.counts[0] + P.counts[1];
             A.pick = pick;
-            if ((e = pool_launch(gpu, pp, SRC_DIRECT, stats, A)) != hipSuccess) return e;
+            if ((e = pool_launch(gpu, pass, pp, SRC_DIRECT, stats, A)) != hipSuccess) return e;
             hipLaunchKernelGGL(k_shade_direct, items_grid, blk, 0, st, P, A.items, nc, pick, hits,
                                out + c0);
         }

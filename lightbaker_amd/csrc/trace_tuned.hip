@@ -513,7 +513,7 @@ struct TunedGrid {
 
 static hipError_t tuned_grid(lb_gpu *gpu, int pass, bool stats, const lb_params &P, uint32_t n,
                              TunedGrid *out) {
-    const Tuning &T = gpu->tuning;
+    const Tuning &T = gpu->tuning[pass];
     out->fn = pick_kernel(pass, T.nodes, stats);
     out->block = (uint32_t)T.workgroup;
     out->lds = stack_lds_bytes(T.lds_stack, (int)out->block);
@@ -536,7 +536,7 @@ static hipError_t tuned_grid(lb_gpu *gpu, int pass, bool stats, const lb_params 
 }
 
 hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n) {
-    if (gpu->tuning.schedule == 1) return stream_prepare(gpu, pass, P, n);
+    if (gpu->tuning[pass].schedule == 1) return stream_prepare(gpu, pass, P, n);
     TunedGrid tg;
     hipError_t e = tuned_grid(gpu, pass, false, P, n, &tg);
     if (e != hipSuccess) return e;
@@ -546,7 +546,7 @@ hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n) 
 hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                         const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
                         float4 *out) {
-    if (gpu->tuning.schedule == 1) return launch_stream(gpu, pass, stats, S, P, items, n, key_base, out);
+    if (gpu->tuning[pass].schedule == 1) return launch_stream(gpu, pass, stats, S, P, items, n, key_base, out);
     TunedGrid tg;
     hipError_t e = tuned_grid(gpu, pass, stats, P, n, &tg);
     if (e != hipSuccess) return e;
@@ -561,7 +561,7 @@ hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
     A.queue = gpu->queue;
     A.spill = static_cast<uint32_t *>(gpu->spill.ptr);
     A.lanes_per_item = tg.lanes;
-    A.depth = (uint32_t)gpu->tuning.lds_stack;
+    A.depth = (uint32_t)gpu->tuning[pass].lds_stack;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, gpu->stream, gpu->queue);
     hipLaunchKernelGGL(tg.fn, dim3(tg.grid), dim3(tg.block), tg.lds, gpu->stream, A);
     return hipGetLastError();

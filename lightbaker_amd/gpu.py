@@ -146,19 +146,27 @@ class Gpu:
 
     def set_tuning(self, workgroup: int = None, rays_per_lane: int = None, lds_stack: int = None,
                    nodes: int = None, refill: int = None, schedule: int = None,
-                   chunk: int = None, leaf_batch: int = None) -> None:
-        """Sets the TUNED form's knobs; the ones not given keep their value."""
-        cur = self.tuning()
+                   chunk: int = None, leaf_batch: int = None, pass_: Pass = None) -> None:
+        """Sets the TUNED form's knobs of one pass, or of all three (starting from the gather
+        pass's) when `pass_` is None; the knobs not given keep their value."""
+        cur = self.tuning(pass_)
         given = dict(workgroup=workgroup, rays_per_lane=rays_per_lane, lds_stack=lds_stack,
                      nodes=nodes, refill=refill, schedule=schedule, chunk=chunk,
                      leaf_batch=leaf_batch)
         cur.update({k: v for k, v in given.items() if v is not None})
         t = N.Tuning(**{k: int(cur[k]) for k in self.TUNING_KEYS})
-        N.check(self._lib.lb_set_tuning(self.handle, C.byref(t)))
+        if pass_ is None:
+            N.check(self._lib.lb_set_tuning(self.handle, C.byref(t)))
+        else:
+            N.check(self._lib.lb_set_pass_tuning(self.handle, int(pass_), C.byref(t)))
 
-    def tuning(self) -> dict:
+    def tuning(self, pass_: Pass = None) -> dict:
+        """The knobs of one pass (the gather pass's when `pass_` is None)."""
         t = N.Tuning()
-        N.check(self._lib.lb_get_tuning(self.handle, C.byref(t)))
+        if pass_ is None:
+            N.check(self._lib.lb_get_tuning(self.handle, C.byref(t)))
+        else:
+            N.check(self._lib.lb_get_pass_tuning(self.handle, int(pass_), C.byref(t)))
         return {k: getattr(t, k) for k in self.TUNING_KEYS}
 
     def run(self, pass_: Pass, scene: SceneBuffers, params: np.ndarray, texels: np.ndarray,
