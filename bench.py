@@ -176,7 +176,7 @@ def main() -> int:
     scene = map_scene(2)
     t_scene = time.perf_counter() - t_scene
     gpu = Gpu()
-    gpu.set_tuning(256, 4, 16, layout, 32, args.schedule, leaf_batch=24)
+    gpu.set_tuning(256, 4, 16, layout, 32, args.schedule, leaf_batch=24, waves=6)
     print(f"# {gpu.adapter}; scene {scene.triangle_count()} triangles ({t_scene:.1f} s)",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()

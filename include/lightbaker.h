@@ -142,6 +142,9 @@ typedef struct lb_tuning {
     int leaf_batch;     /* pooled schedule: lanes (1..64) standing at a leaf before a wave
                            tests leaves rather than stepping its other lanes through inner
                            nodes */
+    int waves;          /* lock-step schedule: waves a SIMD the kernel is compiled to keep
+                           resident: 0 (the compiler's choice), 6 or 8 (fewer registers a lane,
+                           some spilled) */
 } lb_tuning;
 
 /* Sets the knobs of all three passes; lb_get_tuning reads the gather pass's. */

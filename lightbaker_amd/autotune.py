@@ -5,8 +5,8 @@ atlas.
 
 Search: coordinate descent from the current setting, one knob at a time over its range, at most
 `rounds` rounds, each candidate timed as the median of `reps` launches of its pass. A knob that
-the candidate's schedule ignores is skipped (rays per lane for the pooled schedule, refill and
-leaf batch for the lock-step one); a candidate the device cannot launch (an LDS stack larger
+the candidate's schedule ignores is skipped (rays per lane and waves for the pooled schedule,
+refill and leaf batch for the lock-step one); a candidate the device cannot launch (an LDS stack larger
 than a workgroup's LDS) is skipped too.
 """
 from __future__ import annotations
@@ -24,11 +24,14 @@ RANGES = {
     "workgroup": [64, 128, 256, 512],
     "lds_stack": [8, 16, 24, 32, 48],
     "nodes": [N.NODES_WIDE, N.NODES_QUANT],
+    "waves": [0, 6, 8],
 }
 
 
 def _skip(dim: str, cand: dict) -> bool:
     if dim in ("refill", "leaf_batch") and cand["schedule"] == 0:
+        return True
+    if dim == "waves" and cand["schedule"] == 1:
         return True
     return dim == "rays_per_lane" and cand["schedule"] == 1
 

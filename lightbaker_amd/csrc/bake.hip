@@ -549,7 +549,8 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
             bool same = tn.workgroup == tg.workgroup && tn.rays_per_lane == tg.rays_per_lane &&
                         tn.lds_stack == tg.lds_stack && tn.nodes == tg.nodes &&
                         tn.refill == tg.refill && tn.schedule == tg.schedule &&
-                        tn.chunk == tg.chunk && tn.leaf_batch == tg.leaf_batch;
+                        tn.chunk == tg.chunk && tn.leaf_batch == tg.leaf_batch &&
+                        tn.waves == tg.waves;
             if (!B->graph || B->graph_ng != B->ng || !same) {
                 if (B->opt.form == LB_FORM_TUNED &&
                     tuned_prepare(gpu, LB_PASS_GATHER, B->P, (uint32_t)B->ng) != hipSuccess)

@@ -278,6 +278,8 @@ static int check_tuning(const lb_gpu *gpu, const lb_tuning *t, const char *who) 
     if (t->chunk < 0) return fail("%s: chunk %d is negative", who, t->chunk);
     if (t->leaf_batch < 1 || t->leaf_batch > 64)
         return fail("%s: leaf batch %d outside 1..64", who, t->leaf_batch);
+    if (t->waves != 0 && t->waves != 6 && t->waves != 8)
+        return fail("%s: waves %d is not 0, 6 or 8", who, t->waves);
     if (stack_lds_bytes(t->lds_stack, t->workgroup) > gpu->lds_per_block)
         return fail("%s: an LDS stack of %d entries for %d lanes exceeds %zu bytes", who,
                     t->lds_stack, t->workgroup, gpu->lds_per_block);
@@ -293,6 +295,7 @@ static void put_tuning(Tuning &d, const lb_tuning *t) {
     d.schedule = t->schedule;
     d.chunk = t->chunk;
     d.leaf_batch = t->leaf_batch;
+    d.waves = t->waves;
 }
 
 static void get_tuning(const Tuning &s, lb_tuning *t) {
@@ -304,6 +307,7 @@ static void get_tuning(const Tuning &s, lb_tuning *t) {
     t->schedule = s.schedule;
     t->chunk = s.chunk;
     t->leaf_batch = s.leaf_batch;
+    t->waves = s.waves;
 }
 
 int lb_set_tuning(lb_gpu *gpu, const lb_tuning *t) {

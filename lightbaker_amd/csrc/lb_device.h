@@ -133,6 +133,18 @@ __device__ __forceinline__ float slab(f3 o, f3 inv, f3 mn, f3 mx, float tmax) {
     return (hi >= l0 && lo < tmax) ? l0 : FAR;
 }
 
+// slab() as its two parts: the entry max(lo, 0) and whether the box counts as hit (slab() is
+// the entry when hit, FAR when not)
+__device__ __forceinline__ float slab_hit(f3 o, f3 inv, f3 mn, f3 mx, float tmax, bool *hit) {
+    f3 a = mul(sub(mn, o), inv);
+    f3 b = mul(sub(mx, o), inv);
+    float lo = vmax(vmax(vmin(a.x, b.x), vmin(a.y, b.y)), vmin(a.z, b.z));
+    float hi = vmin(vmin(vmax(a.x, b.x), vmax(a.y, b.y)), vmax(a.z, b.z));
+    float l0 = vmax(lo, 0.0f);
+    *hit = hi >= l0 && lo < tmax;
+    return l0;
+}
+
 // trace.wgsl:85
 __device__ __forceinline__ f3 inverse_dir(f3 d) {
     f3 dd = mk(fabsf(d.x) < 1e-12f ? 1e-12f : d.x, fabsf(d.y) < 1e-12f ? 1e-12f : d.y,

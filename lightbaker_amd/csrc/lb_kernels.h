@@ -16,6 +16,7 @@ struct Tuning {
     int schedule = 0;        // 0: lock-step slots, 1: pooled (one ray pool, shaded in order)
     int chunk = 0;           // pooled: hits a chunk holds (0: the default)
     int leaf_batch = 24;     // pooled: lanes at a leaf before the wave tests leaves
+    int waves = 6;           // lock-step: residency target (0: the compiler's; 6; 8)
 };
 
 // LDS words between two lanes' traversal stacks of `depth` entries: odd, so a wave's lanes

@@ -59,6 +59,9 @@ __device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn
     if (LAYOUT == LB_NODES_WIDE) {
         const float4 *w = S.wide + n * 4u;
         float4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+        // the references' load goes out with the boxes' (one round trip a visit): without this
+        // the compiler defers it past the box tests
+        asm volatile("" ::"v"(w3.x), "v"(w3.y));
         amn = mk(w0.x, w0.y, w0.z);
         amx = mk(w0.w, w1.x, w1.y);
         bmn = mk(w1.z, w1.w, w2.x);
@@ -150,14 +153,17 @@ __device__ __forceinline__ bool inner_visit(const SceneView &S, Ray &R, const St
     uint32_t ra, rb;
     children<LAYOUT>(S, R.n, amn, amx, bmn, bmx, ra, rb);
     if (STATS) cnt.inner++;
-    float ta = slab(R.o, R.inv, amn, amx, R.t);
-    float tb = slab(R.o, R.inv, bmn, bmx, R.t);
-    // tn, tf: min and max of (ta, tb), taken by the same comparison (no NaN reaches here)
-    bool an = ta <= tb;
+    // box() (trace.wgsl:74-80) gives the entry l0 when the box is hit and FAR = 1e30 otherwise,
+    // and an entry is always below tmax <= FAR; so with ha, hb the hit tests, the reference's
+    // ta <= tb is ha && (!hb || l0a <= l0b), min(ta, tb) < FAR is ha || hb and
+    // max(ta, tb) < FAR is ha && hb, taken here on the tests themselves
+    bool ha, hb;
+    float la = slab_hit(R.o, R.inv, amn, amx, R.t, &ha);
+    float lb = slab_hit(R.o, R.inv, bmn, bmx, R.t, &hb);
+    if (!(ha || hb)) return false;
+    bool an = ha && (!hb || la <= lb);
     uint32_t near = an ? ra : rb, far = an ? rb : ra;
-    float tn = an ? ta : tb, tf = an ? tb : ta;
-    if (!(tn < FAR)) return false;
-    if (tf < FAR && R.sp < (uint32_t)STACK) {
+    if (ha && hb && R.sp < (uint32_t)STACK) {
         st.put(R.sp, far);
         R.sp++;
     }

@@ -19,13 +19,13 @@
 
 namespace lbk {
 
-// trace.wgsl:83-130 over the tuned node records, to the end, as while-while
+// trace.wgsl:83-130 over the tuned node records, to the end, as while-while: the lanes of a
+// wave step through inner nodes until every one stands at a leaf or is done, then take their
+// leaves. (Switching to the leaves once some of the lanes stand at one measured slower: the
+// extra vote per step costs more than the idle lanes it saves.)
 template <int LAYOUT, bool STATS>
-__device__ Hit trace_t(const SceneView &S, const lb_params &P, uint32_t root, f3 o, f3 d,
-                       float tmax, bool any, uint32_t ray, const StackView &st, Counts &cnt) {
-    Ray R;
-    ray_start(R, root, o, d, tmax, any, ray);
-    if (STATS) cnt.rays++;
+__device__ __forceinline__ void trace_locked(const SceneView &S, const lb_params &P, Ray &R,
+                                             const StackView &st, Counts &cnt) {
     for (;;) {
         bool done = false;
         while (!(R.n & LEAF_BIT)) {
@@ -36,6 +36,15 @@ __device__ Hit trace_t(const SceneView &S, const lb_params &P, uint32_t root, f3
         }
         if (done || leaf_tests<STATS>(S, P, R, cnt) || pop(R, st)) break;
     }
+}
+
+template <int LAYOUT, bool STATS>
+__device__ Hit trace_t(const SceneView &S, const lb_params &P, uint32_t root, f3 o, f3 d,
+                       float tmax, bool any, uint32_t ray, const StackView &st, Counts &cnt) {
+    Ray R;
+    ray_start(R, root, o, d, tmax, any, ray);
+    if (STATS) cnt.rays++;
+    trace_locked<LAYOUT, STATS>(S, P, R, st, cnt);
     return Hit{R.t, R.tri, R.back};
 }
 
@@ -170,10 +179,9 @@ __device__ __forceinline__ uint32_t az_slot(uint32_t s, uint32_t total) {
     return __builtin_bitreverse32(s) >> (32u - b);
 }
 
-// The slot loop of one item's rays: each lane traces its slot to the end as while-while (inner
-// nodes until every lane of the wave stands at a leaf or is done, then the leaves), then takes
-// slot + L. start(slot, R) sets up slot's ray; shade(slot, R) folds its result into the lane's
-// sums.
+// The slot loop of one item's rays: each lane traces its slot to the end (trace_locked), then
+// takes slot + L. start(slot, R) sets up slot's ray; shade(slot, R) folds its result into the
+// lane's sums.
 template <int LAYOUT, bool STATS, typename Start, typename Shade>
 __device__ __forceinline__ void run_slots(const TunedArgs &A, const StackView &st, uint32_t first,
                                           uint32_t total, uint32_t L, Counts &cnt, Start start,
@@ -182,24 +190,16 @@ __device__ __forceinline__ void run_slots(const TunedArgs &A, const StackView &s
     for (uint32_t slot = first; slot < total; slot += L) {
         start(slot, R);
         if (STATS) cnt.rays++;
-        for (;;) {
-            bool done = false;
-            while (!(R.n & LEAF_BIT)) {
-                if (!inner_visit<LAYOUT, STATS>(A.S, R, st, cnt) && pop(R, st)) {
-                    done = true;
-                    break;
-                }
-            }
-            if (done || leaf_tests<STATS>(A.S, A.P, R, cnt) || pop(R, st)) break;
-        }
+        trace_locked<LAYOUT, STATS>(A.S, A.P, R, st, cnt);
         shade(slot, R);
     }
 }
 
 // trace.wgsl:170-243. The inside tests (16 rays, and up to 9 more points) keep the group in
 // step; the sun and occlusion rays (2 g^2 + aoRays slots) run in the chosen schedule.
-template <int LAYOUT, bool STATS>
-__global__ void k_direct_tuned(TunedArgs A) {
+template <int LAYOUT, bool STATS, int WAVES>
+__global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_direct_tuned(
+    TunedArgs A) {
     StackView st = make_stack(A.spill, A.depth);
     const lb_params &P = A.P;
     int L = (int)A.lanes_per_item;
@@ -309,8 +309,10 @@ __global__ void k_direct_tuned(TunedArgs A) {
 }
 
 // trace.wgsl:245-270: lane lig of an item's group takes its rays r = lig, lig + L, ...
-template <int LAYOUT, bool STATS>
-__global__ void k_gather_tuned(TunedArgs A) {
+// WAVES: waves a SIMD the kernel is compiled to keep resident (0: the compiler's choice).
+template <int LAYOUT, bool STATS, int WAVES>
+__global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_gather_tuned(
+    TunedArgs A) {
     StackView st = make_stack(A.spill, A.depth);
     const lb_params &P = A.P;
     int L = (int)A.lanes_per_item;
@@ -373,9 +375,10 @@ __global__ void k_gather_tuned(TunedArgs A) {
     if (STATS) flush_counts(cnt, A.stats);
 }
 
-// trace.wgsl:272-324 over rays + g^2 sun slots
-template <int LAYOUT, bool STATS>
-__global__ void k_volume_tuned(TunedArgs A) {
+// trace.wgsl:272-324 over rays + g^2 sun slots; WAVES as for the gather
+template <int LAYOUT, bool STATS, int WAVES>
+__global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_volume_tuned(
+    TunedArgs A) {
     StackView st = make_stack(A.spill, A.depth);
     const lb_params &P = A.P;
     int L = (int)A.lanes_per_item;
@@ -476,20 +479,23 @@ __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
 
 using KernelFn = void (*)(TunedArgs);
 
-#define LB_KERNEL_FAMILY(NAME)                                                        \
-    static KernelFn pick_##NAME(int layout, bool stats) {                             \
-        if (layout == LB_NODES_WIDE) return stats ? NAME<0, true> : NAME<0, false>;   \
-        return stats ? NAME<1, true> : NAME<1, false>;                                \
-    }
-LB_KERNEL_FAMILY(k_direct_tuned)
-LB_KERNEL_FAMILY(k_gather_tuned)
-LB_KERNEL_FAMILY(k_volume_tuned)
-#undef LB_KERNEL_FAMILY
-
-static KernelFn pick_kernel(int pass, int layout, bool stats) {
-    if (pass == LB_PASS_DIRECT) return pick_k_direct_tuned(layout, stats);
-    if (pass == LB_PASS_GATHER) return pick_k_gather_tuned(layout, stats);
-    return pick_k_volume_tuned(layout, stats);
+// The kernel of a pass, layout, counters and residency target (waves a SIMD: 0, 6 or 8).
+template <int LAYOUT, bool STATS, int WAVES>
+static KernelFn pick_waves(int pass) {
+    if (pass == LB_PASS_DIRECT) return k_direct_tuned<LAYOUT, STATS, WAVES>;
+    if (pass == LB_PASS_GATHER) return k_gather_tuned<LAYOUT, STATS, WAVES>;
+    return k_volume_tuned<LAYOUT, STATS, WAVES>;
+}
+template <int LAYOUT, bool STATS>
+static KernelFn pick_stats(int pass, int waves) {
+    if (waves == 6) return pick_waves<LAYOUT, STATS, 6>(pass);
+    if (waves == 8) return pick_waves<LAYOUT, STATS, 8>(pass);
+    return pick_waves<LAYOUT, STATS, 0>(pass);
+}
+static KernelFn pick_kernel(int pass, int layout, bool stats, int waves) {
+    if (layout == LB_NODES_WIDE)
+        return stats ? pick_stats<0, true>(pass, waves) : pick_stats<0, false>(pass, waves);
+    return stats ? pick_stats<1, true>(pass, waves) : pick_stats<1, false>(pass, waves);
 }
 
 // Lanes sharing one item: the rays an item casts over the rays a lane should take, as a power
@@ -514,7 +520,7 @@ struct TunedGrid {
 static hipError_t tuned_grid(lb_gpu *gpu, int pass, bool stats, const lb_params &P, uint32_t n,
                              TunedGrid *out) {
     const Tuning &T = gpu->tuning[pass];
-    out->fn = pick_kernel(pass, T.nodes, stats);
+    out->fn = pick_kernel(pass, T.nodes, stats, T.waves);
     out->block = (uint32_t)T.workgroup;
     out->lds = stack_lds_bytes(T.lds_stack, (int)out->block);
     int per_cu = 0;

@@ -88,7 +88,7 @@ def test_tuned_matches(gpu, setup, pass_, nodes):
     items = setup["probes"] if pass_ == Pass.VOLUME else setup["items"]
     print(f"tuned {pass_.name} nodes={nodes}: {len(items)} items", flush=True)
     gpu.set_form(N.FORM_TUNED)
-    gpu.set_tuning(256, 2, 16, nodes, 16, 0)
+    gpu.set_tuning(256, 2, 16, nodes, 32, 0, leaf_batch=24)
     try:
         got = gpu.run(pass_, setup["sb"], setup["params"], items, setup["light"])
     finally:
@@ -97,21 +97,23 @@ def test_tuned_matches(gpu, setup, pass_, nodes):
     _compare_tuned(pass_, got, want, nodes == N.NODES_QUANT)
 
 
-@pytest.mark.parametrize("cfg", [(64, 1, 8, 1, 0), (128, 4, 24, 64, 0), (512, 8, 24, 16, 0),
-                                 (256, 2, 32, 32, 0), (256, 8, 48, 48, 0)])
+@pytest.mark.parametrize("cfg", [(64, 1, 8, 0), (128, 4, 24, 6), (512, 8, 24, 8),
+                                 (256, 2, 32, 0), (256, 8, 48, 6), (256, 4, 16, 8)])
 def test_tuned_configs(gpu, setup, cfg):
-    """Every autotuner setting gives the same results (workgroup, rays/lane, LDS stack)."""
-    wg, rpl, depth, refill, sched = cfg
-    print(f"tuned gather cfg={cfg}", flush=True)
-    gpu.set_form(N.FORM_TUNED)
-    gpu.set_tuning(wg, rpl, depth, N.NODES_WIDE, refill, sched)
-    try:
-        got = gpu.run(Pass.GATHER, setup["sb"], setup["params"], setup["items"], setup["light"])
-    finally:
-        gpu.set_form(N.FORM_PLAIN)
-        gpu.set_tuning(256, 2, 16, N.NODES_WIDE, 16, 0)
-    want = _oracle(setup, Pass.GATHER, setup["items"])
-    _compare_tuned(Pass.GATHER, got, want, False)
+    """Every autotuner setting gives the same results (workgroup, rays/lane, LDS stack, waves
+    a SIMD), on every pass."""
+    wg, rpl, depth, waves = cfg
+    for pass_ in (Pass.DIRECT, Pass.GATHER, Pass.VOLUME):
+        print(f"tuned {pass_.name} cfg={cfg}", flush=True)
+        items = setup["probes"] if pass_ == Pass.VOLUME else setup["items"]
+        gpu.set_form(N.FORM_TUNED)
+        gpu.set_tuning(wg, rpl, depth, N.NODES_WIDE, 32, 0, waves=waves)
+        try:
+            got = gpu.run(pass_, setup["sb"], setup["params"], items, setup["light"])
+        finally:
+            gpu.set_form(N.FORM_PLAIN)
+            gpu.set_tuning(**DEFAULT)
+        _compare_tuned(pass_, got, _oracle(setup, pass_, items), False)
 
 
 def test_key_base_slices(gpu, setup):
@@ -149,7 +151,7 @@ def test_counters_match_oracle(gpu, setup):
 STREAM = dict(workgroup=256, rays_per_lane=2, lds_stack=16, nodes=N.NODES_WIDE, refill=32,
               schedule=1, chunk=0, leaf_batch=24)
 DEFAULT = dict(workgroup=256, rays_per_lane=2, lds_stack=16, nodes=N.NODES_WIDE, refill=32,
-               schedule=0, chunk=0, leaf_batch=24)
+               schedule=0, chunk=0, leaf_batch=24, waves=0)
 
 
 def _run_tuned(gpu, setup, pass_, **tuning):
