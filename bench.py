@@ -177,6 +177,9 @@ def main() -> int:
     t_scene = time.perf_counter() - t_scene
     gpu = Gpu()
     gpu.set_tuning(256, 4, 16, layout, 32, args.schedule, leaf_batch=24, waves=6)
+    # the autotuner's usual picks on this map, so a run without it uses the same kernels
+    gpu.set_tuning(pass_=Pass.GATHER, waves=8)
+    gpu.set_tuning(pass_=Pass.DIRECT, rays_per_lane=8, waves=8)
     print(f"# {gpu.adapter}; scene {scene.triangle_count()} triangles ({t_scene:.1f} s)",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -227,7 +230,7 @@ def main() -> int:
     if form == N.FORM_PLAIN:
         kernel = "k_gather_plain<false>"
     elif tuning["schedule"] == 0:
-        kernel = f"k_gather_tuned<{layout}, false>"
+        kernel = f"k_gather_tuned<{layout}, false, {tuning['waves']}>"
     else:
         kernel = f"k_pool<0, {layout}, false>"
     traffic = measured_traffic(kernel)
