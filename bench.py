@@ -176,7 +176,7 @@ def main() -> int:
     scene = map_scene(2)
     t_scene = time.perf_counter() - t_scene
     gpu = Gpu()
-    gpu.set_tuning(256, 4, 16, layout, 32, args.schedule, leaf_batch=24, waves=6)
+    gpu.set_tuning(256, 4, 16, layout, 32, args.schedule, leaf_batch=16, waves=6)
     # the autotuner's usual picks on this map, so a run without it uses the same kernels
     gpu.set_tuning(pass_=Pass.GATHER, waves=8)
     gpu.set_tuning(pass_=Pass.DIRECT, rays_per_lane=8, waves=8)

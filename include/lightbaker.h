@@ -139,9 +139,9 @@ typedef struct lb_tuning {
                            hits are shaded in the reference's order (bit-exact sums) */
     int chunk;          /* pooled schedule: hits one chunk holds in HBM, 0 for 2^28 (2 GiB); a
                            launch runs ceil(rays / chunk) chunks */
-    int leaf_batch;     /* pooled schedule: lanes (1..64) standing at a leaf before a wave
-                           tests leaves rather than stepping its other lanes through inner
-                           nodes */
+    int leaf_batch;     /* lanes (1..64) standing at a leaf before a wave tests leaves rather
+                           than stepping its other lanes through inner nodes (64: only once
+                           none is at an inner node) */
     int waves;          /* lock-step schedule: waves a SIMD the kernel is compiled to keep
                            resident: 0 (the compiler's choice), 6 or 8 (fewer registers a lane,
                            some spilled) */

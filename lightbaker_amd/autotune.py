@@ -6,7 +6,7 @@ atlas.
 Search: coordinate descent from the current setting, one knob at a time over its range, at most
 `rounds` rounds, each candidate timed as the median of `reps` launches of its pass. A knob that
 the candidate's schedule ignores is skipped (rays per lane and waves for the pooled schedule,
-refill and leaf batch for the lock-step one); a candidate the device cannot launch (an LDS stack larger
+refill for the lock-step one); a candidate the device cannot launch (an LDS stack larger
 than a workgroup's LDS) is skipped too.
 """
 from __future__ import annotations
@@ -20,7 +20,7 @@ RANGES = {
     "schedule": [0, 1],
     "rays_per_lane": [1, 2, 4, 8],
     "refill": [16, 32, 48, 56],
-    "leaf_batch": [8, 16, 24, 32, 64],
+    "leaf_batch": [8, 16, 24, 32, 48, 64],
     "workgroup": [64, 128, 256, 512],
     "lds_stack": [8, 16, 24, 32, 48],
     "nodes": [N.NODES_WIDE, N.NODES_QUANT],
@@ -29,7 +29,7 @@ RANGES = {
 
 
 def _skip(dim: str, cand: dict) -> bool:
-    if dim in ("refill", "leaf_batch") and cand["schedule"] == 0:
+    if dim == "refill" and cand["schedule"] == 0:
         return True
     if dim == "waves" and cand["schedule"] == 1:
         return True

@@ -15,7 +15,7 @@ struct Tuning {
     int refill = 32;         // pooled: idle lanes (of 64) a wave waits for before it refills
     int schedule = 0;        // 0: lock-step slots, 1: pooled (one ray pool, shaded in order)
     int chunk = 0;           // pooled: hits a chunk holds (0: the default)
-    int leaf_batch = 24;     // pooled: lanes at a leaf before the wave tests leaves
+    int leaf_batch = 16;     // lanes at a leaf that end a wave's inner phase (64: all)
     int waves = 6;           // lock-step: residency target (0: the compiler's; 6; 8)
 };
 

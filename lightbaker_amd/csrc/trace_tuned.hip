@@ -19,32 +19,38 @@
 
 namespace lbk {
 
-// trace.wgsl:83-130 over the tuned node records, to the end, as while-while: the lanes of a
-// wave step through inner nodes until every one stands at a leaf or is done, then take their
-// leaves. (Switching to the leaves once some of the lanes stand at one measured slower: the
-// extra vote per step costs more than the idle lanes it saves.)
+// trace.wgsl:83-130 over the tuned node records, to the end, the lanes of a wave in step: they
+// go through inner nodes until leaf_batch of the lanes still tracing stand at a leaf (64: until
+// none is at an inner node), then those take their leaves, and again. The count is the exec
+// mask's population, a scalar operation, so a step costs no extra vector instruction. Each lane
+// still takes its own ray's steps in the reference's order.
 template <int LAYOUT, bool STATS>
 __device__ __forceinline__ void trace_locked(const SceneView &S, const lb_params &P, Ray &R,
-                                             const StackView &st, Counts &cnt) {
+                                             const StackView &st, Counts &cnt, int leaf_batch) {
     for (;;) {
         bool done = false;
+        // lanes that may stay at inner nodes before the wave turns to the leaves
+        int keep = __popcll(__ballot(1)) - leaf_batch;
         while (!(R.n & LEAF_BIT)) {
             if (!inner_visit<LAYOUT, STATS>(S, R, st, cnt) && pop(R, st)) {
                 done = true;
                 break;
             }
+            if (__popcll(__ballot(1)) <= keep) break;
         }
-        if (done || leaf_tests<STATS>(S, P, R, cnt) || pop(R, st)) break;
+        if (done) break;
+        if ((R.n & LEAF_BIT) && (leaf_tests<STATS>(S, P, R, cnt) || pop(R, st))) break;
     }
 }
 
 template <int LAYOUT, bool STATS>
 __device__ Hit trace_t(const SceneView &S, const lb_params &P, uint32_t root, f3 o, f3 d,
-                       float tmax, bool any, uint32_t ray, const StackView &st, Counts &cnt) {
+                       float tmax, bool any, uint32_t ray, const StackView &st, Counts &cnt,
+                       int leaf_batch) {
     Ray R;
     ray_start(R, root, o, d, tmax, any, ray);
     if (STATS) cnt.rays++;
-    trace_locked<LAYOUT, STATS>(S, P, R, st, cnt);
+    trace_locked<LAYOUT, STATS>(S, P, R, st, cnt, leaf_batch);
     return Hit{R.t, R.tri, R.back};
 }
 
@@ -133,6 +139,7 @@ struct TunedArgs {
     uint32_t *queue;
     uint32_t *spill;
     uint32_t lanes_per_item, depth;
+    int leaf_batch;  // lanes at a leaf that end a wave's inner phase early (64: never)
 };
 
 
@@ -163,7 +170,7 @@ __device__ __forceinline__ bool inside_g(const TunedArgs &A, const StackView &st
     for (uint32_t r = lig; r < 16u; r += (uint32_t)L) {
         f3 d = cosine(((float)r + 0.5f) / 16.0f, radical(r), f);
         Hit h = trace_t<LAYOUT, STATS>(A.S, A.P, A.S.root, o, d, A.P.ground[3], false,
-                                       key + r + 7000u, st, cnt);
+                                       key + r + 7000u, st, cnt, A.leaf_batch);
         if (h.tri != NONE && h.back) back += 1.0f;
     }
     back = group_sum(back, L);
@@ -190,7 +197,7 @@ __device__ __forceinline__ void run_slots(const TunedArgs &A, const StackView &s
     for (uint32_t slot = first; slot < total; slot += L) {
         start(slot, R);
         if (STATS) cnt.rays++;
-        trace_locked<LAYOUT, STATS>(A.S, A.P, R, st, cnt);
+        trace_locked<LAYOUT, STATS>(A.S, A.P, R, st, cnt, A.leaf_batch);
         shade(slot, R);
     }
 }
@@ -568,6 +575,7 @@ hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
     A.spill = static_cast<uint32_t *>(gpu->spill.ptr);
     A.lanes_per_item = tg.lanes;
     A.depth = (uint32_t)gpu->tuning[pass].lds_stack;
+    A.leaf_batch = gpu->tuning[pass].leaf_batch;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, gpu->stream, gpu->queue);
     hipLaunchKernelGGL(tg.fn, dim3(tg.grid), dim3(tg.block), tg.lds, gpu->stream, A);
     return hipGetLastError();

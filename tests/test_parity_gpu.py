@@ -88,7 +88,7 @@ def test_tuned_matches(gpu, setup, pass_, nodes):
     items = setup["probes"] if pass_ == Pass.VOLUME else setup["items"]
     print(f"tuned {pass_.name} nodes={nodes}: {len(items)} items", flush=True)
     gpu.set_form(N.FORM_TUNED)
-    gpu.set_tuning(256, 2, 16, nodes, 32, 0, leaf_batch=24)
+    gpu.set_tuning(256, 2, 16, nodes, 32, 0, leaf_batch=64)
     try:
         got = gpu.run(pass_, setup["sb"], setup["params"], items, setup["light"])
     finally:
@@ -97,17 +97,17 @@ def test_tuned_matches(gpu, setup, pass_, nodes):
     _compare_tuned(pass_, got, want, nodes == N.NODES_QUANT)
 
 
-@pytest.mark.parametrize("cfg", [(64, 1, 8, 0), (128, 4, 24, 6), (512, 8, 24, 8),
-                                 (256, 2, 32, 0), (256, 8, 48, 6), (256, 4, 16, 8)])
+@pytest.mark.parametrize("cfg", [(64, 1, 8, 0, 64), (128, 4, 24, 6, 16), (512, 8, 24, 8, 1),
+                                 (256, 2, 32, 0, 32), (256, 8, 48, 6, 8), (256, 4, 16, 8, 24)])
 def test_tuned_configs(gpu, setup, cfg):
     """Every autotuner setting gives the same results (workgroup, rays/lane, LDS stack, waves
-    a SIMD), on every pass."""
-    wg, rpl, depth, waves = cfg
+    a SIMD, leaf batch), on every pass."""
+    wg, rpl, depth, waves, leaf_batch = cfg
     for pass_ in (Pass.DIRECT, Pass.GATHER, Pass.VOLUME):
         print(f"tuned {pass_.name} cfg={cfg}", flush=True)
         items = setup["probes"] if pass_ == Pass.VOLUME else setup["items"]
         gpu.set_form(N.FORM_TUNED)
-        gpu.set_tuning(wg, rpl, depth, N.NODES_WIDE, 32, 0, waves=waves)
+        gpu.set_tuning(wg, rpl, depth, N.NODES_WIDE, 32, 0, waves=waves, leaf_batch=leaf_batch)
         try:
             got = gpu.run(pass_, setup["sb"], setup["params"], items, setup["light"])
         finally:
@@ -151,7 +151,7 @@ def test_counters_match_oracle(gpu, setup):
 STREAM = dict(workgroup=256, rays_per_lane=2, lds_stack=16, nodes=N.NODES_WIDE, refill=32,
               schedule=1, chunk=0, leaf_batch=24)
 DEFAULT = dict(workgroup=256, rays_per_lane=2, lds_stack=16, nodes=N.NODES_WIDE, refill=32,
-               schedule=0, chunk=0, leaf_batch=24, waves=0)
+               schedule=0, chunk=0, leaf_batch=64, waves=0)
 
 
 def _run_tuned(gpu, setup, pass_, **tuning):
