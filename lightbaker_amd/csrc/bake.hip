@@ -418,19 +418,16 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
 namespace {
 
 // filter.rs:58-94 over `rings` rings into `v` (C channels, stride 1 or 4) from `valid0`
-void fill(lb_bake *B, int channels, float *v, float *tmp, uint32_t rings, const uint8_t *valid0,
+void fill(lb_bake *B, int channels, float *v, uint32_t rings, const uint8_t *valid0,
           uint8_t *final_mask) {
     hipStream_t st = B->gpu->stream;
     uint32_t T = (uint32_t)B->T;
     uint8_t *ma = (uint8_t *)B->d_m1.ptr, *mb = (uint8_t *)B->d_m2.ptr;
     tp_copy_u8(valid0, ma, T, st);
-    float *a = v, *b = tmp;
     for (uint32_t r = 0; r < rings; r++) {
-        tp_fill_ring(B->V, channels, a, b, ma, mb, st);
-        std::swap(a, b);
+        tp_fill_ring(B->V, channels, v, ma, mb, st);
         std::swap(ma, mb);
     }
-    if (a != v) tp_copy_f32(a, v, T * (channels == 1 ? 1u : 4u), st);
     if (final_mask) tp_copy_u8(ma, final_mask, T, st);
 }
 
@@ -464,7 +461,7 @@ int bounce(lb_bake *B, lb_stats *acc) {
     tp_smooth(V, 4, (const float *)V.indirect, (float *)B->d_f4.ptr,
               (const uint8_t *)B->d_gmask.ptr, (int)B->rg, (const float *)B->d_wg.ptr, st);
     tp_copy_f32((const float *)B->d_f4.ptr, (float *)V.indirect, T * 4u, st);
-    fill(B, 3, (float *)V.indirect, (float *)B->d_f4.ptr, B->grings,
+    fill(B, 3, (float *)V.indirect, B->grings,
          (const uint8_t *)B->d_gmask.ptr, nullptr);
     tp_write_light(V, (float4 *)B->d_light.ptr, B->aw, st);
     hipError_t e = hipGetLastError();
@@ -516,9 +513,9 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
     tp_direct_light(V, (const float *)B->d_f1.ptr,
                     make_float3(sc.sun_irradiance[0], sc.sun_irradiance[1], sc.sun_irradiance[2]),
                     make_float3(B->sun.x, B->sun.y, B->sun.z), st);
-    fill(B, 1, V.ao, (float *)B->d_f1.ptr, B->rings, V.valid, nullptr);
-    fill(B, 1, V.field, (float *)B->d_f1.ptr, B->rings, V.valid, nullptr);
-    fill(B, 3, (float *)V.direct, (float *)B->d_f4.ptr, B->rings, V.valid,
+    fill(B, 1, V.ao, B->rings, V.valid, nullptr);
+    fill(B, 1, V.field, B->rings, V.valid, nullptr);
+    fill(B, 3, (float *)V.direct, B->rings, V.valid,
          (uint8_t *)B->d_filled.ptr);
     tp_known(V, (const uint8_t *)B->d_filled.ptr, st);
     tp_write_light(V, (float4 *)B->d_light.ptr, B->aw, st);  // bake.rs:277-281

@@ -104,8 +104,20 @@ __global__ void k_smooth(TexelView V, const float *__restrict__ in, float *__res
 
 // filter.rs:106-141: distance from each texel's centre to the nearest sub-sample of the other
 // state on the same surface, over the window the field can reach
+// The nearest opposite sample is found on squared distances and rooted once: sqrt and the
+// subtraction of `half` are monotone in FP32 as in exact arithmetic, so the minimum of
+// sqrt(d2) - half over the samples is sqrt(min d2) - half, bit for bit. The sample offsets
+// ((c % g) + 0.5) / g and ((c / g) + 0.5) / g are computed once a workgroup, in LDS.
 __global__ void k_sun_field(TexelView V, const uint8_t *__restrict__ covered, uint32_t g_side,
                             float radius, uint32_t n) {
+    __shared__ float offs[2][32];
+    uint32_t gg = g_side * g_side;
+    if (threadIdx.x < gg) {
+        float gf = (float)g_side;
+        offs[0][threadIdx.x] = ((float)(threadIdx.x % g_side) + 0.5f) / gf;
+        offs[1][threadIdx.x] = ((float)(threadIdx.x / g_side) + 0.5f) / gf;
+    }
+    __syncthreads();
     uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= n) return;
     if (!covered[gi]) {
@@ -114,26 +126,29 @@ __global__ void k_sun_field(TexelView V, const uint8_t *__restrict__ covered, ui
     }
     Where p = where(V, gi);
     int reach = (int)ceilf(radius) + 1;
-    uint32_t gg = g_side * g_side;
     bool lit = (uint32_t)__popc(V.hard[gi]) * 2u >= gg;
-    float best = radius;
     float gf = (float)g_side;
     float half = 0.5f / gf;
     int y0 = max((int)p.y - reach, 0), y1 = min((int)p.y + reach + 1, (int)p.h);
     int x0 = max((int)p.x - reach, 0), x1 = min((int)p.x + reach + 1, (int)p.w);
     float cxf = (float)p.x + 0.5f, cyf = (float)p.y + 0.5f;
+    uint32_t mask = gg >= 32u ? 0xffffffffu : ((1u << gg) - 1u);
+    float d2min = FAR;
     for (int yy = y0; yy < y1; yy++)
         for (int xx = x0; xx < x1; xx++) {
             uint32_t j = p.off + (uint32_t)yy * p.w + (uint32_t)xx;
             if (!covered[j] || !same_surface(V, gi, j)) continue;
-            uint32_t bits = V.hard[j];
-            for (uint32_t c = 0; c < gg; c++) {
-                if (((bits >> c) & 1u) == (lit ? 1u : 0u)) continue;
-                float px = (float)xx + ((float)(c % g_side) + 0.5f) / gf - cxf;
-                float py = (float)yy + ((float)(c / g_side) + 0.5f) / gf - cyf;
-                best = vmin(best, sqrtf(px * px + py * py) - half);
+            // the samples of the opposite state
+            uint32_t opp = (lit ? ~V.hard[j] : V.hard[j]) & mask;
+            while (opp) {
+                uint32_t c = (uint32_t)__builtin_ctz(opp);
+                opp &= opp - 1u;
+                float px = (float)xx + offs[0][c] - cxf;
+                float py = (float)yy + offs[1][c] - cyf;
+                d2min = vmin(d2min, px * px + py * py);
             }
         }
+    float best = d2min < FAR ? vmin(radius, sqrtf(d2min) - half) : radius;
     float d = vmin(vmax(best, 0.0f), radius);
     V.field[gi] = 0.5f + 0.5f * (lit ? d : -d) / radius;
 }
@@ -151,14 +166,13 @@ __global__ void k_direct_light(TexelView V, const float *__restrict__ soft, floa
 
 // filter.rs:58-94, one ring: texels without a value take the mean of their 3 x 3 neighbours that
 // had one (their own chart's first; an empty texel any chart's). C channels in `v` (stride S,
-// first C used), read from `src`, written to `dst`; `was` in, `now` out.
+// first C used), updated in place: a ring writes only texels without a value, and reads only
+// texels that had one before it (`was`), whose values no ring changes; `now` out.
 template <int C, int S>
-__global__ void k_fill_ring(TexelView V, const float *__restrict__ src, float *__restrict__ dst,
-                            const uint8_t *__restrict__ was, uint8_t *__restrict__ now, uint32_t n) {
+__global__ void k_fill_ring(TexelView V, float *v, const uint8_t *__restrict__ was,
+                            uint8_t *__restrict__ now, uint32_t n) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
-    float out[C];
-    for (int c = 0; c < C; c++) out[c] = src[g * S + c];
     uint8_t known = was[g];
     if (!known) {
         Where p = where(V, g);
@@ -173,22 +187,21 @@ __global__ void k_fill_ring(TexelView V, const float *__restrict__ src, float *_
                 uint32_t j = p.off + (uint32_t)yy * p.w + (uint32_t)xx;
                 if (!was[j]) continue;
                 if (ci != NONE && V.chart[j] == ci) {
-                    for (int c = 0; c < C; c++) own[c] += src[j * S + c];
+                    for (int c = 0; c < C; c++) own[c] += v[j * S + c];
                     no++;
                 } else {
-                    for (int c = 0; c < C; c++) any[c] += src[j * S + c];
+                    for (int c = 0; c < C; c++) any[c] += v[j * S + c];
                     na++;
                 }
             }
         if (no > 0) {
-            for (int c = 0; c < C; c++) out[c] = own[c] / (float)no;
+            for (int c = 0; c < C; c++) v[g * S + c] = own[c] / (float)no;
             known = 1;
         } else if (na > 0 && ci == NONE) {
-            for (int c = 0; c < C; c++) out[c] = any[c] / (float)na;
+            for (int c = 0; c < C; c++) v[g * S + c] = any[c] / (float)na;
             known = 1;
         }
     }
-    for (int c = 0; c < C; c++) dst[g * S + c] = out[c];
     now[g] = known;
 }
 
@@ -367,13 +380,13 @@ void tp_sun_field(const TexelView &V, const uint8_t *covered, uint32_t g, float 
 void tp_direct_light(const TexelView &V, const float *soft, float3 irr, float3 sun, hipStream_t s) {
     if (V.n) hipLaunchKernelGGL(k_direct_light, grid_for(V.n), dim3(256), 0, s, V, soft, irr, sun, V.n);
 }
-void tp_fill_ring(const TexelView &V, int channels, const float *src, float *dst,
-                  const uint8_t *was, uint8_t *now, hipStream_t s) {
+void tp_fill_ring(const TexelView &V, int channels, float *v, const uint8_t *was, uint8_t *now,
+                  hipStream_t s) {
     if (!V.n) return;
     if (channels == 1)
-        hipLaunchKernelGGL((k_fill_ring<1, 1>), grid_for(V.n), dim3(256), 0, s, V, src, dst, was, now, V.n);
+        hipLaunchKernelGGL((k_fill_ring<1, 1>), grid_for(V.n), dim3(256), 0, s, V, v, was, now, V.n);
     else
-        hipLaunchKernelGGL((k_fill_ring<3, 4>), grid_for(V.n), dim3(256), 0, s, V, src, dst, was, now, V.n);
+        hipLaunchKernelGGL((k_fill_ring<3, 4>), grid_for(V.n), dim3(256), 0, s, V, v, was, now, V.n);
 }
 void tp_known(const TexelView &V, const uint8_t *filled, hipStream_t s) {
     if (V.n) hipLaunchKernelGGL(k_known, grid_for(V.n), dim3(256), 0, s, V, filled, V.n);
