@@ -178,7 +178,7 @@ def main() -> int:
     gpu = Gpu()
     gpu.set_tuning(256, 4, 16, layout, 32, args.schedule, leaf_batch=16, waves=6)
     # the autotuner's usual picks on this map, so a run without it uses the same kernels
-    gpu.set_tuning(pass_=Pass.GATHER, waves=8)
+    gpu.set_tuning(pass_=Pass.GATHER, rays_per_lane=8, waves=8)
     gpu.set_tuning(pass_=Pass.DIRECT, rays_per_lane=8, waves=8)
     print(f"# {gpu.adapter}; scene {scene.triangle_count()} triangles ({t_scene:.1f} s)",
           file=sys.stderr, flush=True)

@@ -129,7 +129,7 @@ int lb_run_device(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const 
 /* Tuning of the TUNED form. */
 typedef struct lb_tuning {
     int workgroup;      /* threads a workgroup: 64..1024, a multiple of 64 */
-    int rays_per_lane;  /* rays a lane takes of one item (1..8): sets the lanes sharing an item */
+    int rays_per_lane;  /* rays a lane takes of one item (1..16): sets the lanes sharing an item */
     int lds_stack;      /* traversal stack entries in LDS (8..48); deeper ones spill to memory */
     int nodes;          /* LB_NODES_WIDE or LB_NODES_QUANT */
     int refill;         /* pooled schedule: idle lanes (1..64) a wave waits for before it

@@ -18,7 +18,7 @@ from .gpu import Pass
 
 RANGES = {
     "schedule": [0, 1],
-    "rays_per_lane": [1, 2, 4, 8],
+    "rays_per_lane": [1, 2, 4, 8, 16],
     "refill": [16, 32, 48, 56],
     "leaf_batch": [8, 16, 24, 32, 48, 64],
     "workgroup": [64, 128, 256, 512],

@@ -267,8 +267,8 @@ static int check_tuning(const lb_gpu *gpu, const lb_tuning *t, const char *who) 
     if (!gpu || !t) return fail("%s: null handle", who);
     if (t->workgroup < 64 || t->workgroup > 1024 || t->workgroup % 64)
         return fail("%s: workgroup %d is not a multiple of 64 in 64..1024", who, t->workgroup);
-    if (t->rays_per_lane < 1 || t->rays_per_lane > 8)
-        return fail("%s: rays per lane %d outside 1..8", who, t->rays_per_lane);
+    if (t->rays_per_lane < 1 || t->rays_per_lane > 16)
+        return fail("%s: rays per lane %d outside 1..16", who, t->rays_per_lane);
     if (t->lds_stack < 8 || t->lds_stack > 48)
         return fail("%s: LDS stack %d outside 8..48", who, t->lds_stack);
     if (t->nodes != LB_NODES_WIDE && t->nodes != LB_NODES_QUANT)

@@ -177,13 +177,16 @@ __device__ __forceinline__ bool inside_g(const TunedArgs &A, const StackView &st
     return back / 16.0f > bitsf(A.P.atlas[3]);
 }
 
-// Slot s of an item's `total` rays in azimuth order: for a power of two, the ray whose radical
-// inverse is s / total (r = s bit-reversed), so consecutive slots sweep the azimuth parameter v
-// and a round of 64 lanes covers one wedge of directions. Any other count keeps r = s.
-__device__ __forceinline__ uint32_t az_slot(uint32_t s, uint32_t total) {
+// Slot s of an item's `total` rays in azimuth order: for a power of two, slot s is the ray whose
+// azimuth parameter v = fract(radical(r) + rot) is the s-th from 0 (r = (s + m0) bit-reversed,
+// m0 = -rot * total rounded up), so consecutive slots sweep v from 0 whatever the item's
+// rotation `rot`, and the items a wave traces together point into the same wedge of
+// directions. Any other count keeps r = s.
+__device__ __forceinline__ uint32_t az_slot(uint32_t s, uint32_t total, float rot) {
     if (total < 2u || (total & (total - 1u)) != 0u) return s;
     uint32_t b = 31u - (uint32_t)__builtin_clz(total);
-    return __builtin_bitreverse32(s) >> (32u - b);
+    uint32_t m0 = (total - (uint32_t)(rot * (float)total)) & (total - 1u);
+    return __builtin_bitreverse32((s + m0) & (total - 1u)) >> (32u - b);
 }
 
 // The slot loop of one item's rays: each lane traces its slot to the end (trace_locked), then
@@ -282,7 +285,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_di
                 }
             } else {
                 uint32_t r = slot - n_sun;
-                r = az_slot(r, rays);
+                r = az_slot(r, rays, s2);
                 f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
                 ray_start(R, A.S.root, o_ao, d, P.reach[0], false, key + r + 2000u);
             }
@@ -343,7 +346,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
         KSum sr{0, 0}, sg{0, 0}, sb{0, 0};
         float back = 0.0f;
         auto start = [&](uint32_t slot, Ray &R) {
-            uint32_t r = az_slot(slot, rays);
+            uint32_t r = az_slot(slot, rays, s2);
             float jx = rnd(key + r, 19u, seed) - 0.5f;
             float jy = rnd(key + r, 23u, seed) - 0.5f;
             f3 o = add(add(add(tx.pos, scl(tx.dx, jx)), scl(tx.dy, jy)), scl(n, P.reach[2]));
@@ -414,7 +417,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_vo
         float back = 0.0f, sun = 0.0f;
         auto start = [&](uint32_t slot, Ray &R) {
             if (slot < rays) {
-                uint32_t r = az_slot(slot, rays);
+                uint32_t r = az_slot(slot, rays, s2);
                 float u = fract(((float)r + 0.5f) / (float)rays + s1);
                 float v = fract(radical(r) + s2);
                 float z = 1.0f - 2.0f * u;
