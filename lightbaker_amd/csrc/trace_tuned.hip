@@ -19,6 +19,24 @@
 
 namespace lbk {
 
+// Words of LDS a group of lanes keeps its item's state in, after the workgroup's stacks.
+constexpr uint32_t SLOT_WORDS = 24;
+
+// This lane's group's slot: the dynamic LDS holds the stacks, then one slot a group.
+__device__ __forceinline__ lds_u32 *group_slot(uint32_t depth, uint32_t lanes) {
+    extern __shared__ uint32_t lds_stack[];
+    lds_u32 *slots = (lds_u32 *)lds_stack + lane_stride(depth) * blockDim.x;
+    return slots + (threadIdx.x / lanes) * SLOT_WORDS;
+}
+__device__ __forceinline__ void put_f3(lds_u32 *s, f3 v) {
+    s[0] = fbits(v.x);
+    s[1] = fbits(v.y);
+    s[2] = fbits(v.z);
+}
+__device__ __forceinline__ f3 get_f3(const lds_u32 *s) {
+    return mk(bitsf(s[0]), bitsf(s[1]), bitsf(s[2]));
+}
+
 // trace.wgsl:83-130 over the tuned node records, to the end, the lanes of a wave in step: they
 // go through inner nodes until leaf_batch of the lanes still tracing stand at a leaf (64: until
 // none is at an inner node), then those take their leaves, and again. The count is the exec
@@ -332,24 +350,49 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
     uint32_t n_units = (A.n_items + per_wave - 1u) / per_wave;
     uint32_t seed = fbits(P.reach[3]);
     uint32_t rays = P.counts[2];
+    lds_u32 *slot_lds = group_slot(A.depth, (uint32_t)L);
     Counts cnt{0, 0, 0, 0, 0, 0};
     for (;;) {
         uint32_t unit = next_unit(A.queue, n_units);
         if (unit == 0xffffffffu) break;
         uint32_t i = unit * per_wave + grp;
         bool valid = i < A.n_items;
-        ItemIn tx = load_item(A.items, valid ? i : 0u);
-        f3 n = tx.n;
-        frame_t f = frame(n);
-        uint32_t key = (A.key_base + i) * 7919u;
-        float s1 = rnd(key, 11u, seed), s2 = rnd(key, 13u, seed);
+        {
+            // the item's state goes to the group's LDS slot and is read back at each ray start,
+            // so it holds no vector registers across the traversal
+            ItemIn tx = load_item(A.items, valid ? i : 0u);
+            frame_t f = frame(tx.n);
+            uint32_t key = (A.key_base + i) * 7919u;
+            float s1 = rnd(key, 11u, seed), s2 = rnd(key, 13u, seed);
+            if (lig == 0u) {
+                put_f3(slot_lds, tx.pos);
+                put_f3(slot_lds + 3, tx.dx);
+                put_f3(slot_lds + 6, tx.dy);
+                put_f3(slot_lds + 9, tx.n);
+                put_f3(slot_lds + 12, f.t);
+                put_f3(slot_lds + 15, f.b);
+                slot_lds[18] = fbits(s1);
+                slot_lds[19] = fbits(s2);
+                slot_lds[20] = key;
+            }
+        }
         KSum sr{0, 0}, sg{0, 0}, sb{0, 0};
         float back = 0.0f;
         auto start = [&](uint32_t slot, Ray &R) {
+            // read the slot afresh (the compiler would otherwise keep the stored values live)
+            asm volatile("" ::: "memory");
+            f3 pos = get_f3(slot_lds), dx = get_f3(slot_lds + 3), dy = get_f3(slot_lds + 6),
+               n = get_f3(slot_lds + 9);
+            frame_t f;
+            f.t = get_f3(slot_lds + 12);
+            f.b = get_f3(slot_lds + 15);
+            f.n = n;
+            float s1 = bitsf(slot_lds[18]), s2 = bitsf(slot_lds[19]);
+            uint32_t key = slot_lds[20];
             uint32_t r = az_slot(slot, rays, s2);
             float jx = rnd(key + r, 19u, seed) - 0.5f;
             float jy = rnd(key + r, 23u, seed) - 0.5f;
-            f3 o = add(add(add(tx.pos, scl(tx.dx, jx)), scl(tx.dy, jy)), scl(n, P.reach[2]));
+            f3 o = add(add(add(pos, scl(dx, jx)), scl(dy, jy)), scl(n, P.reach[2]));
             f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
             ray_start(R, A.S.root, o, d, P.reach[1], false, key + r + 4000u);
         };
@@ -539,6 +582,7 @@ static hipError_t tuned_grid(lb_gpu *gpu, int pass, bool stats, const lb_params 
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     out->lanes = lanes_per_item(pass, P, T.rays_per_lane);
+    out->lds += (size_t)(out->block / out->lanes) * SLOT_WORDS * 4u;
     uint32_t per_block = (out->block / 64u) * (64u / out->lanes);
     uint32_t need = (n + per_block - 1u) / per_block;
     uint32_t grid = (uint32_t)gpu->cus * (uint32_t)per_cu;
