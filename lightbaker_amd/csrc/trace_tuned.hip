@@ -20,7 +20,7 @@
 namespace lbk {
 
 // Words of LDS a group of lanes keeps its item's state in, after the workgroup's stacks.
-constexpr uint32_t SLOT_WORDS = 24;
+constexpr uint32_t SLOT_WORDS = 36;
 
 // This lane's group's slot: the dynamic LDS holds the stacks, then one slot a group.
 __device__ __forceinline__ lds_u32 *group_slot(uint32_t depth, uint32_t lanes) {
@@ -179,16 +179,35 @@ __device__ __forceinline__ ItemIn load_item(const lb_texel *items, uint32_t i) {
     return ItemIn{xyz(a), xyz(b), xyz(c), xyz(d), fbits(b.w)};
 }
 
-// trace.wgsl:159-168 with the 16 rays over the group
+// trace.wgsl:159-168 with the 16 rays over the group, from point q of the item whose state is in
+// the group's LDS slot (the texel centre, an offset, or the centre pushed along the normal)
 template <int LAYOUT, bool STATS>
-__device__ __forceinline__ bool inside_g(const TunedArgs &A, const StackView &st, f3 o,
-                                         const frame_t &f, uint32_t key, uint32_t lig, int L,
+__device__ __forceinline__ bool inside_g(const TunedArgs &A, const StackView &st,
+                                         const lds_u32 *slot, uint32_t q, uint32_t lig, int L,
                                          Counts &cnt) {
     float back = 0.0f;
     for (uint32_t r = lig; r < 16u; r += (uint32_t)L) {
+        // read the slot afresh (the compiler would otherwise keep the stored values live)
+        asm volatile("" ::: "memory");
+        f3 pos = get_f3(slot), lift = get_f3(slot + 9);
+        frame_t f;
+        f.t = get_f3(slot + 12);
+        f.b = get_f3(slot + 15);
+        f.n = get_f3(slot + 18);
+        uint32_t key = slot[31], kq = key;
+        f3 p = pos;
+        if (q > 0u && q < 9u) {
+            float cx, cy;
+            offset(q, &cx, &cy);
+            p = add(add(pos, scl(get_f3(slot + 3), cx)), scl(get_f3(slot + 6), cy));
+            kq = key + q * 31u;
+        } else if (q == 9u) {
+            p = add(pos, scl(f.n, PUSH));
+            kq = key + 331u;
+        }
         f3 d = cosine(((float)r + 0.5f) / 16.0f, radical(r), f);
-        Hit h = trace_t<LAYOUT, STATS>(A.S, A.P, A.S.root, o, d, A.P.ground[3], false,
-                                       key + r + 7000u, st, cnt, A.leaf_batch);
+        Hit h = trace_t<LAYOUT, STATS>(A.S, A.P, A.S.root, add(p, lift), d, A.P.ground[3], false,
+                                       kq + r + 7000u, st, cnt, A.leaf_batch);
         if (h.tri != NONE && h.back) back += 1.0f;
     }
     back = group_sum(back, L);
@@ -238,61 +257,85 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_di
     uint32_t seed = fbits(P.reach[3]);
     uint32_t g = P.counts[0], gg = g * g;
     f3 sun = mk(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]);
-    frame_t sf = frame(sun);
+    lds_u32 *slot_lds = group_slot(A.depth, (uint32_t)L);
     Counts cnt{0, 0, 0, 0, 0, 0};
     for (;;) {
         uint32_t unit = next_unit(A.queue, n_units);
         if (unit == 0xffffffffu) break;
         uint32_t i = unit * per_wave + grp;
         bool valid = i < A.n_items;
-        ItemIn tx = load_item(A.items, valid ? i : 0u);
-        f3 n = tx.n;
-        uint32_t key = (A.key_base + i) * 7919u;
-        f3 lift = scl(n, P.reach[2]);
-        frame_t f = frame(n);
-        f3 at = tx.pos;
-        uint32_t pick = 0u;
-        float span = 1.0f;
-        if (valid && inside_g<LAYOUT, STATS>(A, st, add(at, lift), f, key, lig, L, cnt)) {
-            for (uint32_t q = 1u; q < 10u; q++) {
-                f3 p;
-                uint32_t kq;
-                if (q < 9u) {
-                    float cx, cy;
-                    offset(q, &cx, &cy);
-                    p = add(add(tx.pos, scl(tx.dx, cx)), scl(tx.dy, cy));
-                    kq = key + q * 31u;
-                } else {
-                    p = add(tx.pos, scl(n, PUSH));
-                    kq = key + 331u;
-                }
-                if (!inside_g<LAYOUT, STATS>(A, st, add(p, lift), f, kq, lig, L, cnt)) {
-                    at = p;
-                    pick = q;
-                    span = 0.4f;
-                    break;
-                }
+        // the item's state goes to the group's LDS slot and is read back where it is used, so
+        // it holds no vector registers across the traversals
+        bool lit;
+        {
+            ItemIn tx = load_item(A.items, valid ? i : 0u);
+            lit = dot3(sun, tx.n) > 0.0f;
+            if (lig == 0u) {
+                uint32_t key = (A.key_base + i) * 7919u;
+                frame_t f = frame(tx.n), sf = frame(sun);
+                put_f3(slot_lds, tx.pos);
+                put_f3(slot_lds + 3, tx.dx);
+                put_f3(slot_lds + 6, tx.dy);
+                put_f3(slot_lds + 9, scl(tx.n, P.reach[2]));
+                put_f3(slot_lds + 12, f.t);
+                put_f3(slot_lds + 15, f.b);
+                put_f3(slot_lds + 18, f.n);
+                put_f3(slot_lds + 21, sf.t);
+                put_f3(slot_lds + 24, sf.b);
+                slot_lds[28] = fbits(rnd(key, 17u, seed));
+                slot_lds[29] = fbits(rnd(key, 3u, seed));
+                slot_lds[30] = fbits(rnd(key, 5u, seed));
+                slot_lds[31] = key;
             }
         }
-        bool lit = dot3(sun, n) > 0.0f;
+        // trace.wgsl:174-193: the point the rays leave from, 0 (the centre) unless the centre
+        // lies inside a solid
+        uint32_t pick = 0u;
+        if (valid && inside_g<LAYOUT, STATS>(A, st, slot_lds, 0u, lig, L, cnt)) {
+            for (uint32_t q = 1u; q < 10u; q++)
+                if (!inside_g<LAYOUT, STATS>(A, st, slot_lds, q, lig, L, cnt)) {
+                    pick = q;
+                    break;
+                }
+        }
+        if (lig == 0u) {
+            asm volatile("" ::: "memory");
+            f3 pos = get_f3(slot_lds), at = pos;
+            if (pick > 0u && pick < 9u) {
+                float cx, cy;
+                offset(pick, &cx, &cy);
+                at = add(add(pos, scl(get_f3(slot_lds + 3), cx)), scl(get_f3(slot_lds + 6), cy));
+            } else if (pick == 9u) {
+                at = add(pos, scl(get_f3(slot_lds + 18), PUSH));
+            }
+            put_f3(slot_lds + 32, at);
+            slot_lds[27] = fbits(pick == 0u ? 1.0f : 0.4f);
+        }
         uint32_t n_sun = lit ? 2u * gg : 0u;
         uint32_t rays = P.counts[1];
         uint32_t total = valid ? n_sun + rays : 0u;
-        float s1 = rnd(key, 3u, seed), s2 = rnd(key, 5u, seed);
-        float kr = rnd(key, 17u, seed);
-        f3 o_ao = add(at, lift);
         uint32_t hard = 0u;
         float soft = 0.0f, back = 0.0f;
         KSum occ{0.0f, 0.0f};
         auto start = [&](uint32_t slot, Ray &R) {
+            // read the slot afresh (the compiler would otherwise keep the stored values live)
+            asm volatile("" ::: "memory");
+            f3 at = get_f3(slot_lds + 32), lift = get_f3(slot_lds + 9);
+            uint32_t key = slot_lds[31];
             if (slot < n_sun) {
+                f3 dx = get_f3(slot_lds + 3), dy = get_f3(slot_lds + 6);
+                float span = bitsf(slot_lds[27]);
                 uint32_t s = slot < gg ? slot : slot - gg;
                 float cx = (((float)(s % g) + 0.5f) / (float)g - 0.5f) * span;
                 float cy = (((float)(s / g) + 0.5f) / (float)g - 0.5f) * span;
-                f3 o = add(add(add(at, scl(tx.dx, cx)), scl(tx.dy, cy)), lift);
+                f3 o = add(add(add(at, scl(dx, cx)), scl(dy, cy)), lift);
                 if (slot < gg) {
                     ray_start(R, A.S.root, o, sun, FAR, true, key + s);
                 } else {
+                    frame_t sf;
+                    sf.t = get_f3(slot_lds + 21);
+                    sf.b = get_f3(slot_lds + 24);
+                    float kr = bitsf(slot_lds[28]);
                     uint32_t k = (s + (uint32_t)(kr * (float)gg)) % gg;
                     float a = (float)k * 2.399963f;
                     float r = P.sun_dir[3] * sqrtf(((float)k + 0.5f) / (float)gg);
@@ -302,10 +345,15 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_di
                     ray_start(R, A.S.root, o, d, FAR, true, key + s + 1000u);
                 }
             } else {
+                frame_t f;
+                f.t = get_f3(slot_lds + 12);
+                f.b = get_f3(slot_lds + 15);
+                f.n = get_f3(slot_lds + 18);
+                float s1 = bitsf(slot_lds[29]), s2 = bitsf(slot_lds[30]);
                 uint32_t r = slot - n_sun;
                 r = az_slot(r, rays, s2);
                 f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
-                ray_start(R, A.S.root, o_ao, d, P.reach[0], false, key + r + 2000u);
+                ray_start(R, A.S.root, add(at, lift), d, P.reach[0], false, key + r + 2000u);
             }
         };
         auto shade = [&](uint32_t slot, const Ray &R) {
