@@ -11,7 +11,13 @@ Also reported: the roofline of the dominant kernel (the gather pass), the PLAIN 
 step on the same map, and the oracle (the C restatement of the passes) timed on the host's cores
 on a bounded sample of the gather pass.
 
+`--workload rebake` measures a rebake instead: lightbaker_amd.scenes.rebake_scene(3), a 200 x
+200 m map in 64 tiles of 25 m whose texel counts differ about fourfold, baked once whole (its
+tiles' light is the cache), then a seeded subset of its tiles and their probe chunks rebaked; a
+step is the device part of that rebake, every dirty tile's items in one list per pass.
+
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--form tuned|plain] [--no-cpu]
+                    [--workload map|rebake]
 """
 from __future__ import annotations
 
@@ -93,7 +99,7 @@ def cpu_baseline(sess, gpu, target_s: float = 12.0) -> dict:
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     base = {"value": n * rays_per_item / dt / 1e9, "unit": "Grays/s", "cores": threads,
             "kind": "port",
-            "sample": f"{n} gather items x {rays_per_item} rays of the map's last bounce "
+            "sample": f"{n} gather items x {rays_per_item} rays of the last bounce "
                       f"(random subset, final light atlas), {dt:.1f} s"}
 
     # error against FP64 on identical rays; the key of an item is its index in the pass
@@ -141,6 +147,7 @@ def main() -> int:
     ap.add_argument("--nodes", choices=["wide", "quant"], default="wide")
     ap.add_argument("--schedule", type=int, choices=[0, 1], default=0,
                     help="TUNED schedule before autotuning: 0 lock-step, 1 pooled")
+    ap.add_argument("--workload", choices=["map", "rebake"], default="map")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -166,14 +173,14 @@ def main() -> int:
         return 0
 
     from lightbaker_amd import _native as N
-    from lightbaker_amd.bake import BakeSession
+    from lightbaker_amd.bake import BakeSession, bake
     from lightbaker_amd.gpu import Gpu, Pass
-    from lightbaker_amd.scenes import map_scene
+    from lightbaker_amd.scenes import map_scene, rebake_scene
 
     form = N.FORM_TUNED if args.form == "tuned" else N.FORM_PLAIN
     layout = N.NODES_WIDE if args.nodes == "wide" else N.NODES_QUANT
     t_scene = time.perf_counter()
-    scene = map_scene(2)
+    scene = map_scene(2) if args.workload == "map" else rebake_scene(3)
     t_scene = time.perf_counter() - t_scene
     gpu = Gpu()
     gpu.set_tuning(256, 4, 16, layout, 32, args.schedule, leaf_batch=16, waves=6)
@@ -182,8 +189,28 @@ def main() -> int:
     gpu.set_tuning(pass_=Pass.DIRECT, rays_per_lane=8, waves=8)
     print(f"# {gpu.adapter}; scene {scene.triangle_count()} triangles ({t_scene:.1f} s)",
           file=sys.stderr, flush=True)
+    rebake = {}
+    if args.workload == "rebake":
+        # the cache: one whole bake; then a seeded share of the tiles and their probe chunks
+        first = bake(scene, gpu, form=N.FORM_TUNED, graph=True)
+        rng = np.random.default_rng(33)
+        nt = len(first.tiles)
+        dirty = np.zeros(nt, bool)
+        dirty[rng.choice(nt, int(rng.integers(1, nt + 1)), replace=False)] = True
+        keys = {(int(first.tiles[t, 0]), int(first.tiles[t, 1])) for t in np.nonzero(dirty)[0]}
+        v = scene.volume
+        nx, ny, nz = first.info["volume_size"]
+        k = np.arange(nx * ny * nz)
+        f = np.float32
+        px = f(v.min[0]) + (k % nx).astype(f) * f(v.step)
+        pz = f(v.min[2]) + ((k // nx) % nz).astype(f) * f(v.step)
+        size = f(scene.settings.tile_size)
+        pk = zip(np.floor(px / size).astype(int), np.floor(pz / size).astype(int))
+        pdirty = np.array([(int(a), int(b)) in keys for a, b in pk], bool)
+        rebake = {"cache": first.cache, "tile_dirty": dirty, "probe_dirty": pdirty}
+        del first
     t0 = time.perf_counter()
-    sess = BakeSession(scene, gpu, form=form, graph=True)
+    sess = BakeSession(scene, gpu, form=form, graph=True, **rebake)
     prep_s = time.perf_counter() - t0
     if args.profile_gather:
         sess.execute()
@@ -233,13 +260,14 @@ def main() -> int:
         kernel = f"k_gather_tuned<{layout}, false, {tuning['waves']}>"
     else:
         kernel = f"k_pool<0, {layout}, false>"
-    traffic = measured_traffic(kernel)
+    # the recorded counters are the map's gather launch
+    traffic = measured_traffic(kernel) if args.workload == "map" else None
     print(f"# gather launch {ms_gather:.2f} ms, {bytes_launch / 1e9:.3f} GB algorithmic, "
           f"{per_launch['rays'] / ms_gather / 1e6:.2f} Grays/s", file=sys.stderr, flush=True)
 
     plain_ms = None
     if not args.no_plain and form == N.FORM_TUNED:
-        ps = BakeSession(scene, gpu, form=N.FORM_PLAIN, graph=True)
+        ps = BakeSession(scene, gpu, form=N.FORM_PLAIN, graph=True, **rebake)
         ps.execute()
         gpu.sync()
         t0 = time.perf_counter()
@@ -249,8 +277,16 @@ def main() -> int:
         ps.close()
 
     cpu = None if args.no_cpu else cpu_baseline(sess, gpu)
+    if args.workload == "map":
+        metric = "Grays/s, 100x100 m 3-bounce bake (direct + 3 gather bounces + volume)"
+        workload = "map_scene(2): 100x100 m, 8 texels/m, 256 GI rays, 3 bounces, 16 tiles"
+    else:
+        metric = "Grays/s, rebake of a seeded share of a 200x200 m map's 64 tiles"
+        workload = (f"rebake_scene(3): 200x200 m, 4 texels/m, 64 GI rays, 1 bounce, "
+                    f"{info['tiles_baked']} of {info['tiles']} tiles dirty, "
+                    f"{info['probes_baked']} of {info['probes']} probes")
     out = {
-        "metric": "Grays/s, 100x100 m 3-bounce bake (direct + 3 gather bounces + volume)",
+        "metric": metric,
         "value": rays / (ms_step / 1e3) / 1e9,
         "unit": "Grays/s",
         "n_gpus": 1,
@@ -261,9 +297,11 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded map generator (fBm terrain + boxes + cylinders), no dataset",
+        "data": ("synthetic: seeded map generator (fBm terrain + boxes + cylinders), no dataset"
+                 if args.workload == "map" else
+                 "synthetic: seeded generator (yawed boxes on a ground of 5 m quads), no dataset"),
         "config": {
-            "workload": "map_scene(2): 100x100 m, 8 texels/m, 256 GI rays, 3 bounces, 16 tiles",
+            "workload": workload,
             "form": args.form, "nodes": "wide" if layout == N.NODES_WIDE else "quant",
             "graph": True, "tuning": tuned,
             "triangles": scene.triangle_count(), "bvh_nodes": info["bvh_nodes"],
@@ -275,6 +313,8 @@ def main() -> int:
             "ms_bounces": info["ms_bounces"], "ms_volume": info["ms_volume"],
             "s_prepare_host": prep_s, "s_finish_host": finish_s,
             "s_per_bake_end_to_end": prep_s + ms_step / 1e3 + finish_s,
+            "tiles_baked": info["tiles_baked"],
+            "ms_per_tile": ms_step / max(info["tiles_baked"], 1),
             "plain_ms_per_step": plain_ms,
             "tuned_vs_plain": (plain_ms / ms_step) if plain_ms else None,
         },

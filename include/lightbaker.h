@@ -266,16 +266,37 @@ int lb_layout_rasterize(const lb_scene_desc *scene, const lb_layout *layout, siz
 
 typedef struct lb_bake lb_bake;
 
+/* What a rebake takes from the cache (bake.rs:123-135's `lights`, volume.rs:44-65's cached
+ * chunks): which tiles and probes to bake, and for the others what an earlier bake left. The
+ * per-texel arrays run over every texel of every tile of the layout, tiles in layout order,
+ * row-major in a tile (the order of the LB_BAKE_TILE_* arrays); only clean tiles' entries are
+ * read. The hashes that decide which tiles are clean (deps.rs) stay with the caller. */
+typedef struct lb_bake_cache {
+    const uint8_t *tile_dirty;  /* per tile: nonzero = bake it; NULL = every tile */
+    /* bake.rs:61-69 `TileLight`, per texel */
+    const uint8_t *known;       /* 2 own light, 1 filled from neighbours, 0 none */
+    const uint8_t *state;       /* output.rs:29-40 validity */
+    const float *field, *ao;    /* f32 */
+    const float *direct, *indirect; /* f32 x3 */
+    size_t n_texels;            /* the arrays' length: every texel of the layout's tiles */
+    const uint8_t *probe_dirty; /* per probe: nonzero = trace it; NULL = every probe */
+    const float *probes;        /* f32 x16 per probe as traced (before the inside fill) */
+    size_t n_probes;
+} lb_bake_cache;
+
 typedef struct lb_bake_options {
     int form;      /* LB_FORM_* of the ray passes */
     int graph;     /* nonzero: the bounce loop (gather + texel passes) replayed from a hipGraph */
     int threads;   /* host threads for the rasterizer (0: all) */
+    const lb_bake_cache *cache; /* NULL: bake every tile and every probe */
 } lb_bake_options;
 
-/* bake() (bake.rs:94): layout, BVH, the tiles' texels, the direct pass, the texel passes, the
- * bounces, the volume, the atlas. Every tile is baked (no cache: tile hashing and the cache are
- * the reference's host bookkeeping). Errors as the reference's: a tile wider than the atlas, a
- * failing GPU call. The scene is borrowed until the call returns. */
+/* bake() (bake.rs:94): layout, BVH, the dirty tiles' texels, the direct pass, the texel passes,
+ * the bounces, the volume, the atlas. The dirty tiles' items go into each pass as one list
+ * (bake.rs:139-149: dirty tiles in layout order, each tile's texels row-major; an item's index
+ * in it is its ray key); the clean tiles' light is in the atlas from the start. Errors as the
+ * reference's: a tile wider than the atlas, a failing GPU call; and a cache whose arrays do not
+ * match the layout. The scene and the cache are borrowed until the call returns. */
 int lb_bake_run(lb_gpu *gpu, const lb_scene_desc *scene, const lb_bake_options *options,
                 lb_bake **out);
 void lb_bake_destroy(lb_bake *bake);
@@ -300,7 +321,7 @@ typedef struct lb_bake_info {
     uint32_t atlas_width, atlas_height;        /* layout.rs:56-66 */
     uint32_t indirect_width, indirect_height;  /* bake.rs:365 */
     uint32_t tiles, tiles_baked, charts;       /* bake.rs:49-58 Stats */
-    uint64_t texels, texels_baked, gather_items, probes;
+    uint64_t texels, texels_baked, gather_items, probes, probes_baked;
     uint32_t volume_size[3];
     float volume_origin[3];
     float volume_step;
@@ -327,8 +348,19 @@ enum lb_bake_array {
     LB_BAKE_TRIS = 10,       /* lb_tri */
     LB_BAKE_MATERIALS = 11,  /* f32 x4 per material */
     LB_BAKE_PARAMS = 12,     /* lb_params of the direct and gather passes */
-    LB_BAKE_TEXELS = 13,     /* lb_texel per texel of every tile, tile after tile, as rasterized */
-    LB_BAKE_GATHER_INDEX = 14 /* u32 per gather item: its texel (after an execute) */
+    LB_BAKE_TEXELS = 13,     /* lb_texel per texel of every baked tile, tile after tile, as rasterized */
+    LB_BAKE_GATHER_INDEX = 14, /* u32 per gather item: its baked texel (after an execute) */
+    /* what bake.rs:360-362 puts in the cache, over every tile (clean tiles as the cache gave
+     * them), texels in the lb_bake_cache order (after a finish) */
+    LB_BAKE_TILE_KNOWN = 15,    /* u8 */
+    LB_BAKE_TILE_STATE = 16,    /* u8 */
+    LB_BAKE_TILE_FIELD = 17,    /* f32 */
+    LB_BAKE_TILE_AO = 18,       /* f32 */
+    LB_BAKE_TILE_DIRECT = 19,   /* f32 x3 */
+    LB_BAKE_TILE_INDIRECT = 20, /* f32 x3 */
+    LB_BAKE_PROBES = 21,        /* f32 x16 per probe before the inside fill (volume.rs:81-84) */
+    LB_BAKE_BAKED_TILES = 22,   /* u32 per baked tile: its index in the layout */
+    LB_BAKE_LIGHT0 = 23         /* f32 x4 per atlas texel: the clean tiles' light, the rest 0 */
 };
 
 /* The bake's array `which`: its bytes, with *data pointing at them (owned by the bake). */

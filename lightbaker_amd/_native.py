@@ -69,8 +69,19 @@ class SceneDesc(C.Structure):
     ]
 
 
+class BakeCache(C.Structure):
+    """lb_bake_cache: which tiles and probes a rebake traces, and the clean ones' values."""
+    _fields_ = [
+        ("tile_dirty", C.c_void_p), ("known", C.c_void_p), ("state", C.c_void_p),
+        ("field", C.c_void_p), ("ao", C.c_void_p), ("direct", C.c_void_p),
+        ("indirect", C.c_void_p), ("n_texels", C.c_size_t), ("probe_dirty", C.c_void_p),
+        ("probes", C.c_void_p), ("n_probes", C.c_size_t),
+    ]
+
+
 class BakeOptions(C.Structure):
-    _fields_ = [("form", C.c_int), ("graph", C.c_int), ("threads", C.c_int)]
+    _fields_ = [("form", C.c_int), ("graph", C.c_int), ("threads", C.c_int),
+                ("cache", C.POINTER(BakeCache))]
 
 
 class BakeInfo(C.Structure):
@@ -79,7 +90,7 @@ class BakeInfo(C.Structure):
         ("indirect_width", C.c_uint32), ("indirect_height", C.c_uint32),
         ("tiles", C.c_uint32), ("tiles_baked", C.c_uint32), ("charts", C.c_uint32),
         ("texels", C.c_uint64), ("texels_baked", C.c_uint64), ("gather_items", C.c_uint64),
-        ("probes", C.c_uint64), ("volume_size", C.c_uint32 * 3), ("volume_origin", C.c_float * 3),
+        ("probes", C.c_uint64), ("probes_baked", C.c_uint64), ("volume_size", C.c_uint32 * 3), ("volume_origin", C.c_float * 3),
         ("volume_step", C.c_float), ("bvh_nodes", C.c_uint32), ("bvh_tris", C.c_uint32),
         ("ms_direct", C.c_float), ("ms_direct_texel", C.c_float), ("ms_bounces", C.c_float),
         ("ms_volume", C.c_float), ("s_layout", C.c_double), ("s_bvh", C.c_double),
@@ -91,6 +102,8 @@ class BakeInfo(C.Structure):
 BAKE_DIRECT, BAKE_INDIRECT, BAKE_VALIDITY, BAKE_VOLUME, BAKE_CHARTS = 0, 1, 2, 3, 4
 BAKE_TRI_CHARTS, BAKE_LIGHT, BAKE_CHART_TILES, BAKE_TILES = 5, 6, 7, 8
 BAKE_NODES, BAKE_TRIS, BAKE_MATERIALS, BAKE_PARAMS, BAKE_TEXELS, BAKE_GATHER_INDEX = 9, 10, 11, 12, 13, 14
+BAKE_TILE_KNOWN, BAKE_TILE_STATE, BAKE_TILE_FIELD, BAKE_TILE_AO = 15, 16, 17, 18
+BAKE_TILE_DIRECT, BAKE_TILE_INDIRECT, BAKE_PROBES, BAKE_BAKED_TILES, BAKE_LIGHT0 = 19, 20, 21, 22, 23
 
 # Every symbol include/lightbaker.h declares, with its ctypes signature.
 _VP, _SZ, _U32, _I = C.c_void_p, C.c_size_t, C.c_uint32, C.c_int

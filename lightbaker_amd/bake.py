@@ -25,6 +25,64 @@ class Bake:
     light: np.ndarray       # (ah, aw, 4)
     chart_tiles: np.ndarray
     tiles: np.ndarray       # (tiles, 6): key x, key z, w, h, atlas x, atlas y
+    cache: "TileCache"      # what the reference's cache would hold after this bake
+
+
+@dataclass
+class TileCache:
+    """bake.rs:61-69 `TileLight` of every tile, texels tile after tile (layout order, row-major
+    in a tile), and volume.rs:81-84's probes as traced (before the inside fill)."""
+    known: np.ndarray       # u8: 2 own light, 1 filled, 0 none
+    state: np.ndarray       # u8 validity
+    field: np.ndarray       # f32
+    ao: np.ndarray          # f32
+    direct: np.ndarray      # (n, 3) f32
+    indirect: np.ndarray    # (n, 3) f32
+    probes: np.ndarray | None  # (probes, 16) f32
+
+
+def _cache_desc(cache: TileCache | None, tile_dirty, probe_dirty):
+    """The lb_bake_cache of a rebake, and the arrays it borrows (None: bake everything)."""
+    if tile_dirty is None and probe_dirty is None:
+        return None, []
+    keep = []
+
+    def ptr(a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(a)
+        return a.ctypes.data
+
+    c = N.BakeCache()
+    if tile_dirty is not None:
+        c.tile_dirty = ptr(np.asarray(tile_dirty).astype(bool), np.uint8)
+    if probe_dirty is not None:
+        c.probe_dirty = ptr(np.asarray(probe_dirty).astype(bool), np.uint8)
+    if cache is not None:
+        c.known = ptr(cache.known, np.uint8)
+        c.state = ptr(cache.state, np.uint8)
+        c.field = ptr(cache.field, np.float32)
+        c.ao = ptr(cache.ao, np.float32)
+        c.direct = ptr(cache.direct, np.float32)
+        c.indirect = ptr(cache.indirect, np.float32)
+        c.n_texels = len(cache.known)
+        if cache.probes is not None:
+            c.probes = ptr(cache.probes, np.float32)
+            c.n_probes = len(cache.probes)
+    keep.append(c)
+    return c, keep
+
+
+def _tile_cache(lib, h) -> "TileCache":
+    pr = _array(lib, h, N.BAKE_PROBES, np.float32)
+    return TileCache(
+        known=_array(lib, h, N.BAKE_TILE_KNOWN, np.uint8),
+        state=_array(lib, h, N.BAKE_TILE_STATE, np.uint8),
+        field=_array(lib, h, N.BAKE_TILE_FIELD, np.float32),
+        ao=_array(lib, h, N.BAKE_TILE_AO, np.float32),
+        direct=_array(lib, h, N.BAKE_TILE_DIRECT, np.float32).reshape(-1, 3),
+        indirect=_array(lib, h, N.BAKE_TILE_INDIRECT, np.float32).reshape(-1, 3),
+        probes=pr.reshape(-1, 16) if len(pr) else None,
+    )
 
 
 def scene_desc(scene: Scene):
@@ -79,10 +137,15 @@ def _array(lib, h, which, dtype):
 
 
 def bake(scene: Scene, gpu: Gpu, form: int = N.FORM_PLAIN, graph: bool = False,
-         threads: int = 0) -> Bake:
+         threads: int = 0, cache: TileCache | None = None, tile_dirty=None,
+         probe_dirty=None) -> Bake:
+    """A bake; with `tile_dirty` / `probe_dirty` (a flag per layout tile / probe) a rebake that
+    traces only the dirty ones and takes the others from `cache` (bake.rs:123-136)."""
     lib = N.lib()
     d, keep = scene_desc(scene)
-    opt = N.BakeOptions(int(form), int(graph), int(threads))
+    cd, ckeep = _cache_desc(cache, tile_dirty, probe_dirty)
+    opt = N.BakeOptions(int(form), int(graph), int(threads),
+                        C.pointer(cd) if cd is not None else None)
     h = C.c_void_p()
     N.check(lib.lb_bake_run(gpu.handle, C.byref(d), C.byref(opt), C.byref(h)))
     try:
@@ -111,10 +174,11 @@ def bake(scene: Scene, gpu: Gpu, form: int = N.FORM_PLAIN, graph: bool = False,
             light=_array(lib, h, N.BAKE_LIGHT, np.float32).reshape(ah, aw, 4),
             chart_tiles=_array(lib, h, N.BAKE_CHART_TILES, np.uint32),
             tiles=_array(lib, h, N.BAKE_TILES, np.int32).reshape(-1, 6),
+            cache=_tile_cache(lib, h),
         )
     finally:
         lib.lb_bake_destroy(h)
-        del keep
+        del keep, ckeep
 
 
 class BakeSession:
@@ -122,12 +186,16 @@ class BakeSession:
     data (the benchmark's step) and timed pass by pass."""
 
     def __init__(self, scene: Scene, gpu: Gpu, form: int = N.FORM_TUNED, graph: bool = True,
-                 threads: int = 0):
+                 threads: int = 0, cache: TileCache | None = None, tile_dirty=None,
+                 probe_dirty=None):
         self.lib = N.lib()
         self.gpu = gpu
         self.scene = scene
         self._desc, self._keep = scene_desc(scene)
-        opt = N.BakeOptions(int(form), int(graph), int(threads))
+        cd, ckeep = _cache_desc(cache, tile_dirty, probe_dirty)
+        self._keep += ckeep
+        opt = N.BakeOptions(int(form), int(graph), int(threads),
+                            C.pointer(cd) if cd is not None else None)
         h = C.c_void_p()
         N.check(self.lib.lb_bake_prepare(gpu.handle, C.byref(self._desc), C.byref(opt),
                                          C.byref(h)))
@@ -160,6 +228,10 @@ class BakeSession:
 
     def array(self, which: int, dtype) -> np.ndarray:
         return _array(self.lib, self.handle, which, dtype)
+
+    def tile_cache(self) -> TileCache:
+        """After finish: every tile's TileLight and the traced probes."""
+        return _tile_cache(self.lib, self.handle)
 
     def close(self) -> None:
         if self.handle:

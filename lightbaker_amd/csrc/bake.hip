@@ -104,20 +104,29 @@ struct lb_bake {
     lb_params P{}, PV{};
     V3 sun{};
     uint32_t g = 0, aw = 0, ah = 0, ds = 1, rg = 0, rings = 0, grings = 0;
-    // host copies
+    // the tiles baked (dirty), in layout order; every tile's first texel in the cache order
+    std::vector<uint32_t> baked;
+    std::vector<size_t> toff_all;
+    // host copies of the baked tiles' texels (toff: each baked tile's first)
     std::vector<uint32_t> toff, chart;
     std::vector<lb_texel> texels;
     std::vector<float> normal;
     std::vector<uint8_t> covered;
     std::vector<uint32_t> cov;
     uint32_t vsz[3] = {0, 0, 0};
-    size_t T = 0, ncov = 0, ng = 0, np = 0;
+    size_t T = 0, ncov = 0, ng = 0, np = 0, npd = 0;
     std::vector<uint32_t> gat;
+    std::vector<uint32_t> pdirty;  // the probes traced, in index order
+    // every tile's TileLight (bake.rs:61-69), clean tiles from the cache, in the cache order
+    std::vector<uint8_t> tl_known, tl_state;
+    std::vector<float> tl_field, tl_ao, tl_direct, tl_indirect;
+    std::vector<float> probes_raw, light0;
+    bool any_clean = false;
     // device state
     DevBuf d_tex0, d_tex, d_tile_of, d_tiles, d_chart, d_normal, d_valid, d_known, d_state,
         d_hard, d_soft, d_ao, d_field, d_direct, d_indirect, d_cov, d_items, d_res, d_light, d_w1,
         d_wg, d_f1, d_f4, d_m1, d_m2, d_covered, d_gat, d_gmask, d_gitems, d_gres, d_filled,
-        d_pitems, d_pres, d_counts, d_total;
+        d_pitems, d_pres, d_counts, d_total, d_light0;
     TexelView V{};
     hipEvent_t ev[8] = {};
     hipGraphExec_t graph = nullptr;
@@ -256,21 +265,78 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     B->PV = P;
     B->PV.counts[2] = sc.volume_rays;  // volume.rs:76
 
-    // the tiles' texels (bake.rs:139-148): every tile is dirty
+    // which tiles to bake, and the clean ones' light from the cache (bake.rs:123-136)
     t0 = clk::now();
     size_t nt = L.tiles.size();
-    I.tiles = I.tiles_baked = (uint32_t)nt;
+    B->toff_all.assign(nt + 1, 0);
+    for (size_t t = 0; t < nt; t++)
+        B->toff_all[t + 1] = B->toff_all[t] + (size_t)L.tiles[t].w * L.tiles[t].h;
+    size_t T_all = B->toff_all[nt];
+    const lb_bake_cache *cache = opt.cache;
+    for (size_t t = 0; t < nt; t++)
+        if (!cache || !cache->tile_dirty || cache->tile_dirty[t]) B->baked.push_back((uint32_t)t);
+    B->any_clean = B->baked.size() < nt;
+    if (B->any_clean) {
+        const lb_bake_cache &c = *cache;
+        if (!c.known || !c.state || !c.field || !c.ao || !c.direct || !c.indirect)
+            return fail("bake cache: clean tiles need known, state, field, ao, direct, indirect");
+        if (c.n_texels != T_all)
+            return fail("bake cache: %zu texels, the layout's tiles hold %zu", c.n_texels, T_all);
+    }
+    B->tl_known.assign(T_all, 0);
+    B->tl_state.assign(T_all, 0);
+    B->tl_field.assign(T_all, 0.0f);
+    B->tl_ao.assign(T_all, 0.0f);
+    B->tl_direct.assign(T_all * 3, 0.0f);
+    B->tl_indirect.assign(T_all * 3, 0.0f);
+    if (B->any_clean) {
+        // write_light (bake.rs:435-444) of the clean tiles: the atlas the first bounce reads
+        const lb_bake_cache &c = *cache;
+        B->light0.assign((size_t)aw * ah * 4, 0.0f);
+        size_t k = 0;
+        for (size_t t = 0; t < nt; t++) {
+            bool dirty = k < B->baked.size() && B->baked[k] == t;
+            if (dirty) {
+                k++;
+                continue;
+            }
+            const Tile &tl = L.tiles[t];
+            for (uint32_t y = 0; y < tl.h; y++)
+                for (uint32_t x = 0; x < tl.w; x++) {
+                    size_t i = B->toff_all[t] + (size_t)y * tl.w + x;
+                    B->tl_known[i] = c.known[i];
+                    B->tl_state[i] = c.state[i];
+                    B->tl_field[i] = c.field[i];
+                    B->tl_ao[i] = c.ao[i];
+                    for (int q = 0; q < 3; q++) {
+                        B->tl_direct[i * 3 + q] = c.direct[i * 3 + q];
+                        B->tl_indirect[i * 3 + q] = c.indirect[i * 3 + q];
+                    }
+                    float *a = &B->light0[(((size_t)tl.y + y) * aw + tl.x + x) * 4];
+                    if (c.known[i] >= 1) {
+                        for (int q = 0; q < 3; q++) a[q] = c.direct[i * 3 + q] + c.indirect[i * 3 + q];
+                        a[3] = 1.0f;
+                    }
+                }
+        }
+    }
+
+    // the dirty tiles' texels (bake.rs:139-148)
+    size_t nd = B->baked.size();
+    I.tiles = (uint32_t)nt;
+    I.tiles_baked = (uint32_t)nd;
     I.charts = (uint32_t)n_charts;
-    std::vector<std::vector<lb_texel>> tex(nt);
+    std::vector<std::vector<lb_texel>> tex(nd);
     {
         unsigned hw = opt.threads > 0 ? (unsigned)opt.threads
                                       : std::max(1u, std::thread::hardware_concurrency());
         std::vector<std::thread> pool;
         std::atomic<size_t> next{0};
-        size_t workers = std::min<size_t>(hw, std::max<size_t>(nt, 1));
+        size_t workers = std::min<size_t>(hw, std::max<size_t>(nd, 1));
         for (size_t w = 0; w < workers; w++)
             pool.emplace_back([&] {
-                for (size_t t; (t = next.fetch_add(1)) < nt;) rasterize(sc, L, L.tiles[t], tex[t]);
+                for (size_t k; (k = next.fetch_add(1)) < nd;)
+                    rasterize(sc, L, L.tiles[B->baked[k]], tex[k]);
             });
         for (auto &th : pool) th.join();
     }
@@ -281,9 +347,12 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
         int32_t v[6] = {tl.kx, tl.kz, (int32_t)tl.w, (int32_t)tl.h, (int32_t)tl.x, (int32_t)tl.y};
         std::memcpy(&B->tiles[t * 6], v, sizeof v);
     }
-    B->toff.assign(nt + 1, 0);
-    for (size_t t = 0; t < nt; t++) B->toff[t + 1] = B->toff[t] + L.tiles[t].w * L.tiles[t].h;
-    size_t T = B->toff[nt];
+    B->toff.assign(nd + 1, 0);
+    for (size_t k = 0; k < nd; k++) {
+        const Tile &tl = L.tiles[B->baked[k]];
+        B->toff[k + 1] = B->toff[k] + tl.w * tl.h;
+    }
+    size_t T = B->toff[nd];
     if (T > 0x7fffffffu) return fail("bake: %zu texels exceed 31-bit indices", T);
     B->T = T;
     B->texels.resize(T);
@@ -291,9 +360,9 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     B->chart.resize(T);
     B->normal.assign(T * 4, 0.0f);
     B->covered.resize(T);
-    std::vector<uint4> tiles(std::max<size_t>(nt, 1));
-    for (size_t t = 0; t < nt; t++) {
-        const Tile &tl = L.tiles[t];
+    std::vector<uint4> tiles(std::max<size_t>(nd, 1));
+    for (size_t t = 0; t < nd; t++) {
+        const Tile &tl = L.tiles[B->baked[t]];
         uint32_t o = B->toff[t];
         tiles[t] = make_uint4(o, tl.w, tl.h, tl.x | (tl.y << 16));
         if (!tex[t].empty()) std::memcpy(&B->texels[o], tex[t].data(), tex[t].size() * sizeof(lb_texel));
@@ -316,7 +385,8 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     I.texels_baked = B->ncov;
     I.s_raster = secs(t0);
 
-    // the probes (volume.rs:36-75): every probe, in index order (their keys are their indices)
+    // the probes (volume.rs:36-75): the dirty ones in index order (a probe's key is its index, so
+    // the reference's chunk order gives the same rays); the clean ones from the cache
     std::vector<lb_texel> pitems;
     if (sc.has_volume) {
         for (int a = 0; a < 3; a++)
@@ -325,17 +395,41 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
         size_t nx = B->vsz[0], ny = B->vsz[1], nz = B->vsz[2];
         B->np = nx * ny * nz;
         if (B->np > 0x7fffffffu) return fail("volume: %zu probes exceed 31-bit indices", B->np);
-        pitems.resize(B->np);
+        B->probes_raw.assign(B->np * 16, 0.0f);
+        bool some_clean = false;
         for (size_t k = 0; k < B->np; k++) {
+            if (cache && cache->probe_dirty && !cache->probe_dirty[k]) {
+                some_clean = true;
+                continue;
+            }
+            B->pdirty.push_back((uint32_t)k);
+        }
+        if (some_clean) {
+            if (!cache->probes) return fail("bake cache: clean probes need their values");
+            if (cache->n_probes != B->np)
+                return fail("bake cache: %zu probes, the volume holds %zu", cache->n_probes, B->np);
+            for (size_t k = 0, d = 0; k < B->np; k++) {
+                if (d < B->pdirty.size() && B->pdirty[d] == k) {
+                    d++;
+                    continue;
+                }
+                std::memcpy(&B->probes_raw[k * 16], &cache->probes[k * 16], 64);
+            }
+        }
+        B->npd = B->pdirty.size();
+        pitems.resize(B->npd);
+        for (size_t d = 0; d < B->npd; d++) {
+            size_t k = B->pdirty[d];
             float px = sc.volume_min[0] + (float)(k % nx) * sc.volume_step;
             float py = sc.volume_min[1] + (float)(k / (nx * nz)) * sc.volume_step;
             float pz = sc.volume_min[2] + (float)((k / nx) % nz) * sc.volume_step;
             uint32_t kk = (uint32_t)k;
             float kf;
             std::memcpy(&kf, &kk, 4);
-            pitems[k] = lb_texel{{px, py, pz, 1.0f}, {0.0f, 1.0f, 0.0f, kf}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+            pitems[d] = lb_texel{{px, py, pz, 1.0f}, {0.0f, 1.0f, 0.0f, kf}, {0, 0, 0, 0}, {0, 0, 0, 0}};
         }
         I.probes = B->np;
+        I.probes_baked = B->npd;
         for (int a = 0; a < 3; a++) {
             I.volume_size[a] = B->vsz[a];
             I.volume_origin[a] = sc.volume_min[a];
@@ -382,10 +476,11 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     TRY(put(B->d_wg, gauss((int)B->rg), st));
     B->rings = L.padding + 4;                  // bake.rs:243
     B->grings = L.padding + 2 * B->ds + 2;     // bake.rs:320
-    if (sc.has_volume) {
+    if (B->npd) {
         TRY(put(B->d_pitems, pitems, st));
-        TRY(B->d_pres.reserve(B->np * 64));
+        TRY(B->d_pres.reserve(B->npd * 64));
     }
+    if (B->any_clean) TRY(put(B->d_light0, B->light0, st));
     TexelView &V = B->V;
     V.n = (uint32_t)T;
     V.tile_of = (const uint32_t *)B->d_tile_of.ptr;
@@ -405,7 +500,7 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     for (auto &e : B->ev) TRYH(hipEventCreate(&e));
     if (opt.form == LB_FORM_TUNED) {
         // the tuned form's spill area, sized for the largest launch, before any capture
-        uint32_t most = (uint32_t)std::max<size_t>({B->ncov, T, B->np, 1});
+        uint32_t most = (uint32_t)std::max<size_t>({B->ncov, T, B->npd, 1});
         for (int pass = 0; pass < 3; pass++)
             if (tuned_prepare(gpu, pass, pass == LB_PASS_VOLUME ? B->PV : P, most) != hipSuccess)
                 return fail("GPU: tuned spill area");
@@ -487,7 +582,11 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
     // the prepared state (bake.rs:204-205: soft 0, ao 1, hard 0, no state; valid = covered)
     TRYH(hipEventRecord(B->ev[0], st));
     TRYH(hipMemcpyAsync(V.texels, B->d_tex0.ptr, B->T * sizeof(lb_texel), hipMemcpyDeviceToDevice, st));
-    TRYH(hipMemsetAsync(B->d_light.ptr, 0, (size_t)B->aw * B->ah * 16, st));
+    if (B->any_clean)
+        TRYH(hipMemcpyAsync(B->d_light.ptr, B->d_light0.ptr, (size_t)B->aw * B->ah * 16,
+                            hipMemcpyDeviceToDevice, st));
+    else
+        TRYH(hipMemsetAsync(B->d_light.ptr, 0, (size_t)B->aw * B->ah * 16, st));
     TRYH(hipMemcpyAsync(V.valid, B->d_covered.ptr, B->T, hipMemcpyDeviceToDevice, st));
     TRYH(hipMemsetAsync(V.state, 0, Tn, st));
     TRYH(hipMemsetAsync(V.hard, 0, Tn * 4, st));
@@ -574,8 +673,8 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
     TRYH(hipEventRecord(B->ev[4], st));
 
     // the volume (volume.rs:77)
-    if (sc.has_volume)
-        TRY(pass_stats(B, LB_PASS_VOLUME, &B->PV, (const lb_texel *)B->d_pitems.ptr, B->np,
+    if (B->npd)
+        TRY(pass_stats(B, LB_PASS_VOLUME, &B->PV, (const lb_texel *)B->d_pitems.ptr, B->npd,
                        (const float *)B->d_light.ptr, (size_t)B->aw * B->ah,
                        (float *)B->d_pres.ptr, sv));
     TRYH(hipEventRecord(B->ev[5], st));
@@ -599,7 +698,7 @@ int bake_finish(lb_bake *B) {
     std::vector<uint8_t> known(T), state(T);
     std::vector<float> field(T), ao(T), dir4(T * 4), ind4(T * 4);
     B->light.resize((size_t)aw * ah * 4);
-    std::vector<float> probes(B->np * 16);
+    std::vector<float> pres(B->npd * 16);
     if (T) {
         TRYH(hipMemcpyAsync(known.data(), B->d_known.ptr, T, hipMemcpyDeviceToHost, st));
         TRYH(hipMemcpyAsync(state.data(), B->d_state.ptr, T, hipMemcpyDeviceToHost, st));
@@ -609,8 +708,12 @@ int bake_finish(lb_bake *B) {
         TRYH(hipMemcpyAsync(ind4.data(), B->d_indirect.ptr, T * 16, hipMemcpyDeviceToHost, st));
     }
     TRYH(hipMemcpyAsync(B->light.data(), B->d_light.ptr, (size_t)aw * ah * 16, hipMemcpyDeviceToHost, st));
-    if (B->np) TRYH(hipMemcpyAsync(probes.data(), B->d_pres.ptr, B->np * 64, hipMemcpyDeviceToHost, st));
+    if (B->npd) TRYH(hipMemcpyAsync(pres.data(), B->d_pres.ptr, B->npd * 64, hipMemcpyDeviceToHost, st));
     TRYH(hipStreamSynchronize(st));
+    // the traced probes beside the cached ones: what volume.rs:81-84 caches
+    for (size_t d = 0; d < B->npd; d++)
+        std::memcpy(&B->probes_raw[(size_t)B->pdirty[d] * 16], &pres[d * 16], 64);
+    std::vector<float> probes = B->probes_raw;
 
     // probes inside solids from their neighbours, 8 rings (volume.rs:86-117)
     size_t nx = B->vsz[0], ny = B->vsz[1], nz = B->vsz[2], np = B->np;
@@ -664,8 +767,24 @@ int bake_finish(lb_bake *B) {
         }
     }
 
-    // the atlas from every tile (bake.rs:364-421)
+    // the baked tiles' TileLight beside the cached ones (what bake.rs:360-362 caches)
     const Layout &L = B->layout;
+    for (size_t k = 0; k < B->baked.size(); k++) {
+        size_t o = B->toff_all[B->baked[k]], n = B->toff[k + 1] - B->toff[k];
+        for (size_t i = 0; i < n; i++) {
+            size_t gi = B->toff[k] + i;
+            B->tl_known[o + i] = known[gi];
+            B->tl_state[o + i] = validity_t[gi];
+            B->tl_field[o + i] = field[gi];
+            B->tl_ao[o + i] = ao[gi];
+            for (int q = 0; q < 3; q++) {
+                B->tl_direct[(o + i) * 3 + q] = dir4[gi * 4 + q];
+                B->tl_indirect[(o + i) * 3 + q] = ind4[gi * 4 + q];
+            }
+        }
+    }
+
+    // the atlas from every tile (bake.rs:364-421)
     uint32_t ds = B->ds;
     uint32_t iw = (aw + ds - 1) / ds, ih = (ah + ds - 1) / ds;
     B->direct.assign((size_t)aw * ah * 2, 1.0f);
@@ -675,15 +794,16 @@ int bake_finish(lb_bake *B) {
         const Tile &tl = L.tiles[t];
         for (uint32_t y = 0; y < tl.h; y++)
             for (uint32_t x = 0; x < tl.w; x++) {
-                size_t gi = B->toff[t] + (size_t)y * tl.w + x;
-                if (known[gi] == 0) continue;
+                size_t gi = B->toff_all[t] + (size_t)y * tl.w + x;
+                uint8_t kn = B->tl_known[gi];
+                if (kn == 0) continue;
                 size_t ax = tl.x + x, ay = tl.y + y;
-                B->direct[(ay * aw + ax) * 2] = field[gi];
-                B->direct[(ay * aw + ax) * 2 + 1] = ao[gi];
-                B->validity[ay * aw + ax] = validity_t[gi];
-                float k = known[gi] == 2 ? 1.0f : 1e-3f;
+                B->direct[(ay * aw + ax) * 2] = B->tl_field[gi];
+                B->direct[(ay * aw + ax) * 2 + 1] = B->tl_ao[gi];
+                B->validity[ay * aw + ax] = B->tl_state[gi];
+                float k = kn == 2 ? 1.0f : 1e-3f;
                 float *s = &sum[((ay / ds) * iw + ax / ds) * 4];
-                for (int c = 0; c < 3; c++) s[c] += ind4[gi * 4 + c] * k;
+                for (int c = 0; c < 3; c++) s[c] += B->tl_indirect[gi * 3 + c] * k;
                 s[3] += k;
             }
     }
@@ -743,7 +863,7 @@ int bake_time_pass(lb_bake *B, int pass, int reps, float *ms_avg) {
         out = (float *)B->d_gres.ptr;
     } else {
         items = (const lb_texel *)B->d_pitems.ptr;
-        n = B->np;
+        n = B->npd;
         P = &B->PV;
         out = (float *)B->d_pres.ptr;
     }
@@ -770,7 +890,7 @@ int lb_bake_prepare(lb_gpu *gpu, const lb_scene_desc *scene, const lb_bake_optio
                     lb_bake **out) {
     if (!gpu || !scene || !out) return fail("lb_bake_prepare: null handle");
     *out = nullptr;
-    lb_bake_options opt{LB_FORM_PLAIN, 0, 0};
+    lb_bake_options opt{LB_FORM_PLAIN, 0, 0, nullptr};
     if (options) opt = *options;
     if (opt.form != LB_FORM_PLAIN && opt.form != LB_FORM_TUNED)
         return fail("lb_bake_prepare: no form %d", opt.form);
@@ -839,6 +959,15 @@ size_t lb_bake_array(const lb_bake *b, int which, const void **data) {
     case LB_BAKE_MATERIALS: p = b->mats.data(); n = b->mats.size() * 4; break;
     case LB_BAKE_PARAMS: p = &b->P; n = sizeof(lb_params); break;
     case LB_BAKE_TEXELS: p = b->texels.data(); n = b->texels.size() * sizeof(lb_texel); break;
+    case LB_BAKE_TILE_KNOWN: p = b->tl_known.data(); n = b->tl_known.size(); break;
+    case LB_BAKE_TILE_STATE: p = b->tl_state.data(); n = b->tl_state.size(); break;
+    case LB_BAKE_TILE_FIELD: p = b->tl_field.data(); n = b->tl_field.size() * 4; break;
+    case LB_BAKE_TILE_AO: p = b->tl_ao.data(); n = b->tl_ao.size() * 4; break;
+    case LB_BAKE_TILE_DIRECT: p = b->tl_direct.data(); n = b->tl_direct.size() * 4; break;
+    case LB_BAKE_TILE_INDIRECT: p = b->tl_indirect.data(); n = b->tl_indirect.size() * 4; break;
+    case LB_BAKE_PROBES: p = b->probes_raw.data(); n = b->probes_raw.size() * 4; break;
+    case LB_BAKE_BAKED_TILES: p = b->baked.data(); n = b->baked.size() * 4; break;
+    case LB_BAKE_LIGHT0: p = b->light0.data(); n = b->light0.size() * 4; break;
     case LB_BAKE_GATHER_INDEX:
         if (b->gat.size() != b->ng) {
             auto *m = const_cast<lb_bake *>(b);

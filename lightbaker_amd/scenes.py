@@ -199,6 +199,42 @@ def small_scene(seed: int = 1) -> Scene:
         settings=st, volume=Volume((-10.0, 0.0, -10.0), (10.0, 8.0, 10.0), 1.0, 64))
 
 
+def rebake_scene(seed: int = 3, size: float = 200.0) -> Scene:
+    """A size x size m map built like small_scene (yawed boxes, edges U[0.5, 4] m, on a ground of
+    5 m quads, grey albedo 0.5, constant sky 0.5, sun at 45 degrees), in 25 m tiles whose box
+    counts run from 10 to 150 (50 per 400 m^2 on average, small_scene's density), so the tiles'
+    texel counts differ about fourfold."""
+    rng = np.random.default_rng(seed)
+    cells = int(size // 5)
+    ground = []
+    for i in range(cells):
+        for j in range(cells):
+            x, z = i * 5.0, j * 5.0
+            ground += quad((x, 0, z), (x, 0, z + 5), (x + 5, 0, z + 5), (x + 5, 0, z))
+    tiles = int(math.ceil(size / 25.0))
+    boxes = []
+    for tx in range(tiles):
+        for tz in range(tiles):
+            for _ in range(int(rng.integers(10, 151))):
+                sx, sy, sz = rng.uniform(0.5, 4.0, 3)
+                # 3 m from the tile's edges: every face of a box falls in its tile
+                cx = tx * 25.0 + rng.uniform(3.0, 22.0)
+                cz = tz * 25.0 + rng.uniform(3.0, 22.0)
+                boxes.append(yawed_box(cx, cz, 0.0, sx, sy, sz, rng.uniform(0, 2 * math.pi)))
+    az = rng.uniform(0, 2 * math.pi)
+    el = math.radians(45.0)
+    sun = (math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az))
+    st = Settings(texels_per_meter=4.0, tile_size=25.0, atlas_width=4096, sun_samples=16,
+                  ao_rays=64, gi_rays=64, gi_bounces=1)
+    return Scene(
+        sun_direction=sun, sun_radius_deg=0.5, sun_irradiance=(3.0, 3.0, 3.0),
+        zenith=(0.5, 0.5, 0.5), horizon=(0.5, 0.5, 0.5), ground=(0.5, 0.5, 0.5),
+        materials=[Material("grey", (0.5, 0.5, 0.5))],
+        meshes=[Mesh("ground", 0, np.array(ground, np.float32)),
+                Mesh("boxes", 0, np.concatenate(boxes))],
+        settings=st, volume=Volume((0.0, 0.0, 0.0), (size, 8.0, size), 1.0, 64))
+
+
 def fbm(n: int, octaves: int, rng) -> np.ndarray:
     """(n+1)^2 heights in [-1, 1]: octaves of bilinearly upsampled value noise, halving amplitude."""
     h = np.zeros((n + 1, n + 1))

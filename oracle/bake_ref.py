@@ -68,9 +68,10 @@ def offset(q):
 
 
 def bake(inp: dict) -> dict:
-    """inp: texels (T,) TEXEL tile after tile; tiles (nt, 6); aw, ah; nodes, tris, mats, charts,
-    params; settings (lightmap/bake fields); sun (3,), irr (3,); probes (np,) TEXEL or None,
-    volume_params; volume (min, step, size)."""
+    """inp: texels (T,) TEXEL of the dirty tiles, tile after tile; tiles (nt, 6) of the dirty
+    tiles; light0 (aw * ah, 4) the clean tiles' light or None; aw, ah; nodes, tris, mats, charts,
+    params; settings (lightmap/bake fields); sun (3,), irr (3,); probes (np,) TEXEL of the dirty
+    probes or None, volume_params."""
     st = inp["settings"]
     tex = inp["texels"].copy()
     tiles = inp["tiles"]
@@ -150,7 +151,13 @@ def bake(inp: dict) -> dict:
             val[..., 3] = np.where(kn >= 1, 1.0, 0.0)
             light[y0:y0 + h, x0:x0 + w] = val
 
-    light = np.zeros((ah, aw, 4), np.float32)
+    # the clean tiles' light from the cache (bake.rs:276-278 writes every tile; the tiles given
+    # here are the dirty ones)
+    l0 = inp.get("light0")
+    if l0 is not None and len(l0):
+        light = np.array(l0, np.float32).reshape(ah, aw, 4)
+    else:
+        light = np.zeros((ah, aw, 4), np.float32)
     write_light(light)
 
     # the gather set (bake.rs:282-299)
@@ -197,3 +204,24 @@ def bake(inp: dict) -> dict:
     return {"light": light, "known": known, "state": state, "field": field, "ao": ao,
             "direct": direct, "indirect": indirect, "probes": probes, "gathered": gat,
             "texels": tex}
+
+
+def clean_light(tiles_all, tile_dirty, cache, aw, ah):
+    """write_light (bake.rs:435-444) of the clean tiles from their cached TileLight: the atlas a
+    rebake's first bounce starts from. cache: known, direct, indirect over every tile's texels,
+    tiles in layout order."""
+    light = np.zeros((ah, aw, 4), np.float32)
+    o = 0
+    for t in range(len(tiles_all)):
+        w, h, x0, y0 = (int(v) for v in tiles_all[t, 2:6])
+        sl = slice(o, o + w * h)
+        o += w * h
+        if tile_dirty[t]:
+            continue
+        kn = cache.known[sl].reshape(h, w)
+        dd = (cache.direct[sl] + cache.indirect[sl]).reshape(h, w, 3)
+        val = np.zeros((h, w, 4), np.float32)
+        val[..., :3] = np.where(kn[..., None] >= 1, dd, 0)
+        val[..., 3] = np.where(kn >= 1, 1.0, 0.0)
+        light[y0:y0 + h, x0:x0 + w] = val
+    return light

@@ -1,8 +1,9 @@
 """The reference's integration test (crates/lightbaker/tests/bake.rs:95-152) on this bake: the same
 scene and the same thresholds, through every kernel form and with the bounce loop replayed from
-a hipGraph. The cache half of that test (re-bakes reading tiles back) belongs to the host's tile
-cache, which this package leaves to the caller; its determinism half (a second bake gives the same
-`direct`, bake.rs:150) is kept."""
+a hipGraph. Its determinism half (a second bake gives the same `direct`, bake.rs:150) is here; its
+cache half (a rebake with nothing dirty traces nothing and gives the same `direct`) is in
+test_rebake_gpu.py, with the dirty tiles named by the caller: the hashes that pick them (deps.rs)
+stay with the reference's host."""
 from __future__ import annotations
 
 import math
