@@ -366,6 +366,45 @@ enum lb_bake_array {
 /* The bake's array `which`: its bytes, with *data pointing at them (owned by the bake). */
 size_t lb_bake_array(const lb_bake *bake, int which, const void **data);
 
+/* ---- the adjoint of the bounces ------------------------------------------------------------ *
+ * No counterpart in the reference, which has no gradients; it serves callers that fit the
+ * scene's light to a target lightmap (inverse lighting: albedos, sky and sun from a bake). The
+ * light atlas after the bounces (bake.rs:300-330: the atlas the last bounce writes, direct +
+ * indirect per known texel) is a function of the materials' albedo, the sky (trace.wgsl:63-66:
+ * zenith, horizon, ground; the fallback irradiance pi (zenith + horizon) / 2 follows them) and
+ * the sun's irradiance (bake.rs:247-252), with visibility held fixed: every gather ray's hit is
+ * recorded once, and each forward replays those records through the bounces. */
+
+typedef struct lb_adjoint lb_adjoint;
+
+typedef struct lb_light_params {
+    const float *albedo;   /* rgb per material, n_materials x 3 (host) */
+    size_t n_materials;    /* the scene's */
+    float zenith[3], horizon[3], ground[3];
+    float sun[3];          /* irradiance */
+} lb_light_params;
+
+/* After lb_bake_execute: records the gather rays' hits (the bake must outlive the adjoint).
+ * Each record is two u32 per ray, ray r of gather item g at (r * items + g) * 2: a word (kind 0:
+ * the atlas texel bounced() reads; kind 3: the bits of the direction's y; else 0) and a tag
+ * (kind << 30 | material; kinds 0 a front hit on a chart, 1 a front hit off the charts, 2 a back
+ * face, 3 the sky). */
+int lb_adjoint_create(lb_bake *bake, lb_adjoint **out);
+void lb_adjoint_destroy(lb_adjoint *adj);
+/* atlas texels, parameters (n_materials * 3 + 12), gather items, rays per item */
+int lb_adjoint_info(const lb_adjoint *adj, size_t *atlas_texels, size_t *n_params, size_t *items,
+                    size_t *rays);
+/* The records, copied to `out` (host, cap u32 words). */
+int lb_adjoint_records(const lb_adjoint *adj, uint32_t *out, size_t cap);
+/* The bounces replayed with `params`: the final light atlas into `light` (device memory,
+ * atlas_texels x 4 f32). With the scene's own parameters it equals the bake's atlas bit for bit
+ * in the PLAIN form. The bounces' atlases are kept for lb_adjoint_backward. */
+int lb_adjoint_forward(lb_adjoint *adj, const lb_light_params *params, float *light);
+/* The gradient of a loss through the last forward: `dlight` (device, atlas_texels x 4 f32, the
+ * loss's derivative by each atlas texel's rgb; w unused) in, `grad` (device, n_params f32) out:
+ * albedo rgb per material, then zenith, horizon, ground and sun rgb. */
+int lb_adjoint_backward(lb_adjoint *adj, const float *dlight, float *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,13 @@ class BakeInfo(C.Structure):
     ]
 
 
+class LightParams(C.Structure):
+    """lb_light_params: the parameters the adjoint differentiates by."""
+    _fields_ = [("albedo", C.c_void_p), ("n_materials", C.c_size_t),
+                ("zenith", C.c_float * 3), ("horizon", C.c_float * 3), ("ground", C.c_float * 3),
+                ("sun", C.c_float * 3)]
+
+
 BAKE_DIRECT, BAKE_INDIRECT, BAKE_VALIDITY, BAKE_VOLUME, BAKE_CHARTS = 0, 1, 2, 3, 4
 BAKE_TRI_CHARTS, BAKE_LIGHT, BAKE_CHART_TILES, BAKE_TILES = 5, 6, 7, 8
 BAKE_NODES, BAKE_TRIS, BAKE_MATERIALS, BAKE_PARAMS, BAKE_TEXELS, BAKE_GATHER_INDEX = 9, 10, 11, 12, 13, 14
@@ -139,6 +146,13 @@ SIGNATURES = {
     "lb_bake_array": (_SZ, [_VP, _I, C.POINTER(_VP)]),
     "lb_bake_prepare": (_I, [_VP, C.POINTER(SceneDesc), C.POINTER(BakeOptions), C.POINTER(_VP)]),
     "lb_bake_execute": (_I, [_VP, _VP]),
+    "lb_adjoint_create": (_I, [_VP, C.POINTER(_VP)]),
+    "lb_adjoint_destroy": (None, [_VP]),
+    "lb_adjoint_info": (_I, [_VP, C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_SZ),
+                             C.POINTER(_SZ)]),
+    "lb_adjoint_records": (_I, [_VP, _VP, _SZ]),
+    "lb_adjoint_forward": (_I, [_VP, C.POINTER(LightParams), _VP]),
+    "lb_adjoint_backward": (_I, [_VP, _VP, _VP]),
     "lb_bake_finish": (_I, [_VP]),
     "lb_bake_time_pass": (_I, [_VP, _I, _I, C.POINTER(C.c_float)]),
     "lb_layout_build": (_I, [C.POINTER(SceneDesc), C.POINTER(_VP)]),

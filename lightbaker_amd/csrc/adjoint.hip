@@ -1,0 +1,654 @@
+// The adjoint of the bake's bounces (crates/lightbaker/src/bake.rs:243-330 with the gather pass of
+// trace.wgsl:245-270): the gradient of a loss on the final light atlas by the materials' albedo,
+// the sky's zenith, horizon and ground colours and the sun's irradiance, with visibility held
+// fixed. The reference has no gradients; this follows its forward exactly.
+//
+// Visibility is fixed once: k_record traces every gather ray as the gather does and keeps what
+// its light depends on (the sky direction's y, the atlas texel a front hit reads, the material).
+// A forward replays those records through the bounces with new parameters and keeps each
+// bounce's atlas; the backward runs the bounces in reverse:
+//   atlas -> texels      (write_light, bake.rs:435-444): a known texel's light is direct + indirect
+//   fill^T, smooth^T     (filter.rs:20-94; weights and masks depend on geometry only)
+//   scatter^T            (bake.rs:309-316)
+//   gather^T             per ray: sky, albedo and the atlas texel it read (k_gather_t)
+// and the sun's share through the direct light (linear in the irradiance: its derivative is the
+// direct pass's filled cos x sun share).
+//
+// Sums of the backward are in FP64 where they run over many terms (atlas and parameter
+// accumulators); the forward replay repeats the gather's FP32 arithmetic in the reference's order.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "lb_bake_state.h"
+#include "lb_trace_plain.h"
+
+using namespace lbk;
+
+namespace lbk {
+namespace {
+
+enum : uint32_t { REC_ATLAS = 0u, REC_FALLBACK = 1u, REC_BACK = 2u, REC_MISS = 3u };
+constexpr int8_t LEVEL_NEVER = 127;  // a texel no ring fills
+constexpr uint32_t BLOCK = 256;
+
+struct SkyP {
+    f3 zenith, horizon, ground, fallback;
+};
+
+__device__ __forceinline__ f3 f3of(float4 v) { return mk(v.x, v.y, v.z); }
+
+// the atlas texel bounced() (trace.wgsl:137-145) reads for a hit at o + d t on chart c
+__device__ __forceinline__ uint32_t hit_texel(const SceneView &S, const lb_params &P, f3 o, f3 d,
+                                              float t, uint32_t c) {
+    f3 p = add(o, scl(d, t));
+    float4 r0 = S.charts[c * 2u], r1 = S.charts[c * 2u + 1u];
+    float x = r0.x * p.x + r0.y * p.y + r0.z * p.z + r0.w * 1.0f;
+    float y = r1.x * p.x + r1.y * p.y + r1.z * p.z + r1.w * 1.0f;
+    float fx = floorf(x), fy = floorf(y);
+    int hx = (int)P.atlas[0] - 1, hy = (int)P.atlas[1] - 1;
+    int ix = !(fx >= 0.0f) ? 0 : (fx >= (float)hx ? hx : (int)fx);
+    int iy = !(fy >= 0.0f) ? 0 : (fy >= (float)hy ? hy : (int)fy);
+    return (uint32_t)iy * P.atlas[0] + (uint32_t)ix;
+}
+
+// Ray r of gather item i, traced as trace.wgsl:248-262 traces it (item keys from 0, as the bake
+// runs the pass), one lane a ray; its record at r * n + i.
+__global__ __launch_bounds__(BLOCK) void k_record(SceneView S, lb_params P,
+                                                  const lb_texel *__restrict__ items, uint32_t n,
+                                                  uint32_t rays, uint2 *__restrict__ rec) {
+    uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= (uint64_t)n * rays) return;
+    uint32_t r = (uint32_t)(t / n), i = (uint32_t)(t % n);
+    const float4 *tp = reinterpret_cast<const float4 *>(items + i);
+    float4 tpos = tp[0], tnrm = tp[1], tdx = tp[2], tdy = tp[3];
+    f3 nrm = xyz(tnrm);
+    frame_t f = frame(nrm);
+    uint32_t key = i * 7919u;
+    uint32_t seed = fbits(P.reach[3]);
+    float s1 = rnd(key, 11u, seed), s2 = rnd(key, 13u, seed);
+    float jx = rnd(key + r, 19u, seed) - 0.5f;
+    float jy = rnd(key + r, 23u, seed) - 0.5f;
+    f3 o = add(add(add(xyz(tpos), scl(xyz(tdx), jx)), scl(xyz(tdy), jy)), scl(nrm, P.reach[2]));
+    f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
+    Counts cnt{0, 0, 0, 0, 0, 0};
+    Hit h = trace_plain<false, false>(S, P, o, d, P.reach[1], key + r + 4000u, cnt);
+    uint2 out;
+    if (h.tri == NONE) {
+        out = make_uint2(fbits(d.y), REC_MISS << 30);
+    } else if (h.back) {
+        out = make_uint2(0u, REC_BACK << 30);
+    } else {
+        uint32_t m = fbits(S.tris[h.tri * 3u].w), c = fbits(S.tris[h.tri * 3u + 1u].w);
+        out = c == NONE ? make_uint2(0u, (REC_FALLBACK << 30) | m)
+                        : make_uint2(hit_texel(S, P, o, d, h.t, c), (REC_ATLAS << 30) | m);
+    }
+    rec[t] = out;
+}
+
+// The gather's result from the records: trace.wgsl:256-269's sum in ray order, with its
+// arithmetic (sky = mix(horizon, zenith, y) or ground; bounced = albedo / pi * light or the
+// fallback where the atlas has none).
+__global__ __launch_bounds__(BLOCK) void k_replay(const uint2 *__restrict__ rec, uint32_t n,
+                                                  uint32_t rays, const float4 *__restrict__ albedo,
+                                                  const float4 *__restrict__ light, SkyP K,
+                                                  float4 *__restrict__ out) {
+    uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
+    if (g >= n) return;
+    f3 sum = mk(0.0f, 0.0f, 0.0f);
+    float back = 0.0f;
+    for (uint32_t r = 0; r < rays; r++) {
+        uint2 q = rec[(size_t)r * n + g];
+        uint32_t kind = q.y >> 30;
+        if (kind == REC_MISS) {
+            float y = bitsf(q.x);
+            sum = add(sum, y < 0.0f ? K.ground : add(scl(K.horizon, 1.0f - y), scl(K.zenith, y)));
+            continue;
+        }
+        if (kind == REC_BACK) {
+            back += 1.0f;
+            continue;
+        }
+        f3 e = K.fallback;
+        if (kind == REC_ATLAS) {
+            float4 l = light[q.x];
+            if (l.w > 0.5f) e = f3of(l);
+        }
+        float4 a = albedo[q.y & 0x3fffffffu];
+        sum = add(sum, mul(mk(a.x / PI, a.y / PI, a.z / PI), e));
+    }
+    out[g] = make_float4(sum.x * PI / (float)rays, sum.y * PI / (float)rays,
+                         sum.z * PI / (float)rays, back / (float)rays);
+}
+
+// One ring of filter.rs:58-94 on the masks alone: the ring (level) at which each texel gets a
+// value and how many neighbours it averages. level -1: a value from the start.
+__global__ void k_fill_plan(TexelView V, int8_t *level, uint8_t *count, int ring, uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n || level[g] != LEVEL_NEVER) return;
+    Where p = where(V, g);
+    uint32_t ci = V.chart[g];
+    int no = 0, na = 0;
+    int y0 = max((int)p.y - 1, 0), y1 = min((int)p.y + 2, (int)p.h);
+    int x0 = max((int)p.x - 1, 0), x1 = min((int)p.x + 2, (int)p.w);
+    for (int yy = y0; yy < y1; yy++)
+        for (int xx = x0; xx < x1; xx++) {
+            uint32_t j = p.off + (uint32_t)yy * p.w + (uint32_t)xx;
+            // had a value before this ring (a texel this ring fills reads as not yet)
+            if (!(level[j] < ring)) continue;
+            if (ci != NONE && V.chart[j] == ci) no++;
+            else na++;
+        }
+    if (no > 0) {
+        level[g] = (int8_t)ring;
+        count[g] = (uint8_t)no;
+    } else if (na > 0 && ci == NONE) {
+        level[g] = (int8_t)ring;
+        count[g] = (uint8_t)na;
+    }
+}
+
+__global__ void k_level_init(const uint8_t *__restrict__ valid, int8_t *level, uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) level[g] = valid[g] ? (int8_t)-1 : LEVEL_NEVER;
+}
+
+// The transpose of one fill ring: a texel the ring filled hands its adjoint to the neighbours it
+// averaged, 1 / count each; every other texel keeps its own.
+__global__ void k_fill_t(TexelView V, const int8_t *__restrict__ level,
+                         const uint8_t *__restrict__ count, int ring,
+                         const float4 *__restrict__ din, float4 *__restrict__ dout, uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int lv = level[g];
+    f3 acc = lv == ring ? mk(0.0f, 0.0f, 0.0f) : f3of(din[g]);
+    if (lv < ring) {
+        Where p = where(V, g);
+        uint32_t cg = V.chart[g];
+        int y0 = max((int)p.y - 1, 0), y1 = min((int)p.y + 2, (int)p.h);
+        int x0 = max((int)p.x - 1, 0), x1 = min((int)p.x + 2, (int)p.w);
+        for (int yy = y0; yy < y1; yy++)
+            for (int xx = x0; xx < x1; xx++) {
+                uint32_t i = p.off + (uint32_t)yy * p.w + (uint32_t)xx;
+                if (level[i] != ring) continue;
+                uint32_t ci = V.chart[i];
+                // a chart's texel averages only its chart's neighbours; an empty one any
+                if (ci != NONE && ci != cg) continue;
+                acc = add(acc, scl(f3of(din[i]), 1.0f / (float)count[i]));
+            }
+    }
+    dout[g] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+}
+
+// filter.rs:20-53's weight sums: texel g's Gaussian weights over its valid same-surface window
+__global__ void k_wsum(TexelView V, const uint8_t *__restrict__ valid, int r,
+                       const float *__restrict__ w, float *__restrict__ wsum, uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    float s = 1.0f;
+    if (valid[g] && r > 0) {
+        Where p = where(V, g);
+        s = 0.0f;
+        int y0 = max((int)p.y - r, 0), y1 = min((int)p.y + r + 1, (int)p.h);
+        int x0 = max((int)p.x - r, 0), x1 = min((int)p.x + r + 1, (int)p.w);
+        for (int yy = y0; yy < y1; yy++)
+            for (int xx = x0; xx < x1; xx++) {
+                uint32_t j = p.off + (uint32_t)yy * p.w + (uint32_t)xx;
+                if (!valid[j] || !same_surface(V, g, j)) continue;
+                s += w[(yy - (int)p.y + r) * (2 * r + 1) + (xx - (int)p.x + r)];
+            }
+    }
+    wsum[g] = s;
+}
+
+// The transpose of filter.rs:20-53: a valid texel j gets k(g, j) / wsum(g) of each valid
+// same-surface g whose window holds it (windows and the surface test are symmetric); an invalid
+// texel passed through unchanged.
+__global__ void k_smooth_t(TexelView V, const uint8_t *__restrict__ valid, int r,
+                           const float *__restrict__ w, const float *__restrict__ wsum,
+                           const float4 *__restrict__ din, float4 *__restrict__ dout, uint32_t n) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    if (!valid[j] || r <= 0) {
+        float4 v = din[j];
+        dout[j] = make_float4(v.x, v.y, v.z, 0.0f);
+        return;
+    }
+    Where p = where(V, j);
+    f3 acc = mk(0.0f, 0.0f, 0.0f);
+    int y0 = max((int)p.y - r, 0), y1 = min((int)p.y + r + 1, (int)p.h);
+    int x0 = max((int)p.x - r, 0), x1 = min((int)p.x + r + 1, (int)p.w);
+    for (int yy = y0; yy < y1; yy++)
+        for (int xx = x0; xx < x1; xx++) {
+            uint32_t g = p.off + (uint32_t)yy * p.w + (uint32_t)xx;
+            if (!valid[g] || !same_surface(V, g, j)) continue;
+            float k = w[((int)p.y - yy + r) * (2 * r + 1) + ((int)p.x - xx + r)];
+            acc = add(acc, scl(f3of(din[g]), k / wsum[g]));
+        }
+    dout[j] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+}
+
+// write_light's transpose (bake.rs:435-444): a known texel's direct and indirect each get the
+// adjoint of its atlas texel; `di` (when given) is set, `dd` accumulates.
+__global__ void k_light_t(TexelView V, const double *__restrict__ dl, uint32_t aw,
+                          float4 *__restrict__ di, float4 *__restrict__ dd, uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    f3 v = mk(0.0f, 0.0f, 0.0f);
+    if (V.known[g] >= 1) {
+        Where p = where(V, g);
+        uint32_t corner = V.tiles[p.tile].w;
+        size_t a = (size_t)((corner >> 16) + p.y) * aw + (corner & 0xffffu) + p.x;
+        v = mk((float)dl[a * 3], (float)dl[a * 3 + 1], (float)dl[a * 3 + 2]);
+    }
+    if (di) di[g] = make_float4(v.x, v.y, v.z, 0.0f);
+    float4 o = dd[g];
+    dd[g] = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, 0.0f);
+}
+
+__global__ void k_to_f64(const float4 *__restrict__ src, double *__restrict__ dst, uint32_t n) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    float4 v = src[a];
+    dst[(size_t)a * 3] = v.x;
+    dst[(size_t)a * 3 + 1] = v.y;
+    dst[(size_t)a * 3 + 2] = v.z;
+}
+
+__global__ void k_zero_f64(double *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_zero_f4(float4 *p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// The gather's transpose for one bounce. Item g's result E = pi / rays * sum over its rays; with
+// dE its adjoint, each ray hands on
+//   sky:        pi / rays * dE * (y, 1 - y) to zenith and horizon (y >= 0), to ground (y < 0)
+//   atlas hit:  dE * light / rays to its material's albedo, dE * albedo / rays to the texel
+//   fallback:   dE * fallback / rays to the albedo, dE * albedo / rays * pi / 2 to zenith and
+//               horizon (fallback = pi (zenith + horizon) / 2)
+// Parameter sums go to this block's row of `part` (albedo rgb per material, then zenith, horizon,
+// ground): LDS accumulators in FP64, added to the row, so the rows of several bounces add up.
+__global__ __launch_bounds__(BLOCK) void k_gather_t(
+    const uint2 *__restrict__ rec, uint32_t n, uint32_t rays, const uint32_t *__restrict__ gat,
+    const float4 *__restrict__ de, const float4 *__restrict__ albedo,
+    const float4 *__restrict__ light, SkyP K, double *__restrict__ dl, double *__restrict__ part,
+    uint32_t width) {
+    extern __shared__ double acc[];
+    for (uint32_t k = threadIdx.x; k < width; k += BLOCK) acc[k] = 0.0;
+    __syncthreads();
+    uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
+    if (g < n) {
+        f3 dE = f3of(de[gat[g]]);
+        float inv = 1.0f / (float)rays;
+        float c = PI / (float)rays;
+        f3 gz = mk(0, 0, 0), gh = mk(0, 0, 0), gg = mk(0, 0, 0);
+        uint32_t M3 = width - 9u;
+        for (uint32_t r = 0; r < rays; r++) {
+            uint2 q = rec[(size_t)r * n + g];
+            uint32_t kind = q.y >> 30;
+            if (kind == REC_BACK) continue;
+            if (kind == REC_MISS) {
+                float y = bitsf(q.x);
+                if (y < 0.0f) {
+                    gg = add(gg, scl(dE, c));
+                } else {
+                    gz = add(gz, scl(dE, c * y));
+                    gh = add(gh, scl(dE, c * (1.0f - y)));
+                }
+                continue;
+            }
+            uint32_t m = q.y & 0x3fffffffu;
+            f3 a = f3of(albedo[m]);
+            f3 da = scl(dE, inv);  // dE / rays
+            bool lit = false;
+            if (kind == REC_ATLAS) {
+                float4 l = light[q.x];
+                if (l.w > 0.5f) {
+                    lit = true;
+                    f3 e = f3of(l);
+                    f3 ga = mul(da, e);
+                    atomicAdd(&acc[m * 3u], (double)ga.x);
+                    atomicAdd(&acc[m * 3u + 1u], (double)ga.y);
+                    atomicAdd(&acc[m * 3u + 2u], (double)ga.z);
+                    f3 gl = mul(da, a);
+                    unsafeAtomicAdd(&dl[(size_t)q.x * 3], (double)gl.x);
+                    unsafeAtomicAdd(&dl[(size_t)q.x * 3 + 1], (double)gl.y);
+                    unsafeAtomicAdd(&dl[(size_t)q.x * 3 + 2], (double)gl.z);
+                }
+            }
+            if (!lit) {
+                f3 ga = mul(da, K.fallback);
+                atomicAdd(&acc[m * 3u], (double)ga.x);
+                atomicAdd(&acc[m * 3u + 1u], (double)ga.y);
+                atomicAdd(&acc[m * 3u + 2u], (double)ga.z);
+                f3 gf = scl(mul(da, a), 0.5f * PI);
+                gz = add(gz, gf);
+                gh = add(gh, gf);
+            }
+        }
+        for (int k = 0; k < 3; k++) {
+            float z = k == 0 ? gz.x : (k == 1 ? gz.y : gz.z);
+            float h = k == 0 ? gh.x : (k == 1 ? gh.y : gh.z);
+            float q = k == 0 ? gg.x : (k == 1 ? gg.y : gg.z);
+            atomicAdd(&acc[M3 + k], (double)z);
+            atomicAdd(&acc[M3 + 3 + k], (double)h);
+            atomicAdd(&acc[M3 + 6 + k], (double)q);
+        }
+    }
+    __syncthreads();
+    double *row = part + (size_t)blockIdx.x * width;
+    for (uint32_t k = threadIdx.x; k < width; k += BLOCK) row[k] += acc[k];
+}
+
+// The sun's share: sum over texels of dD . G (G the filled cos x sun share), a block's partial
+// sums of the three channels in its row of `part` (a fixed-order tree in FP64).
+__global__ __launch_bounds__(BLOCK) void k_sun_t(const float4 *__restrict__ dd,
+                                                 const float4 *__restrict__ G, uint32_t n,
+                                                 double *__restrict__ part) {
+    __shared__ double s[3][BLOCK];
+    uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (g < n) {
+        float4 d = dd[g];
+        double k = G[g].x;
+        v[0] = d.x * k;
+        v[1] = d.y * k;
+        v[2] = d.z * k;
+    }
+    for (int c = 0; c < 3; c++) s[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (uint32_t h = BLOCK / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int c = 0; c < 3; c++) s[c][threadIdx.x] += s[c][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[(size_t)blockIdx.x * 3 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// out[k] = sum over rows of part[row * width + k], rows in order
+__global__ void k_rows(const double *__restrict__ part, uint32_t rows, uint32_t width,
+                       float *__restrict__ out) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= width) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < rows; b++) s += part[(size_t)b * width + k];
+    out[k] = (float)s;
+}
+
+inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
+
+}  // namespace
+}  // namespace lbk
+
+struct lb_adjoint {
+    lb_bake *B = nullptr;
+    uint32_t ng = 0, rays = 0, bounces = 0, M = 0, T = 0;
+    size_t atlas = 0;
+    lbk::DevBuf rec, level, count, wsum, soft1, G, D, I, F4, E, m1, m2, albedo, ckpt, dl, dI, dI2,
+        dD, part_g, part_s;
+    lbk::SkyP sky{};
+    uint32_t blocks_g = 0, blocks_s = 0;
+
+    float4 *ck(uint32_t k) const { return (float4 *)ckpt.ptr + (size_t)k * atlas; }
+};
+
+namespace lbk {
+namespace {
+
+// filter.rs:58-94 over `rings` rings, 3 channels of float4 `v`, from `valid0`, masks in m1/m2
+void fill3(lb_adjoint *A, const TexelView &V, float4 *v, uint32_t rings, const uint8_t *valid0,
+           hipStream_t st) {
+    uint8_t *ma = (uint8_t *)A->m1.ptr, *mb = (uint8_t *)A->m2.ptr;
+    tp_copy_u8(valid0, ma, A->T, st);
+    for (uint32_t r = 0; r < rings; r++) {
+        tp_fill_ring(V, 3, (float *)v, ma, mb, st);
+        std::swap(ma, mb);
+    }
+}
+
+TexelView view_with(const lb_adjoint *A, float4 *direct, float4 *indirect) {
+    TexelView V = A->B->V;
+    V.direct = direct;
+    V.indirect = indirect;
+    return V;
+}
+
+int adjoint_create(lb_bake *B, lb_adjoint *A) {
+    lb_gpu *gpu = B->gpu;
+    hipStream_t st = gpu->stream;
+    hipSetDevice(gpu->device);
+    if (!B->scene) return fail("lb_adjoint_create: the bake is not prepared");
+    if (B->sc.n_materials >= (1u << 30)) return fail("lb_adjoint_create: too many materials");
+    A->B = B;
+    A->ng = (uint32_t)B->ng;
+    A->rays = B->P.counts[2];
+    A->bounces = B->sc.gi_bounces;
+    A->M = (uint32_t)B->sc.n_materials;
+    A->T = (uint32_t)B->T;
+    A->atlas = (size_t)B->aw * B->ah;
+    if (B->grings >= (uint32_t)LEVEL_NEVER) return fail("lb_adjoint_create: %u fill rings", B->grings);
+    uint32_t width = A->M * 3u + 9u;
+    if ((size_t)width * sizeof(double) > 64 * 1024)
+        return fail("lb_adjoint_create: %u materials exceed the LDS accumulators", A->M);
+    uint64_t nrays = (uint64_t)A->ng * A->rays;
+    if (nrays >= (1ull << 40)) return fail("lb_adjoint_create: %llu rays", (unsigned long long)nrays);
+    size_t Tn = std::max<size_t>(A->T, 1), Gn = std::max<size_t>(A->ng, 1);
+    if (A->rec.reserve(std::max<uint64_t>(nrays, 1) * 8) || A->level.reserve(Tn) ||
+        A->count.reserve(Tn) || A->wsum.reserve(Tn * 4) || A->soft1.reserve(Tn * 4) ||
+        A->G.reserve(Tn * 16) || A->D.reserve(Tn * 16) || A->I.reserve(Tn * 16) ||
+        A->F4.reserve(Tn * 16) || A->E.reserve(Gn * 16) || A->m1.reserve(Tn) ||
+        A->m2.reserve(Tn) || A->albedo.reserve(std::max<uint32_t>(A->M, 1) * 16) ||
+        A->ckpt.reserve(A->atlas * 16 * (A->bounces + 1)) || A->dl.reserve(A->atlas * 24) ||
+        A->dI.reserve(Tn * 16) || A->dI2.reserve(Tn * 16) || A->dD.reserve(Tn * 16))
+        return 1;
+    A->blocks_g = (uint32_t)((Gn + BLOCK - 1) / BLOCK);
+    A->blocks_s = (uint32_t)((Tn + BLOCK - 1) / BLOCK);
+    if (A->part_g.reserve((size_t)A->blocks_g * width * 8) || A->part_s.reserve((size_t)A->blocks_s * 24))
+        return 1;
+    const TexelView &V = B->V;
+
+    // the records (the gather items are the bake's, resident since its execute)
+    SceneView S = scene_view(B->scene, nullptr);
+    if (nrays)
+        hipLaunchKernelGGL(k_record, grid_for(nrays), dim3(BLOCK), 0, st, S, B->P,
+                           (const lb_texel *)B->d_gitems.ptr, A->ng, A->rays, (uint2 *)A->rec.ptr);
+
+    // the bounce fill's plan and the bounce smoothing's weight sums (bake.rs:318-320)
+    if (A->T) {
+        hipLaunchKernelGGL(k_level_init, grid_for(A->T), dim3(BLOCK), 0, st,
+                           (const uint8_t *)B->d_gmask.ptr, (int8_t *)A->level.ptr, A->T);
+        for (uint32_t r = 0; r < B->grings; r++)
+            hipLaunchKernelGGL(k_fill_plan, grid_for(A->T), dim3(BLOCK), 0, st, V,
+                               (int8_t *)A->level.ptr, (uint8_t *)A->count.ptr, (int)r, A->T);
+        hipLaunchKernelGGL(k_wsum, grid_for(A->T), dim3(BLOCK), 0, st, V,
+                           (const uint8_t *)B->d_gmask.ptr, (int)B->rg, (const float *)B->d_wg.ptr,
+                           (float *)A->wsum.ptr, A->T);
+        // the smoothed sun share, and G: the direct light at unit irradiance (bake.rs:243-256)
+        tp_smooth(V, 1, V.soft, (float *)A->soft1.ptr, V.valid, 1, (const float *)B->d_w1.ptr, st);
+        TexelView VG = view_with(A, (float4 *)A->G.ptr, (float4 *)A->I.ptr);
+        tp_direct_light(VG, (const float *)A->soft1.ptr, make_float3(1.0f, 1.0f, 1.0f),
+                        make_float3(B->sun.x, B->sun.y, B->sun.z), st);
+        fill3(A, VG, (float4 *)A->G.ptr, B->rings, V.valid, st);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e == hipSuccess ? 0 : hip_fail(e, "lb_adjoint_create");
+}
+
+int adjoint_forward(lb_adjoint *A, const lb_light_params &p, float *light) {
+    lb_bake *B = A->B;
+    hipStream_t st = B->gpu->stream;
+    hipSetDevice(B->gpu->device);
+    if (p.n_materials != A->M)
+        return fail("lb_adjoint_forward: %zu materials, the scene has %u", p.n_materials, A->M);
+    if (A->M && !p.albedo) return fail("lb_adjoint_forward: no albedo");
+    std::vector<float4> alb(std::max<uint32_t>(A->M, 1), make_float4(0, 0, 0, 0));
+    for (uint32_t m = 0; m < A->M; m++)
+        alb[m] = make_float4(p.albedo[m * 3], p.albedo[m * 3 + 1], p.albedo[m * 3 + 2], 0.0f);
+    if (A->albedo.upload(alb.data(), alb.size() * 16, st)) return 1;
+    const float PIF = 3.14159265358979323846f;
+    SkyP &K = A->sky;
+    K.zenith = f3{p.zenith[0], p.zenith[1], p.zenith[2]};
+    K.horizon = f3{p.horizon[0], p.horizon[1], p.horizon[2]};
+    K.ground = f3{p.ground[0], p.ground[1], p.ground[2]};
+    // bake.rs:106-116: fallback = (zenith + horizon) * 0.5 * pi, in that order
+    K.fallback = f3{(p.zenith[0] + p.horizon[0]) * 0.5f * PIF,
+                    (p.zenith[1] + p.horizon[1]) * 0.5f * PIF,
+                    (p.zenith[2] + p.horizon[2]) * 0.5f * PIF};
+    const TexelView &V0 = B->V;
+    TexelView V = view_with(A, (float4 *)A->D.ptr, (float4 *)A->I.ptr);
+    uint32_t T = A->T;
+    size_t bytes = A->atlas * 16;
+    // the direct light (bake.rs:247-256) at this irradiance
+    if (T) {
+        tp_direct_light(V, (const float *)A->soft1.ptr, make_float3(p.sun[0], p.sun[1], p.sun[2]),
+                        make_float3(B->sun.x, B->sun.y, B->sun.z), st);
+        fill3(A, V, (float4 *)A->D.ptr, B->rings, V0.valid, st);
+        tp_fill_f32((float *)A->I.ptr, 0.0f, T * 4u, st);
+    }
+    // the first atlas: the clean tiles' light, then every dirty texel's (bake.rs:276-278)
+    hipError_t e;
+    if (B->any_clean)
+        e = hipMemcpyAsync(A->ck(0), B->d_light0.ptr, bytes, hipMemcpyDeviceToDevice, st);
+    else
+        e = hipMemsetAsync(A->ck(0), 0, bytes, st);
+    if (e != hipSuccess) return hip_fail(e, "lb_adjoint_forward");
+    tp_write_light(V, A->ck(0), B->aw, st);
+    for (uint32_t k = 1; k <= A->bounces && A->ng; k++) {
+        e = hipMemcpyAsync(A->ck(k), A->ck(k - 1), bytes, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return hip_fail(e, "lb_adjoint_forward");
+        hipLaunchKernelGGL(k_replay, grid_for(A->ng), dim3(BLOCK), 0, st,
+                           (const uint2 *)A->rec.ptr, A->ng, A->rays, (const float4 *)A->albedo.ptr,
+                           A->ck(k - 1), K, (float4 *)A->E.ptr);
+        // bake.rs:301-328 as the bake's bounce runs it
+        tp_fill_f32((float *)A->I.ptr, 0.0f, T * 4u, st);
+        tp_scatter((const float4 *)A->E.ptr, (const uint32_t *)B->d_gat.ptr, A->ng,
+                   (float4 *)A->I.ptr, st);
+        tp_smooth(V, 4, (const float *)A->I.ptr, (float *)A->F4.ptr,
+                  (const uint8_t *)B->d_gmask.ptr, (int)B->rg, (const float *)B->d_wg.ptr, st);
+        tp_copy_f32((const float *)A->F4.ptr, (float *)A->I.ptr, T * 4u, st);
+        fill3(A, V, (float4 *)A->I.ptr, B->grings, (const uint8_t *)B->d_gmask.ptr, st);
+        tp_write_light(V, A->ck(k), B->aw, st);
+    }
+    uint32_t last = A->ng ? A->bounces : 0;
+    if (light) {
+        e = hipMemcpyAsync(light, A->ck(last), bytes, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return hip_fail(e, "lb_adjoint_forward");
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e == hipSuccess ? 0 : hip_fail(e, "lb_adjoint_forward");
+}
+
+int adjoint_backward(lb_adjoint *A, const float *dlight, float *grad) {
+    lb_bake *B = A->B;
+    hipStream_t st = B->gpu->stream;
+    hipSetDevice(B->gpu->device);
+    const TexelView &V = B->V;
+    uint32_t T = A->T, width = A->M * 3u + 9u;
+    double *dl = (double *)A->dl.ptr;
+    float4 *dI = (float4 *)A->dI.ptr, *dI2 = (float4 *)A->dI2.ptr, *dD = (float4 *)A->dD.ptr;
+    size_t pg = (size_t)A->blocks_g * width, ps = (size_t)A->blocks_s * 3;
+    hipLaunchKernelGGL(k_zero_f64, grid_for(pg), dim3(BLOCK), 0, st, (double *)A->part_g.ptr, pg);
+    if (T) hipLaunchKernelGGL(k_zero_f4, grid_for(T), dim3(BLOCK), 0, st, dD, T);
+    hipLaunchKernelGGL(k_to_f64, grid_for(A->atlas), dim3(BLOCK), 0, st, (const float4 *)dlight,
+                       dl, (uint32_t)A->atlas);
+    uint32_t last = A->ng ? A->bounces : 0;
+    for (uint32_t k = last; k >= 1; k--) {
+        if (T) {
+            hipLaunchKernelGGL(k_light_t, grid_for(T), dim3(BLOCK), 0, st, V, (const double *)dl,
+                               B->aw, dI, dD, T);
+            for (int r = (int)B->grings - 1; r >= 0; r--) {
+                hipLaunchKernelGGL(k_fill_t, grid_for(T), dim3(BLOCK), 0, st, V,
+                                   (const int8_t *)A->level.ptr, (const uint8_t *)A->count.ptr, r,
+                                   (const float4 *)dI, dI2, T);
+                std::swap(dI, dI2);
+            }
+            hipLaunchKernelGGL(k_smooth_t, grid_for(T), dim3(BLOCK), 0, st, V,
+                               (const uint8_t *)B->d_gmask.ptr, (int)B->rg,
+                               (const float *)B->d_wg.ptr, (const float *)A->wsum.ptr,
+                               (const float4 *)dI, dI2, T);
+            std::swap(dI, dI2);
+        }
+        hipLaunchKernelGGL(k_zero_f64, grid_for(A->atlas * 3), dim3(BLOCK), 0, st, dl, A->atlas * 3);
+        hipLaunchKernelGGL(k_gather_t, dim3(A->blocks_g), dim3(BLOCK), width * sizeof(double), st,
+                           (const uint2 *)A->rec.ptr, A->ng, A->rays,
+                           (const uint32_t *)B->d_gat.ptr, (const float4 *)dI,
+                           (const float4 *)A->albedo.ptr, (const float4 *)A->ck(k - 1), A->sky,
+                           dl, (double *)A->part_g.ptr, width);
+    }
+    if (T) {
+        hipLaunchKernelGGL(k_light_t, grid_for(T), dim3(BLOCK), 0, st, V, (const double *)dl,
+                           B->aw, (float4 *)nullptr, dD, T);
+        hipLaunchKernelGGL(k_sun_t, dim3(A->blocks_s), dim3(BLOCK), 0, st, (const float4 *)dD,
+                           (const float4 *)A->G.ptr, T, (double *)A->part_s.ptr);
+    } else {
+        hipLaunchKernelGGL(k_zero_f64, grid_for(ps), dim3(BLOCK), 0, st, (double *)A->part_s.ptr, ps);
+    }
+    hipLaunchKernelGGL(k_rows, grid_for(width), dim3(BLOCK), 0, st, (const double *)A->part_g.ptr,
+                       A->blocks_g, width, grad);
+    hipLaunchKernelGGL(k_rows, dim3(1), dim3(BLOCK), 0, st, (const double *)A->part_s.ptr,
+                       T ? A->blocks_s : 1u, 3u, grad + width);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e == hipSuccess ? 0 : hip_fail(e, "lb_adjoint_backward");
+}
+
+}  // namespace
+}  // namespace lbk
+
+extern "C" {
+
+int lb_adjoint_create(lb_bake *bake, lb_adjoint **out) {
+    if (!bake || !out) return fail("lb_adjoint_create: null handle");
+    *out = nullptr;
+    auto *a = new lb_adjoint();
+    if (lbk::adjoint_create(bake, a)) {
+        delete a;
+        return 1;
+    }
+    *out = a;
+    return 0;
+}
+
+void lb_adjoint_destroy(lb_adjoint *adj) {
+    if (adj && adj->B && adj->B->gpu) hipStreamSynchronize(adj->B->gpu->stream);
+    delete adj;
+}
+
+int lb_adjoint_info(const lb_adjoint *adj, size_t *atlas_texels, size_t *n_params, size_t *items,
+                    size_t *rays) {
+    if (!adj) return fail("lb_adjoint_info: null handle");
+    if (atlas_texels) *atlas_texels = adj->atlas;
+    if (n_params) *n_params = (size_t)adj->M * 3 + 12;
+    if (items) *items = adj->ng;
+    if (rays) *rays = adj->rays;
+    return 0;
+}
+
+int lb_adjoint_records(const lb_adjoint *adj, uint32_t *out, size_t cap) {
+    if (!adj || !out) return fail("lb_adjoint_records: null handle");
+    size_t words = (size_t)adj->ng * adj->rays * 2;
+    if (cap < words) return fail("lb_adjoint_records: %zu words, %zu needed", cap, words);
+    if (!words) return 0;
+    hipError_t e = hipMemcpy(out, adj->rec.ptr, words * 4, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? 0 : hip_fail(e, "lb_adjoint_records");
+}
+
+int lb_adjoint_forward(lb_adjoint *adj, const lb_light_params *params, float *light) {
+    if (!adj || !params) return fail("lb_adjoint_forward: null handle");
+    return lbk::adjoint_forward(adj, *params, light);
+}
+
+int lb_adjoint_backward(lb_adjoint *adj, const float *dlight, float *grad) {
+    if (!adj || !dlight || !grad) return fail("lb_adjoint_backward: null handle");
+    return lbk::adjoint_backward(adj, dlight, grad);
+}
+
+}  // extern "C"

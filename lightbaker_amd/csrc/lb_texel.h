@@ -37,6 +37,29 @@ struct TexelView {
     float4 *indirect;        // rgb
 };
 
+// filter.rs:13-16: texels i and j lie on one surface (same chart, normals within ~25 degrees)
+__device__ __forceinline__ bool same_surface(const TexelView &V, uint32_t i, uint32_t j) {
+    float4 a = V.normal[i], b = V.normal[j];
+    return V.chart[i] == V.chart[j] && a.x * b.x + a.y * b.y + a.z * b.z >= 0.9f;
+}
+
+// A texel's tile and its place in it.
+struct Where {
+    uint32_t tile, off, w, h, x, y;
+};
+__device__ __forceinline__ Where where(const TexelView &V, uint32_t g) {
+    Where r;
+    r.tile = V.tile_of[g];
+    uint4 t = V.tiles[r.tile];  // off, w, h, atlas corner (x | y << 16)
+    r.off = t.x;
+    r.w = t.y;
+    r.h = t.z;
+    uint32_t i = g - r.off;
+    r.x = i % r.w;
+    r.y = i / r.w;
+    return r;
+}
+
 struct DecodeArgs {
     float inside_share, ao_strength, ao_contrast, ao_floor;
 };

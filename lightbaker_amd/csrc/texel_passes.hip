@@ -14,32 +14,6 @@
 
 namespace lbk {
 
-namespace {
-
-__device__ __forceinline__ bool same_surface(const TexelView &V, uint32_t i, uint32_t j) {
-    // filter.rs:13-16
-    float4 a = V.normal[i], b = V.normal[j];
-    return V.chart[i] == V.chart[j] && a.x * b.x + a.y * b.y + a.z * b.z >= 0.9f;
-}
-
-struct Where {
-    uint32_t tile, off, w, h, x, y;
-};
-__device__ __forceinline__ Where where(const TexelView &V, uint32_t g) {
-    Where r;
-    r.tile = V.tile_of[g];
-    uint4 t = V.tiles[r.tile];  // off, w, h, atlas corner (x | y << 16)
-    r.off = t.x;
-    r.w = t.y;
-    r.h = t.z;
-    uint32_t i = g - r.off;
-    r.x = i % r.w;
-    r.y = i / r.w;
-    return r;
-}
-
-}  // namespace
-
 // bake.rs:207-241 for the covered item k (texel cov[k])
 __global__ void k_decode(TexelView V, const float4 *__restrict__ res, const uint32_t *cov,
                          uint32_t n_cov, DecodeArgs a) {

@@ -199,6 +199,33 @@ def small_scene(seed: int = 1) -> Scene:
         settings=st, volume=Volume((-10.0, 0.0, -10.0), (10.0, 8.0, 10.0), 1.0, 64))
 
 
+def fit_scene(seed: int = 5) -> Scene:
+    """small_scene's layout (a 20 x 20 m ground and 50 yawed boxes) with 16 materials (albedo
+    U[0.2, 0.8] per channel; the ground is material 0, each box one of the 16; material 15
+    receives no light map, so rays that meet it take the fallback irradiance), a sky graded from
+    horizon to zenith over a darker ground, and 3 bounces: the scene light is fitted on."""
+    rng = np.random.default_rng(seed)
+    ground = np.array(quad((-10, 0, -10), (-10, 0, 10), (10, 0, 10), (10, 0, -10)), np.float32)
+    meshes = [Mesh("ground", 0, ground)]
+    for k in range(50):
+        sx, sy, sz = rng.uniform(0.5, 4.0, 3)
+        cx, cz = rng.uniform(-9.0, 9.0, 2)
+        geo = yawed_box(cx, cz, 0.0, sx, sy, sz, rng.uniform(0, 2 * math.pi))
+        meshes.append(Mesh(f"box{k}", int(rng.integers(0, 16)), geo))
+    mats = [Material(f"m{i}", tuple(float(v) for v in rng.uniform(0.2, 0.8, 3)),
+                     receives=i != 15)
+            for i in range(16)]
+    az = rng.uniform(0, 2 * math.pi)
+    el = math.radians(45.0)
+    sun = (math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az))
+    st = Settings(texels_per_meter=4.0, tile_size=16.0, atlas_width=1024, sun_samples=16,
+                  ao_rays=64, gi_rays=64, gi_bounces=3)
+    return Scene(
+        sun_direction=sun, sun_radius_deg=0.5, sun_irradiance=(3.0, 3.0, 3.0),
+        zenith=(0.35, 0.45, 0.6), horizon=(0.5, 0.5, 0.5), ground=(0.25, 0.22, 0.2),
+        materials=mats, meshes=meshes, settings=st, volume=None)
+
+
 def rebake_scene(seed: int = 3, size: float = 200.0) -> Scene:
     """A size x size m map built like small_scene (yawed boxes, edges U[0.5, 4] m, on a ground of
     5 m quads, grey albedo 0.5, constant sky 0.5, sun at 45 degrees), in 25 m tiles whose box

@@ -203,7 +203,7 @@ def bake(inp: dict) -> dict:
                      light.reshape(-1, 4)).reshape(-1, 16)
     return {"light": light, "known": known, "state": state, "field": field, "ao": ao,
             "direct": direct, "indirect": indirect, "probes": probes, "gathered": gat,
-            "texels": tex}
+            "texels": tex, "valid": valid, "soft": soft}
 
 
 def clean_light(tiles_all, tile_dirty, cache, aw, ah):

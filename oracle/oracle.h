@@ -34,6 +34,12 @@ int orc_run(int pass, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
             const float *materials, const float *charts, const lb_params *params,
             const lb_texel *items, size_t n, uint32_t key_base, const float *light, float *out,
             orc_counters *cnt);
+int orc_gather_records(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+                       const float *materials, const float *charts, const lb_params *params,
+                       const lb_texel *items, size_t n, uint32_t key_base, uint32_t *out);
+void orc_smooth64(int w, int h, const uint32_t *chart, const float *normal, const double *v,
+                  int C, const uint8_t *valid, int r, double *out);
+void orc_fill64(int w, int h, const uint32_t *chart, double *v, int C, uint8_t *valid, int rings);
 void orc_trace(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, const float *materials,
                const lb_params *params, const float o[3], const float d[3], float tmax, int any,
                uint32_t ray, float *t_out, uint32_t *tri_out, int *back_out, orc_counters *cnt);

@@ -476,6 +476,70 @@ int orc_run(int pass, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
     return 0;
 }
 
+/* What each gather ray's light depends on (trace.wgsl:248-268), for the adjoint's checker: per
+ * item i and ray r, at (i * rays + r) * 2: a word (kind 0: the atlas texel bounced() reads,
+ * trace.wgsl:137-145; kind 3: the bits of the direction's y; else 0) and a tag (kind << 30 |
+ * material; kinds 0 a front hit on a chart, 1 a front hit off the charts, 2 a back face, 3 the
+ * sky). */
+int orc_gather_records(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+                       const float *materials, const float *charts, const lb_params *params,
+                       const lb_texel *items, size_t n, uint32_t key_base, uint32_t *out) {
+    static const float zero_chart[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    lb_node tmp;
+    ctx_t c0;
+    c0.nodes = empty_fix(nodes, n_nodes, &tmp); c0.tris = tris; c0.materials = materials;
+    c0.charts = charts ? charts : zero_chart;
+    c0.light = NULL;
+    c0.P = params; c0.cnt = NULL;
+    uint32_t rays = params->counts[2];
+    long nn = (long)n;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < nn; i++) {
+        const ctx_t *c = &c0;
+        const lb_texel *tx = &items[i];
+        const lb_params *P = c->P;
+        v3 nrm = ld3(tx->normal);
+        frame_t f = frame(nrm);
+        uint32_t key = (key_base + (uint32_t)i) * 7919u;
+        float s1 = rnd(c, key, 11u), s2 = rnd(c, key, 13u);
+        v3 pos = ld3(tx->pos), dx = ld3(tx->dx), dy = ld3(tx->dy);
+        for (uint32_t r = 0u; r < rays; r++) {
+            float jx = rnd(c, key + r, 19u) - 0.5f;
+            float jy = rnd(c, key + r, 23u) - 0.5f;
+            v3 o = add(add(add(pos, scl(dx, jx)), scl(dy, jy)), scl(nrm, P->reach[2]));
+            float u0 = ((float)r + 0.5f) / (float)rays + s1;
+            float v0 = orc_radical(r) + s2;
+            v3 d = cosine(u0 - floorf(u0), v0 - floorf(v0), &f);
+            hit_t h = trace(c, o, d, P->reach[1], 0, key + r + 4000u);
+            uint32_t *q = out + ((size_t)i * rays + r) * 2u;
+            if (h.tri == NONE_U) {
+                q[0] = fbits(d.y);
+                q[1] = 3u << 30;
+            } else if (h.back) {
+                q[0] = 0u;
+                q[1] = 2u << 30;
+            } else {
+                const lb_tri *tr = &c->tris[h.tri];
+                uint32_t m = fbits(tr->v0[3]), ch = fbits(tr->e1[3]);
+                if (ch == NONE_U) {
+                    q[0] = 0u;
+                    q[1] = (1u << 30) | m;
+                } else {
+                    v3 p = add(o, scl(d, h.t));
+                    const float *r0 = &c->charts[ch * 8u], *r1 = &c->charts[ch * 8u + 4u];
+                    float x = r0[0] * p.x + r0[1] * p.y + r0[2] * p.z + r0[3] * 1.0f;
+                    float y = r1[0] * p.x + r1[1] * p.y + r1[2] * p.z + r1[3] * 1.0f;
+                    int ix = orc_sat_floor(x, (int)P->atlas[0] - 1);
+                    int iy = orc_sat_floor(y, (int)P->atlas[1] - 1);
+                    q[0] = (uint32_t)iy * P->atlas[0] + (uint32_t)ix;
+                    q[1] = m;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
 /* One ray through trace() (trace.wgsl:83-130): t, triangle, back. */
 void orc_trace(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, const float *materials,
                const lb_params *params, const float o[3], const float d[3], float tmax, int any,

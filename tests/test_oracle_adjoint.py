@@ -1,0 +1,76 @@
+"""The adjoint's checker (oracle/adjoint_ref.py) pinned on the CPU, before it checks the GPU:
+  * its FP64 replay of the gather records through the bounces gives the oracle's FP32 bake
+    (oracle/bake_ref.py, which traces every ray) within FP32 rounding: the records hold what the
+    gather's light depends on and the replay combines them as trace.wgsl:256-269 does;
+  * its forward-mode derivatives equal FP64 central differences of its own forward, for albedo,
+    sky and sun parameters (a dropped or misplaced tangent fails);
+  * the light is linear in the sun's irradiance and in the sky at fixed albedo (closed forms of
+    the forward's structure)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle.adjoint_ref as aref
+import oracle.bake_ref as bref
+from lightbaker_amd.scenes import fit_scene
+
+from .kit import host_bake_inputs
+
+
+@pytest.fixture(scope="module")
+def setup():
+    scene = fit_scene(5)
+    scene.settings.gi_bounces = 2
+    inp = host_bake_inputs(scene)
+    out = bref.bake(inp)
+    S = aref.setup(inp, out)
+    alb = np.array([m.albedo for m in scene.materials])
+    sky = np.array([scene.zenith, scene.horizon, scene.ground])
+    sun = np.array(scene.sun_irradiance)
+    return scene, out, S, alb, sky, sun
+
+
+def test_replay_matches_traced_bake(setup):
+    scene, out, S, alb, sky, sun = setup
+    Lt = aref.forward(S, alb, sky[0], sky[1], sky[2], sun, tangents=False)
+    got = aref.atlas_light(S, Lt)
+    want = out["light"].reshape(-1, 4).astype(np.float64)
+    assert np.array_equal(got[:, 3], want[:, 3])
+    err = np.abs(got[:, :3] - want[:, :3]).max()
+    assert err < 1e-5 * max(1.0, np.abs(want).max()), err
+
+
+def test_tangents_match_central_differences(setup):
+    scene, out, S, alb, sky, sun = setup
+    M = len(alb)
+    Lt = aref.forward(S, alb, sky[0], sky[1], sky[2], sun)
+    h = 1e-4
+
+    def value(a, s, u):
+        return aref.forward(S, a, s[0], s[1], s[2], u, tangents=False)[:, :, 0]
+
+    for p in (0, 3 * 3 + 1, 3 * 7 + 2, 3 * M + 2, 3 * M + 3, 3 * M + 7, 3 * M + 11):
+        th = [alb.copy(), sky.copy(), sun.copy()]
+        lo = [alb.copy(), sky.copy(), sun.copy()]
+        if p < 3 * M:
+            th[0].flat[p] += h
+            lo[0].flat[p] -= h
+        elif p < 3 * M + 9:
+            th[1].flat[p - 3 * M] += h
+            lo[1].flat[p - 3 * M] -= h
+        else:
+            th[2][p - 3 * M - 9] += h
+            lo[2][p - 3 * M - 9] -= h
+        fd = (value(*th) - value(*lo)) / (2 * h)
+        tg = Lt[:, :, 1 + p]
+        assert np.abs(fd - tg).max() <= 1e-6 * max(1.0, np.abs(tg).max()), p
+
+
+def test_linear_in_sun_and_sky(setup):
+    scene, out, S, alb, sky, sun = setup
+    a = aref.forward(S, alb, sky[0], sky[1], sky[2], sun, tangents=False)[:, :, 0]
+    b = aref.forward(S, alb, 2 * sky[0], 2 * sky[1], 2 * sky[2], 2 * sun, tangents=False)[:, :, 0]
+    assert np.allclose(b, 2 * a, rtol=1e-12, atol=1e-12)
+    z = aref.forward(S, alb, 0 * sky[0], 0 * sky[1], 0 * sky[2], 0 * sun, tangents=False)
+    assert np.abs(z).max() == 0.0
