@@ -35,15 +35,13 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 def node_bytes(stats: dict, form: int, nodes_layout: int) -> int:
     """Algorithmic bytes of one pass from its traversal counters, with each form's record sizes:
     PLAIN reads the node (32 B) at every visit and both children (2 x 32 B) at an inner visit
-    (trace.wgsl:90,111-112); TUNED reads one record with both children's boxes at an inner visit
-    (WIDE 2 x 32 B, QUANT 32 B) and, for QUANT, a leaf's (first, count) (8 B). Every triangle test
-    48 B, every material load 16 B, every bounced() 32 B of chart rows and 16 B of light texel."""
+    (trace.wgsl:90,111-112); TUNED reads one record with both children's boxes and references at
+    an inner visit (WIDE 2 x 32 B, QUANT 32 B) and nothing at a leaf. Every triangle test 48 B,
+    every material load 16 B, every bounced() 32 B of chart rows and 16 B of light texel."""
     if form == 0:
         nb = (stats["inner_visits"] + stats["leaf_visits"]) * 32 + stats["inner_visits"] * 64
     else:
-        # WIDE carries a leaf child's (first, count) in the parent's record; QUANT reads 8 B
         nb = stats["inner_visits"] * (64 if nodes_layout == 0 else 32)
-        nb += 0 if nodes_layout == 0 else stats["leaf_visits"] * 8
     return nb + stats["tri_tests"] * 48 + stats["material_loads"] * 16 + stats["light_gathers"] * 48
 
 
@@ -144,7 +142,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--form", choices=["tuned", "plain"], default="tuned")
-    ap.add_argument("--nodes", choices=["wide", "quant"], default="wide")
+    ap.add_argument("--nodes", choices=["wide", "quant"], default="quant")
     ap.add_argument("--schedule", type=int, choices=[0, 1], default=0,
                     help="TUNED schedule before autotuning: 0 lock-step, 1 pooled")
     ap.add_argument("--workload", choices=["map", "rebake"], default="map")

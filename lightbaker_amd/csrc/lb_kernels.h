@@ -11,7 +11,7 @@ struct Tuning {
     int workgroup = 256;     // threads a workgroup (multiple of 64)
     int rays_per_lane = 2;   // rays one lane traces of one item before the wave reduces
     int lds_stack = 16;      // traversal stack entries a lane keeps in LDS
-    int nodes = LB_NODES_WIDE;
+    int nodes = LB_NODES_QUANT;
     int refill = 32;         // pooled: idle lanes (of 64) a wave waits for before it refills
     int schedule = 0;        // 0: lock-step slots, 1: pooled (one ray pool, shaded in order)
     int chunk = 0;           // pooled: hits a chunk holds (0: the default)

@@ -71,17 +71,24 @@ __device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn
     } else {
         const uint4 *q = S.quant + n * 2u;
         uint4 q0 = q[0], q1 = q[1];
-        float ox = bitsf(q0.x), oy = bitsf(q0.y), oz = bitsf(q0.z);
-        float sx = bitsf((q0.w & 0xffu) << 23), sy = bitsf(((q0.w >> 8) & 0xffu) << 23),
-              sz = bitsf(((q0.w >> 16) & 0xffu) << 23);
+        asm volatile("" ::"v"(q1.w));
+        // origin: the word's top 24 bits; step 2^e: its low byte as a biased exponent
+        float ox = bitsf(q0.x & 0xffffff00u), oy = bitsf(q0.y & 0xffffff00u),
+              oz = bitsf(q0.z & 0xffffff00u);
+        float sx = bitsf((q0.x & 0xffu) << 23), sy = bitsf((q0.y & 0xffu) << 23),
+              sz = bitsf((q0.z & 0xffu) << 23);
         auto b = [](uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); };
-        amn = mk(ox + b(q1.x, 0) * sx, oy + b(q1.x, 1) * sy, oz + b(q1.x, 2) * sz);
-        amx = mk(ox + b(q1.x, 3) * sx, oy + b(q1.y, 0) * sy, oz + b(q1.y, 1) * sz);
-        bmn = mk(ox + b(q1.y, 2) * sx, oy + b(q1.y, 3) * sy, oz + b(q1.z, 0) * sz);
-        bmx = mk(ox + b(q1.z, 1) * sx, oy + b(q1.z, 2) * sy, oz + b(q1.z, 3) * sz);
-        uint32_t first = q1.w & 0x3fffffffu, flags = q1.w >> 30;
-        ra = flags & 1u ? (LEAF_BIT | (31u << 26) | first) : first;
-        rb = flags & 2u ? (LEAF_BIT | (31u << 26) | (first + 1u)) : first + 1u;
+        // q * 2^e is exact, so the fused form rounds once, as origin + q * 2^e does
+        amn = mk(__builtin_fmaf(b(q1.x, 0), sx, ox), __builtin_fmaf(b(q1.x, 1), sy, oy),
+                 __builtin_fmaf(b(q1.x, 2), sz, oz));
+        amx = mk(__builtin_fmaf(b(q1.x, 3), sx, ox), __builtin_fmaf(b(q1.y, 0), sy, oy),
+                 __builtin_fmaf(b(q1.y, 1), sz, oz));
+        bmn = mk(__builtin_fmaf(b(q1.y, 2), sx, ox), __builtin_fmaf(b(q1.y, 3), sy, oy),
+                 __builtin_fmaf(b(q1.z, 0), sz, oz));
+        bmx = mk(__builtin_fmaf(b(q1.z, 1), sx, ox), __builtin_fmaf(b(q1.z, 2), sy, oy),
+                 __builtin_fmaf(b(q1.z, 3), sz, oz));
+        ra = q0.w;
+        rb = q1.w;
     }
 }
 

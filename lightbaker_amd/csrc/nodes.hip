@@ -4,10 +4,13 @@
 // its two children (first, first + 1) so one fetch serves the pair the reference reads as three
 // 32 B records (trace.wgsl:90,111-112):
 //   WIDE   4 x 16 B: a.min xyz a.max.x | a.max.yz b.min.xy | b.min.z b.max xyz | a ref, b ref
-//   QUANT  2 x 16 B: origin xyz, exponents | 12 bytes of children's boxes, first + leaf flags
-// QUANT boxes are 8-bit multiples of a power of two per axis above the parent's min corner, each
-// rounded outward and checked against the device's own decode (origin + q * 2^e in FP32), so
-// every decoded box contains its FP32 box and culling stays conservative.
+//   QUANT  2 x 16 B: origin | exponent per axis, a ref | 12 bytes of children's boxes, b ref
+// QUANT boxes are 8-bit multiples of a power of two per axis above an origin at or below the
+// pair's min corner (an FP32 value whose low 8 bits are zero, so the axis's exponent byte fits
+// beside it), each rounded outward and checked against the device's own decode (origin + q * 2^e
+// in FP32: the product is exact, so one rounding), so every decoded box contains its FP32 box and
+// culling stays conservative. Both layouts carry each child's reference (a leaf's triangles, or
+// the inner child's index), so no leaf visit reads another record.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -90,6 +93,14 @@ bool validate_bvh(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, size
 
 // The power-of-two step of a quantized axis and the outward-rounded 8-bit corners of a child.
 // Returns false if 255 steps do not reach (the caller widens the step).
+// The largest FP32 value with its low 8 bits zero that is at most x (x finite).
+static float origin_below(float x) {
+    uint32_t u = bits(x);
+    if (!(u & 0xffu)) return x;
+    if (x >= 0.0f) return fl(u & 0xffffff00u);
+    return fl((u & 0xffffff00u) + 0x100u);  // a larger magnitude: further below
+}
+
 static bool quantize_axis(float origin, uint32_t ebits, float lo, float hi, uint8_t *qlo,
                           uint8_t *qhi) {
     float step = fl(ebits << 23);
@@ -126,23 +137,24 @@ int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const l
         const lb_node &n = nodes[i];
         if (n.count > 0) continue;
         const lb_node &a = nodes[n.first], &b = nodes[n.first + 1];
-        uint32_t flags = (a.count > 0 ? 1u : 0u) | (b.count > 0 ? 2u : 0u);
         float *w = &wide[i * 16];
         const float vals[12] = {a.min[0], a.min[1], a.min[2], a.max[0], a.max[1], a.max[2],
                                 b.min[0], b.min[1], b.min[2], b.max[0], b.max[1], b.max[2]};
         std::memcpy(w, vals, sizeof vals);
         // each child's reference, a leaf's carrying its (first, count): a leaf visit then reads
         // no node record
-        w[12] = fl(a.count > 0 ? leaf_ref(a, n.first) : n.first);
-        w[13] = fl(b.count > 0 ? leaf_ref(b, n.first + 1) : n.first + 1);
-        // quantized: origin at the union's min corner, one exponent an axis
+        uint32_t ra = a.count > 0 ? leaf_ref(a, n.first) : n.first;
+        uint32_t rb = b.count > 0 ? leaf_ref(b, n.first + 1) : n.first + 1;
+        w[12] = fl(ra);
+        w[13] = fl(rb);
+        // quantized: origin at or below the union's min corner, one exponent an axis
         uint32_t *q = &quant[i * 8];
         uint8_t qb[12];
         uint32_t ex[3];
         for (int ax = 0; ax < 3; ax++) {
             float lo = std::fmin(a.min[ax], b.min[ax]);
             float hi = std::fmax(a.max[ax], b.max[ax]);
-            float origin = lo;
+            float origin = origin_below(lo);
             float ext = hi - lo;
             int e = ext > 0.0f ? (int)std::ceil(std::log2((double)ext / 250.0)) : -100;
             if (e < -100) e = -100;
@@ -155,12 +167,11 @@ int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const l
                 }
                 if (e > 120) return fail("scene: node %zu is too large to quantize", i);
             }
-            q[ax] = bits(origin);
+            q[ax] = bits(origin) | ex[ax];
         }
-        q[3] = ex[0] | (ex[1] << 8) | (ex[2] << 16);
+        q[3] = ra;
         std::memcpy(&q[4], qb, 12);
-        q[7] = n.first | (flags << 30);
-        if (n.first >= (1u << 30)) return fail("scene: %zu nodes exceed 30-bit indices", n_nodes);
+        q[7] = rb;
     }
     // leaves: (first triangle, count) at the leaf's own index
     for (size_t i = 0; i < n_nodes; i++) {
