@@ -16,8 +16,13 @@ on a bounded sample of the gather pass.
 tiles' light is the cache), then a seeded subset of its tiles and their probe chunks rebaked; a
 step is the device part of that rebake, every dirty tile's items in one list per pass.
 
+`--workload city` measures the 400 x 400 m city (lightbaker_amd.scenes.city_scene(4): 2 000
+buildings, 1.24 M triangles, 8 texels/m, 3 bounces) the same way, with the map's tuning; its CPU
+leg compares the PLAIN form and the TUNED form with quantized and FP32 nodes against the FP64
+tracer.
+
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--form tuned|plain] [--no-cpu]
-                    [--workload map|rebake]
+                    [--workload map|rebake|city]
 """
 from __future__ import annotations
 
@@ -60,11 +65,12 @@ def measured_traffic(kernel: str) -> dict | None:
     return rec if rec.get("kernel") == kernel else None
 
 
-def cpu_baseline(sess, gpu, target_s: float = 12.0) -> dict:
+def cpu_baseline(sess, gpu, target_s: float = 12.0, max_items: int = 500000) -> dict:
     """The CPU leg: the oracle (oracle/trace_ref.c, FP32, OpenMP over items) timed on a sample of
-    the gather pass, and the error of the PLAIN and TUNED forms against the oracle's FP64 tracer
-    on identical rays (gather: every gather item; direct: a sample of texels), with the final
-    light atlas."""
+    the gather pass, and the error of the PLAIN form and of the TUNED form with each node layout
+    against the oracle's FP64 tracer on identical rays (gather: the gather items, a random
+    `max_items` of them where there are more; direct: 200 000 random covered texels), with the
+    final light atlas."""
     import oracle
     from lightbaker_amd import _native as N
     from lightbaker_amd.gpu import Bvh, Pass
@@ -105,9 +111,14 @@ def cpu_baseline(sess, gpu, target_s: float = 12.0) -> dict:
 
     def forms(pass_, its, lt):
         out = {}
-        for name, form in (("plain", N.FORM_PLAIN), ("tuned", N.FORM_TUNED)):
-            gpu.set_form(form)
+        gpu.set_form(N.FORM_PLAIN)
+        out["plain"] = gpu.run(pass_, sb, params, its, lt)
+        gpu.set_form(N.FORM_TUNED)
+        keep = gpu.tuning(pass_)["nodes"]
+        for name, layout in (("tuned_quant", N.NODES_QUANT), ("tuned_wide", N.NODES_WIDE)):
+            gpu.set_tuning(pass_=pass_, nodes=layout)
             out[name] = gpu.run(pass_, sb, params, its, lt)
+        gpu.set_tuning(pass_=pass_, nodes=keep)
         gpu.set_form(N.FORM_PLAIN)
         return out
 
@@ -122,11 +133,14 @@ def cpu_baseline(sess, gpu, target_s: float = 12.0) -> dict:
         return res
 
     acc = {}
-    g_items = np.ascontiguousarray(texels[gat])
+    # a sample keeps its items' keys: each run gets the same items in the same order
+    pick = np.sort(rng.permutation(len(gat))[:max_items]) if len(gat) > max_items else slice(None)
+    g_items = np.ascontiguousarray(texels[gat][pick])
     got = forms(Pass.GATHER, g_items, light)
     ref = oracle.run64(1, nodes, tris, mats, charts, params, g_items, 0, light)
     acc["gather_irradiance"] = dict(errors(got, ref, [0, 1, 2]), items=len(g_items))
-    d_items = np.ascontiguousarray(texels[rng.permutation(len(texels))[:200000]])
+    cov = np.nonzero(texels["pos"][:, 3] > 0.5)[0]
+    d_items = np.ascontiguousarray(texels[cov[rng.permutation(len(cov))[:200000]]])
     got = forms(Pass.DIRECT, d_items, None)
     ref = oracle.run64(0, nodes, tris, mats, charts, params, d_items, 0, None)
     acc["direct_occlusion"] = dict(errors(got, ref, [2]), items=len(d_items))
@@ -145,10 +159,12 @@ def main() -> int:
     ap.add_argument("--nodes", choices=["wide", "quant"], default="quant")
     ap.add_argument("--schedule", type=int, choices=[0, 1], default=0,
                     help="TUNED schedule before autotuning: 0 lock-step, 1 pooled")
-    ap.add_argument("--workload", choices=["map", "rebake"], default="map")
+    ap.add_argument("--workload", choices=["map", "rebake", "city"], default="map")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune also where the workload skips it by default (city)")
     ap.add_argument("--profile-gather", type=int, default=0, metavar="N",
                     help="prepare, execute once, then N gather launches only (for counters)")
     ap.add_argument("--copy-kernel", type=int, default=0, metavar="MIB",
@@ -173,12 +189,17 @@ def main() -> int:
     from lightbaker_amd import _native as N
     from lightbaker_amd.bake import BakeSession, bake
     from lightbaker_amd.gpu import Gpu, Pass
-    from lightbaker_amd.scenes import map_scene, rebake_scene
+    from lightbaker_amd.scenes import city_scene, map_scene, rebake_scene
 
     form = N.FORM_TUNED if args.form == "tuned" else N.FORM_PLAIN
     layout = N.NODES_WIDE if args.nodes == "wide" else N.NODES_QUANT
     t_scene = time.perf_counter()
-    scene = map_scene(2) if args.workload == "map" else rebake_scene(3)
+    scene = {"map": lambda: map_scene(2), "rebake": lambda: rebake_scene(3),
+             "city": lambda: city_scene(4)}[args.workload]()
+    if args.workload == "city" and not args.autotune:
+        # one city bake is 15 G rays: the autotuner's hundred trial launches take minutes, so by
+        # default the map's picks are used
+        args.no_autotune = True
     t_scene = time.perf_counter() - t_scene
     gpu = Gpu()
     gpu.set_tuning(256, 4, 16, layout, 32, args.schedule, leaf_batch=16, waves=6)
@@ -278,6 +299,10 @@ def main() -> int:
     if args.workload == "map":
         metric = "Grays/s, 100x100 m 3-bounce bake (direct + 3 gather bounces + volume)"
         workload = "map_scene(2): 100x100 m, 8 texels/m, 256 GI rays, 3 bounces, 16 tiles"
+    elif args.workload == "city":
+        metric = "Grays/s, 400x400 m 1.2M-triangle city 3-bounce bake"
+        workload = ("city_scene(4): 400x400 m, 2 000 buildings, 1.24 M triangles, 8 texels/m, "
+                    "64 GI rays, 3 bounces, 256 tiles")
     else:
         metric = "Grays/s, rebake of a seeded share of a 200x200 m map's 64 tiles"
         workload = (f"rebake_scene(3): 200x200 m, 4 texels/m, 64 GI rays, 1 bounce, "
@@ -295,9 +320,12 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": ("synthetic: seeded map generator (fBm terrain + boxes + cylinders), no dataset"
-                 if args.workload == "map" else
-                 "synthetic: seeded generator (yawed boxes on a ground of 5 m quads), no dataset"),
+        "data": {"map": "synthetic: seeded map generator (fBm terrain + boxes + cylinders), "
+                        "no dataset",
+                 "rebake": "synthetic: seeded generator (yawed boxes on a ground of 5 m quads), "
+                           "no dataset",
+                 "city": "synthetic: seeded city generator (prisms of 14-400 sides on a plane), "
+                         "no dataset"}[args.workload],
         "config": {
             "workload": workload,
             "form": args.form, "nodes": "wide" if layout == N.NODES_WIDE else "quant",

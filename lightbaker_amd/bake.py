@@ -133,7 +133,9 @@ def _array(lib, h, which, dtype):
     n = lib.lb_bake_array(h, which, C.byref(p))
     if n == 0:
         return np.zeros(0, dtype=dtype)
-    return np.frombuffer(C.string_at(p, n), dtype=dtype).copy()
+    # a ctypes array over the bake's bytes (string_at stops at 2 GiB)
+    buf = (C.c_ubyte * n).from_address(p.value)
+    return np.frombuffer(buf, dtype=np.uint8).view(dtype).copy()
 
 
 def bake(scene: Scene, gpu: Gpu, form: int = N.FORM_PLAIN, graph: bool = False,

@@ -262,6 +262,56 @@ def rebake_scene(seed: int = 3, size: float = 200.0) -> Scene:
         settings=st, volume=Volume((0.0, 0.0, 0.0), (size, 8.0, size), 1.0, 64))
 
 
+def building(cx, cz, radius, height, sides, yaw) -> np.ndarray:
+    """A prism over a regular `sides`-gon (walls, a fan roof, no floor): 3 * sides - 2
+    triangles, counter-clockwise seen from outside."""
+    a = yaw + np.arange(sides) * (2 * math.pi / sides)
+    ring = np.stack([cx + radius * np.cos(a), cz + radius * np.sin(a)], axis=1)
+    tris = []
+    for k in range(sides):
+        (x0, z0), (x1, z1) = ring[k], ring[(k + 1) % sides]
+        tris += [(x0, 0.0, z0), (x0, height, z0), (x1, height, z1)]
+        tris += [(x0, 0.0, z0), (x1, height, z1), (x1, 0.0, z1)]
+    for k in range(1, sides - 1):
+        tris += [(ring[0][0], height, ring[0][1]), (ring[k + 1][0], height, ring[k + 1][1]),
+                 (ring[k][0], height, ring[k][1])]
+    return np.array(tris, dtype=np.float32)
+
+
+def city_scene(seed: int = 4, buildings: int = 2000, size: float = 400.0,
+               texels_per_meter: float = 8.0) -> Scene:
+    """A size x size m plane (10 m quads) with `buildings` prisms of 40 to 1 200 triangles
+    (14 to 400 sides), footprint radius U[3, 9] m, heights lognormal (mu 2.5, sigma 0.6) m,
+    one of 16 materials each: 1.24 M triangles at the defaults, flattened (the reference has no
+    instancing). 3 bounces, 8 texels/m."""
+    rng = np.random.default_rng(seed)
+    cells = int(size // 10)
+    ground = []
+    for i in range(cells):
+        for j in range(cells):
+            x, z = i * 10.0, j * 10.0
+            ground += quad((x, 0, z), (x, 0, z + 10), (x + 10, 0, z + 10), (x + 10, 0, z))
+    meshes = [Mesh("ground", 0, np.array(ground, np.float32))]
+    for k in range(buildings):
+        sides = int(rng.integers(14, 401))
+        r = rng.uniform(3.0, 9.0)
+        h = float(np.exp(rng.normal(2.5, 0.6)))
+        cx, cz = rng.uniform(r, size - r, 2)
+        meshes.append(Mesh(f"b{k}", int(rng.integers(0, 16)),
+                           building(cx, cz, r, h, sides, rng.uniform(0, 2 * math.pi))))
+    mats = [Material(f"m{i}", tuple(float(v) for v in rng.uniform(0.15, 0.85, 3)))
+            for i in range(16)]
+    el = math.radians(rng.uniform(20.0, 70.0))
+    az = rng.uniform(0, 2 * math.pi)
+    sun = (math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az))
+    st = Settings(texels_per_meter=texels_per_meter, tile_size=25.0, atlas_width=16384,
+                  sun_samples=16, ao_rays=64, gi_rays=64, gi_bounces=3)
+    return Scene(
+        sun_direction=sun, sun_radius_deg=0.5, sun_irradiance=(3.0, 3.0, 3.0),
+        zenith=(0.2, 0.25, 0.35), horizon=(0.3, 0.3, 0.32), ground=(0.1, 0.1, 0.1),
+        materials=mats, meshes=meshes, settings=st, volume=None)
+
+
 def fbm(n: int, octaves: int, rng) -> np.ndarray:
     """(n+1)^2 heights in [-1, 1]: octaves of bilinearly upsampled value noise, halving amplitude."""
     h = np.zeros((n + 1, n + 1))
