@@ -113,6 +113,8 @@ def cpu_baseline(sess, gpu, target_s: float = 12.0, max_items: int = 500000) -> 
         out = {}
         gpu.set_form(N.FORM_PLAIN)
         out["plain"] = gpu.run(pass_, sb, params, its, lt)
+        gpu.set_form(N.FORM_WATERTIGHT)
+        out["plain_watertight"] = gpu.run(pass_, sb, params, its, lt)
         gpu.set_form(N.FORM_TUNED)
         keep = gpu.tuning(pass_)["nodes"]
         for name, layout in (("tuned_quant", N.NODES_QUANT), ("tuned_wide", N.NODES_WIDE)):
@@ -122,7 +124,7 @@ def cpu_baseline(sess, gpu, target_s: float = 12.0, max_items: int = 500000) -> 
         gpu.set_form(N.FORM_PLAIN)
         return out
 
-    def errors(got, ref, cols):
+    def errors(got, ref, cols, direct=False):
         r = ref[:, cols]
         scale = float(np.max(np.abs(r))) or 1.0
         res = {}
@@ -130,6 +132,17 @@ def cpu_baseline(sess, gpu, target_s: float = 12.0, max_items: int = 500000) -> 
             d = g[:, cols].astype(np.float64) - r
             res[name] = {"rmse": float(np.sqrt(np.mean(d * d))),
                          "max_abs_of_full_scale": float(np.max(np.abs(d))) / scale}
+        # items the watertight test changes in a hit-derived count (back share; for the direct
+        # pass also shadow bits, pick and sun share) or beyond its rounding of t (1e-4 relative):
+        # rays that met another triangle or none
+        a, b = got["plain"], got["plain_watertight"]
+        c = [0, 2] if direct else [0, 1, 2]
+        differ = ~np.isclose(a[:, c], b[:, c], rtol=1e-4, atol=1e-5).all(axis=1)
+        differ |= a[:, 3] != b[:, 3]
+        if direct:
+            differ |= a[:, 1].view(np.uint32) != b[:, 1].view(np.uint32)
+            differ |= a[:, 0] != b[:, 0]
+        res["watertight_items_differing"] = float(np.mean(differ))
         return res
 
     acc = {}
@@ -143,8 +156,8 @@ def cpu_baseline(sess, gpu, target_s: float = 12.0, max_items: int = 500000) -> 
     d_items = np.ascontiguousarray(texels[cov[rng.permutation(len(cov))[:200000]]])
     got = forms(Pass.DIRECT, d_items, None)
     ref = oracle.run64(0, nodes, tris, mats, charts, params, d_items, 0, None)
-    acc["direct_occlusion"] = dict(errors(got, ref, [2]), items=len(d_items))
-    acc["direct_soft_sun"] = dict(errors(got, ref, [0]), items=len(d_items))
+    acc["direct_occlusion"] = dict(errors(got, ref, [2], True), items=len(d_items))
+    acc["direct_soft_sun"] = dict(errors(got, ref, [0], True), items=len(d_items))
     sb.close()
     base["error_vs_fp64"] = acc
     return base

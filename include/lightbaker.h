@@ -71,8 +71,11 @@ enum lb_pass { LB_PASS_DIRECT = 0, LB_PASS_GATHER = 1, LB_PASS_VOLUME = 2 };
 
 /* Kernel form. PLAIN: one item per lane, the reference's serial ray loop, FP32 node records.
  * TUNED: rays spread over the lanes of a wave with Kahan-compensated wave reductions, an LDS
- * traversal stack, and the node/triangle layout chosen by lb_set_tuning. */
-enum lb_form { LB_FORM_PLAIN = 0, LB_FORM_TUNED = 1 };
+ * traversal stack, and the node/triangle layout chosen by lb_set_tuning. WATERTIGHT: PLAIN with
+ * the watertight ray-triangle test of Woop, Benthin and Wald (2013) in place of the reference's
+ * Moller-Trumbore (trace.wgsl:93-104): another hit on rays through shared edges, so it is not
+ * bit-exact with the reference; it is there to measure how often the two tests differ. */
+enum lb_form { LB_FORM_PLAIN = 0, LB_FORM_TUNED = 1, LB_FORM_WATERTIGHT = 2 };
 
 /* Node layout for the TUNED form. WIDE: both children's FP32 boxes stored in the parent (64 B
  * per inner node visit instead of the reference's 96 B). QUANT: children's boxes as 8-bit

@@ -26,7 +26,7 @@ assert NODE.itemsize == 32 and TRI.itemsize == 48 and TEXEL.itemsize == 64
 assert PARAMS.itemsize == 144
 
 PASS_DIRECT, PASS_GATHER, PASS_VOLUME = 0, 1, 2
-FORM_PLAIN, FORM_TUNED = 0, 1
+FORM_PLAIN, FORM_TUNED, FORM_WATERTIGHT = 0, 1, 2
 NODES_WIDE, NODES_QUANT = 0, 1
 
 

@@ -834,7 +834,7 @@ int lb_bake_prepare(lb_gpu *gpu, const lb_scene_desc *scene, const lb_bake_optio
     *out = nullptr;
     lb_bake_options opt{LB_FORM_PLAIN, 0, 0, nullptr};
     if (options) opt = *options;
-    if (opt.form != LB_FORM_PLAIN && opt.form != LB_FORM_TUNED)
+    if (opt.form != LB_FORM_PLAIN && opt.form != LB_FORM_TUNED && opt.form != LB_FORM_WATERTIGHT)
         return fail("lb_bake_prepare: no form %d", opt.form);
     auto *b = new lb_bake();
     if (bake_prepare(gpu, *scene, opt, b)) {

@@ -258,7 +258,8 @@ int lb_run(lb_gpu *gpu, int pass, const lb_scene *scene, const lb_params *params
 
 int lb_set_form(lb_gpu *gpu, int form) {
     if (!gpu) return fail("lb_set_form: null handle");
-    if (form != LB_FORM_PLAIN && form != LB_FORM_TUNED) return fail("lb_set_form: no form %d", form);
+    if (form != LB_FORM_PLAIN && form != LB_FORM_TUNED && form != LB_FORM_WATERTIGHT)
+        return fail("lb_set_form: no form %d", form);
     gpu->form = form;
     return 0;
 }
@@ -424,7 +425,8 @@ int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_pa
              size_t n_light, float *out, bool stats) {
     if (!gpu || !scene || !params) return fail("run: null handle");
     if (pass < LB_PASS_DIRECT || pass > LB_PASS_VOLUME) return fail("run: no pass %d", pass);
-    if (form != LB_FORM_PLAIN && form != LB_FORM_TUNED) return fail("run: no form %d", form);
+    if (form != LB_FORM_PLAIN && form != LB_FORM_TUNED && form != LB_FORM_WATERTIGHT)
+        return fail("run: no form %d", form);
     if (n == 0) return 0;
     if (n > 0xffffffffu) return fail("run: %zu items exceed 32-bit item indices", n);
     if (!items || !out) return fail("run: null items or output");
@@ -441,9 +443,9 @@ int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_pa
     P.atlas[2] = key_base;
     SceneView S = scene_view(scene, light ? light : static_cast<const float *>(gpu->zero.ptr));
     hipError_t e;
-    if (form == LB_FORM_PLAIN)
-        e = launch_plain(pass, stats, S, P, items, (uint32_t)n, key_base,
-                         reinterpret_cast<float4 *>(out), gpu->counters, gpu->stream);
+    if (form != LB_FORM_TUNED)
+        e = launch_plain(pass, stats, form == LB_FORM_WATERTIGHT, S, P, items, (uint32_t)n,
+                         key_base, reinterpret_cast<float4 *>(out), gpu->counters, gpu->stream);
     else
         e = launch_tuned(gpu, pass, stats, S, P, items, (uint32_t)n, key_base,
                          reinterpret_cast<float4 *>(out));

@@ -198,6 +198,41 @@ __device__ __forceinline__ float mt(f3 o, f3 d, float4 v0, float4 e1v, float4 e2
     return dot3(e2, q) * id;
 }
 
+__device__ __forceinline__ float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// The watertight ray-triangle test of Woop, Benthin and Wald ("Watertight Ray/Triangle
+// Intersection", JCGT 2(1), 2013), without its double-precision fallback: the ray's dominant axis
+// becomes z, the triangle is sheared so the ray runs along +z, and the signs of the projected
+// triangle's edge functions U, V, W decide the hit, so a ray through a shared edge meets one of
+// its two triangles. The vertices are v0, v0 + e1, v0 + e2 (the record keeps edges). Returns t =
+// T / det or -1; *det_out is the reference's orientation e1 . (d x e2) (a back face below 0).
+__device__ __forceinline__ float wt(f3 o, f3 d, float4 v0v, float4 e1v, float4 e2v,
+                                    float *det_out) {
+    f3 e1 = xyz(e1v), e2 = xyz(e2v), v0 = xyz(v0v);
+    *det_out = dot3(e1, cross3(d, e2));
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dz = comp(d, kz);
+    if (dz < 0.0f) {
+        int k = kx;
+        kx = ky;
+        ky = k;
+    }
+    float sx = comp(d, kx) / dz, sy = comp(d, ky) / dz, sz = 1.0f / dz;
+    f3 A = sub(v0, o), B = sub(add(v0, e1), o), C = sub(add(v0, e2), o);
+    float Ax = comp(A, kx) - sx * comp(A, kz), Ay = comp(A, ky) - sy * comp(A, kz);
+    float Bx = comp(B, kx) - sx * comp(B, kz), By = comp(B, ky) - sy * comp(B, kz);
+    float Cx = comp(C, kx) - sx * comp(C, kz), Cy = comp(C, ky) - sy * comp(C, kz);
+    float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return -1.0f;
+    float det = U + V + W;
+    if (det == 0.0f) return -1.0f;
+    float T = U * (sz * comp(A, kz)) + V * (sz * comp(B, kz)) + W * (sz * comp(C, kz));
+    return T / det;
+}
+
 // Moller-Trumbore with every product of the three edge vectors taken before any test, so the
 // triangle's three loads are consumed together (one memory round trip); the same values and
 // rejections as mt().

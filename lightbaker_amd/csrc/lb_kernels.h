@@ -27,8 +27,8 @@ inline size_t stack_lds_bytes(int depth, int workgroup) {
     return (size_t)lane_stride((uint32_t)depth) * (size_t)workgroup * 4u;
 }
 
-hipError_t launch_plain(int pass, bool stats, const SceneView &S, const lb_params &P,
-                        const lb_texel *items, uint32_t n, uint32_t key_base, float4 *out,
-                        unsigned long long *counters, hipStream_t stream);
+hipError_t launch_plain(int pass, bool stats, bool watertight, const SceneView &S,
+                        const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
+                        float4 *out, unsigned long long *counters, hipStream_t stream);
 
 }  // namespace lbk

@@ -7,8 +7,8 @@
 
 namespace lbk {
 
-// trace.wgsl:83-130
-template <bool ANY, bool STATS>
+// trace.wgsl:83-130; WT: the watertight triangle test (wt()) in place of Moller-Trumbore
+template <bool ANY, bool STATS, bool WT = false>
 __device__ Hit trace_plain(const SceneView &S, const lb_params &P, f3 o, f3 d, float tmax,
                            uint32_t ray, Counts &cnt) {
     Hit hit{tmax, NONE, false};
@@ -26,7 +26,7 @@ __device__ Hit trace_plain(const SceneView &S, const lb_params &P, f3 o, f3 d, f
                 float4 v0 = S.tris[i * 3u], e1 = S.tris[i * 3u + 1u], e2 = S.tris[i * 3u + 2u];
                 if (STATS) cnt.tris++;
                 float det;
-                float t = mt(o, d, v0, e1, e2, &det);
+                float t = WT ? wt(o, d, v0, e1, e2, &det) : mt(o, d, v0, e1, e2, &det);
                 if (t > 1e-4f && t < hit.t) {
                     if (STATS) cnt.mats++;
                     if (stops(S, P, i, v0, ray)) {

@@ -9,21 +9,21 @@
 namespace lbk {
 
 // trace.wgsl:159-168
-template <bool STATS>
+template <bool STATS, bool WT>
 __device__ bool inside_plain(const SceneView &S, const lb_params &P, f3 o, f3 n, uint32_t key,
                              Counts &cnt) {
     frame_t f = frame(n);
     float back = 0.0f;
     for (uint32_t r = 0; r < 16u; r++) {
         f3 d = cosine(((float)r + 0.5f) / 16.0f, radical(r), f);
-        Hit h = trace_plain<false, STATS>(S, P, o, d, P.ground[3], key + r + 7000u, cnt);
+        Hit h = trace_plain<false, STATS, WT>(S, P, o, d, P.ground[3], key + r + 7000u, cnt);
         if (h.tri != NONE && h.back) back += 1.0f;
     }
     return back / 16.0f > bitsf(P.atlas[3]);
 }
 
 // trace.wgsl:170-243
-template <bool STATS>
+template <bool STATS, bool WT>
 __global__ __launch_bounds__(64) void k_direct_plain(SceneView S, lb_params P,
                                                      const lb_texel *__restrict__ items,
                                                      uint32_t n_items, uint32_t key_base,
@@ -46,12 +46,12 @@ __global__ __launch_bounds__(64) void k_direct_plain(SceneView S, lb_params P,
     f3 at = pos;
     uint32_t pick = 0u;
     float span = 1.0f;
-    if (inside_plain<STATS>(S, P, add(at, lift), n, key, cnt)) {
+    if (inside_plain<STATS, WT>(S, P, add(at, lift), n, key, cnt)) {
         for (uint32_t q = 1u; q < 9u; q++) {
             float cx, cy;
             offset(q, &cx, &cy);
             f3 p = add(add(pos, scl(dx, cx)), scl(dy, cy));
-            if (!inside_plain<STATS>(S, P, add(p, lift), n, key + q * 31u, cnt)) {
+            if (!inside_plain<STATS, WT>(S, P, add(p, lift), n, key + q * 31u, cnt)) {
                 at = p;
                 pick = q;
                 span = 0.4f;
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(64) void k_direct_plain(SceneView S, lb_params P,
         }
         if (pick == 0u) {
             f3 p = add(pos, scl(n, PUSH));
-            if (!inside_plain<STATS>(S, P, add(p, lift), n, key + 331u, cnt)) {
+            if (!inside_plain<STATS, WT>(S, P, add(p, lift), n, key + 331u, cnt)) {
                 at = p;
                 pick = 9u;
                 span = 0.4f;
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(64) void k_direct_plain(SceneView S, lb_params P,
             float cx = (((float)(s % g) + 0.5f) / (float)g - 0.5f) * span;
             float cy = (((float)(s / g) + 0.5f) / (float)g - 0.5f) * span;
             f3 o = add(add(add(at, scl(dx, cx)), scl(dy, cy)), lift);
-            if (trace_plain<true, STATS>(S, P, o, sun, FAR, key + s, cnt).tri == NONE)
+            if (trace_plain<true, STATS, WT>(S, P, o, sun, FAR, key + s, cnt).tri == NONE)
                 hard |= 1u << s;
             uint32_t k = (s + (uint32_t)(rnd(key, 17u, seed) * (float)gg)) % gg;
             float a = (float)k * 2.399963f;
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(64) void k_direct_plain(SceneView S, lb_params P,
             float sa, ca;
             sincos_lb(a, &sa, &ca);
             f3 d = normalize3(add(add(sun, scl(scl(sf.t, ca), r)), scl(scl(sf.b, sa), r)));
-            if (trace_plain<true, STATS>(S, P, o, d, FAR, key + s + 1000u, cnt).tri == NONE)
+            if (trace_plain<true, STATS, WT>(S, P, o, d, FAR, key + s + 1000u, cnt).tri == NONE)
                 soft += 1.0f;
         }
         soft /= (float)gg;
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(64) void k_direct_plain(SceneView S, lb_params P,
     float occ = 0.0f, back = 0.0f;
     for (uint32_t r = 0u; r < rays; r++) {
         f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
-        Hit h = trace_plain<false, STATS>(S, P, o, d, P.reach[0], key + r + 2000u, cnt);
+        Hit h = trace_plain<false, STATS, WT>(S, P, o, d, P.reach[0], key + r + 2000u, cnt);
         if (h.tri == NONE) continue;
         float k = h.t / P.reach[0];
         occ += 1.0f - k * k;
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(64) void k_direct_plain(SceneView S, lb_params P,
 }
 
 // trace.wgsl:245-270
-template <bool STATS>
+template <bool STATS, bool WT>
 __global__ __launch_bounds__(64) void k_gather_plain(SceneView S, lb_params P,
                                                      const lb_texel *__restrict__ items,
                                                      uint32_t n_items, uint32_t key_base,
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(64) void k_gather_plain(SceneView S, lb_params P,
         float jy = rnd(key + r, 23u, seed) - 0.5f;
         f3 o = add(add(add(pos, scl(dx, jx)), scl(dy, jy)), scl(n, P.reach[2]));
         f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
-        Hit h = trace_plain<false, STATS>(S, P, o, d, P.reach[1], key + r + 4000u, cnt);
+        Hit h = trace_plain<false, STATS, WT>(S, P, o, d, P.reach[1], key + r + 4000u, cnt);
         if (h.tri == NONE) {
             sum = add(sum, sky(P, d));
             continue;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(64) void k_gather_plain(SceneView S, lb_params P,
 }
 
 // trace.wgsl:272-324
-template <bool STATS>
+template <bool STATS, bool WT>
 __global__ __launch_bounds__(64) void k_volume_plain(SceneView S, lb_params P,
                                                      const lb_texel *__restrict__ items,
                                                      uint32_t n_items, uint32_t key_base,
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(64) void k_volume_plain(SceneView S, lb_params P,
         float sa, ca;
         sincos_lb(a, &sa, &ca);
         f3 d = mk(rr * ca, z, rr * sa);
-        Hit h = trace_plain<false, STATS>(S, P, o, d, P.reach[1], key + r, cnt);
+        Hit h = trace_plain<false, STATS, WT>(S, P, o, d, P.reach[1], key + r, cnt);
         f3 l = sky(P, d);
         if (h.tri != NONE) {
             if (h.back) back += 1.0f;
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(64) void k_volume_plain(SceneView S, lb_params P,
         float sa, ca;
         sincos_lb(a, &sa, &ca);
         f3 d = normalize3(add(add(sund, scl(scl(sf.t, ca), r)), scl(scl(sf.b, sa), r)));
-        if (trace_plain<true, STATS>(S, P, o, d, FAR, key + s + 9000u, cnt).tri == NONE)
+        if (trace_plain<true, STATS, WT>(S, P, o, d, FAR, key + s + 9000u, cnt).tri == NONE)
             sun += 1.0f;
     }
     out[i * 4u] = make_float4(l0.x * w, l0.y * w, l0.z * w, sun / (float)gg);
@@ -214,31 +214,30 @@ __global__ __launch_bounds__(64) void k_volume_plain(SceneView S, lb_params P,
     if (STATS) flush_counts(cnt, stats);
 }
 
-hipError_t launch_plain(int pass, bool stats, const SceneView &S, const lb_params &P,
-                        const lb_texel *items, uint32_t n, uint32_t key_base, float4 *out,
-                        unsigned long long *counters, hipStream_t stream) {
+hipError_t launch_plain(int pass, bool stats, bool watertight, const SceneView &S,
+                        const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
+                        float4 *out, unsigned long long *counters, hipStream_t stream) {
     dim3 grid((n + 63u) / 64u), block(64);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, block, 0, stream, S, P, items, n, key_base, out,
+                           counters);
+    };
+    int k = (stats ? 1 : 0) | (watertight ? 2 : 0);
     if (pass == LB_PASS_DIRECT) {
-        if (stats)
-            hipLaunchKernelGGL(k_direct_plain<true>, grid, block, 0, stream, S, P, items, n,
-                               key_base, out, counters);
-        else
-            hipLaunchKernelGGL(k_direct_plain<false>, grid, block, 0, stream, S, P, items, n,
-                               key_base, out, counters);
+        if (k == 0) go(k_direct_plain<false, false>);
+        else if (k == 1) go(k_direct_plain<true, false>);
+        else if (k == 2) go(k_direct_plain<false, true>);
+        else go(k_direct_plain<true, true>);
     } else if (pass == LB_PASS_GATHER) {
-        if (stats)
-            hipLaunchKernelGGL(k_gather_plain<true>, grid, block, 0, stream, S, P, items, n,
-                               key_base, out, counters);
-        else
-            hipLaunchKernelGGL(k_gather_plain<false>, grid, block, 0, stream, S, P, items, n,
-                               key_base, out, counters);
+        if (k == 0) go(k_gather_plain<false, false>);
+        else if (k == 1) go(k_gather_plain<true, false>);
+        else if (k == 2) go(k_gather_plain<false, true>);
+        else go(k_gather_plain<true, true>);
     } else {
-        if (stats)
-            hipLaunchKernelGGL(k_volume_plain<true>, grid, block, 0, stream, S, P, items, n,
-                               key_base, out, counters);
-        else
-            hipLaunchKernelGGL(k_volume_plain<false>, grid, block, 0, stream, S, P, items, n,
-                               key_base, out, counters);
+        if (k == 0) go(k_volume_plain<false, false>);
+        else if (k == 1) go(k_volume_plain<true, false>);
+        else if (k == 2) go(k_volume_plain<false, true>);
+        else go(k_volume_plain<true, true>);
     }
     return hipGetLastError();
 }
