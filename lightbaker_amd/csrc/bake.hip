@@ -484,14 +484,40 @@ int pass_stats(lb_bake *B, int pass, const lb_params *P, const lb_texel *items, 
     return 0;
 }
 
-// one bounce (bake.rs:301-328): gather, the light at its texels, smooth, fill, the atlas
-int bounce(lb_bake *B, lb_stats *acc) {
+// Whether the TUNED lock-step gather takes each item's rays cheapest first: every bounce traces
+// the same rays (the same keys and sample points; only the light they read changes), so the
+// first bounce's step counts rank the rays for the others, and the lanes of a round then trace
+// rays of like length. The ranking changes only which lane traces which ray when.
+bool ordered_gather(const lb_bake *B) {
+    const Tuning &t = B->gpu->tuning[LB_PASS_GATHER];
+    return B->opt.form == LB_FORM_TUNED && t.schedule == 0 && B->P.counts[2] <= 256u &&
+           B->sc.gi_bounces > 1 && B->ng > 0 && B->d_perm.ptr;
+}
+
+// one bounce (bake.rs:301-328): gather, the light at its texels, smooth, fill, the atlas; k its
+// index
+int bounce(lb_bake *B, lb_stats *acc, uint32_t k) {
     hipStream_t st = B->gpu->stream;
     TexelView &V = B->V;
     uint32_t T = (uint32_t)B->T;
-    TRY(pass_stats(B, LB_PASS_GATHER, &B->P, (const lb_texel *)B->d_gitems.ptr, B->ng,
-                   (const float *)B->d_light.ptr, (size_t)B->aw * B->ah, (float *)B->d_gres.ptr,
-                   acc));
+    if (!acc && ordered_gather(B)) {
+        // the first bounce counts each ray's steps, the rest take the rays ranked by them
+        GatherOrder o;
+        o.cost = k == 0 ? (uint8_t *)B->d_cost.ptr : nullptr;
+        o.perm = k == 0 ? nullptr : (const uint8_t *)B->d_perm.ptr;
+        TRY(run_pass(B->gpu, LB_PASS_GATHER, B->opt.form, B->scene, &B->P,
+                     (const lb_texel *)B->d_gitems.ptr, B->ng, 0, (const float *)B->d_light.ptr,
+                     (size_t)B->aw * B->ah, (float *)B->d_gres.ptr, k == 0, &o));
+        if (k == 0) {
+            hipError_t e = rank_slots((const uint8_t *)B->d_cost.ptr, (uint32_t)B->ng,
+                                      B->P.counts[2], (uint8_t *)B->d_perm.ptr, st);
+            if (e != hipSuccess) return hip_fail(e, "rank_slots");
+        }
+    } else {
+        TRY(pass_stats(B, LB_PASS_GATHER, &B->P, (const lb_texel *)B->d_gitems.ptr, B->ng,
+                       (const float *)B->d_light.ptr, (size_t)B->aw * B->ah,
+                       (float *)B->d_gres.ptr, acc));
+    }
     tp_fill_f32((float *)V.indirect, 0.0f, T * 4u, st);
     tp_scatter((const float4 *)B->d_gres.ptr, (const uint32_t *)B->d_gat.ptr, (uint32_t)B->ng,
                V.indirect, st);
@@ -579,9 +605,14 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
     // the bounces (bake.rs:300-330)
     TRYH(hipEventRecord(B->ev[3], st));
     if (sc.gi_bounces > 0 && B->ng > 0) {
+        // the ordered gather's step counts and ranks, one byte a ray
+        size_t nr = B->ng * (size_t)B->P.counts[2];
+        if (B->opt.form == LB_FORM_TUNED && B->P.counts[2] <= 256u && sc.gi_bounces > 1 &&
+            (B->d_cost.reserve(nr) || B->d_perm.reserve(nr)))
+            return 1;
         if (B->opt.graph && !per_pass) {
-            // one bounce captured, replayed gi_bounces times (same buffers each time); the item
-            // count is part of the capture, so a new count captures again
+            // the bounces captured once and replayed (same buffers each time); the item count is
+            // part of the capture, so a new count captures again
             const Tuning &tn = gpu->tuning[LB_PASS_GATHER];
             const Tuning &tg = B->graph_tuning;
             bool same = tn.workgroup == tg.workgroup && tn.rays_per_lane == tg.rays_per_lane &&
@@ -597,7 +628,8 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
                 B->graph = nullptr;
                 hipGraph_t gr;
                 TRYH(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-                int rc = bounce(B, nullptr);
+                int rc = 0;
+                for (uint32_t b = 0; b < sc.gi_bounces && !rc; b++) rc = bounce(B, nullptr, b);
                 hipError_t ce = hipStreamEndCapture(st, &gr);
                 if (rc) return 1;
                 if (ce != hipSuccess) return hip_fail(ce, "graph capture");
@@ -607,9 +639,9 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
                 B->graph_ng = B->ng;
                 B->graph_tuning = tn;
             }
-            for (uint32_t b = 0; b < sc.gi_bounces; b++) TRYH(hipGraphLaunch(B->graph, st));
+            TRYH(hipGraphLaunch(B->graph, st));
         } else {
-            for (uint32_t b = 0; b < sc.gi_bounces; b++) TRY(bounce(B, sg));
+            for (uint32_t b = 0; b < sc.gi_bounces; b++) TRY(bounce(B, sg, b));
         }
     }
     TRYH(hipEventRecord(B->ev[4], st));
@@ -811,11 +843,14 @@ int bake_time_pass(lb_bake *B, int pass, int reps, float *ms_avg) {
     }
     if (!n) return fail("bake: pass %d has no items (execute first)", pass);
     if (reps < 1) return fail("bake: reps %d", reps);
+    // the gather as the bake's later bounces run it (its rays ranked by the first bounce's cost)
+    GatherOrder o;
+    o.perm = pass == LB_PASS_GATHER && ordered_gather(B) ? (const uint8_t *)B->d_perm.ptr : nullptr;
     TRYH(hipEventRecord(B->ev[6], st));
     for (int r = 0; r < reps; r++)
         TRY(run_pass(gpu, pass, B->opt.form, B->scene, P, items, n, 0,
                      pass == LB_PASS_DIRECT ? nullptr : (const float *)B->d_light.ptr,
-                     (size_t)B->aw * B->ah, out, false));
+                     (size_t)B->aw * B->ah, out, false, &o));
     TRYH(hipEventRecord(B->ev[7], st));
     TRYH(hipEventSynchronize(B->ev[7]));
     float ms = 0.0f;

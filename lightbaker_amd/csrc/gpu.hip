@@ -422,7 +422,7 @@ int check_params(const lb_scene *scene, const lb_params *P, int pass, size_t n_l
 
 int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_params *params,
              const lb_texel *items, size_t n, uint32_t key_base, const float *light,
-             size_t n_light, float *out, bool stats) {
+             size_t n_light, float *out, bool stats, const GatherOrder *order) {
     if (!gpu || !scene || !params) return fail("run: null handle");
     if (pass < LB_PASS_DIRECT || pass > LB_PASS_VOLUME) return fail("run: no pass %d", pass);
     if (form != LB_FORM_PLAIN && form != LB_FORM_TUNED && form != LB_FORM_WATERTIGHT)
@@ -448,7 +448,7 @@ int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_pa
                          key_base, reinterpret_cast<float4 *>(out), gpu->counters, gpu->stream);
     else
         e = launch_tuned(gpu, pass, stats, S, P, items, (uint32_t)n, key_base,
-                         reinterpret_cast<float4 *>(out));
+                         reinterpret_cast<float4 *>(out), order);
     if (e != hipSuccess) return hip_fail(e, "launch");
     return 0;
 }

@@ -64,9 +64,19 @@ bool validate_bvh(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, size
 int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
                       size_t n_tris, const float *materials);
 SceneView scene_view(const lb_scene *s, const float *light);
+// The order a TUNED lock-step gather takes each item's rays in (one byte a ray, item-major): with
+// `cost`, the launch records each ray's traversal steps by its azimuth slot (a counting launch);
+// with `perm`, slot s of item i traces azimuth slot perm[i * rays + s]. Either needs rays <= 256.
+struct GatherOrder {
+    uint8_t *cost = nullptr;
+    const uint8_t *perm = nullptr;
+};
 int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_params *params,
              const lb_texel *items, size_t n, uint32_t key_base, const float *light,
-             size_t n_light, float *out, bool stats);
+             size_t n_light, float *out, bool stats, const GatherOrder *order = nullptr);
+// each item's azimuth slots ranked by recorded cost, into perm (rays <= 256)
+hipError_t rank_slots(const uint8_t *cost, uint32_t n_items, uint32_t rays, uint8_t *perm,
+                      hipStream_t stream);
 bool is_empty_bvh(const lb_node *nodes, size_t n_nodes);
 // sizes the tuned form's spill area for a launch over n items (before a graph capture)
 hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n);
@@ -76,6 +86,6 @@ hipError_t launch_stream(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                          float4 *out);
 hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                         const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
-                        float4 *out);
+                        float4 *out, const GatherOrder *order = nullptr);
 
 }  // namespace lbk

@@ -158,6 +158,8 @@ struct TunedArgs {
     uint32_t *spill;
     uint32_t lanes_per_item, depth;
     int leaf_batch;  // lanes at a leaf that end a wave's inner phase early (64: never)
+    uint8_t *cost;        // gather, counting launch: each ray's steps by azimuth slot (or null)
+    const uint8_t *perm;  // gather: slot s of item i takes azimuth slot perm[i * rays + s]
 };
 
 
@@ -426,6 +428,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
         }
         KSum sr{0, 0}, sg{0, 0}, sb{0, 0};
         float back = 0.0f;
+        uint64_t steps0 = 0;
         auto start = [&](uint32_t slot, Ray &R) {
             // read the slot afresh (the compiler would otherwise keep the stored values live)
             asm volatile("" ::: "memory");
@@ -437,14 +440,21 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
             f.n = n;
             float s1 = bitsf(slot_lds[18]), s2 = bitsf(slot_lds[19]);
             uint32_t key = slot_lds[20];
-            uint32_t r = az_slot(slot, rays, s2);
+            uint32_t s = A.perm ? (uint32_t)A.perm[(size_t)i * rays + slot] : slot;
+            uint32_t r = az_slot(s, rays, s2);
+            if (STATS) steps0 = cnt.inner + cnt.tris;
             float jx = rnd(key + r, 19u, seed) - 0.5f;
             float jy = rnd(key + r, 23u, seed) - 0.5f;
             f3 o = add(add(add(pos, scl(dx, jx)), scl(dy, jy)), scl(n, P.reach[2]));
             f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
             ray_start(R, A.S.root, o, d, P.reach[1], false, key + r + 4000u);
         };
-        auto shade = [&](uint32_t, const Ray &R) {
+        auto shade = [&](uint32_t slot, const Ray &R) {
+            if (STATS && A.cost) {
+                uint64_t steps = cnt.inner + cnt.tris - steps0;
+                uint32_t s = A.perm ? (uint32_t)A.perm[(size_t)i * rays + slot] : slot;
+                A.cost[(size_t)i * rays + s] = (uint8_t)(steps < 255u ? steps : 255u);
+            }
             if (R.tri == NONE) {
                 f3 l = sky(P, R.d);
                 sr.add(l.x);
@@ -643,6 +653,40 @@ static hipError_t tuned_grid(lb_gpu *gpu, int pass, bool stats, const lb_params 
     return hipSuccess;
 }
 
+// Each item's azimuth slots sorted by (recorded cost, slot): a bitonic sort of 256 keys in LDS,
+// one workgroup an item. Slot s of an ordered gather then takes the s-th cheapest ray, so the
+// lanes of one round trace rays of like length.
+__global__ __launch_bounds__(256) void k_rank_slots(const uint8_t *__restrict__ cost,
+                                                    uint32_t rays, uint8_t *__restrict__ perm) {
+    __shared__ uint32_t key[256];
+    uint32_t t = threadIdx.x;
+    size_t base = (size_t)blockIdx.x * rays;
+    key[t] = t < rays ? ((uint32_t)cost[base + t] << 16) | t : 0xffffffffu;
+    __syncthreads();
+    for (uint32_t size = 2; size <= 256u; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            uint32_t j = t ^ stride;
+            if (j > t) {
+                bool up = (t & size) == 0u;
+                uint32_t a = key[t], b = key[j];
+                if ((a > b) == up) {
+                    key[t] = b;
+                    key[j] = a;
+                }
+            }
+            __syncthreads();
+        }
+    if (t < rays) perm[base + t] = (uint8_t)(key[t] & 0xffffu);
+}
+
+hipError_t rank_slots(const uint8_t *cost, uint32_t n_items, uint32_t rays, uint8_t *perm,
+                      hipStream_t stream) {
+    if (!n_items) return hipSuccess;
+    if (rays > 256u) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_rank_slots, dim3(n_items), dim3(256), 0, stream, cost, rays, perm);
+    return hipGetLastError();
+}
+
 hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n) {
     if (gpu->tuning[pass].schedule == 1) return stream_prepare(gpu, pass, P, n);
     TunedGrid tg;
@@ -653,7 +697,7 @@ hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n) 
 
 hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                         const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
-                        float4 *out) {
+                        float4 *out, const GatherOrder *order) {
     if (gpu->tuning[pass].schedule == 1) return launch_stream(gpu, pass, stats, S, P, items, n, key_base, out);
     TunedGrid tg;
     hipError_t e = tuned_grid(gpu, pass, stats, P, n, &tg);
@@ -671,6 +715,9 @@ hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
     A.lanes_per_item = tg.lanes;
     A.depth = (uint32_t)gpu->tuning[pass].lds_stack;
     A.leaf_batch = gpu->tuning[pass].leaf_batch;
+    bool ordered = order && pass == LB_PASS_GATHER && P.counts[2] <= 256u;
+    A.cost = ordered && stats ? order->cost : nullptr;
+    A.perm = ordered ? order->perm : nullptr;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, gpu->stream, gpu->queue);
     hipLaunchKernelGGL(tg.fn, dim3(tg.grid), dim3(tg.block), tg.lds, gpu->stream, A);
     return hipGetLastError();

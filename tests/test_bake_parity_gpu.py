@@ -1,9 +1,9 @@
 """The whole device half of the bake against the oracle's restatement of it (oracle/bake_ref.py:
 bake.rs:149-330 and volume.rs:69-117 over the oracle's C passes), on the same prepared inputs.
 
-Bar: the PLAIN form, and the TUNED form's pooled schedule (which shades in the reference's
-order), give the final light atlas (direct + last bounce per texel) bit for bit. The TUNED form's
-lock-step schedule (compensated sums in another order) within 1e-4 relative.
+Bar: the PLAIN form, and the TUNED form's pooled schedule with FP32 node records (it shades in
+the reference's order), give the final light atlas (direct + last bounce per texel) bit for bit.
+The TUNED form's lock-step schedule (compensated sums in another order) within 1e-4 relative.
 """
 from __future__ import annotations
 
@@ -58,7 +58,8 @@ def oracle_inputs(scene, sess):
 def test_bake_matches_oracle(gpu, gen, form, schedule):
     scene = gen()
     print(f"bake parity form={form} schedule={schedule}", flush=True)
-    gpu.set_tuning(schedule=schedule)
+    # the pooled schedule is bit-exact with FP32 node records (the reference's boxes)
+    gpu.set_tuning(schedule=schedule, nodes=N.NODES_WIDE if schedule == 1 else N.NODES_QUANT)
     try:
         sess = BakeSession(scene, gpu, form=form, graph=form == N.FORM_TUNED)
         sess.execute()

@@ -4,8 +4,9 @@ tiles in layout order, each tile's texels row-major; an item's index in it is it
 
 Bar: against the oracle's restatement of the same rebake (oracle/bake_ref.py over the dirty
 tiles, its first atlas written from the cached TileLight by bake.rs:435-444), the PLAIN form and
-the pooled schedule give the atlas light, every dirty tile's TileLight and every traced probe bit
-for bit; the lock-step schedule within 1e-4 relative. A rebake with nothing dirty returns the
+the pooled schedule (with FP32 node records, the reference's boxes) give the atlas light, every
+dirty tile's TileLight and every traced probe bit for bit; the lock-step schedule (quantized
+nodes, the default) within 1e-4 relative. A rebake with nothing dirty returns the
 cache and the first bake's outputs exactly (tests/bake.rs:147-150: "baked again unchanged:
 nothing to bake"), and one with everything dirty equals a plain bake.
 """
@@ -59,7 +60,8 @@ def test_rebake_matches_oracle(gpu, first, form, schedule):
     dirty, pdirty = dirty_masks(scene, b0.tiles, 33, 0.3)
     print(f"rebake parity form={form} schedule={schedule}: {dirty.sum()} of {len(dirty)} tiles, "
           f"{pdirty.sum()} of {len(pdirty)} probes", flush=True)
-    gpu.set_tuning(schedule=schedule)
+    # the pooled schedule is bit-exact with FP32 node records (the reference's boxes)
+    gpu.set_tuning(schedule=schedule, nodes=N.NODES_WIDE if schedule == 1 else N.NODES_QUANT)
     try:
         sess = BakeSession(scene, gpu, form=form, graph=form == N.FORM_TUNED, cache=b0.cache,
                            tile_dirty=dirty, probe_dirty=pdirty)
@@ -168,7 +170,7 @@ def test_rebake_uneven_tiles(gpu):
     out = {}
     for name, form, schedule in (("plain", N.FORM_PLAIN, 0), ("pooled", N.FORM_TUNED, 1)):
         print(f"rebake uneven tiles: {name}", flush=True)
-        gpu.set_tuning(schedule=schedule)
+        gpu.set_tuning(schedule=schedule, nodes=N.NODES_WIDE)
         try:
             out[name] = bake(scene, gpu, form=form, graph=form == N.FORM_TUNED, cache=b0.cache,
                              tile_dirty=dirty, probe_dirty=pdirty)
