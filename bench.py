@@ -178,6 +178,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="autotune also where the workload skips it by default (city)")
+    ap.add_argument("--profile-pass", choices=["direct", "gather", "volume"], default="gather",
+                    help="the pass --profile-gather launches")
     ap.add_argument("--profile-gather", type=int, default=0, metavar="N",
                     help="prepare, execute once, then N gather launches only (for counters)")
     ap.add_argument("--copy-kernel", type=int, default=0, metavar="MIB",
@@ -246,8 +248,10 @@ def main() -> int:
     prep_s = time.perf_counter() - t0
     if args.profile_gather:
         sess.execute()
-        ms = sess.time_pass(Pass.GATHER, reps=args.profile_gather)
-        print(f"# {args.profile_gather} gather launches, {ms:.2f} ms each", file=sys.stderr)
+        pp = {"direct": Pass.DIRECT, "gather": Pass.GATHER, "volume": Pass.VOLUME}[args.profile_pass]
+        ms = sess.time_pass(pp, reps=args.profile_gather)
+        print(f"# {args.profile_gather} {args.profile_pass} launches, {ms:.2f} ms each",
+              file=sys.stderr)
         return 0
     tuned = None
     if form == N.FORM_TUNED and not args.no_autotune:
@@ -308,6 +312,21 @@ def main() -> int:
         plain_ms = (time.perf_counter() - t0) * 1000.0
         ps.close()
 
+    # the same bake with the later bounces replaying the first bounce's ray records (the same
+    # rays, so the same results; the traced rays a step drop to direct + one bounce + volume)
+    replay_ms = None
+    if form == N.FORM_TUNED and scene.settings.gi_bounces > 1:
+        rs = BakeSession(scene, gpu, form=form, graph=True, replay=True, **rebake)
+        rs.execute()
+        gpu.sync()
+        t0 = time.perf_counter()
+        for _ in range(max(args.steps, 1)):
+            rs.execute()
+        gpu.sync()
+        replay_ms = (time.perf_counter() - t0) * 1000.0 / max(args.steps, 1)
+        rs.close()
+        print(f"# replayed bounces: {replay_ms:.2f} ms a step", file=sys.stderr, flush=True)
+
     cpu = None if args.no_cpu else cpu_baseline(sess, gpu)
     if args.workload == "map":
         metric = "Grays/s, 100x100 m 3-bounce bake (direct + 3 gather bounces + volume)"
@@ -356,6 +375,10 @@ def main() -> int:
             "ms_per_tile": ms_step / max(info["tiles_baked"], 1),
             "plain_ms_per_step": plain_ms,
             "tuned_vs_plain": (plain_ms / ms_step) if plain_ms else None,
+            "replay_ms_per_step": replay_ms,
+            "replay_rays_traced_per_step": (stats[0]["rays"] + stats[1]["rays"] // bounces
+                                            + stats[2]["rays"]) if replay_ms else None,
+            "replay_vs_plain": (plain_ms / replay_ms) if plain_ms and replay_ms else None,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -292,6 +292,12 @@ typedef struct lb_bake_options {
     int graph;     /* nonzero: the bounce loop (gather + texel passes) replayed from a hipGraph */
     int threads;   /* host threads for the rasterizer (0: all) */
     const lb_bake_cache *cache; /* NULL: bake every tile and every probe */
+    /* nonzero: the first bounce records what each gather ray meets and the later bounces replay
+     * those records with the new light instead of tracing the same rays again (every bounce
+     * traces the same rays: the same keys and sample points). The results are the same as
+     * traced bounces: bit for bit in the PLAIN form, within the TUNED form's sum tolerance.
+     * 8 B a gather ray of device memory. */
+    int replay;
 } lb_bake_options;
 
 /* bake() (bake.rs:94): layout, BVH, the dirty tiles' texels, the direct pass, the texel passes,

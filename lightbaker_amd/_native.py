@@ -81,7 +81,7 @@ class BakeCache(C.Structure):
 
 class BakeOptions(C.Structure):
     _fields_ = [("form", C.c_int), ("graph", C.c_int), ("threads", C.c_int),
-                ("cache", C.POINTER(BakeCache))]
+                ("cache", C.POINTER(BakeCache)), ("replay", C.c_int)]
 
 
 class BakeInfo(C.Structure):

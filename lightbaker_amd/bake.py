@@ -140,14 +140,15 @@ def _array(lib, h, which, dtype):
 
 def bake(scene: Scene, gpu: Gpu, form: int = N.FORM_PLAIN, graph: bool = False,
          threads: int = 0, cache: TileCache | None = None, tile_dirty=None,
-         probe_dirty=None) -> Bake:
+         probe_dirty=None, replay: bool = False) -> Bake:
     """A bake; with `tile_dirty` / `probe_dirty` (a flag per layout tile / probe) a rebake that
-    traces only the dirty ones and takes the others from `cache` (bake.rs:123-136)."""
+    traces only the dirty ones and takes the others from `cache` (bake.rs:123-136). With
+    `replay`, the bounces after the first replay the first one's ray records."""
     lib = N.lib()
     d, keep = scene_desc(scene)
     cd, ckeep = _cache_desc(cache, tile_dirty, probe_dirty)
     opt = N.BakeOptions(int(form), int(graph), int(threads),
-                        C.pointer(cd) if cd is not None else None)
+                        C.pointer(cd) if cd is not None else None, int(replay))
     h = C.c_void_p()
     N.check(lib.lb_bake_run(gpu.handle, C.byref(d), C.byref(opt), C.byref(h)))
     try:
@@ -189,7 +190,7 @@ class BakeSession:
 
     def __init__(self, scene: Scene, gpu: Gpu, form: int = N.FORM_TUNED, graph: bool = True,
                  threads: int = 0, cache: TileCache | None = None, tile_dirty=None,
-                 probe_dirty=None):
+                 probe_dirty=None, replay: bool = False):
         self.lib = N.lib()
         self.gpu = gpu
         self.scene = scene
@@ -197,7 +198,7 @@ class BakeSession:
         cd, ckeep = _cache_desc(cache, tile_dirty, probe_dirty)
         self._keep += ckeep
         opt = N.BakeOptions(int(form), int(graph), int(threads),
-                            C.pointer(cd) if cd is not None else None)
+                            C.pointer(cd) if cd is not None else None, int(replay))
         h = C.c_void_p()
         N.check(self.lib.lb_bake_prepare(gpu.handle, C.byref(self._desc), C.byref(opt),
                                          C.byref(h)))

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "lb_bake_state.h"
+#include "lb_records.h"
 #include "lb_trace_plain.h"
 
 using namespace lbk;
@@ -29,29 +30,10 @@ using namespace lbk;
 namespace lbk {
 namespace {
 
-enum : uint32_t { REC_ATLAS = 0u, REC_FALLBACK = 1u, REC_BACK = 2u, REC_MISS = 3u };
 constexpr int8_t LEVEL_NEVER = 127;  // a texel no ring fills
 constexpr uint32_t BLOCK = 256;
 
-struct SkyP {
-    f3 zenith, horizon, ground, fallback;
-};
-
 __device__ __forceinline__ f3 f3of(float4 v) { return mk(v.x, v.y, v.z); }
-
-// the atlas texel bounced() (trace.wgsl:137-145) reads for a hit at o + d t on chart c
-__device__ __forceinline__ uint32_t hit_texel(const SceneView &S, const lb_params &P, f3 o, f3 d,
-                                              float t, uint32_t c) {
-    f3 p = add(o, scl(d, t));
-    float4 r0 = S.charts[c * 2u], r1 = S.charts[c * 2u + 1u];
-    float x = r0.x * p.x + r0.y * p.y + r0.z * p.z + r0.w * 1.0f;
-    float y = r1.x * p.x + r1.y * p.y + r1.z * p.z + r1.w * 1.0f;
-    float fx = floorf(x), fy = floorf(y);
-    int hx = (int)P.atlas[0] - 1, hy = (int)P.atlas[1] - 1;
-    int ix = !(fx >= 0.0f) ? 0 : (fx >= (float)hx ? hx : (int)fx);
-    int iy = !(fy >= 0.0f) ? 0 : (fy >= (float)hy ? hy : (int)fy);
-    return (uint32_t)iy * P.atlas[0] + (uint32_t)ix;
-}
 
 // Ray r of gather item i, traced as trace.wgsl:248-262 traces it (item keys from 0, as the bake
 // runs the pass), one lane a ray; its record at r * n + i.
@@ -74,17 +56,7 @@ __global__ __launch_bounds__(BLOCK) void k_record(SceneView S, lb_params P,
     f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
     Counts cnt{0, 0, 0, 0, 0, 0};
     Hit h = trace_plain<false, false>(S, P, o, d, P.reach[1], key + r + 4000u, cnt);
-    uint2 out;
-    if (h.tri == NONE) {
-        out = make_uint2(fbits(d.y), REC_MISS << 30);
-    } else if (h.back) {
-        out = make_uint2(0u, REC_BACK << 30);
-    } else {
-        uint32_t m = fbits(S.tris[h.tri * 3u].w), c = fbits(S.tris[h.tri * 3u + 1u].w);
-        out = c == NONE ? make_uint2(0u, (REC_FALLBACK << 30) | m)
-                        : make_uint2(hit_texel(S, P, o, d, h.t, c), (REC_ATLAS << 30) | m);
-    }
-    rec[t] = out;
+    rec[t] = gather_record(S, P, o, d, h.tri, h.back, h.t);
 }
 
 // The gather's result from the records: trace.wgsl:256-269's sum in ray order, with its
@@ -121,6 +93,27 @@ __global__ __launch_bounds__(BLOCK) void k_replay(const uint2 *__restrict__ rec,
     out[g] = make_float4(sum.x * PI / (float)rays, sum.y * PI / (float)rays,
                          sum.z * PI / (float)rays, back / (float)rays);
 }
+
+}  // namespace
+
+hipError_t record_gather(const SceneView &S, const lb_params &P, const lb_texel *items,
+                         uint32_t n, uint32_t rays, uint2 *rec, hipStream_t stream) {
+    uint64_t nrays = (uint64_t)n * rays;
+    if (!nrays) return hipSuccess;
+    hipLaunchKernelGGL(k_record, dim3((unsigned)((nrays + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
+                       stream, S, P, items, n, rays, rec);
+    return hipGetLastError();
+}
+
+hipError_t replay_gather(const uint2 *rec, uint32_t n, uint32_t rays, const float4 *albedo,
+                         const float4 *light, const SkyP &K, float4 *out, hipStream_t stream) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_replay, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, rec, n,
+                       rays, albedo, light, K, out);
+    return hipGetLastError();
+}
+
+namespace {
 
 // One ring of filter.rs:58-94 on the masks alone: the ring (level) at which each texel gets a
 // value and how many neighbours it averages. level -1: a value from the start.

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "lb_bake_state.h"
+#include "lb_records.h"
 
 using namespace lbk;
 
@@ -500,7 +501,34 @@ int bounce(lb_bake *B, lb_stats *acc, uint32_t k) {
     hipStream_t st = B->gpu->stream;
     TexelView &V = B->V;
     uint32_t T = (uint32_t)B->T;
-    if (!acc && ordered_gather(B)) {
+    bool replay = !acc && B->opt.replay && B->sc.gi_bounces > 1 && B->d_rec.ptr;
+    if (replay && k > 0) {
+        // the rays of the first bounce, met again: their records with this bounce's atlas
+        hipError_t e = replay_gather((const uint2 *)B->d_rec.ptr, (uint32_t)B->ng,
+                                     B->P.counts[2], (const float4 *)B->scene->materials.ptr,
+                                     (const float4 *)B->d_light.ptr, sky_of(B->P),
+                                     (float4 *)B->d_gres.ptr, st);
+        if (e != hipSuccess) return hip_fail(e, "replay_gather");
+    } else if (replay) {
+        const Tuning &t = B->gpu->tuning[LB_PASS_GATHER];
+        if (B->opt.form == LB_FORM_TUNED && t.schedule == 0) {
+            // the lock-step gather writes each ray's record as it shades it
+            GatherOrder o;
+            o.rec = (uint2 *)B->d_rec.ptr;
+            TRY(run_pass(B->gpu, LB_PASS_GATHER, B->opt.form, B->scene, &B->P,
+                         (const lb_texel *)B->d_gitems.ptr, B->ng, 0,
+                         (const float *)B->d_light.ptr, (size_t)B->aw * B->ah,
+                         (float *)B->d_gres.ptr, false, &o));
+        } else {
+            TRY(pass_stats(B, LB_PASS_GATHER, &B->P, (const lb_texel *)B->d_gitems.ptr, B->ng,
+                           (const float *)B->d_light.ptr, (size_t)B->aw * B->ah,
+                           (float *)B->d_gres.ptr, nullptr));
+            hipError_t e = record_gather(scene_view(B->scene, nullptr), B->P,
+                                         (const lb_texel *)B->d_gitems.ptr, (uint32_t)B->ng,
+                                         B->P.counts[2], (uint2 *)B->d_rec.ptr, st);
+            if (e != hipSuccess) return hip_fail(e, "record_gather");
+        }
+    } else if (!acc && ordered_gather(B)) {
         // the first bounce counts each ray's steps, the rest take the rays ranked by them
         GatherOrder o;
         o.cost = k == 0 ? (uint8_t *)B->d_cost.ptr : nullptr;
@@ -610,6 +638,7 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
         if (B->opt.form == LB_FORM_TUNED && B->P.counts[2] <= 256u && sc.gi_bounces > 1 &&
             (B->d_cost.reserve(nr) || B->d_perm.reserve(nr)))
             return 1;
+        if (B->opt.replay && sc.gi_bounces > 1 && B->d_rec.reserve(nr * 8)) return 1;
         if (B->opt.graph && !per_pass) {
             // the bounces captured once and replayed (same buffers each time); the item count is
             // part of the capture, so a new count captures again
@@ -867,7 +896,7 @@ int lb_bake_prepare(lb_gpu *gpu, const lb_scene_desc *scene, const lb_bake_optio
                     lb_bake **out) {
     if (!gpu || !scene || !out) return fail("lb_bake_prepare: null handle");
     *out = nullptr;
-    lb_bake_options opt{LB_FORM_PLAIN, 0, 0, nullptr};
+    lb_bake_options opt{LB_FORM_PLAIN, 0, 0, nullptr, 0};
     if (options) opt = *options;
     if (opt.form != LB_FORM_PLAIN && opt.form != LB_FORM_TUNED && opt.form != LB_FORM_WATERTIGHT)
         return fail("lb_bake_prepare: no form %d", opt.form);

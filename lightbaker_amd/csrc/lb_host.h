@@ -67,9 +67,11 @@ SceneView scene_view(const lb_scene *s, const float *light);
 // The order a TUNED lock-step gather takes each item's rays in (one byte a ray, item-major): with
 // `cost`, the launch records each ray's traversal steps by its azimuth slot (a counting launch);
 // with `perm`, slot s of item i traces azimuth slot perm[i * rays + s]. Either needs rays <= 256.
+// With `rec`, the launch also writes each ray's record (lb_records.h) at r * items + i.
 struct GatherOrder {
     uint8_t *cost = nullptr;
     const uint8_t *perm = nullptr;
+    uint2 *rec = nullptr;
 };
 int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_params *params,
              const lb_texel *items, size_t n, uint32_t key_base, const float *light,

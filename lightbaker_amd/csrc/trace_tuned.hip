@@ -15,6 +15,7 @@
 // FP32 (WIDE, 64 B a visit) or 8-bit outward-rounded (QUANT, 32 B a visit). The stack's first
 // `lds_stack` entries live in LDS; deeper entries spill to a per-lane global area.
 #include "lb_host.h"
+#include "lb_records.h"
 #include "lb_traverse.h"
 
 namespace lbk {
@@ -160,6 +161,7 @@ struct TunedArgs {
     int leaf_batch;  // lanes at a leaf that end a wave's inner phase early (64: never)
     uint8_t *cost;        // gather, counting launch: each ray's steps by azimuth slot (or null)
     const uint8_t *perm;  // gather: slot s of item i takes azimuth slot perm[i * rays + s]
+    uint2 *rec;           // gather: each ray's record at r * n_items + i (or null)
 };
 
 
@@ -429,6 +431,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
         KSum sr{0, 0}, sg{0, 0}, sb{0, 0};
         float back = 0.0f;
         uint64_t steps0 = 0;
+        uint32_t ray_of_slot = 0;
         auto start = [&](uint32_t slot, Ray &R) {
             // read the slot afresh (the compiler would otherwise keep the stored values live)
             asm volatile("" ::: "memory");
@@ -442,6 +445,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
             uint32_t key = slot_lds[20];
             uint32_t s = A.perm ? (uint32_t)A.perm[(size_t)i * rays + slot] : slot;
             uint32_t r = az_slot(s, rays, s2);
+            ray_of_slot = r;
             if (STATS) steps0 = cnt.inner + cnt.tris;
             float jx = rnd(key + r, 19u, seed) - 0.5f;
             float jy = rnd(key + r, 23u, seed) - 0.5f;
@@ -450,6 +454,9 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
             ray_start(R, A.S.root, o, d, P.reach[1], false, key + r + 4000u);
         };
         auto shade = [&](uint32_t slot, const Ray &R) {
+            if (A.rec)
+                A.rec[(size_t)ray_of_slot * A.n_items + i] =
+                    gather_record(A.S, P, R.o, R.d, R.tri, R.back, R.t);
             if (STATS && A.cost) {
                 uint64_t steps = cnt.inner + cnt.tris - steps0;
                 uint32_t s = A.perm ? (uint32_t)A.perm[(size_t)i * rays + slot] : slot;
@@ -718,6 +725,7 @@ hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
     bool ordered = order && pass == LB_PASS_GATHER && P.counts[2] <= 256u;
     A.cost = ordered && stats ? order->cost : nullptr;
     A.perm = ordered ? order->perm : nullptr;
+    A.rec = order && pass == LB_PASS_GATHER ? order->rec : nullptr;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, gpu->stream, gpu->queue);
     hipLaunchKernelGGL(tg.fn, dim3(tg.grid), dim3(tg.block), tg.lds, gpu->stream, A);
     return hipGetLastError();
