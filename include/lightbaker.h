@@ -290,7 +290,7 @@ typedef struct lb_bake_cache {
 typedef struct lb_bake_options {
     int form;      /* LB_FORM_* of the ray passes */
     int graph;     /* nonzero: the bounce loop (gather + texel passes) replayed from a hipGraph */
-    int threads;   /* host threads for the rasterizer (0: all) */
+    int threads;   /* host threads (0: all); the tiles are rasterized on the device */
     const lb_bake_cache *cache; /* NULL: bake every tile and every probe */
     /* nonzero: the first bounce records what each gather ray meets and the later bounces replay
      * those records with the new light instead of tracing the same rays again (every bounce

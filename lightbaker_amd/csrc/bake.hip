@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "lb_bake_state.h"
+#include "lb_raster.h"
 #include "lb_records.h"
 
 using namespace lbk;
@@ -264,25 +265,11 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
         }
     }
 
-    // the dirty tiles' texels (bake.rs:139-148)
+    // the dirty tiles' texels (bake.rs:139-148), rasterized on the device (raster.rs:39-114)
     size_t nd = B->baked.size();
     I.tiles = (uint32_t)nt;
     I.tiles_baked = (uint32_t)nd;
     I.charts = (uint32_t)n_charts;
-    std::vector<std::vector<lb_texel>> tex(nd);
-    {
-        unsigned hw = opt.threads > 0 ? (unsigned)opt.threads
-                                      : std::max(1u, std::thread::hardware_concurrency());
-        std::vector<std::thread> pool;
-        std::atomic<size_t> next{0};
-        size_t workers = std::min<size_t>(hw, std::max<size_t>(nd, 1));
-        for (size_t w = 0; w < workers; w++)
-            pool.emplace_back([&] {
-                for (size_t k; (k = next.fetch_add(1)) < nd;)
-                    rasterize(sc, L, L.tiles[B->baked[k]], tex[k]);
-            });
-        for (auto &th : pool) th.join();
-    }
     B->chart_tiles = L.chart_tile;
     B->tiles.resize(nt * 6);
     for (size_t t = 0; t < nt; t++) {
@@ -298,31 +285,46 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     size_t T = B->toff[nd];
     if (T > 0x7fffffffu) return fail("bake: %zu texels exceed 31-bit indices", T);
     B->T = T;
-    B->texels.resize(T);
     std::vector<uint32_t> tile_of(T);
-    B->chart.resize(T);
-    B->normal.assign(T * 4, 0.0f);
-    B->covered.resize(T);
     std::vector<uint4> tiles(std::max<size_t>(nd, 1));
     for (size_t t = 0; t < nd; t++) {
         const Tile &tl = L.tiles[B->baked[t]];
         uint32_t o = B->toff[t];
         tiles[t] = make_uint4(o, tl.w, tl.h, tl.x | (tl.y << 16));
-        if (!tex[t].empty()) std::memcpy(&B->texels[o], tex[t].data(), tex[t].size() * sizeof(lb_texel));
-        for (uint32_t i = 0; i < tl.w * tl.h; i++) {
-            size_t gi = o + i;
-            const lb_texel &x = tex[t][i];
-            tile_of[gi] = (uint32_t)t;
-            bool cv = x.pos[3] > 0.5f;
-            B->covered[gi] = cv;
-            uint32_t c;
-            std::memcpy(&c, &x.normal[3], 4);
-            B->chart[gi] = cv ? c : 0xffffffffu;
-            for (int k = 0; k < 3; k++) B->normal[gi * 4 + k] = x.normal[k];
-            if (cv) B->cov.push_back((uint32_t)gi);
-        }
+        std::fill(tile_of.begin() + o, tile_of.begin() + o + (size_t)tl.w * tl.h, (uint32_t)t);
     }
-    tex.clear();
+    {
+        std::vector<RasterTri> recs;
+        raster_records(sc, L, B->baked, recs);
+        if (recs.size() > 0x7fffffffu) return fail("bake: %zu triangles to rasterize", recs.size());
+        size_t Tn0 = std::max<size_t>(T, 1);
+        TRY(put(B->d_tiles, tiles, st));
+        TRY(put(B->d_tile_of, tile_of, st));
+        TRY(put(B->d_raster, recs, st));
+        TRY(B->d_owner.reserve(Tn0 * 4));
+        TRY(B->d_tex0.reserve(Tn0 * sizeof(lb_texel)));
+        TRYH(raster_tiles((const RasterTri *)B->d_raster.ptr, (uint32_t)recs.size(),
+                          (const uint4 *)B->d_tiles.ptr, (const uint32_t *)B->d_tile_of.ptr,
+                          (uint32_t)T, (uint32_t *)B->d_owner.ptr, (lb_texel *)B->d_tex0.ptr, st));
+        B->texels.resize(T);
+        if (T)
+            TRYH(hipMemcpyAsync(B->texels.data(), B->d_tex0.ptr, T * sizeof(lb_texel),
+                                hipMemcpyDeviceToHost, st));
+        TRYH(hipStreamSynchronize(st));
+    }
+    B->chart.resize(T);
+    B->normal.assign(T * 4, 0.0f);
+    B->covered.resize(T);
+    for (size_t gi = 0; gi < T; gi++) {
+        const lb_texel &x = B->texels[gi];
+        bool cv = x.pos[3] > 0.5f;
+        B->covered[gi] = cv;
+        uint32_t c;
+        std::memcpy(&c, &x.normal[3], 4);
+        B->chart[gi] = cv ? c : 0xffffffffu;
+        for (int k = 0; k < 3; k++) B->normal[gi * 4 + k] = x.normal[k];
+        if (cv) B->cov.push_back((uint32_t)gi);
+    }
     B->ncov = B->cov.size();
     I.texels = T;
     I.texels_baked = B->ncov;
@@ -382,10 +384,7 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
 
     // device state
     size_t Tn = std::max<size_t>(T, 1);
-    TRY(put(B->d_tex0, B->texels, st));
     TRY(B->d_tex.reserve(Tn * sizeof(lb_texel)));
-    TRY(put(B->d_tile_of, tile_of, st));
-    TRY(put(B->d_tiles, tiles, st));
     TRY(put(B->d_chart, B->chart, st));
     TRY(put(B->d_normal, B->normal, st));
     TRY(put(B->d_covered, B->covered, st));

@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "lb_bake.h"
+#include "lb_raster.h"
 
 namespace lbk {
 
@@ -102,6 +103,59 @@ void rasterize(const lb_scene_desc &s, const Layout &L, const Tile &tile, std::v
                                       {dy.x, dy.y, dy.z, 0.0f}};
                     }
                 }
+            }
+        }
+    }
+}
+
+void raster_records(const lb_scene_desc &s, const Layout &L, const std::vector<uint32_t> &tiles,
+                    std::vector<RasterTri> &out) {
+    out.clear();
+    for (size_t k = 0; k < tiles.size(); k++) {
+        const Tile &tile = L.tiles[tiles[k]];
+        int32_t w = (int32_t)tile.w, h = (int32_t)tile.h;
+        for (uint32_t ci : tile.charts) {
+            const Chart &c = L.charts[ci];
+            for (uint32_t t : c.tris) {
+                const float *pp = s.meshes[c.mesh].positions + (size_t)t * 9;
+                V3 pa{pp[0], pp[1], pp[2]}, pb{pp[3], pp[4], pp[5]}, pc{pp[6], pp[7], pp[8]};
+                V2 ua = chart_local(c, pa, L.density, L.padding);
+                V2 ub = chart_local(c, pb, L.density, L.padding);
+                V2 uc = chart_local(c, pc, L.density, L.padding);
+                float det = (ub.y - uc.y) * (ua.x - uc.x) + (uc.x - ub.x) * (ua.y - uc.y);
+                if (std::fabs(det) < 1e-12f) continue;
+                auto at = [&](V2 q) {
+                    float l1 = ((ub.y - uc.y) * (q.x - uc.x) + (uc.x - ub.x) * (q.y - uc.y)) / det;
+                    float l2 = ((uc.y - ua.y) * (q.x - uc.x) + (ua.x - uc.x) * (q.y - uc.y)) / det;
+                    float l3 = 1.0f - l1 - l2;
+                    return pa * l1 + pb * l2 + pc * l3;
+                };
+                V3 n = face_normal(s, c.mesh, t);
+                V3 o = at(V2{0, 0});
+                V3 dx = at(V2{1, 0}) - o, dy = at(V2{0, 1}) - o;
+                V2 lo{std::fmax(std::floor(std::fmin(std::fmin(ua.x, ub.x), uc.x)), 0.0f),
+                      std::fmax(std::floor(std::fmin(std::fmin(ua.y, ub.y), uc.y)), 0.0f)};
+                V2 hi{std::fmin(std::ceil(std::fmax(std::fmax(ua.x, ub.x), uc.x)), (float)(w - 1)),
+                      std::fmin(std::ceil(std::fmax(std::fmax(ua.y, ub.y), uc.y)), (float)(h - 1))};
+                RasterTri r{};
+                r.ua[0] = ua.x; r.ua[1] = ua.y;
+                r.ub[0] = ub.x; r.ub[1] = ub.y;
+                r.uc[0] = uc.x; r.uc[1] = uc.y;
+                r.det = det;
+                const V3 *src[6] = {&pa, &pb, &pc, &n, &dx, &dy};
+                float *dst[6] = {r.pa, r.pb, r.pc, r.n, r.dx, r.dy};
+                for (int q = 0; q < 6; q++) {
+                    dst[q][0] = src[q]->x;
+                    dst[q][1] = src[q]->y;
+                    dst[q][2] = src[q]->z;
+                }
+                r.chart = ci;
+                r.tile = (uint32_t)k;
+                r.lo[0] = (int32_t)lo.x;
+                r.lo[1] = (int32_t)lo.y;
+                r.hi[0] = (int32_t)hi.x;
+                r.hi[1] = (int32_t)hi.y;
+                out.push_back(r);
             }
         }
     }
