@@ -40,23 +40,25 @@ __device__ __forceinline__ f3 f3of(float4 v) { return mk(v.x, v.y, v.z); }
 __global__ __launch_bounds__(BLOCK) void k_record(SceneView S, lb_params P,
                                                   const lb_texel *__restrict__ items, uint32_t n,
                                                   uint32_t rays, uint2 *__restrict__ rec) {
-    uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= (uint64_t)n * rays) return;
-    uint32_t r = (uint32_t)(t / n), i = (uint32_t)(t % n);
-    const float4 *tp = reinterpret_cast<const float4 *>(items + i);
-    float4 tpos = tp[0], tnrm = tp[1], tdx = tp[2], tdy = tp[3];
-    f3 nrm = xyz(tnrm);
-    frame_t f = frame(nrm);
-    uint32_t key = i * 7919u;
-    uint32_t seed = fbits(P.reach[3]);
-    float s1 = rnd(key, 11u, seed), s2 = rnd(key, 13u, seed);
-    float jx = rnd(key + r, 19u, seed) - 0.5f;
-    float jy = rnd(key + r, 23u, seed) - 0.5f;
-    f3 o = add(add(add(xyz(tpos), scl(xyz(tdx), jx)), scl(xyz(tdy), jy)), scl(nrm, P.reach[2]));
-    f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
-    Counts cnt{0, 0, 0, 0, 0, 0};
-    Hit h = trace_plain<false, false>(S, P, o, d, P.reach[1], key + r + 4000u, cnt);
-    rec[t] = gather_record(S, P, o, d, h.tri, h.back, h.t);
+    // rays a lane over the grid (a launch holds under 2^32 work-items)
+    for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < (uint64_t)n * rays;
+         t += (uint64_t)gridDim.x * BLOCK) {
+        uint32_t r = (uint32_t)(t / n), i = (uint32_t)(t % n);
+        const float4 *tp = reinterpret_cast<const float4 *>(items + i);
+        float4 tpos = tp[0], tnrm = tp[1], tdx = tp[2], tdy = tp[3];
+        f3 nrm = xyz(tnrm);
+        frame_t f = frame(nrm);
+        uint32_t key = i * 7919u;
+        uint32_t seed = fbits(P.reach[3]);
+        float s1 = rnd(key, 11u, seed), s2 = rnd(key, 13u, seed);
+        float jx = rnd(key + r, 19u, seed) - 0.5f;
+        float jy = rnd(key + r, 23u, seed) - 0.5f;
+        f3 o = add(add(add(xyz(tpos), scl(xyz(tdx), jx)), scl(xyz(tdy), jy)), scl(nrm, P.reach[2]));
+        f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
+        Counts cnt{0, 0, 0, 0, 0, 0};
+        Hit h = trace_plain<false, false>(S, P, o, d, P.reach[1], key + r + 4000u, cnt);
+        rec[t] = gather_record(S, P, o, d, h.tri, h.back, h.t);
+    }
 }
 
 // The gather's result from the records: trace.wgsl:256-269's sum in ray order, with its
@@ -100,8 +102,9 @@ hipError_t record_gather(const SceneView &S, const lb_params &P, const lb_texel 
                          uint32_t n, uint32_t rays, uint2 *rec, hipStream_t stream) {
     uint64_t nrays = (uint64_t)n * rays;
     if (!nrays) return hipSuccess;
-    hipLaunchKernelGGL(k_record, dim3((unsigned)((nrays + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
-                       stream, S, P, items, n, rays, rec);
+    uint64_t blocks = (nrays + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(k_record, dim3((unsigned)(blocks < (1u << 22) ? blocks : (1u << 22))),
+                       dim3(BLOCK), 0, stream, S, P, items, n, rays, rec);
     return hipGetLastError();
 }
 
@@ -449,7 +452,7 @@ int adjoint_create(lb_bake *B, lb_adjoint *A) {
     // the records (the gather items are the bake's, resident since its execute)
     SceneView S = scene_view(B->scene, nullptr);
     if (nrays)
-        hipLaunchKernelGGL(k_record, grid_for(nrays), dim3(BLOCK), 0, st, S, B->P,
+        hipLaunchKernelGGL(k_record, dim3((unsigned)std::min<uint64_t>((nrays + BLOCK - 1) / BLOCK, 1u << 22)), dim3(BLOCK), 0, st, S, B->P,
                            (const lb_texel *)B->d_gitems.ptr, A->ng, A->rays, (uint2 *)A->rec.ptr);
 
     // the bounce fill's plan and the bounce smoothing's weight sums (bake.rs:318-320)

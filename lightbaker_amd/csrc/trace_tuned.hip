@@ -663,34 +663,40 @@ static hipError_t tuned_grid(lb_gpu *gpu, int pass, bool stats, const lb_params 
 // Each item's azimuth slots sorted by (recorded cost, slot): a bitonic sort of 256 keys in LDS,
 // one workgroup an item. Slot s of an ordered gather then takes the s-th cheapest ray, so the
 // lanes of one round trace rays of like length.
+// Items a workgroup at a time over the grid (a launch holds under 2^32 work-items).
 __global__ __launch_bounds__(256) void k_rank_slots(const uint8_t *__restrict__ cost,
-                                                    uint32_t rays, uint8_t *__restrict__ perm) {
+                                                    uint32_t n_items, uint32_t rays,
+                                                    uint8_t *__restrict__ perm) {
     __shared__ uint32_t key[256];
     uint32_t t = threadIdx.x;
-    size_t base = (size_t)blockIdx.x * rays;
-    key[t] = t < rays ? ((uint32_t)cost[base + t] << 16) | t : 0xffffffffu;
-    __syncthreads();
-    for (uint32_t size = 2; size <= 256u; size <<= 1)
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            uint32_t j = t ^ stride;
-            if (j > t) {
-                bool up = (t & size) == 0u;
-                uint32_t a = key[t], b = key[j];
-                if ((a > b) == up) {
-                    key[t] = b;
-                    key[j] = a;
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        size_t base = (size_t)item * rays;
+        key[t] = t < rays ? ((uint32_t)cost[base + t] << 16) | t : 0xffffffffu;
+        __syncthreads();
+        for (uint32_t size = 2; size <= 256u; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                uint32_t j = t ^ stride;
+                if (j > t) {
+                    bool up = (t & size) == 0u;
+                    uint32_t a = key[t], b = key[j];
+                    if ((a > b) == up) {
+                        key[t] = b;
+                        key[j] = a;
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
-        }
-    if (t < rays) perm[base + t] = (uint8_t)(key[t] & 0xffffu);
+        if (t < rays) perm[base + t] = (uint8_t)(key[t] & 0xffffu);
+        __syncthreads();
+    }
 }
 
 hipError_t rank_slots(const uint8_t *cost, uint32_t n_items, uint32_t rays, uint8_t *perm,
                       hipStream_t stream) {
     if (!n_items) return hipSuccess;
     if (rays > 256u) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_rank_slots, dim3(n_items), dim3(256), 0, stream, cost, rays, perm);
+    uint32_t grid = n_items < (1u << 20) ? n_items : (1u << 20);
+    hipLaunchKernelGGL(k_rank_slots, dim3(grid), dim3(256), 0, stream, cost, n_items, rays, perm);
     return hipGetLastError();
 }
 
