@@ -317,7 +317,11 @@ void lb_bake_destroy(lb_bake *bake);
  *            volume); repeatable: each call starts from the prepared state. With `per_pass`
  *            (3 entries: direct, gather over all bounces, volume) the counting kernels run and
  *            the traversal counters are returned;
- *   finish   read-back and host work after the rays (volume.rs:86-117, bake.rs:337-421). */
+ *   finish   after the rays: the probes' inside fill on the host (volume.rs:86-117), then on the
+ *            device every tile's TileLight with the texels lit from the volume (bake.rs:337-362),
+ *            the atlas, the indirect atlas and its fill (bake.rs:364-421) and the shipped
+ *            encodings (output.rs:95-110); the direct, indirect and validity atlases are read
+ *            back, every other array when lb_bake_array asks for it. */
 int lb_bake_prepare(lb_gpu *gpu, const lb_scene_desc *scene, const lb_bake_options *options,
                     lb_bake **out);
 int lb_bake_execute(lb_bake *bake, lb_stats *per_pass);
@@ -339,6 +343,8 @@ typedef struct lb_bake_info {
     float ms_direct, ms_direct_texel, ms_bounces, ms_volume;
     /* host time of each stage, s */
     double s_layout, s_bvh, s_raster, s_gather_set, s_execute, s_finish, s_total;
+    float volume_scale;  /* output.rs:108: the brightest L0 channel, what a volume byte of 255 is */
+    float ms_finish;     /* device time of finish's kernels, ms */
 } lb_bake_info;
 
 int lb_bake_info_get(const lb_bake *bake, lb_bake_info *info);
@@ -369,11 +375,31 @@ enum lb_bake_array {
     LB_BAKE_TILE_INDIRECT = 20, /* f32 x3 */
     LB_BAKE_PROBES = 21,        /* f32 x16 per probe before the inside fill (volume.rs:81-84) */
     LB_BAKE_BAKED_TILES = 22,   /* u32 per baked tile: its index in the layout */
-    LB_BAKE_LIGHT0 = 23         /* f32 x4 per atlas texel: the clean tiles' light, the rest 0 */
+    LB_BAKE_LIGHT0 = 23,        /* f32 x4 per atlas texel: the clean tiles' light, the rest 0 */
+    /* the atlases as they ship (output.rs:99-110, after a finish) */
+    LB_BAKE_DIRECT_BC5 = 24,       /* BC5 blocks of the direct atlas (encode.rs:37-50), w * h bytes */
+    LB_BAKE_INDIRECT_RGB9E5 = 25,  /* u32 per indirect texel (encode.rs:53-69) */
+    LB_BAKE_VOLUME_BYTES = 26      /* 16 bytes per probe (output.rs:165-181) */
 };
 
-/* The bake's array `which`: its bytes, with *data pointing at them (owned by the bake). */
+/* The bake's array `which`: its bytes, with *data pointing at them (owned by the bake, valid
+ * until the next execute, finish or destroy). Arrays that stay on the device until asked for are
+ * read back here; 0 when the array is empty or the read-back fails (lb_last_error says which). */
 size_t lb_bake_array(const lb_bake *bake, int which, const void **data);
+
+/* ---- the output codecs over device memory (encode.rs:7-69, output.rs:101,108,165-181) ------ *
+ * Queued on the gpu's stream; lb_encode_volume waits for its scale. Buffers are device
+ * pointers: `rg` 8-byte aligned, `out` of BC5 and of the volume 16-byte aligned. */
+
+/* encode.rs:37-50 `bc5`: rg (w * h byte pairs, row-major) -> w * h bytes of blocks (each 4 x 4
+ * block's red then green BC4 block, blocks row-major). w and h multiples of 4 (the reference
+ * asserts it; here an error). */
+int lb_encode_bc5(lb_gpu *gpu, const uint8_t *rg, uint32_t w, uint32_t h, uint8_t *out);
+/* encode.rs:53-69 `rgb9e5` over n rgb triples (f32 x3) -> n words */
+int lb_encode_rgb9e5(lb_gpu *gpu, const float *rgb, size_t n, uint32_t *out);
+/* output.rs:108,165-181: n probes (f32 x16, volume.rs:25-31) -> 16 bytes each, and the scale
+ * (host) the L0 bytes are relative to */
+int lb_encode_volume(lb_gpu *gpu, const float *probes, size_t n, uint8_t *out, float *scale);
 
 /* ---- the adjoint of the bounces ------------------------------------------------------------ *
  * No counterpart in the reference, which has no gradients; it serves callers that fit the

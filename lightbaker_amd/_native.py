@@ -95,7 +95,8 @@ class BakeInfo(C.Structure):
         ("ms_direct", C.c_float), ("ms_direct_texel", C.c_float), ("ms_bounces", C.c_float),
         ("ms_volume", C.c_float), ("s_layout", C.c_double), ("s_bvh", C.c_double),
         ("s_raster", C.c_double), ("s_gather_set", C.c_double), ("s_execute", C.c_double),
-        ("s_finish", C.c_double), ("s_total", C.c_double),
+        ("s_finish", C.c_double), ("s_total", C.c_double), ("volume_scale", C.c_float),
+        ("ms_finish", C.c_float),
     ]
 
 
@@ -111,6 +112,7 @@ BAKE_TRI_CHARTS, BAKE_LIGHT, BAKE_CHART_TILES, BAKE_TILES = 5, 6, 7, 8
 BAKE_NODES, BAKE_TRIS, BAKE_MATERIALS, BAKE_PARAMS, BAKE_TEXELS, BAKE_GATHER_INDEX = 9, 10, 11, 12, 13, 14
 BAKE_TILE_KNOWN, BAKE_TILE_STATE, BAKE_TILE_FIELD, BAKE_TILE_AO = 15, 16, 17, 18
 BAKE_TILE_DIRECT, BAKE_TILE_INDIRECT, BAKE_PROBES, BAKE_BAKED_TILES, BAKE_LIGHT0 = 19, 20, 21, 22, 23
+BAKE_DIRECT_BC5, BAKE_INDIRECT_RGB9E5, BAKE_VOLUME_BYTES = 24, 25, 26
 
 # Every symbol include/lightbaker.h declares, with its ctypes signature.
 _VP, _SZ, _U32, _I = C.c_void_p, C.c_size_t, C.c_uint32, C.c_int
@@ -160,6 +162,9 @@ SIGNATURES = {
     "lb_layout_size": (_I, [_VP, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_SZ), C.POINTER(_SZ)]),
     "lb_layout_get": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "lb_layout_rasterize": (_I, [C.POINTER(SceneDesc), _VP, _SZ, _VP, _SZ]),
+    "lb_encode_bc5": (_I, [_VP, _VP, _U32, _U32, _VP]),
+    "lb_encode_rgb9e5": (_I, [_VP, _VP, _SZ, _VP]),
+    "lb_encode_volume": (_I, [_VP, _VP, _SZ, _VP, C.POINTER(C.c_float)]),
 }
 
 

@@ -26,6 +26,10 @@ class Bake:
     chart_tiles: np.ndarray
     tiles: np.ndarray       # (tiles, 6): key x, key z, w, h, atlas x, atlas y
     cache: "TileCache"      # what the reference's cache would hold after this bake
+    # the atlases as they ship (output.rs:99-110), encoded on the device
+    direct_bc5: np.ndarray | None = None        # (aw * ah,) bytes of BC5 blocks
+    indirect_rgb9e5: np.ndarray | None = None   # (ih, iw) u32
+    volume_bytes: np.ndarray | None = None      # (probes, 16) u8; scale in info["volume_scale"]
 
 
 @dataclass
@@ -178,6 +182,10 @@ def bake(scene: Scene, gpu: Gpu, form: int = N.FORM_PLAIN, graph: bool = False,
             chart_tiles=_array(lib, h, N.BAKE_CHART_TILES, np.uint32),
             tiles=_array(lib, h, N.BAKE_TILES, np.int32).reshape(-1, 6),
             cache=_tile_cache(lib, h),
+            direct_bc5=_array(lib, h, N.BAKE_DIRECT_BC5, np.uint8),
+            indirect_rgb9e5=_array(lib, h, N.BAKE_INDIRECT_RGB9E5, np.uint32).reshape(ih, iw),
+            volume_bytes=(_array(lib, h, N.BAKE_VOLUME_BYTES, np.uint8).reshape(-1, 16)
+                          if vol is not None else None),
         )
     finally:
         lib.lb_bake_destroy(h)

@@ -4,8 +4,9 @@
 //   execute  every device pass on resident data, in one stream: the direct pass, its texel
 //            passes, the gather set, the bounces (optionally replayed from a captured hipGraph)
 //            and the volume pass; repeatable (it starts from the prepared state each time);
-//   finish   the read-back and the host's part after the rays: probes inside solids, the
-//            volume's light for texels with none, the atlas (volume.rs:86-117, bake.rs:337-421).
+//   finish   after the rays: probes inside solids on the host (volume.rs:86-117), then on the
+//            device the volume's light for texels with none, every tile's TileLight, the atlas
+//            and the shipped encodings (bake.rs:337-421, output.rs:95-110), read back.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,8 +17,10 @@
 #include <memory>
 #include <set>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
+#include "lb_atlas.h"
 #include "lb_bake_state.h"
 #include "lb_raster.h"
 #include "lb_records.h"
@@ -37,6 +40,12 @@ double secs(clk::time_point a) { return std::chrono::duration<double>(clk::now()
 template <typename T>
 int put(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
     return b.upload(v.data(), v.size() * sizeof(T), s);
+}
+
+TileLightView tile_light_view(const lb_bake *B) {
+    return TileLightView{(uint8_t *)B->d_tl_known.ptr, (uint8_t *)B->d_tl_state.ptr,
+                         (float *)B->d_tl_field.ptr,   (float *)B->d_tl_ao.ptr,
+                         (float *)B->d_tl_direct.ptr,  (float *)B->d_tl_indirect.ptr};
 }
 
 #define TRYH(expr)                                        \
@@ -62,34 +71,6 @@ std::vector<float> gauss(int r) {
     return w;
 }
 
-// volume.rs:125-147
-void volume_irradiance(const std::vector<float> &probes, const uint32_t size[3],
-                       const float origin[3], float step, V3 p, V3 n, float e[3], float *sun) {
-    size_t nx = size[0], ny = size[1], nz = size[2];
-    V3 f = p - V3{origin[0], origin[1], origin[2]};
-    f = {f.x / step, f.y / step, f.z / step};
-    auto cl = [](float v, size_t k) { return std::fmin(std::fmax(v, 0.0f), (float)(k - 1)); };
-    float fx = cl(f.x, nx), fy = cl(f.y, ny), fz = cl(f.z, nz);
-    size_t x0 = std::min((size_t)fx, nx - 2), y0 = std::min((size_t)fy, ny - 2),
-           z0 = std::min((size_t)fz, nz - 2);
-    float tx = fx - (float)x0, ty = fy - (float)y0, tz = fz - (float)z0;
-    float p4[4][4] = {};
-    for (int k = 0; k < 8; k++) {
-        int dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
-        float w = (dx ? tx : 1.0f - tx) * (dy ? ty : 1.0f - ty) * (dz ? tz : 1.0f - tz);
-        const float *q = &probes[(((y0 + dy) * nz + (z0 + dz)) * nx + (x0 + dx)) * 16];
-        for (int a = 0; a < 4; a++)
-            for (int c = 0; c < 4; c++) p4[a][c] += q[a * 4 + c] * w;
-    }
-    const float PIF = 3.14159265358979323846f;
-    float a0 = PIF * 0.282095f, a1 = 2.0f * PIF / 3.0f * 0.488603f;
-    for (int c = 0; c < 3; c++)
-        e[c] = 0.5f * a0 * p4[0][c] +
-               0.5f * std::fmax(a0 * p4[0][c] +
-                                    a1 * (p4[1][c] * n.x + p4[2][c] * n.y + p4[3][c] * n.z),
-                                0.0f);
-    *sun = p4[0][3];
-}
 
 }  // namespace
 }  // namespace lbk
@@ -216,6 +197,8 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     for (size_t t = 0; t < nt; t++)
         B->toff_all[t + 1] = B->toff_all[t] + (size_t)L.tiles[t].w * L.tiles[t].h;
     size_t T_all = B->toff_all[nt];
+    if (T_all > 0x7fffffffu) return fail("bake: %zu texels exceed 31-bit indices", T_all);
+    B->T_all = T_all;
     const lb_bake_cache *cache = opt.cache;
     for (size_t t = 0; t < nt; t++)
         if (!cache || !cache->tile_dirty || cache->tile_dirty[t]) B->baked.push_back((uint32_t)t);
@@ -227,42 +210,51 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
         if (c.n_texels != T_all)
             return fail("bake cache: %zu texels, the layout's tiles hold %zu", c.n_texels, T_all);
     }
-    B->tl_known.assign(T_all, 0);
-    B->tl_state.assign(T_all, 0);
-    B->tl_field.assign(T_all, 0.0f);
-    B->tl_ao.assign(T_all, 0.0f);
-    B->tl_direct.assign(T_all * 3, 0.0f);
-    B->tl_indirect.assign(T_all * 3, 0.0f);
-    if (B->any_clean) {
-        // write_light (bake.rs:435-444) of the clean tiles: the atlas the first bounce reads
-        const lb_bake_cache &c = *cache;
-        B->light0.assign((size_t)aw * ah * 4, 0.0f);
+    // every tile's TileLight stays on the device (what the cache receives): a rebake's clean
+    // tiles from the cache, the baked tiles' written by finish
+    {
+        size_t Ta = std::max<size_t>(T_all, 1);
+        TRY(B->d_tl_known.reserve(Ta));
+        TRY(B->d_tl_state.reserve(Ta));
+        TRY(B->d_tl_field.reserve(Ta * 4));
+        TRY(B->d_tl_ao.reserve(Ta * 4));
+        TRY(B->d_tl_direct.reserve(Ta * 12));
+        TRY(B->d_tl_indirect.reserve(Ta * 12));
+    }
+    // the atlas' 4 x 4 blocks -> their tile (tiles are 4-aligned: layout.rs:18,216,236,260)
+    {
+        std::vector<AtlasTile> at(std::max<size_t>(nt, 1));
+        uint32_t bw = (aw + 3) / 4, bh = (ah + 3) / 4;
+        std::vector<uint32_t> bmap((size_t)bw * bh, ATLAS_NONE);
         size_t k = 0;
         for (size_t t = 0; t < nt; t++) {
-            bool dirty = k < B->baked.size() && B->baked[k] == t;
-            if (dirty) {
-                k++;
-                continue;
-            }
             const Tile &tl = L.tiles[t];
-            for (uint32_t y = 0; y < tl.h; y++)
-                for (uint32_t x = 0; x < tl.w; x++) {
-                    size_t i = B->toff_all[t] + (size_t)y * tl.w + x;
-                    B->tl_known[i] = c.known[i];
-                    B->tl_state[i] = c.state[i];
-                    B->tl_field[i] = c.field[i];
-                    B->tl_ao[i] = c.ao[i];
-                    for (int q = 0; q < 3; q++) {
-                        B->tl_direct[i * 3 + q] = c.direct[i * 3 + q];
-                        B->tl_indirect[i * 3 + q] = c.indirect[i * 3 + q];
-                    }
-                    float *a = &B->light0[(((size_t)tl.y + y) * aw + tl.x + x) * 4];
-                    if (c.known[i] >= 1) {
-                        for (int q = 0; q < 3; q++) a[q] = c.direct[i * 3 + q] + c.indirect[i * 3 + q];
-                        a[3] = 1.0f;
-                    }
-                }
+            bool dirty = k < B->baked.size() && B->baked[k] == t;
+            if (dirty) k++;
+            if ((tl.x | tl.y | tl.w | tl.h) & 3u) return fail("layout: tile %zu is not 4-aligned", t);
+            at[t] = AtlasTile{(uint32_t)B->toff_all[t], tl.w, tl.h, tl.x, tl.y, dirty ? 1u : 0u};
+            for (uint32_t by = tl.y / 4; by < (tl.y + tl.h) / 4; by++)
+                for (uint32_t bx = tl.x / 4; bx < (tl.x + tl.w) / 4; bx++)
+                    bmap[(size_t)by * bw + bx] = (uint32_t)t;
         }
+        TRY(put(B->d_atiles, at, st));
+        TRY(put(B->d_bmap, bmap, st));
+        B->amap = AtlasMap{(const AtlasTile *)B->d_atiles.ptr, (const uint32_t *)B->d_bmap.ptr,
+                           aw, ah, bw};
+    }
+    if (B->any_clean) {
+        // the clean tiles' TileLight from the cache, and their light (bake.rs:435-444): the
+        // atlas the first bounce reads
+        const lb_bake_cache &c = *cache;
+        TRYH(hipMemcpyAsync(B->d_tl_known.ptr, c.known, T_all, hipMemcpyHostToDevice, st));
+        TRYH(hipMemcpyAsync(B->d_tl_state.ptr, c.state, T_all, hipMemcpyHostToDevice, st));
+        TRYH(hipMemcpyAsync(B->d_tl_field.ptr, c.field, T_all * 4, hipMemcpyHostToDevice, st));
+        TRYH(hipMemcpyAsync(B->d_tl_ao.ptr, c.ao, T_all * 4, hipMemcpyHostToDevice, st));
+        TRYH(hipMemcpyAsync(B->d_tl_direct.ptr, c.direct, T_all * 12, hipMemcpyHostToDevice, st));
+        TRYH(hipMemcpyAsync(B->d_tl_indirect.ptr, c.indirect, T_all * 12, hipMemcpyHostToDevice, st));
+        TRY(B->d_light0.reserve((size_t)aw * ah * 16));
+        TRYH(atlas_clean_light(B->amap, tile_light_view(B), (float4 *)B->d_light0.ptr, st));
+        TRYH(hipStreamSynchronize(st));  // the cache is borrowed until prepare returns
     }
 
     // the dirty tiles' texels (bake.rs:139-148), rasterized on the device (raster.rs:39-114)
@@ -306,26 +298,34 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
         TRYH(raster_tiles((const RasterTri *)B->d_raster.ptr, (uint32_t)recs.size(),
                           (const uint4 *)B->d_tiles.ptr, (const uint32_t *)B->d_tile_of.ptr,
                           (uint32_t)T, (uint32_t *)B->d_owner.ptr, (lb_texel *)B->d_tex0.ptr, st));
-        B->texels.resize(T);
-        if (T)
-            TRYH(hipMemcpyAsync(B->texels.data(), B->d_tex0.ptr, T * sizeof(lb_texel),
-                                hipMemcpyDeviceToHost, st));
+    }
+    // each texel's covered flag, chart and normal, and the covered texels' list (bake.rs:149),
+    // on the device
+    {
+        size_t Tn = std::max<size_t>(T, 1);
+        TRY(B->d_chart.reserve(Tn * 4));
+        TRY(B->d_normal.reserve(Tn * 16));
+        TRY(B->d_covered.reserve(Tn));
+        TRY(B->d_cov.reserve(Tn * 4));
+        TRY(B->d_counts.reserve((size_t)tp_gather_blocks((uint32_t)Tn) * 4 + 16));
+        TRY(B->d_total.reserve(16));
+        TRYH(raster_derive((const lb_texel *)B->d_tex0.ptr, (uint32_t)T, (uint32_t *)B->d_chart.ptr,
+                           (float4 *)B->d_normal.ptr, (uint8_t *)B->d_covered.ptr, st));
+        tp_compact((const uint8_t *)B->d_covered.ptr, (uint32_t)T, (uint32_t *)B->d_counts.ptr,
+                   (uint32_t *)B->d_total.ptr, (uint32_t *)B->d_cov.ptr, st);
+        TRYH(hipGetLastError());
+        uint32_t ncov = 0;
+        TRYH(hipMemcpyAsync(&ncov, B->d_total.ptr, 4, hipMemcpyDeviceToHost, st));
         TRYH(hipStreamSynchronize(st));
+        B->ncov = ncov;
     }
-    B->chart.resize(T);
-    B->normal.assign(T * 4, 0.0f);
-    B->covered.resize(T);
-    for (size_t gi = 0; gi < T; gi++) {
-        const lb_texel &x = B->texels[gi];
-        bool cv = x.pos[3] > 0.5f;
-        B->covered[gi] = cv;
-        uint32_t c;
-        std::memcpy(&c, &x.normal[3], 4);
-        B->chart[gi] = cv ? c : 0xffffffffu;
-        for (int k = 0; k < 3; k++) B->normal[gi * 4 + k] = x.normal[k];
-        if (cv) B->cov.push_back((uint32_t)gi);
+    {
+        // baked texel g of baked tile k is all-tile texel g + dst_base[k]
+        std::vector<int64_t> dst(std::max<size_t>(nd, 1), 0);
+        for (size_t k = 0; k < nd; k++)
+            dst[k] = (int64_t)B->toff_all[B->baked[k]] - (int64_t)B->toff[k];
+        TRY(put(B->d_dst_base, dst, st));
     }
-    B->ncov = B->cov.size();
     I.texels = T;
     I.texels_baked = B->ncov;
     I.s_raster = secs(t0);
@@ -385,9 +385,6 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     // device state
     size_t Tn = std::max<size_t>(T, 1);
     TRY(B->d_tex.reserve(Tn * sizeof(lb_texel)));
-    TRY(put(B->d_chart, B->chart, st));
-    TRY(put(B->d_normal, B->normal, st));
-    TRY(put(B->d_covered, B->covered, st));
     TRY(B->d_valid.reserve(Tn));
     TRY(B->d_known.reserve(Tn));
     TRY(B->d_state.reserve(Tn));
@@ -403,7 +400,6 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     TRY(B->d_f4.reserve(Tn * 16));
     TRY(B->d_m1.reserve(Tn));
     TRY(B->d_m2.reserve(Tn));
-    TRY(put(B->d_cov, B->cov, st));
     TRY(B->d_items.reserve(std::max<size_t>(B->ncov, 1) * sizeof(lb_texel)));
     TRY(B->d_res.reserve(std::max<size_t>(B->ncov, 1) * 16));
     TRY(B->d_light.reserve((size_t)aw * ah * 16));
@@ -422,7 +418,6 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
         TRY(put(B->d_pitems, pitems, st));
         TRY(B->d_pres.reserve(B->npd * 64));
     }
-    if (B->any_clean) TRY(put(B->d_light0, B->light0, st));
     TexelView &V = B->V;
     V.n = (uint32_t)T;
     V.tile_of = (const uint32_t *)B->d_tile_of.ptr;
@@ -573,6 +568,7 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
     lb_stats *sg = per_pass ? &per_pass[1] : nullptr;
     lb_stats *sv = per_pass ? &per_pass[2] : nullptr;
     auto t_exec = clk::now();
+    B->held = 0;  // the host copies of device arrays are stale from here
 
     // the prepared state (bake.rs:204-205: soft 0, ao 1, hard 0, no state; valid = covered)
     TRYH(hipEventRecord(B->ev[0], st));
@@ -692,29 +688,22 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
 int bake_finish(lb_bake *B) {
     lb_gpu *gpu = B->gpu;
     hipStream_t st = gpu->stream;
+    hipSetDevice(gpu->device);
     const lb_scene_desc &sc = B->sc;
     lb_bake_info &I = B->info;
     auto t0 = clk::now();
-    size_t T = B->T;
-    uint32_t aw = B->aw, ah = B->ah;
-    std::vector<uint8_t> known(T), state(T);
-    std::vector<float> field(T), ao(T), dir4(T * 4), ind4(T * 4);
-    B->light.resize((size_t)aw * ah * 4);
-    std::vector<float> pres(B->npd * 16);
-    if (T) {
-        TRYH(hipMemcpyAsync(known.data(), B->d_known.ptr, T, hipMemcpyDeviceToHost, st));
-        TRYH(hipMemcpyAsync(state.data(), B->d_state.ptr, T, hipMemcpyDeviceToHost, st));
-        TRYH(hipMemcpyAsync(field.data(), B->d_field.ptr, T * 4, hipMemcpyDeviceToHost, st));
-        TRYH(hipMemcpyAsync(ao.data(), B->d_ao.ptr, T * 4, hipMemcpyDeviceToHost, st));
-        TRYH(hipMemcpyAsync(dir4.data(), B->d_direct.ptr, T * 16, hipMemcpyDeviceToHost, st));
-        TRYH(hipMemcpyAsync(ind4.data(), B->d_indirect.ptr, T * 16, hipMemcpyDeviceToHost, st));
-    }
-    TRYH(hipMemcpyAsync(B->light.data(), B->d_light.ptr, (size_t)aw * ah * 16, hipMemcpyDeviceToHost, st));
-    if (B->npd) TRYH(hipMemcpyAsync(pres.data(), B->d_pres.ptr, B->npd * 64, hipMemcpyDeviceToHost, st));
-    TRYH(hipStreamSynchronize(st));
+    B->held = 0;
+    uint32_t aw = B->aw, ah = B->ah, ds = B->ds;
+    uint32_t iw = (aw + ds - 1) / ds, ih = (ah + ds - 1) / ds;
+    size_t na = (size_t)aw * ah, ni = (size_t)iw * ih;
     // the traced probes beside the cached ones: what volume.rs:81-84 caches
-    for (size_t d = 0; d < B->npd; d++)
-        std::memcpy(&B->probes_raw[(size_t)B->pdirty[d] * 16], &pres[d * 16], 64);
+    if (B->npd) {
+        std::vector<float> pres(B->npd * 16);
+        TRYH(hipMemcpyAsync(pres.data(), B->d_pres.ptr, B->npd * 64, hipMemcpyDeviceToHost, st));
+        TRYH(hipStreamSynchronize(st));
+        for (size_t d = 0; d < B->npd; d++)
+            std::memcpy(&B->probes_raw[(size_t)B->pdirty[d] * 16], &pres[d * 16], 64);
+    }
     std::vector<float> probes = B->probes_raw;
 
     // probes inside solids from their neighbours, 8 rings (volume.rs:86-117)
@@ -748,97 +737,76 @@ int bake_finish(lb_bake *B) {
         }
     }
 
-    // texels with no light of their own take the volume's (bake.rs:337-359)
-    std::vector<uint8_t> validity_t(state);
+    // on the device: the baked tiles' TileLight with the volume's light for texels with none
+    // (bake.rs:337-362), the atlas from every tile (bake.rs:364-421), the shipped encodings
+    TRYH(hipEventRecord(B->ev[8], st));
+    VolumeView vol{};
     if (np) {
-        for (size_t gi = 0; gi < T; gi++) {
-            if (!B->covered[gi] || known[gi] != 0) continue;
-            V3 n{B->normal[gi * 4], B->normal[gi * 4 + 1], B->normal[gi * 4 + 2]};
-            V3 p{B->texels[gi].pos[0], B->texels[gi].pos[1], B->texels[gi].pos[2]};
-            float e[3], s;
-            volume_irradiance(probes, B->vsz, sc.volume_min, sc.volume_step, p + n * 0.15f, n, e, &s);
-            float c = std::fmax(dot(n, B->sun), 0.0f);
-            for (int k = 0; k < 3; k++) {
-                ind4[gi * 4 + k] = e[k];
-                dir4[gi * 4 + k] = sc.sun_irradiance[k] * c * s;
-            }
-            field[gi] = s > 0.5f ? 1.0f : 0.0f;
-            ao[gi] = 1.0f;
-            known[gi] = 1;
-            validity_t[gi] = VALIDITY_VOLUME;
-        }
+        TRY(put(B->d_probes, probes, st));
+        vol.probes = (const float4 *)B->d_probes.ptr;
+        vol.nx = (uint32_t)nx;
+        vol.ny = (uint32_t)ny;
+        vol.nz = (uint32_t)nz;
+        vol.ox = sc.volume_min[0];
+        vol.oy = sc.volume_min[1];
+        vol.oz = sc.volume_min[2];
+        vol.step = sc.volume_step;
+        vol.sun_dir = make_float3(B->sun.x, B->sun.y, B->sun.z);
+        vol.sun_irr = make_float3(sc.sun_irradiance[0], sc.sun_irradiance[1], sc.sun_irradiance[2]);
     }
+    BakedView bv{};
+    bv.n = (uint32_t)B->T;
+    bv.tile_of = (const uint32_t *)B->d_tile_of.ptr;
+    bv.dst_base = (const int64_t *)B->d_dst_base.ptr;
+    bv.covered = (const uint8_t *)B->d_covered.ptr;
+    bv.known = (const uint8_t *)B->d_known.ptr;
+    bv.state = (const uint8_t *)B->d_state.ptr;
+    bv.field = (const float *)B->d_field.ptr;
+    bv.ao = (const float *)B->d_ao.ptr;
+    bv.direct = (const float4 *)B->d_direct.ptr;
+    bv.indirect = (const float4 *)B->d_indirect.ptr;
+    bv.pos = (const float4 *)B->d_tex0.ptr;
+    bv.normal = (const float4 *)B->d_normal.ptr;
+    TileLightView tl = tile_light_view(B);
+    TRYH(atlas_tile_light(bv, vol, tl, st));
+    TRY(B->d_adirect.reserve(std::max<size_t>(na, 1) * 8));
+    TRY(B->d_avalid.reserve(std::max<size_t>(na, 1)));
+    TRY(B->d_bc5.reserve(std::max<size_t>(na, 16)));
+    TRY(B->d_aind.reserve(std::max<size_t>(ni, 1) * 12));
+    TRY(B->d_atmp.reserve(std::max<size_t>(ni, 1) * 12));
+    TRY(B->d_has0.reserve(std::max<size_t>(ni, 1)));
+    TRY(B->d_has1.reserve(std::max<size_t>(ni, 1)));
+    TRY(B->d_rgb.reserve(std::max<size_t>(ni, 1) * 4));
+    TRY(B->d_vbytes.reserve(std::max<size_t>(np, 1) * 16));
+    TRY(B->d_word.reserve(16));
+    TRYH(atlas_assemble(B->amap, tl, ds, (float2 *)B->d_adirect.ptr, (uint8_t *)B->d_avalid.ptr,
+                        (float *)B->d_aind.ptr, (float *)B->d_atmp.ptr, (uint8_t *)B->d_has0.ptr,
+                        (uint8_t *)B->d_has1.ptr, st));
+    // output.rs:99-110 (the atlas width and height are multiples of 4: ALIGN and atlasWidth)
+    TRYH(encode_bc5_f32((const float2 *)B->d_adirect.ptr, aw, ah, (uint8_t *)B->d_bc5.ptr, st));
+    TRYH(encode_rgb9e5((const float *)B->d_aind.ptr, ni, (uint32_t *)B->d_rgb.ptr, st));
+    if (np)
+        TRYH(encode_volume((const float4 *)B->d_probes.ptr, np, (uint8_t *)B->d_vbytes.ptr,
+                           (uint32_t *)B->d_word.ptr, st));
+    TRYH(hipEventRecord(B->ev[9], st));
 
-    // the baked tiles' TileLight beside the cached ones (what bake.rs:360-362 caches)
-    const Layout &L = B->layout;
-    for (size_t k = 0; k < B->baked.size(); k++) {
-        size_t o = B->toff_all[B->baked[k]], n = B->toff[k + 1] - B->toff[k];
-        for (size_t i = 0; i < n; i++) {
-            size_t gi = B->toff[k] + i;
-            B->tl_known[o + i] = known[gi];
-            B->tl_state[o + i] = validity_t[gi];
-            B->tl_field[o + i] = field[gi];
-            B->tl_ao[o + i] = ao[gi];
-            for (int q = 0; q < 3; q++) {
-                B->tl_direct[(o + i) * 3 + q] = dir4[gi * 4 + q];
-                B->tl_indirect[(o + i) * 3 + q] = ind4[gi * 4 + q];
-            }
-        }
+    // the bake's outputs (bake.rs:36-47): the atlases and the volume
+    B->direct.resize(na * 2);
+    B->validity.resize(na);
+    B->indirect.resize(ni * 3);
+    uint32_t scale_bits = 0;
+    if (na) {
+        TRYH(hipMemcpyAsync(B->direct.data(), B->d_adirect.ptr, na * 8, hipMemcpyDeviceToHost, st));
+        TRYH(hipMemcpyAsync(B->validity.data(), B->d_avalid.ptr, na, hipMemcpyDeviceToHost, st));
+        TRYH(hipMemcpyAsync(B->indirect.data(), B->d_aind.ptr, ni * 12, hipMemcpyDeviceToHost, st));
     }
-
-    // the atlas from every tile (bake.rs:364-421)
-    uint32_t ds = B->ds;
-    uint32_t iw = (aw + ds - 1) / ds, ih = (ah + ds - 1) / ds;
-    B->direct.assign((size_t)aw * ah * 2, 1.0f);
-    B->validity.assign((size_t)aw * ah, VALIDITY_NONE);
-    std::vector<float> sum((size_t)iw * ih * 4, 0.0f);
-    for (size_t t = 0; t < L.tiles.size(); t++) {
-        const Tile &tl = L.tiles[t];
-        for (uint32_t y = 0; y < tl.h; y++)
-            for (uint32_t x = 0; x < tl.w; x++) {
-                size_t gi = B->toff_all[t] + (size_t)y * tl.w + x;
-                uint8_t kn = B->tl_known[gi];
-                if (kn == 0) continue;
-                size_t ax = tl.x + x, ay = tl.y + y;
-                B->direct[(ay * aw + ax) * 2] = B->tl_field[gi];
-                B->direct[(ay * aw + ax) * 2 + 1] = B->tl_ao[gi];
-                B->validity[ay * aw + ax] = B->tl_state[gi];
-                float k = kn == 2 ? 1.0f : 1e-3f;
-                float *s = &sum[((ay / ds) * iw + ax / ds) * 4];
-                for (int c = 0; c < 3; c++) s[c] += B->tl_indirect[gi * 3 + c] * k;
-                s[3] += k;
-            }
-    }
-    B->indirect.assign((size_t)iw * ih * 3, 0.0f);
-    std::vector<uint8_t> has((size_t)iw * ih);
-    for (size_t i = 0; i < (size_t)iw * ih; i++) {
-        has[i] = sum[i * 4 + 3] > 0.0f;
-        if (has[i])
-            for (int c = 0; c < 3; c++) B->indirect[i * 3 + c] = sum[i * 4 + c] / sum[i * 4 + 3];
-    }
-    for (int ring = 0; ring < 2; ring++) {
-        std::vector<uint8_t> was = has;
-        std::vector<float> src = B->indirect;
-        for (uint32_t y = 0; y < ih; y++)
-            for (uint32_t x = 0; x < iw; x++) {
-                size_t i = (size_t)y * iw + x;
-                if (was[i]) continue;
-                float acc[3] = {0, 0, 0};
-                int n = 0;
-                for (uint32_t yy = y > 0 ? y - 1 : 0; yy < std::min(y + 2, ih); yy++)
-                    for (uint32_t xx = x > 0 ? x - 1 : 0; xx < std::min(x + 2, iw); xx++) {
-                        size_t j = (size_t)yy * iw + xx;
-                        if (!was[j]) continue;
-                        for (int c = 0; c < 3; c++) acc[c] += src[j * 3 + c];
-                        n++;
-                    }
-                if (n > 0) {
-                    for (int c = 0; c < 3; c++) B->indirect[i * 3 + c] = acc[c] / (float)n;
-                    has[i] = 1;
-                }
-            }
-    }
+    if (np) TRYH(hipMemcpyAsync(&scale_bits, B->d_word.ptr, 4, hipMemcpyDeviceToHost, st));
+    TRYH(hipStreamSynchronize(st));
+    TRYH(hipEventElapsedTime(&I.ms_finish, B->ev[8], B->ev[9]));
     B->volume = std::move(probes);
+    float scale = 1.0f;  // output.rs:108: 1 without a volume
+    if (np) std::memcpy(&scale, &scale_bits, 4);
+    I.volume_scale = scale;
     I.atlas_width = aw;
     I.atlas_height = ah;
     I.indirect_width = iw;
@@ -946,9 +914,34 @@ int lb_bake_info_get(const lb_bake *bake, lb_bake_info *info) {
 }
 
 size_t lb_bake_array(const lb_bake *b, int which, const void **data) {
+    if (data) *data = nullptr;
     if (!b) return 0;
     const void *p = nullptr;
     size_t n = 0;
+    // an array the bake keeps on the device, read back on the first request
+    auto lazy = [&](auto &host, const DevBuf &dev, size_t count) -> int {
+        using T = typename std::remove_reference_t<decltype(host)>::value_type;
+        uint64_t bit = 1ull << which;
+        if (!(b->held & bit)) {
+            host.resize(count);
+            if (count) {
+                hipSetDevice(b->gpu->device);
+                hipStream_t st = b->gpu->stream;
+                hipError_t e = hipMemcpyAsync(host.data(), dev.ptr, count * sizeof(T),
+                                              hipMemcpyDeviceToHost, st);
+                if (e == hipSuccess) e = hipStreamSynchronize(st);
+                if (e != hipSuccess) {
+                    host.clear();
+                    return hip_fail(e, "bake: reading an array back");
+                }
+            }
+            b->held |= bit;
+        }
+        p = host.data();
+        n = host.size() * sizeof(T);
+        return 0;
+    };
+    size_t na = (size_t)b->aw * b->ah, ni = b->indirect.size() / 3, T = b->T_all;
     switch (which) {
     case LB_BAKE_DIRECT: p = b->direct.data(); n = b->direct.size() * 4; break;
     case LB_BAKE_INDIRECT: p = b->indirect.data(); n = b->indirect.size() * 4; break;
@@ -956,23 +949,26 @@ size_t lb_bake_array(const lb_bake *b, int which, const void **data) {
     case LB_BAKE_VOLUME: p = b->volume.data(); n = b->volume.size() * 4; break;
     case LB_BAKE_CHARTS: p = b->charts.data(); n = b->charts.size() * 4; break;
     case LB_BAKE_TRI_CHARTS: p = b->tri_charts.data(); n = b->tri_charts.size() * 4; break;
-    case LB_BAKE_LIGHT: p = b->light.data(); n = b->light.size() * 4; break;
+    case LB_BAKE_LIGHT: lazy(b->h_light, b->d_light, na * 4); break;
     case LB_BAKE_CHART_TILES: p = b->chart_tiles.data(); n = b->chart_tiles.size() * 4; break;
     case LB_BAKE_TILES: p = b->tiles.data(); n = b->tiles.size() * 4; break;
     case LB_BAKE_NODES: p = b->nodes.data(); n = b->nodes.size() * sizeof(lb_node); break;
     case LB_BAKE_TRIS: p = b->tris.data(); n = b->tris.size() * sizeof(lb_tri); break;
     case LB_BAKE_MATERIALS: p = b->mats.data(); n = b->mats.size() * 4; break;
     case LB_BAKE_PARAMS: p = &b->P; n = sizeof(lb_params); break;
-    case LB_BAKE_TEXELS: p = b->texels.data(); n = b->texels.size() * sizeof(lb_texel); break;
-    case LB_BAKE_TILE_KNOWN: p = b->tl_known.data(); n = b->tl_known.size(); break;
-    case LB_BAKE_TILE_STATE: p = b->tl_state.data(); n = b->tl_state.size(); break;
-    case LB_BAKE_TILE_FIELD: p = b->tl_field.data(); n = b->tl_field.size() * 4; break;
-    case LB_BAKE_TILE_AO: p = b->tl_ao.data(); n = b->tl_ao.size() * 4; break;
-    case LB_BAKE_TILE_DIRECT: p = b->tl_direct.data(); n = b->tl_direct.size() * 4; break;
-    case LB_BAKE_TILE_INDIRECT: p = b->tl_indirect.data(); n = b->tl_indirect.size() * 4; break;
+    case LB_BAKE_TEXELS: lazy(b->h_texels, b->d_tex0, b->T); break;
+    case LB_BAKE_TILE_KNOWN: lazy(b->h_tl_known, b->d_tl_known, T); break;
+    case LB_BAKE_TILE_STATE: lazy(b->h_tl_state, b->d_tl_state, T); break;
+    case LB_BAKE_TILE_FIELD: lazy(b->h_tl_field, b->d_tl_field, T); break;
+    case LB_BAKE_TILE_AO: lazy(b->h_tl_ao, b->d_tl_ao, T); break;
+    case LB_BAKE_TILE_DIRECT: lazy(b->h_tl_direct, b->d_tl_direct, T * 3); break;
+    case LB_BAKE_TILE_INDIRECT: lazy(b->h_tl_indirect, b->d_tl_indirect, T * 3); break;
     case LB_BAKE_PROBES: p = b->probes_raw.data(); n = b->probes_raw.size() * 4; break;
     case LB_BAKE_BAKED_TILES: p = b->baked.data(); n = b->baked.size() * 4; break;
-    case LB_BAKE_LIGHT0: p = b->light0.data(); n = b->light0.size() * 4; break;
+    case LB_BAKE_LIGHT0: lazy(b->h_light0, b->d_light0, b->any_clean ? na * 4 : 0); break;
+    case LB_BAKE_DIRECT_BC5: lazy(b->h_bc5, b->d_bc5, b->direct.empty() ? 0 : na); break;
+    case LB_BAKE_INDIRECT_RGB9E5: lazy(b->h_rgb, b->d_rgb, ni); break;
+    case LB_BAKE_VOLUME_BYTES: lazy(b->h_vbytes, b->d_vbytes, b->volume.empty() ? 0 : b->np * 16); break;
     case LB_BAKE_GATHER_INDEX:
         if (b->gat.size() != b->ng) {
             auto *m = const_cast<lb_bake *>(b);
@@ -985,6 +981,7 @@ size_t lb_bake_array(const lb_bake *b, int which, const void **data) {
         break;
     default: break;
     }
+    if (!n) p = nullptr;
     if (data) *data = p;
     return n;
 }

@@ -86,6 +86,7 @@ void lb_close(lb_gpu *gpu) {
     gpu->run_light.release();
     gpu->run_out.release();
     gpu->zero.release();
+    gpu->word.release();
     gpu->spill.release();
     gpu->stream_hits.release();
     gpu->stream_aux.release();

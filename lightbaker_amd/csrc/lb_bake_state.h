@@ -6,6 +6,7 @@
 
 #include <vector>
 
+#include "lb_atlas.h"
 #include "lb_bake.h"
 #include "lb_host.h"
 #include "lb_texel.h"
@@ -23,33 +24,38 @@ struct lb_bake {
     // the tiles baked (dirty), in layout order; every tile's first texel in the cache order
     std::vector<uint32_t> baked;
     std::vector<size_t> toff_all;
-    // host copies of the baked tiles' texels (toff: each baked tile's first)
-    std::vector<uint32_t> toff, chart;
-    std::vector<lb_texel> texels;
-    std::vector<float> normal;
-    std::vector<uint8_t> covered;
-    std::vector<uint32_t> cov;
+    // each baked tile's first texel in the baked list
+    std::vector<uint32_t> toff;
     uint32_t vsz[3] = {0, 0, 0};
-    size_t T = 0, ncov = 0, ng = 0, np = 0, npd = 0;
+    size_t T = 0, T_all = 0, ncov = 0, ng = 0, np = 0, npd = 0;
     std::vector<uint32_t> gat;
     std::vector<uint32_t> pdirty;  // the probes traced, in index order
-    // every tile's TileLight (bake.rs:61-69), clean tiles from the cache, in the cache order
-    std::vector<uint8_t> tl_known, tl_state;
-    std::vector<float> tl_field, tl_ao, tl_direct, tl_indirect;
-    std::vector<float> probes_raw, light0;
+    std::vector<float> probes_raw;  // every probe as traced (the cache's for clean ones)
     bool any_clean = false;
+    lbk::AtlasMap amap{};
+    // host copies of device arrays, read back when lb_bake_array asks (bit k: array k is held)
+    mutable uint64_t held = 0;
+    mutable std::vector<uint8_t> h_tl_known, h_tl_state, h_bc5, h_vbytes;
+    mutable std::vector<float> h_tl_field, h_tl_ao, h_tl_direct, h_tl_indirect, h_light, h_light0;
+    mutable std::vector<uint32_t> h_rgb;
+    mutable std::vector<lb_texel> h_texels;
     // device state
     lbk::DevBuf d_tex0, d_tex, d_tile_of, d_tiles, d_chart, d_normal, d_valid, d_known, d_state,
         d_hard, d_soft, d_ao, d_field, d_direct, d_indirect, d_cov, d_items, d_res, d_light, d_w1,
         d_wg, d_f1, d_f4, d_m1, d_m2, d_covered, d_gat, d_gmask, d_gitems, d_gres, d_filled,
         d_pitems, d_pres, d_counts, d_total, d_light0, d_cost, d_perm, d_rec, d_raster, d_owner;
+    // every tile's TileLight (bake.rs:61-69) in the cache order, the atlas' tile map, and the
+    // finish's outputs
+    lbk::DevBuf d_tl_known, d_tl_state, d_tl_field, d_tl_ao, d_tl_direct, d_tl_indirect, d_atiles,
+        d_bmap, d_dst_base, d_probes, d_adirect, d_avalid, d_aind, d_atmp, d_has0, d_has1, d_bc5,
+        d_rgb, d_vbytes, d_word;
     lbk::TexelView V{};
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};
     hipGraphExec_t graph = nullptr;
     size_t graph_ng = 0;  // the gather item count the graph was captured with
     lbk::Tuning graph_tuning;  // and the tuning
     // results
-    std::vector<float> direct, indirect, volume, charts, light;
+    std::vector<float> direct, indirect, volume, charts;
     std::vector<uint8_t> validity;
     std::vector<uint32_t> tri_charts, chart_tiles;
     std::vector<int32_t> tiles;

@@ -38,6 +38,7 @@ struct lb_gpu {
     size_t lds_per_block = 0;
     lbk::DevBuf run_items, run_light, run_out;  // lb_run's staging
     lbk::DevBuf zero;                           // one zero vec4 (no light)
+    lbk::DevBuf word;                           // lb_encode_volume's scale
     lbk::DevBuf spill;                          // TUNED: stack entries beyond the LDS part
     lbk::DevBuf stream_hits;                    // TUNED, pooled: a chunk's hits
     lbk::DevBuf stream_aux;                     // TUNED, pooled: picks, item lists, counts

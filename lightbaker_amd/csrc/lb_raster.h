@@ -34,5 +34,8 @@ void raster_records(const lb_scene_desc &s, const Layout &L, const std::vector<u
 hipError_t raster_tiles(const RasterTri *rec, uint32_t n_rec, const uint4 *tiles,
                         const uint32_t *tile_of, uint32_t n_texels, uint32_t *owner,
                         lb_texel *out, hipStream_t stream);
+// Each texel's covered flag, chart (0xffffffff where uncovered) and normal (w 0).
+hipError_t raster_derive(const lb_texel *tex, uint32_t n, uint32_t *chart, float4 *normal,
+                         uint8_t *covered, hipStream_t stream);
 
 }  // namespace lbk

@@ -84,6 +84,10 @@ void tp_fill_f32(float *p, float v, uint32_t n, hipStream_t s);
 void tp_gather_set(const TexelView &V, uint32_t ds, uint8_t *flag, uint32_t *counts,
                    uint32_t *total, uint32_t *out, hipStream_t s);
 uint32_t tp_gather_blocks(uint32_t n);
+// The indices of the flagged entries of flag[0..n), in order, into `out`, their count into
+// `total`; `counts` holds tp_gather_blocks(n) words
+void tp_compact(const uint8_t *flag, uint32_t n, uint32_t *counts, uint32_t *total, uint32_t *out,
+                hipStream_t s);
 void tp_copy_f32(const float *src, float *dst, uint32_t n, hipStream_t s);
 void tp_copy_u8(const uint8_t *src, uint8_t *dst, uint32_t n, hipStream_t s);
 

@@ -126,12 +126,36 @@ __global__ __launch_bounds__(BLOCK) void k_raster_write(const RasterTri *__restr
     o[3] = make_float4(r.dy[0], r.dy[1], r.dy[2], 0.0f);
 }
 
+// What the texel passes read of each texel (bake.rs:139-148 with raster.rs:25-36): covered
+// (pos.w > 0.5), its chart (none where uncovered) and its normal (w 0).
+__global__ __launch_bounds__(BLOCK) void k_raster_derive(const lb_texel *__restrict__ tex,
+                                                         uint32_t n, uint32_t *__restrict__ chart,
+                                                         float4 *__restrict__ normal,
+                                                         uint8_t *__restrict__ covered) {
+    uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
+    if (g >= n) return;
+    const float4 *t = reinterpret_cast<const float4 *>(tex + g);
+    float4 p = t[0], nm = t[1];
+    bool cv = p.w > 0.5f;
+    covered[g] = cv;
+    chart[g] = cv ? __float_as_uint(nm.w) : EMPTY;
+    normal[g] = make_float4(nm.x, nm.y, nm.z, 0.0f);
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
 
 }  // namespace
+
+hipError_t raster_derive(const lb_texel *tex, uint32_t n, uint32_t *chart, float4 *normal,
+                         uint8_t *covered, hipStream_t stream) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_raster_derive, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream,
+                       tex, n, chart, normal, covered);
+    return hipGetLastError();
+}
 
 hipError_t raster_tiles(const RasterTri *rec, uint32_t n_rec, const uint4 *tiles,
                         const uint32_t *tile_of, uint32_t n_texels, uint32_t *owner,

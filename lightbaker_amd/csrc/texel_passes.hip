@@ -392,6 +392,13 @@ void tp_gather_set(const TexelView &V, uint32_t ds, uint8_t *flag, uint32_t *cou
     hipLaunchKernelGGL(k_compact, dim3(nb), dim3(CBLOCK), 0, s, flag, n, counts, out);
 }
 uint32_t tp_gather_blocks(uint32_t n) { return (n + CBLOCK - 1) / CBLOCK; }
+void tp_compact(const uint8_t *flag, uint32_t n, uint32_t *counts, uint32_t *total, uint32_t *out,
+                hipStream_t s) {
+    uint32_t nb = (n + CBLOCK - 1) / CBLOCK;
+    if (nb) hipLaunchKernelGGL(k_count, dim3(nb), dim3(CBLOCK), 0, s, flag, n, counts);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(CBLOCK), 0, s, counts, nb, total);
+    if (nb) hipLaunchKernelGGL(k_compact, dim3(nb), dim3(CBLOCK), 0, s, flag, n, counts, out);
+}
 void tp_copy_f32(const float *src, float *dst, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_copy_f32, grid_for(n), dim3(256), 0, s, src, dst, n);
 }

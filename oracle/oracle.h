@@ -50,6 +50,27 @@ void orc_trace_brute(const lb_tri *tris, size_t n_tris, const float *materials,
                      const lb_params *params, const float o[3], const float d[3], float tmax,
                      uint32_t ray, float *t_out, uint32_t *tri_out, int *back_out);
 
+/* the bake's tail and the output codecs (finish_ref.c) */
+void orc_probe_fill(float *probes, size_t nx, size_t ny, size_t nz);
+void orc_volume_irr(const float *probes, const uint32_t size[3], const float origin[3], float step,
+                    const float p[3], const float n[3], float e[3], float *sun);
+size_t orc_volume_lit(const float *probes, const uint32_t size[3], const float origin[3],
+                      float step, const float sun_dir[3], const float sun_irr[3], size_t n,
+                      const float *texels, uint8_t *known, uint8_t *state, float *field, float *ao,
+                      float *direct, float *indirect);
+void orc_atlas(const int32_t *tiles, size_t nt, const uint8_t *known, const uint8_t *state,
+               const float *field, const float *ao, const float *indirect, uint32_t aw,
+               uint32_t ah, uint32_t ds, float *direct_out, uint8_t *validity, float *ind_out);
+uint8_t orc_unorm8(float v);
+void orc_bc4_block(const uint8_t v[16], uint8_t out[8]);
+int orc_bc5(const uint8_t *rg, size_t w, size_t h, uint8_t *out);
+uint32_t orc_rgb9e5(const float c[3]);
+void orc_rgb9e5_decode(uint32_t p, float out[3]);
+float orc_volume_bytes(const float *probes, size_t n, uint8_t *out);
+void orc_unorm8_n(const float *v, size_t n, uint8_t *out);
+void orc_rgb9e5_n(const float *rgb, size_t n, uint32_t *out);
+void orc_rgb9e5_decode_n(const uint32_t *p, size_t n, float *out);
+
 #ifdef __cplusplus
 }
 #endif
