@@ -170,3 +170,20 @@ def test_volume_entry_point(gpu):
     assert same(got, want)
     z, zs = encode.volume_bytes(gpu, np.zeros((3, 16), np.float32))
     assert np.float32(zs) == np.float32(1e-6) and (z[:, :3] == 0).all() and (z[:, 4:13] == 128).all()
+
+
+def test_lbake_file_of_a_bake(gpu):
+    """output.rs:95-152 over a real bake: every chunk decodes to the device's encodings."""
+    from lightbaker_amd import output
+    scene = reference_test_scene(8.0)
+    b = bake(scene, gpu, form=N.FORM_TUNED, graph=True)
+    r = output.read(output.write(scene, b, level=3))
+    h = r["header"]
+    assert r["direct"]["data"] == b.direct_bc5.tobytes()
+    assert (r["direct"]["width"], r["direct"]["height"]) == (b.info["atlas_width"],
+                                                             b.info["atlas_height"])
+    assert r["indirect"]["data"] == b.indirect_rgb9e5.astype("<u4").tobytes()
+    assert r["validity"] == b.validity.tobytes() and r["volume"] == b.volume_bytes.tobytes()
+    assert r["charts"] == b.charts.tobytes() and r["triangleCharts"] == b.tri_charts.tobytes()
+    assert h["volume"]["scale"] == float(np.float32(b.info["volume_scale"]))
+    assert h["stats"]["texelsBaked"] == b.info["texels_baked"]
