@@ -282,7 +282,16 @@ def main() -> int:
     sess.finish()
     finish_s = time.perf_counter() - t0
 
-    # the dominant kernel: the gather pass (one launch = one bounce)
+    # the tail on the device (csrc/atlas.hip): algorithmic bytes of its kernels, per baked texel
+    # (TileLight: 47 B read, 34 B written), per atlas texel (the atlas: 22 B read where a tile
+    # lies, 9 B written; BC5: 8 B read, 1 B written), per indirect texel (3 x 13 B for the sums'
+    # output and two rings, RGB9E5: 12 B read, 4 B written) and per probe (64 B read, 16 written)
+    fin = sess.info()
+    n_atlas = fin["atlas_width"] * fin["atlas_height"]
+    n_ind = fin["indirect_width"] * fin["indirect_height"]
+    fin_bytes = (fin["texels"] * 81 + fin["texels"] * 22 + n_atlas * (9 + 9)
+                 + n_ind * (3 * 13 + 16) + fin["probes"] * 80)
+
     bounces = scene.settings.gi_bounces
     ms_gather = sess.time_pass(Pass.GATHER, reps=5)
     g = stats[1]
@@ -370,6 +379,9 @@ def main() -> int:
             "ms_direct": info["ms_direct"], "ms_direct_texel": info["ms_direct_texel"],
             "ms_bounces": info["ms_bounces"], "ms_volume": info["ms_volume"],
             "s_prepare_host": prep_s, "s_finish_host": finish_s,
+            "ms_finish_device": fin["ms_finish"],
+            "finish_bytes": fin_bytes, "finish_gbps": fin_bytes / (fin["ms_finish"] / 1e3) / 1e9
+            if fin["ms_finish"] > 0 else None,
             "s_per_bake_end_to_end": prep_s + ms_step / 1e3 + finish_s,
             "tiles_baked": info["tiles_baked"],
             "ms_per_tile": ms_step / max(info["tiles_baked"], 1),

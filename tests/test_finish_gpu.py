@@ -24,7 +24,7 @@ from oracle import finish_ref as F
 from lightbaker_amd import _native as N
 from lightbaker_amd import encode
 from lightbaker_amd.bake import BakeSession, bake
-from lightbaker_amd.scenes import rebake_scene, reference_test_scene
+from lightbaker_amd.scenes import Mesh, cube, rebake_scene, reference_test_scene
 
 from .test_bake_parity_gpu import oracle_inputs
 from .test_oracle_finish import rgb_cases
@@ -58,10 +58,19 @@ def check_atlas_and_codecs(sess_or_bake, info, tiles, tc, direct, validity, indi
         assert np.float32(info["volume_scale"]) == scale
 
 
+def wedged_scene():
+    """The reference test scene with a cube wedged inside its box, 0.1 m from every wall: the
+    inner cube's texels lie inside a solid (the walls' backs within the 0.3 m inside reach), so
+    where a chart keeps no light of its own the volume lights it (bake.rs:337-356)."""
+    scene = reference_test_scene(8.0)
+    scene.meshes.append(Mesh("wedged", 1, cube((8.1, 0.1, 10.1), (9.9, 1.9, 11.9))))
+    return scene
+
+
 @pytest.fixture(scope="module")
 def plain(gpu):
-    scene = reference_test_scene(8.0)
-    print("finish: PLAIN bake of the reference test scene", flush=True)
+    scene = wedged_scene()
+    print("finish: PLAIN bake of the reference test scene with a wedged cube", flush=True)
     sess = BakeSession(scene, gpu, form=N.FORM_PLAIN, graph=False)
     sess.execute()
     sess.finish()
@@ -83,8 +92,8 @@ def test_tile_light_matches_oracle(plain):
     P = inp["params"]
     lit = F.volume_lit(probes, size, origin, step, P["sun_dir"][0][:3], inp["irr"],
                        inp["texels"], tl)
-    # the box's floor face lies on the ground: inside, with no lit neighbour on its chart
-    assert lit > 0, "no texel for the volume to light: the fallback is not exercised"
+    # some of the wedged cube's texels (33 on this scene): the fallback is exercised
+    assert lit > 0, "no texel lit from the volume: the fallback is not exercised"
     tc = sess.tile_cache()
     for k in ("known", "state", "field", "ao", "direct", "indirect"):
         assert same(getattr(tc, k).reshape(tl[k].shape), tl[k]), k
@@ -185,5 +194,6 @@ def test_lbake_file_of_a_bake(gpu):
     assert r["indirect"]["data"] == b.indirect_rgb9e5.astype("<u4").tobytes()
     assert r["validity"] == b.validity.tobytes() and r["volume"] == b.volume_bytes.tobytes()
     assert r["charts"] == b.charts.tobytes() and r["triangleCharts"] == b.tri_charts.tobytes()
-    assert h["volume"]["scale"] == float(np.float32(b.info["volume_scale"]))
+    # the header prints the f32's shortest decimal, which reads back as the same f32
+    assert np.float32(h["volume"]["scale"]) == np.float32(b.info["volume_scale"])
     assert h["stats"]["texelsBaked"] == b.info["texels_baked"]
