@@ -160,7 +160,65 @@ def cpu_baseline(sess, gpu, target_s: float = 12.0, max_items: int = 500000) -> 
     acc["direct_soft_sun"] = dict(errors(got, ref, [0], True), items=len(d_items))
     sb.close()
     base["error_vs_fp64"] = acc
+
+    # the tile dependencies' filter (deps.rs:104-121) in the oracle, one core, 4 zones
+    from oracle import deps_ref
+    pos = tris_of(sess)
+    zones = np.tile(np.array([[0, 0, 0, 30, 10, 30]], np.float32), (4, 1))
+    zones[:, [0, 2, 3, 5]] += np.arange(4, dtype=np.float32)[:, None] * 20
+    sun = params["sun_dir"][0][:3].astype(np.float32)
+    t0 = time.perf_counter()
+    deps_ref.bits(pos, None, zones, sun, np.float32(pos[:, 1::3].max()))
+    dt = time.perf_counter() - t0
+    base["deps_filter"] = {"value": len(zones) * len(pos) / dt / 1e9, "unit": "Gpairs/s",
+                           "cores": 1, "sample": f"4 zones x {len(pos)} triangles"}
     return base
+
+
+def tris_of(sess) -> np.ndarray:
+    """The scene's triangles' corners (n, 9), from the bake's triangle records."""
+    from lightbaker_amd import _native as N
+    t = sess.array(N.BAKE_TRIS, N.TRI)
+    v0 = t["v0"][:, :3]
+    return np.concatenate([v0, v0 + t["e1"][:, :3], v0 + t["e2"][:, :3]], 1).astype(np.float32)
+
+
+def deps_timing(gpu, scene) -> dict:
+    """The tile dependencies (deps.rs:72-121, bake.rs:124-128): every tile's and probe chunk's
+    hash with the triangle filter on the device, end to end (host streams and BLAKE3 included)."""
+    from lightbaker_amd import deps
+    g = deps.global_hash(scene)
+    deps.tile_hashes(gpu, scene, g)  # warm
+    t0 = time.perf_counter()
+    h = deps.tile_hashes(gpu, scene, g)
+    s_tiles = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    keys, _ = deps.probe_hashes(gpu, scene, g)
+    s_probes = time.perf_counter() - t0
+    n_tris = scene.triangle_count()
+    # the filter alone (lb_deps_swept: upload, one launch, read-back) over as many zones, each a
+    # tile-sized box somewhere on the scene, as the bake has tiles and chunks
+    import ctypes as C
+    from lightbaker_amd import _native as N
+    pos = np.concatenate([m.positions.reshape(-1, 9) for m in scene.meshes]).astype(np.float32)
+    nz = len(h) + len(keys)
+    rng = np.random.default_rng(0)
+    lo, hi = pos.reshape(-1, 3).min(0), pos.reshape(-1, 3).max(0)
+    c = rng.uniform(lo, hi, (nz, 3)).astype(np.float32)
+    half = np.float32(scene.settings.tile_size / 2 + scene.settings.gi_reach)
+    zones = np.ascontiguousarray(np.concatenate([c - half, c + half], 1), np.float32)
+    sun = np.asarray(scene.sun_direction, np.float32)
+    sun = np.ascontiguousarray(sun / np.linalg.norm(sun), np.float32)
+    bits = np.zeros((nz, (len(pos) + 31) // 32), np.uint32)
+    top = float(pos[:, 1::3].max())
+    for _ in range(2):
+        t0 = time.perf_counter()
+        N.check(N.lib().lb_deps_swept(gpu.handle, N.ptr(pos), len(pos), None, N.ptr(zones), nz,
+                                      N.ptr(sun), C.c_float(top), N.ptr(bits)))
+        s_filter = time.perf_counter() - t0
+    return {"tiles": len(h), "probe_chunks": len(keys), "triangles": n_tris,
+            "pairs": nz * n_tris, "s_tile_hashes": s_tiles, "s_probe_hashes": s_probes,
+            "s_filter": s_filter, "filter_gpairs_per_s": nz * n_tris / s_filter / 1e9}
 
 
 def main() -> int:
@@ -337,6 +395,7 @@ def main() -> int:
         print(f"# replayed bounces: {replay_ms:.2f} ms a step", file=sys.stderr, flush=True)
 
     cpu = None if args.no_cpu else cpu_baseline(sess, gpu)
+    dep = deps_timing(gpu, scene)
     if args.workload == "map":
         metric = "Grays/s, 100x100 m 3-bounce bake (direct + 3 gather bounces + volume)"
         workload = "map_scene(2): 100x100 m, 8 texels/m, 256 GI rays, 3 bounces, 16 tiles"
@@ -391,6 +450,7 @@ def main() -> int:
             "replay_rays_traced_per_step": (stats[0]["rays"] + stats[1]["rays"] // bounces
                                             + stats[2]["rays"]) if replay_ms else None,
             "replay_vs_plain": (plain_ms / replay_ms) if plain_ms and replay_ms else None,
+            "deps": dep,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -401,6 +401,36 @@ int lb_encode_rgb9e5(lb_gpu *gpu, const float *rgb, size_t n, uint32_t *out);
  * (host) the L0 bytes are relative to */
 int lb_encode_volume(lb_gpu *gpu, const float *probes, size_t n, uint8_t *out, float *scale);
 
+/* ---- what a tile's light depends on (deps.rs:72-121), the filter on the device ------------- *
+ * The reference hashes, for every tile and every probe chunk, every casting triangle that can
+ * shade it (its box within the zone, or between the zone and the sun) and rebakes the tiles whose
+ * hash changed. It tests every triangle against every zone on the host; here one launch tests
+ * every pair. Host buffers in and out; the hashes are BLAKE3 as the reference computes them. */
+
+/* BLAKE3 (unkeyed, 32 bytes) of n bytes: the hash the reference keys its cache with. */
+int lb_blake3(const void *data, size_t n, uint8_t out[32]);
+/* deps.rs:83's filter for many zones: for zone z (n_zones x 6 f32: min xyz, max xyz, already
+ * grown by the reach) and triangle t (positions: n_tris x 9 f32, meshes in order), bit t % 32 of
+ * bits[z * ceil(n_tris / 32) + t / 32] is set when `casts` is NULL or casts[t] is nonzero, and
+ * the triangle's box meets the zone swept along `sun` (normalized) up to `top` (deps.rs:104-121;
+ * `top` is deps.rs:47-49's highest box top over every triangle). */
+int lb_deps_swept(lb_gpu *gpu, const float *positions, size_t n_tris, const uint8_t *casts,
+                  const float *zones, size_t n_zones, const float sun[3], float top,
+                  uint32_t *bits);
+/* bake.rs:124-128 over every tile of `layout` (NULL: built from the scene): each tile's
+ * hash_reach (deps.rs:72-102) of its surfaces' box (bake.rs:447-458) grown by the reach
+ * (occlusion's, or the bounces' when further), seeded with tile_seed (bake.rs:461-477), from
+ * `global` (deps.rs:60-70's hash of the settings, the caller's). hashes: 32 bytes a tile. */
+int lb_deps_tile_hashes(lb_gpu *gpu, const lb_scene_desc *scene, const lb_layout *layout,
+                        const uint8_t global[32], uint8_t *hashes);
+/* volume.rs:36-56: the probes grouped by ground-plan tile (keys in order), each group's
+ * hash_reach with the bounces' reach, seeded with its key and `volume_json` (the volume settings
+ * as serde_json writes them). keys (2 i32) and hashes (32 bytes) a chunk, up to cap; *n_chunks
+ * the count (also when it exceeds cap, with an error). */
+int lb_deps_probe_hashes(lb_gpu *gpu, const lb_scene_desc *scene, const uint8_t global[32],
+                         const char *volume_json, size_t json_len, int32_t *keys,
+                         uint8_t *hashes, size_t cap, size_t *n_chunks);
+
 /* ---- the adjoint of the bounces ------------------------------------------------------------ *
  * No counterpart in the reference, which has no gradients; it serves callers that fit the
  * scene's light to a target lightmap (inverse lighting: albedos, sky and sun from a bake). The

@@ -165,6 +165,11 @@ SIGNATURES = {
     "lb_encode_bc5": (_I, [_VP, _VP, _U32, _U32, _VP]),
     "lb_encode_rgb9e5": (_I, [_VP, _VP, _SZ, _VP]),
     "lb_encode_volume": (_I, [_VP, _VP, _SZ, _VP, C.POINTER(C.c_float)]),
+    "lb_blake3": (_I, [_VP, _SZ, _VP]),
+    "lb_deps_swept": (_I, [_VP, _VP, _SZ, _VP, _VP, _SZ, _VP, C.c_float, _VP]),
+    "lb_deps_tile_hashes": (_I, [_VP, C.POINTER(SceneDesc), _VP, _VP, _VP]),
+    "lb_deps_probe_hashes": (_I, [_VP, C.POINTER(SceneDesc), _VP, _VP, _SZ, _VP, _VP, _SZ,
+                                  C.POINTER(_SZ)]),
 }
 
 

@@ -11,10 +11,6 @@ namespace lbk {
 int fail(const char *fmt, ...);
 }
 
-struct lb_layout {
-    lbk::Layout L;
-};
-
 struct lb_bvh {
     std::vector<lb_node> nodes;
     std::vector<lb_tri> tris;

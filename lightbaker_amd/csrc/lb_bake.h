@@ -89,3 +89,8 @@ V3 face_normal(const lb_scene_desc &s, uint32_t mesh, uint32_t t);
 void rasterize(const lb_scene_desc &s, const Layout &l, const Tile &t, std::vector<lb_texel> &out);
 
 }  // namespace lbk
+
+// the C-ABI's layout handle (lb_layout_build)
+struct lb_layout {
+    lbk::Layout L;
+};

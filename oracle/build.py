@@ -12,7 +12,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 OUT = HERE / "liboracle.so"
-SOURCES = ["trace_ref.c", "filter_ref.c", "fp64_ref.c", "adjoint_ref.c", "finish_ref.c"]
+SOURCES = ["trace_ref.c", "filter_ref.c", "fp64_ref.c", "adjoint_ref.c", "finish_ref.c", "deps_ref.c"]
 
 
 def build(verbose: bool = False) -> Path:

@@ -71,6 +71,11 @@ void orc_unorm8_n(const float *v, size_t n, uint8_t *out);
 void orc_rgb9e5_n(const float *rgb, size_t n, uint32_t *out);
 void orc_rgb9e5_decode_n(const uint32_t *p, size_t n, float *out);
 
+/* deps.rs's triangle filter (deps_ref.c) */
+int orc_swept(const float z[6], const float sun[3], float top, const float b[6]);
+void orc_deps_bits(const float *pos, size_t n_tris, const uint8_t *casts, const float *zones,
+                   size_t n_zones, const float sun[3], float top, uint32_t *bits);
+
 #ifdef __cplusplus
 }
 #endif
