@@ -234,6 +234,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--xcd-split", type=int, choices=[0, 1], default=None,
+                    help="lock-step passes: items split over the 8 chiplets' queues (1) or one "
+                         "queue (0); default: the library's")
     ap.add_argument("--autotune", action="store_true",
                     help="autotune also where the workload skips it by default (city)")
     ap.add_argument("--profile-pass", choices=["direct", "gather", "volume"], default="gather",
@@ -277,8 +280,11 @@ def main() -> int:
     gpu = Gpu()
     gpu.set_tuning(256, 4, 16, layout, 32, args.schedule, leaf_batch=16, waves=6)
     # the autotuner's usual picks on this map, so a run without it uses the same kernels
-    gpu.set_tuning(pass_=Pass.GATHER, rays_per_lane=8, waves=8)
+    gpu.set_tuning(pass_=Pass.GATHER, rays_per_lane=8, waves=8, xcd_split=1)
     gpu.set_tuning(pass_=Pass.DIRECT, rays_per_lane=8, waves=8)
+    if args.xcd_split is not None:
+        for p in Pass:
+            gpu.set_tuning(pass_=p, xcd_split=args.xcd_split)
     print(f"# {gpu.adapter}; scene {scene.triangle_count()} triangles ({t_scene:.1f} s)",
           file=sys.stderr, flush=True)
     rebake = {}

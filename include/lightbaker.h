@@ -148,6 +148,9 @@ typedef struct lb_tuning {
     int waves;          /* lock-step schedule: waves a SIMD the kernel is compiled to keep
                            resident: 0 (the compiler's choice), 6 or 8 (fewer registers a lane,
                            some spilled) */
+    int xcd_split;      /* lock-step schedule: 1: the items split into 8 contiguous ranges, one a
+                           chiplet (XCD), each chiplet's waves taking their own range first, so a
+                           chiplet's L2 holds the part of the scene its texels see; 0: one queue */
 } lb_tuning;
 
 /* Sets the knobs of all three passes; lb_get_tuning reads the gather pass's. */

@@ -1,5 +1,6 @@
 """The TUNED form's autotuner: picks, for each pass on its own, the schedule, workgroup size, rays
-per lane, LDS stack depth, refill threshold, leaf batch and node layout with the best time on a
+per lane, LDS stack depth, refill threshold, leaf batch, node layout, residency and the queue's
+split over the chiplets with the best time on a
 prepared bake's own launches, then checks that the bake still gives the starting setting's light
 atlas.
 
@@ -25,13 +26,14 @@ RANGES = {
     "lds_stack": [8, 16, 24, 32, 48],
     "nodes": [N.NODES_WIDE, N.NODES_QUANT],
     "waves": [0, 6, 8],
+    "xcd_split": [0, 1],
 }
 
 
 def _skip(dim: str, cand: dict) -> bool:
     if dim == "refill" and cand["schedule"] == 0:
         return True
-    if dim == "waves" and cand["schedule"] == 1:
+    if dim in ("waves", "xcd_split") and cand["schedule"] == 1:
         return True
     return dim == "rays_per_lane" and cand["schedule"] == 1
 

@@ -529,7 +529,7 @@ int bounce(lb_bake *B, lb_stats *acc, uint32_t k) {
         o.perm = k == 0 ? nullptr : (const uint8_t *)B->d_perm.ptr;
         TRY(run_pass(B->gpu, LB_PASS_GATHER, B->opt.form, B->scene, &B->P,
                      (const lb_texel *)B->d_gitems.ptr, B->ng, 0, (const float *)B->d_light.ptr,
-                     (size_t)B->aw * B->ah, (float *)B->d_gres.ptr, k == 0, &o));
+                     (size_t)B->aw * B->ah, (float *)B->d_gres.ptr, false, &o));
         if (k == 0) {
             hipError_t e = rank_slots((const uint8_t *)B->d_cost.ptr, (uint32_t)B->ng,
                                       B->P.counts[2], (uint8_t *)B->d_perm.ptr, st);

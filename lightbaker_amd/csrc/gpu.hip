@@ -67,8 +67,8 @@ int lb_open(lb_gpu **out) {
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess ||
         hipMalloc(&g->counters, 8 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&g->queue, 64) != hipSuccess || g->zero.reserve(64) ||
-        hipMemset(g->zero.ptr, 0, 64) != hipSuccess || hipMemset(g->queue, 0, 64) != hipSuccess) {
+        hipMalloc(&g->queue, 1024) != hipSuccess || g->zero.reserve(64) ||
+        hipMemset(g->zero.ptr, 0, 64) != hipSuccess || hipMemset(g->queue, 0, 1024) != hipSuccess) {
         delete g;
         return fail("no GPU device");
     }
@@ -282,6 +282,8 @@ static int check_tuning(const lb_gpu *gpu, const lb_tuning *t, const char *who) 
         return fail("%s: leaf batch %d outside 1..64", who, t->leaf_batch);
     if (t->waves != 0 && t->waves != 6 && t->waves != 8)
         return fail("%s: waves %d is not 0, 6 or 8", who, t->waves);
+    if (t->xcd_split != 0 && t->xcd_split != 1)
+        return fail("%s: xcd_split %d is not 0 or 1", who, t->xcd_split);
     if (stack_lds_bytes(t->lds_stack, t->workgroup) > gpu->lds_per_block)
         return fail("%s: an LDS stack of %d entries for %d lanes exceeds %zu bytes", who,
                     t->lds_stack, t->workgroup, gpu->lds_per_block);
@@ -298,6 +300,7 @@ static void put_tuning(Tuning &d, const lb_tuning *t) {
     d.chunk = t->chunk;
     d.leaf_batch = t->leaf_batch;
     d.waves = t->waves;
+    d.xcd_split = t->xcd_split;
 }
 
 static void get_tuning(const Tuning &s, lb_tuning *t) {
@@ -310,6 +313,7 @@ static void get_tuning(const Tuning &s, lb_tuning *t) {
     t->chunk = s.chunk;
     t->leaf_batch = s.leaf_batch;
     t->waves = s.waves;
+    t->xcd_split = s.xcd_split;
 }
 
 int lb_set_tuning(lb_gpu *gpu, const lb_tuning *t) {

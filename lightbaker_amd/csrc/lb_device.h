@@ -291,9 +291,15 @@ __device__ __forceinline__ void offset(uint32_t q, float *cx, float *cy) {
 }
 
 // Per-thread traversal counters, flushed to the lb_stats block at the end of a kernel.
-struct Counts {
-    uint64_t rays, inner, tris, mats, gathers, leaves;
+template <typename T>
+struct CountsT {
+    T rays, inner, tris, mats, gathers, leaves;
 };
+// the launch's traversal counters (flushed to lb_stats)
+using Counts = CountsT<uint64_t>;
+// a lane's own step count only (the counting gather's ray costs), never flushed: what is not
+// read is not computed
+using Counts32 = CountsT<uint32_t>;
 
 __device__ __forceinline__ void flush_counts(const Counts &c, unsigned long long *dst) {
     atomicAdd(dst + 0, (unsigned long long)c.rays);

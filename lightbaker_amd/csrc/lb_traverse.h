@@ -118,9 +118,9 @@ __device__ __forceinline__ void ray_start(Ray &R, uint32_t root, f3 o, f3 d, flo
 }
 
 // The triangles of leaf R.n (trace.wgsl:91-104); true when an any-hit ray accepted one.
-template <bool STATS>
+template <bool STATS, typename C>
 __device__ __forceinline__ bool leaf_tests(const SceneView &S, const lb_params &P, Ray &R,
-                                           Counts &cnt) {
+                                           C &cnt) {
     uint32_t first = R.n & 0x03ffffffu, count = (R.n >> 26) & 31u;
     if (count == 31u) {
         uint2 li = S.leaves[first];
@@ -153,9 +153,9 @@ __device__ __forceinline__ bool leaf_tests(const SceneView &S, const lb_params &
 
 // Inner node R.n (trace.wgsl:105-123): true when the ray moved on to its near child (the far one
 // pushed if it was hit too), false when it missed both.
-template <int LAYOUT, bool STATS>
+template <int LAYOUT, bool STATS, typename C>
 __device__ __forceinline__ bool inner_visit(const SceneView &S, Ray &R, const StackView &st,
-                                            Counts &cnt) {
+                                            C &cnt) {
     f3 amn, amx, bmn, bmx;
     uint32_t ra, rb;
     children<LAYOUT>(S, R.n, amn, amx, bmn, bmx, ra, rb);
@@ -187,9 +187,9 @@ __device__ __forceinline__ bool pop(Ray &R, const StackView &st) {
 }
 
 // One node visit in the reference's order; true when the ray is finished.
-template <int LAYOUT, bool STATS>
+template <int LAYOUT, bool STATS, typename C>
 __device__ __forceinline__ bool ray_step(const SceneView &S, const lb_params &P, Ray &R,
-                                         const StackView &st, Counts &cnt) {
+                                         const StackView &st, C &cnt) {
     if (R.n & LEAF_BIT) {
         if (leaf_tests<STATS>(S, P, R, cnt)) return true;
     } else if (inner_visit<LAYOUT, STATS>(S, R, st, cnt)) {

@@ -14,6 +14,8 @@
 // 48 entries, a push beyond them dropped) over a node record that holds both children's boxes,
 // FP32 (WIDE, 64 B a visit) or 8-bit outward-rounded (QUANT, 32 B a visit). The stack's first
 // `lds_stack` entries live in LDS; deeper entries spill to a per-lane global area.
+#include <type_traits>
+
 #include "lb_host.h"
 #include "lb_records.h"
 #include "lb_traverse.h"
@@ -43,9 +45,9 @@ __device__ __forceinline__ f3 get_f3(const lds_u32 *s) {
 // none is at an inner node), then those take their leaves, and again. The count is the exec
 // mask's population, a scalar operation, so a step costs no extra vector instruction. Each lane
 // still takes its own ray's steps in the reference's order.
-template <int LAYOUT, bool STATS>
+template <int LAYOUT, bool STATS, typename C>
 __device__ __forceinline__ void trace_locked(const SceneView &S, const lb_params &P, Ray &R,
-                                             const StackView &st, Counts &cnt, int leaf_batch) {
+                                             const StackView &st, C &cnt, int leaf_batch) {
     for (;;) {
         bool done = false;
         // lanes that may stay at inner nodes before the wave turns to the leaves
@@ -62,9 +64,9 @@ __device__ __forceinline__ void trace_locked(const SceneView &S, const lb_params
     }
 }
 
-template <int LAYOUT, bool STATS>
+template <int LAYOUT, bool STATS, typename C>
 __device__ Hit trace_t(const SceneView &S, const lb_params &P, uint32_t root, f3 o, f3 d,
-                       float tmax, bool any, uint32_t ray, const StackView &st, Counts &cnt,
+                       float tmax, bool any, uint32_t ray, const StackView &st, C &cnt,
                        int leaf_batch) {
     Ray R;
     ray_start(R, root, o, d, tmax, any, ray);
@@ -162,6 +164,7 @@ struct TunedArgs {
     uint8_t *cost;        // gather, counting launch: each ray's steps by azimuth slot (or null)
     const uint8_t *perm;  // gather: slot s of item i takes azimuth slot perm[i * rays + s]
     uint2 *rec;           // gather: each ray's record at r * n_items + i (or null)
+    uint32_t xcd_split;   // units split over the chiplets' queues (next_unit_split)
 };
 
 
@@ -171,6 +174,33 @@ __device__ __forceinline__ uint32_t next_unit(uint32_t *queue, uint32_t n_units)
     if ((threadIdx.x & 63u) == 0u) u = atomicAdd(queue, 1u);
     u = __builtin_amdgcn_readfirstlane(u);
     return u < n_units ? u : 0xffffffffu;
+}
+
+// The units split into XCDS contiguous ranges with a counter each (128 bytes apart). The
+// dispatcher deals workgroups to the chiplets in turn, so workgroup b runs on chiplet b % XCDS:
+// a wave takes units from its own chiplet's range, then from the next ranges as they run dry,
+// and ends when it comes back round. Neighbouring items trace into neighbouring geometry, so a
+// chiplet's L2 holds the part of the scene its range sees. `range` is the wave's current range
+// (blockIdx.x % XCDS to start).
+constexpr uint32_t XCDS = 8, QUEUE_STRIDE = 32;
+__device__ __forceinline__ uint32_t next_unit_split(uint32_t *queue, uint32_t n_units,
+                                                    uint32_t &range) {
+    uint32_t home = blockIdx.x % XCDS;
+    for (;;) {
+        uint32_t lo = (uint32_t)((uint64_t)n_units * range / XCDS);
+        uint32_t hi = (uint32_t)((uint64_t)n_units * (range + 1u) / XCDS);
+        uint32_t u = 0;
+        if ((threadIdx.x & 63u) == 0u) u = atomicAdd(queue + range * QUEUE_STRIDE, 1u);
+        u = __builtin_amdgcn_readfirstlane(u);
+        if (u < hi - lo) return lo + u;
+        range = (range + 1u) % XCDS;
+        if (range == home) return 0xffffffffu;
+    }
+}
+
+__device__ __forceinline__ uint32_t take_unit(const TunedArgs &A, uint32_t n_units,
+                                              uint32_t &range) {
+    return A.xcd_split ? next_unit_split(A.queue, n_units, range) : next_unit(A.queue, n_units);
 }
 
 struct ItemIn {
@@ -233,9 +263,9 @@ __device__ __forceinline__ uint32_t az_slot(uint32_t s, uint32_t total, float ro
 // The slot loop of one item's rays: each lane traces its slot to the end (trace_locked), then
 // takes slot + L. start(slot, R) sets up slot's ray; shade(slot, R) folds its result into the
 // lane's sums.
-template <int LAYOUT, bool STATS, typename Start, typename Shade>
+template <int LAYOUT, bool STATS, typename C, typename Start, typename Shade>
 __device__ __forceinline__ void run_slots(const TunedArgs &A, const StackView &st, uint32_t first,
-                                          uint32_t total, uint32_t L, Counts &cnt, Start start,
+                                          uint32_t total, uint32_t L, C &cnt, Start start,
                                           Shade shade) {
     Ray R;
     for (uint32_t slot = first; slot < total; slot += L) {
@@ -263,8 +293,9 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_di
     f3 sun = mk(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]);
     lds_u32 *slot_lds = group_slot(A.depth, (uint32_t)L);
     Counts cnt{0, 0, 0, 0, 0, 0};
+    uint32_t range = blockIdx.x % XCDS;
     for (;;) {
-        uint32_t unit = next_unit(A.queue, n_units);
+        uint32_t unit = take_unit(A, n_units, range);
         if (unit == 0xffffffffu) break;
         uint32_t i = unit * per_wave + grp;
         bool valid = i < A.n_items;
@@ -390,9 +421,13 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_di
 
 // trace.wgsl:245-270: lane lig of an item's group takes its rays r = lig, lig + L, ...
 // WAVES: waves a SIMD the kernel is compiled to keep resident (0: the compiler's choice).
-template <int LAYOUT, bool STATS, int WAVES>
+// MODE: 0 plain; 1 the counting launch (each ray's steps into A.cost, nothing else counted);
+// 2 the launch's traversal counters as well (lb_stats).
+template <int LAYOUT, int MODE, int WAVES>
 __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_gather_tuned(
     TunedArgs A) {
+    constexpr bool STATS = MODE != 0;
+    using C = typename std::conditional<MODE == 2, Counts, Counts32>::type;
     StackView st = make_stack(A.spill, A.depth);
     const lb_params &P = A.P;
     int L = (int)A.lanes_per_item;
@@ -403,9 +438,10 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
     uint32_t seed = fbits(P.reach[3]);
     uint32_t rays = P.counts[2];
     lds_u32 *slot_lds = group_slot(A.depth, (uint32_t)L);
-    Counts cnt{0, 0, 0, 0, 0, 0};
+    C cnt{0, 0, 0, 0, 0, 0};
+    uint32_t range = blockIdx.x % XCDS;
     for (;;) {
-        uint32_t unit = next_unit(A.queue, n_units);
+        uint32_t unit = take_unit(A, n_units, range);
         if (unit == 0xffffffffu) break;
         uint32_t i = unit * per_wave + grp;
         bool valid = i < A.n_items;
@@ -430,7 +466,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
         }
         KSum sr{0, 0}, sg{0, 0}, sb{0, 0};
         float back = 0.0f;
-        uint64_t steps0 = 0;
+        decltype(cnt.inner) steps0 = 0;
         uint32_t ray_of_slot = 0;
         auto start = [&](uint32_t slot, Ray &R) {
             // read the slot afresh (the compiler would otherwise keep the stored values live)
@@ -458,7 +494,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
                 A.rec[(size_t)ray_of_slot * A.n_items + i] =
                     gather_record(A.S, P, R.o, R.d, R.tri, R.back, R.t);
             if (STATS && A.cost) {
-                uint64_t steps = cnt.inner + cnt.tris - steps0;
+                uint32_t steps = (uint32_t)(cnt.inner + cnt.tris - steps0);
                 uint32_t s = A.perm ? (uint32_t)A.perm[(size_t)i * rays + slot] : slot;
                 A.cost[(size_t)i * rays + s] = (uint8_t)(steps < 255u ? steps : 255u);
             }
@@ -490,7 +526,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
             A.out[i] = make_float4(sr.value() * PI / (float)rays, sg.value() * PI / (float)rays,
                                    sb.value() * PI / (float)rays, back / (float)rays);
     }
-    if (STATS) flush_counts(cnt, A.stats);
+    if constexpr (MODE == 2) flush_counts(cnt, A.stats);
 }
 
 // trace.wgsl:272-324 over rays + g^2 sun slots; WAVES as for the gather
@@ -510,8 +546,9 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_vo
     f3 sund = mk(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]);
     frame_t sf = frame(sund);
     Counts cnt{0, 0, 0, 0, 0, 0};
+    uint32_t range = blockIdx.x % XCDS;
     for (;;) {
-        uint32_t unit = next_unit(A.queue, n_units);
+        uint32_t unit = take_unit(A, n_units, range);
         if (unit == 0xffffffffu) break;
         uint32_t i = unit * per_wave + grp;
         bool valid = i < A.n_items;
@@ -594,26 +631,36 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_vo
 }
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
+// the chiplets' counters (XCDS of them, QUEUE_STRIDE words apart: 1 KiB)
+__global__ void k_zero_queue(uint32_t *q) {
+    if (threadIdx.x < XCDS) q[threadIdx.x * QUEUE_STRIDE] = 0u;
+}
 
 using KernelFn = void (*)(TunedArgs);
 
 // The kernel of a pass, layout, counters and residency target (waves a SIMD: 0, 6 or 8).
-template <int LAYOUT, bool STATS, int WAVES>
+// MODE as k_gather_tuned's: the other passes have no counting launch (1 is taken as 0)
+template <int LAYOUT, int MODE, int WAVES>
 static KernelFn pick_waves(int pass) {
-    if (pass == LB_PASS_DIRECT) return k_direct_tuned<LAYOUT, STATS, WAVES>;
-    if (pass == LB_PASS_GATHER) return k_gather_tuned<LAYOUT, STATS, WAVES>;
-    return k_volume_tuned<LAYOUT, STATS, WAVES>;
+    if (pass == LB_PASS_DIRECT) return k_direct_tuned<LAYOUT, MODE == 2, WAVES>;
+    if (pass == LB_PASS_GATHER) return k_gather_tuned<LAYOUT, MODE, WAVES>;
+    return k_volume_tuned<LAYOUT, MODE == 2, WAVES>;
 }
-template <int LAYOUT, bool STATS>
-static KernelFn pick_stats(int pass, int waves) {
-    if (waves == 6) return pick_waves<LAYOUT, STATS, 6>(pass);
-    if (waves == 8) return pick_waves<LAYOUT, STATS, 8>(pass);
-    return pick_waves<LAYOUT, STATS, 0>(pass);
+template <int LAYOUT, int MODE>
+static KernelFn pick_mode(int pass, int waves) {
+    if (waves == 6) return pick_waves<LAYOUT, MODE, 6>(pass);
+    if (waves == 8) return pick_waves<LAYOUT, MODE, 8>(pass);
+    return pick_waves<LAYOUT, MODE, 0>(pass);
 }
-static KernelFn pick_kernel(int pass, int layout, bool stats, int waves) {
-    if (layout == LB_NODES_WIDE)
-        return stats ? pick_stats<0, true>(pass, waves) : pick_stats<0, false>(pass, waves);
-    return stats ? pick_stats<1, true>(pass, waves) : pick_stats<1, false>(pass, waves);
+template <int LAYOUT>
+static KernelFn pick_layout(int pass, int mode, int waves) {
+    if (mode == 2) return pick_mode<LAYOUT, 2>(pass, waves);
+    if (mode == 1) return pick_mode<LAYOUT, 1>(pass, waves);
+    return pick_mode<LAYOUT, 0>(pass, waves);
+}
+static KernelFn pick_kernel(int pass, int layout, int mode, int waves) {
+    return layout == LB_NODES_WIDE ? pick_layout<0>(pass, mode, waves)
+                                   : pick_layout<1>(pass, mode, waves);
 }
 
 // Lanes sharing one item: the rays an item casts over the rays a lane should take, as a power
@@ -635,10 +682,11 @@ struct TunedGrid {
     size_t lds;
 };
 
-static hipError_t tuned_grid(lb_gpu *gpu, int pass, bool stats, const lb_params &P, uint32_t n,
+// mode: 0 plain, 1 the gather's counting launch, 2 with the traversal counters
+static hipError_t tuned_grid(lb_gpu *gpu, int pass, int mode, const lb_params &P, uint32_t n,
                              TunedGrid *out) {
     const Tuning &T = gpu->tuning[pass];
-    out->fn = pick_kernel(pass, T.nodes, stats, T.waves);
+    out->fn = pick_kernel(pass, T.nodes, mode, T.waves);
     out->block = (uint32_t)T.workgroup;
     out->lds = stack_lds_bytes(T.lds_stack, (int)out->block);
     int per_cu = 0;
@@ -703,17 +751,23 @@ hipError_t rank_slots(const uint8_t *cost, uint32_t n_items, uint32_t rays, uint
 hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n) {
     if (gpu->tuning[pass].schedule == 1) return stream_prepare(gpu, pass, P, n);
     TunedGrid tg;
-    hipError_t e = tuned_grid(gpu, pass, false, P, n, &tg);
-    if (e != hipSuccess) return e;
-    return tuned_grid(gpu, pass, true, P, n, &tg);
+    for (int mode = 0; mode < 3; mode++) {
+        hipError_t e = tuned_grid(gpu, pass, mode, P, n, &tg);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                         const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
                         float4 *out, const GatherOrder *order) {
     if (gpu->tuning[pass].schedule == 1) return launch_stream(gpu, pass, stats, S, P, items, n, key_base, out);
+    // the gather's counting launch records each ray's steps and counts nothing else
+    bool ordered = order && pass == LB_PASS_GATHER && P.counts[2] <= 256u;
+    uint8_t *cost = ordered ? order->cost : nullptr;
+    int mode = stats ? 2 : (cost ? 1 : 0);
     TunedGrid tg;
-    hipError_t e = tuned_grid(gpu, pass, stats, P, n, &tg);
+    hipError_t e = tuned_grid(gpu, pass, mode, P, n, &tg);
     if (e != hipSuccess) return e;
     TunedArgs A;
     A.S = S;
@@ -728,11 +782,11 @@ hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
     A.lanes_per_item = tg.lanes;
     A.depth = (uint32_t)gpu->tuning[pass].lds_stack;
     A.leaf_batch = gpu->tuning[pass].leaf_batch;
-    bool ordered = order && pass == LB_PASS_GATHER && P.counts[2] <= 256u;
-    A.cost = ordered && stats ? order->cost : nullptr;
+    A.cost = cost;
     A.perm = ordered ? order->perm : nullptr;
     A.rec = order && pass == LB_PASS_GATHER ? order->rec : nullptr;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, gpu->stream, gpu->queue);
+    A.xcd_split = (uint32_t)gpu->tuning[pass].xcd_split;
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, gpu->stream, gpu->queue);
     hipLaunchKernelGGL(tg.fn, dim3(tg.grid), dim3(tg.block), tg.lds, gpu->stream, A);
     return hipGetLastError();
 }

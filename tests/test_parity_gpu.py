@@ -223,3 +223,18 @@ def test_pooled_counters(gpu, setup, pass_):
                          setup["charts"], setup["params"], items, 0, setup["light"],
                          counters=True)
     assert st == want
+
+
+@pytest.mark.parametrize("pass_", [Pass.DIRECT, Pass.GATHER, Pass.VOLUME])
+def test_tuned_xcd_split_same_results(gpu, setup, pass_):
+    """The chiplet-split queue hands out the same items in another order; every item's rays and
+    sums are its own, so the results are bit for bit those of one queue, and match the oracle
+    to the TUNED bar."""
+    items = setup["probes"] if pass_ == Pass.VOLUME else setup["items"]
+    gpu.set_form(N.FORM_TUNED)
+    out = {}
+    for x in (0, 1):
+        gpu.set_tuning(pass_=pass_, xcd_split=x)
+        out[x] = gpu.run(pass_, setup["sb"], setup["params"], items, setup["light"])
+    assert np.array_equal(kit.bits(out[0]), kit.bits(out[1]))
+    _compare_tuned(pass_, out[1], _oracle(setup, pass_, items), gpu.tuning(pass_)["nodes"] == 1)
