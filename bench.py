@@ -234,6 +234,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--gather-order", type=int, choices=[0, 1, 2], default=None,
+                    help="lock-step gather: 0 azimuth then ranked, 1 elevation bands then "
+                         "ranked, 2 elevation bands every bounce; default: the bench's")
     ap.add_argument("--xcd-split", type=int, choices=[0, 1], default=None,
                     help="lock-step passes: items split over the 8 chiplets' queues (1) or one "
                          "queue (0); default: the library's")
@@ -285,6 +288,8 @@ def main() -> int:
     if args.xcd_split is not None:
         for p in Pass:
             gpu.set_tuning(pass_=p, xcd_split=args.xcd_split)
+    if args.gather_order is not None:
+        gpu.set_tuning(pass_=Pass.GATHER, gather_order=args.gather_order)
     print(f"# {gpu.adapter}; scene {scene.triangle_count()} triangles ({t_scene:.1f} s)",
           file=sys.stderr, flush=True)
     rebake = {}

@@ -151,6 +151,11 @@ typedef struct lb_tuning {
     int xcd_split;      /* lock-step schedule: 1: the items split into 8 contiguous ranges, one a
                            chiplet (XCD), each chiplet's waves taking their own range first, so a
                            chiplet's L2 holds the part of the scene its texels see; 0: one queue */
+    int gather_order;   /* lock-step gather, the order an item's rays go to its lanes: 0: by
+                           azimuth, later bounces of a bake ranked by the first bounce's steps; 1:
+                           in elevation bands (32 rays a band, one an azimuth sector), later
+                           bounces ranked; 2: elevation bands in every bounce, nothing counted or
+                           ranked */
 } lb_tuning;
 
 /* Sets the knobs of all three passes; lb_get_tuning reads the gather pass's. */

@@ -40,7 +40,7 @@ class Tuning(C.Structure):
     _fields_ = [("workgroup", C.c_int), ("rays_per_lane", C.c_int), ("lds_stack", C.c_int),
                 ("nodes", C.c_int), ("refill", C.c_int), ("schedule", C.c_int),
                 ("chunk", C.c_int), ("leaf_batch", C.c_int), ("waves", C.c_int),
-                ("xcd_split", C.c_int)]
+                ("xcd_split", C.c_int), ("gather_order", C.c_int)]
 
 
 class MaterialDesc(C.Structure):

@@ -485,7 +485,8 @@ int pass_stats(lb_bake *B, int pass, const lb_params *P, const lb_texel *items, 
 // rays of like length. The ranking changes only which lane traces which ray when.
 bool ordered_gather(const lb_bake *B) {
     const Tuning &t = B->gpu->tuning[LB_PASS_GATHER];
-    return B->opt.form == LB_FORM_TUNED && t.schedule == 0 && B->P.counts[2] <= 256u &&
+    return B->opt.form == LB_FORM_TUNED && t.schedule == 0 && t.gather_order != 2 &&
+           B->P.counts[2] <= 256u &&
            B->sc.gi_bounces > 1 && B->ng > 0 && B->d_perm.ptr;
 }
 

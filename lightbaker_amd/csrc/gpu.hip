@@ -284,6 +284,8 @@ static int check_tuning(const lb_gpu *gpu, const lb_tuning *t, const char *who) 
         return fail("%s: waves %d is not 0, 6 or 8", who, t->waves);
     if (t->xcd_split != 0 && t->xcd_split != 1)
         return fail("%s: xcd_split %d is not 0 or 1", who, t->xcd_split);
+    if (t->gather_order < 0 || t->gather_order > 2)
+        return fail("%s: gather_order %d is not 0, 1 or 2", who, t->gather_order);
     if (stack_lds_bytes(t->lds_stack, t->workgroup) > gpu->lds_per_block)
         return fail("%s: an LDS stack of %d entries for %d lanes exceeds %zu bytes", who,
                     t->lds_stack, t->workgroup, gpu->lds_per_block);
@@ -301,6 +303,7 @@ static void put_tuning(Tuning &d, const lb_tuning *t) {
     d.leaf_batch = t->leaf_batch;
     d.waves = t->waves;
     d.xcd_split = t->xcd_split;
+    d.gather_order = t->gather_order;
 }
 
 static void get_tuning(const Tuning &s, lb_tuning *t) {
@@ -314,6 +317,7 @@ static void get_tuning(const Tuning &s, lb_tuning *t) {
     t->leaf_batch = s.leaf_batch;
     t->waves = s.waves;
     t->xcd_split = s.xcd_split;
+    t->gather_order = s.gather_order;
 }
 
 int lb_set_tuning(lb_gpu *gpu, const lb_tuning *t) {

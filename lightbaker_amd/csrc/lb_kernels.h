@@ -18,6 +18,7 @@ struct Tuning {
     int leaf_batch = 16;     // lanes at a leaf that end a wave's inner phase (64: all)
     int waves = 6;           // lock-step: residency target (0: the compiler's; 6; 8)
     int xcd_split = 0;       // lock-step: items split over the 8 chiplets' queues (0: one queue)
+    int gather_order = 0;    // lock-step gather: 0 azimuth, 1 elevation bands then ranked, 2 bands
 };
 
 // LDS words between two lanes' traversal stacks of `depth` entries: odd, so a wave's lanes

@@ -153,20 +153,58 @@ __device__ __forceinline__ bool leaf_tests(const SceneView &S, const lb_params &
 
 // Inner node R.n (trace.wgsl:105-123): true when the ray moved on to its near child (the far one
 // pushed if it was hit too), false when it missed both.
+// A QUANT record's two slab tests without decoding the corners: the plane of byte q on axis a
+// lies at t = q * (2^e_a * inv_a) + (origin_a - o_a) * inv_a, one fused multiply-add a plane
+// after six products a node. The planes' t round differently from (corner - o) * inv on the
+// decoded corners, by a few ulps; the encoder keeps every corner outside its FP32 box in exact
+// arithmetic as well as after rounding, so the boxes stay conservative.
+__device__ __forceinline__ void quant_slabs(const SceneView &S, const Ray &R, float &la, bool &ha,
+                                            float &lb, bool &hb, uint32_t &ra, uint32_t &rb) {
+    const uint4 *q = S.quant + R.n * 2u;
+    uint4 q0 = q[0], q1 = q[1];
+    asm volatile("" ::"v"(q1.w));
+    float kx = bitsf((q0.x & 0xffu) << 23) * R.inv.x, ky = bitsf((q0.y & 0xffu) << 23) * R.inv.y,
+          kz = bitsf((q0.z & 0xffu) << 23) * R.inv.z;
+    float cx = (bitsf(q0.x & 0xffffff00u) - R.o.x) * R.inv.x,
+          cy = (bitsf(q0.y & 0xffffff00u) - R.o.y) * R.inv.y,
+          cz = (bitsf(q0.z & 0xffffff00u) - R.o.z) * R.inv.z;
+    auto b = [](uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); };
+    auto slab = [&](float x0, float x1, float y0, float y1, float z0, float z1, bool *hit) {
+        float lo = vmax(vmax(vmin(x0, x1), vmin(y0, y1)), vmin(z0, z1));
+        float hi = vmin(vmin(vmax(x0, x1), vmax(y0, y1)), vmax(z0, z1));
+        float l0 = vmax(lo, 0.0f);
+        *hit = hi >= l0 && lo < R.t;
+        return l0;
+    };
+    la = slab(__builtin_fmaf(b(q1.x, 0), kx, cx), __builtin_fmaf(b(q1.x, 3), kx, cx),
+              __builtin_fmaf(b(q1.x, 1), ky, cy), __builtin_fmaf(b(q1.y, 0), ky, cy),
+              __builtin_fmaf(b(q1.x, 2), kz, cz), __builtin_fmaf(b(q1.y, 1), kz, cz), &ha);
+    lb = slab(__builtin_fmaf(b(q1.y, 2), kx, cx), __builtin_fmaf(b(q1.z, 1), kx, cx),
+              __builtin_fmaf(b(q1.y, 3), ky, cy), __builtin_fmaf(b(q1.z, 2), ky, cy),
+              __builtin_fmaf(b(q1.z, 0), kz, cz), __builtin_fmaf(b(q1.z, 3), kz, cz), &hb);
+    ra = q0.w;
+    rb = q1.w;
+}
+
 template <int LAYOUT, bool STATS, typename C>
 __device__ __forceinline__ bool inner_visit(const SceneView &S, Ray &R, const StackView &st,
                                             C &cnt) {
-    f3 amn, amx, bmn, bmx;
     uint32_t ra, rb;
-    children<LAYOUT>(S, R.n, amn, amx, bmn, bmx, ra, rb);
     if (STATS) cnt.inner++;
     // box() (trace.wgsl:74-80) gives the entry l0 when the box is hit and FAR = 1e30 otherwise,
     // and an entry is always below tmax <= FAR; so with ha, hb the hit tests, the reference's
     // ta <= tb is ha && (!hb || l0a <= l0b), min(ta, tb) < FAR is ha || hb and
     // max(ta, tb) < FAR is ha && hb, taken here on the tests themselves
     bool ha, hb;
-    float la = slab_hit(R.o, R.inv, amn, amx, R.t, &ha);
-    float lb = slab_hit(R.o, R.inv, bmn, bmx, R.t, &hb);
+    float la, lb;
+    if constexpr (LAYOUT == LB_NODES_QUANT) {
+        quant_slabs(S, R, la, ha, lb, hb, ra, rb);
+    } else {
+        f3 amn, amx, bmn, bmx;
+        children<LAYOUT>(S, R.n, amn, amx, bmn, bmx, ra, rb);
+        la = slab_hit(R.o, R.inv, amn, amx, R.t, &ha);
+        lb = slab_hit(R.o, R.inv, bmn, bmx, R.t, &hb);
+    }
     if (!(ha || hb)) return false;
     bool an = ha && (!hb || la <= lb);
     uint32_t near = an ? ra : rb, far = an ? rb : ra;

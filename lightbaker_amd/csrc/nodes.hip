@@ -104,14 +104,19 @@ static float origin_below(float x) {
 static bool quantize_axis(float origin, uint32_t ebits, float lo, float hi, uint8_t *qlo,
                           uint8_t *qhi) {
     float step = fl(ebits << 23);
+    // the corner after rounding (origin + q * step, one rounding) and in exact arithmetic (the
+    // traversal's fused form skips the rounding); both must lie outside [lo, hi]
     auto dec = [&](int q) { return origin + (float)q * step; };
+    auto ex = [&](int q) { return (long double)origin + (long double)q * (long double)step; };
+    auto below = [&](int q) { return dec(q) <= lo && ex(q) <= (long double)lo; };
+    auto above = [&](int q) { return dec(q) >= hi && ex(q) >= (long double)hi; };
     int a = (int)std::floor((lo - origin) / step);
     if (a < 0) a = 0;
-    while (a > 0 && dec(a) > lo) a--;
-    if (dec(a) > lo) return false;
+    while (a > 0 && !below(a)) a--;
+    if (!below(a)) return false;
     int b = (int)std::ceil((hi - origin) / step);
     if (b < 0) b = 0;
-    while (b <= 255 && dec(b) < hi) b++;
+    while (b <= 255 && !above(b)) b++;
     if (b > 255) return false;
     *qlo = (uint8_t)a;
     *qhi = (uint8_t)b;

@@ -165,6 +165,7 @@ struct TunedArgs {
     const uint8_t *perm;  // gather: slot s of item i takes azimuth slot perm[i * rays + s]
     uint2 *rec;           // gather: each ray's record at r * n_items + i (or null)
     uint32_t xcd_split;   // units split over the chiplets' queues (next_unit_split)
+    uint32_t el_order;    // gather without perm: slots in elevation bands (el_az_slot)
 };
 
 
@@ -258,6 +259,32 @@ __device__ __forceinline__ uint32_t az_slot(uint32_t s, uint32_t total, float ro
     uint32_t b = 31u - (uint32_t)__builtin_clz(total);
     uint32_t m0 = (total - (uint32_t)(rot * (float)total)) & (total - 1u);
     return __builtin_bitreverse32((s + m0) & (total - 1u)) >> (32u - b);
+}
+
+// az_slot's inverse: the slot of ray r
+__device__ __forceinline__ uint32_t az_slot_of(uint32_t r, uint32_t total, float rot) {
+    if (total < 2u || (total & (total - 1u)) != 0u) return r;
+    uint32_t b = 31u - (uint32_t)__builtin_clz(total);
+    uint32_t m0 = (total - (uint32_t)(rot * (float)total)) & (total - 1u);
+    return ((__builtin_bitreverse32(r) >> (32u - b)) - m0) & (total - 1u);
+}
+
+// Slot s of `total` gather rays (a power of two, at least 64) in elevation bands: band s / 32
+// holds the 32 rays of the next smallest elevation parameters u = fract((r + 0.5) / total +
+// rot_u) (from the normal out to the horizon), and within a band position s % 32 takes the ray
+// of the p-th azimuth sector from v = 0, v = fract(radical(r) + rot_v). Any 32 consecutive r hold
+// every residue of r mod 32 once, and radical's top 5 bits are those bits reversed, so a band
+// holds one ray of each of the 32 sectors. The rays of a round (one band) then end at like
+// depths without counting them first.
+__device__ __forceinline__ uint32_t el_az_slot(uint32_t s, uint32_t total, float rot_u,
+                                               float rot_v) {
+    uint32_t mask = total - 1u;
+    // the ray of smallest u: where (r + 0.5) / total + rot_u passes 1
+    uint32_t r_min = (uint32_t)ceilf((float)total * (1.0f - rot_u) - 0.5f) & mask;
+    uint32_t r0 = (r_min + (s & ~31u)) & mask;
+    uint32_t m = (32u - (uint32_t)(rot_v * 32.0f)) & 31u;
+    uint32_t low = __builtin_bitreverse32((s + m) & 31u) >> 27;
+    return (r0 + ((low - r0) & 31u)) & mask;
 }
 
 // The slot loop of one item's rays: each lane traces its slot to the end (trace_locked), then
@@ -467,7 +494,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
         KSum sr{0, 0}, sg{0, 0}, sb{0, 0};
         float back = 0.0f;
         decltype(cnt.inner) steps0 = 0;
-        uint32_t ray_of_slot = 0;
+        uint32_t ray_of_slot = 0, cost_slot = 0;
         auto start = [&](uint32_t slot, Ray &R) {
             // read the slot afresh (the compiler would otherwise keep the stored values live)
             asm volatile("" ::: "memory");
@@ -479,9 +506,19 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
             f.n = n;
             float s1 = bitsf(slot_lds[18]), s2 = bitsf(slot_lds[19]);
             uint32_t key = slot_lds[20];
-            uint32_t s = A.perm ? (uint32_t)A.perm[(size_t)i * rays + slot] : slot;
-            uint32_t r = az_slot(s, rays, s2);
+            uint32_t s, r;
+            if (A.perm) {
+                s = (uint32_t)A.perm[(size_t)i * rays + slot];
+                r = az_slot(s, rays, s2);
+            } else if (A.el_order) {
+                r = el_az_slot(slot, rays, s1, s2);
+                s = az_slot_of(r, rays, s2);
+            } else {
+                s = slot;
+                r = az_slot(s, rays, s2);
+            }
             ray_of_slot = r;
+            cost_slot = s;
             if (STATS) steps0 = cnt.inner + cnt.tris;
             float jx = rnd(key + r, 19u, seed) - 0.5f;
             float jy = rnd(key + r, 23u, seed) - 0.5f;
@@ -495,8 +532,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
                     gather_record(A.S, P, R.o, R.d, R.tri, R.back, R.t);
             if (STATS && A.cost) {
                 uint32_t steps = (uint32_t)(cnt.inner + cnt.tris - steps0);
-                uint32_t s = A.perm ? (uint32_t)A.perm[(size_t)i * rays + slot] : slot;
-                A.cost[(size_t)i * rays + s] = (uint8_t)(steps < 255u ? steps : 255u);
+                A.cost[(size_t)i * rays + cost_slot] = (uint8_t)(steps < 255u ? steps : 255u);
             }
             if (R.tri == NONE) {
                 f3 l = sky(P, R.d);
@@ -786,6 +822,9 @@ hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
     A.perm = ordered ? order->perm : nullptr;
     A.rec = order && pass == LB_PASS_GATHER ? order->rec : nullptr;
     A.xcd_split = (uint32_t)gpu->tuning[pass].xcd_split;
+    uint32_t rays = P.counts[2];
+    A.el_order = pass == LB_PASS_GATHER && !A.perm && gpu->tuning[pass].gather_order >= 1 &&
+                 rays >= 64u && rays <= 65536u && (rays & (rays - 1u)) == 0u;
     hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, gpu->stream, gpu->queue);
     hipLaunchKernelGGL(tg.fn, dim3(tg.grid), dim3(tg.block), tg.lds, gpu->stream, A);
     return hipGetLastError();

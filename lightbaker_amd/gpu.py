@@ -142,18 +142,20 @@ class Gpu:
         N.check(self._lib.lb_set_form(self.handle, int(form)))
 
     TUNING_KEYS = ("workgroup", "rays_per_lane", "lds_stack", "nodes", "refill", "schedule",
-                   "chunk", "leaf_batch", "waves", "xcd_split")
+                   "chunk", "leaf_batch", "waves", "xcd_split", "gather_order")
 
     def set_tuning(self, workgroup: int = None, rays_per_lane: int = None, lds_stack: int = None,
                    nodes: int = None, refill: int = None, schedule: int = None,
                    chunk: int = None, leaf_batch: int = None, waves: int = None,
-                   xcd_split: int = None, pass_: Pass = None) -> None:
+                   xcd_split: int = None, gather_order: int = None,
+                   pass_: Pass = None) -> None:
         """Sets the TUNED form's knobs of one pass, or of all three (starting from the gather
         pass's) when `pass_` is None; the knobs not given keep their value."""
         cur = self.tuning(pass_)
         given = dict(workgroup=workgroup, rays_per_lane=rays_per_lane, lds_stack=lds_stack,
                      nodes=nodes, refill=refill, schedule=schedule, chunk=chunk,
-                     leaf_batch=leaf_batch, waves=waves, xcd_split=xcd_split)
+                     leaf_batch=leaf_batch, waves=waves, xcd_split=xcd_split,
+                     gather_order=gather_order)
         cur.update({k: v for k, v in given.items() if v is not None})
         t = N.Tuning(**{k: int(cur[k]) for k in self.TUNING_KEYS})
         if pass_ is None:

@@ -238,3 +238,14 @@ def test_tuned_xcd_split_same_results(gpu, setup, pass_):
         out[x] = gpu.run(pass_, setup["sb"], setup["params"], items, setup["light"])
     assert np.array_equal(kit.bits(out[0]), kit.bits(out[1]))
     _compare_tuned(pass_, out[1], _oracle(setup, pass_, items), gpu.tuning(pass_)["nodes"] == 1)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_tuned_gather_elevation_bands(gpu, setup, order):
+    """The gather's rays in elevation bands go to other lanes in another order: the same rays,
+    so the same hits, and sums within the TUNED bar."""
+    gpu.set_form(N.FORM_TUNED)
+    gpu.set_tuning(pass_=Pass.GATHER, gather_order=order)
+    got = gpu.run(Pass.GATHER, setup["sb"], setup["params"], setup["items"], setup["light"])
+    _compare_tuned(Pass.GATHER, got, _oracle(setup, Pass.GATHER, setup["items"]),
+                   gpu.tuning(Pass.GATHER)["nodes"] == 1)

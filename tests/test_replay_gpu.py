@@ -56,3 +56,16 @@ def test_tuned_replay_within_tolerance(gpu, traced):
     print(f"tuned replay: max relative difference {err.max():.2e}", flush=True)
     assert err.max() < 1e-4
     assert same(got.validity, want.validity)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_tuned_gather_orders_within_tolerance(gpu, traced, order):
+    """Elevation-band order (then ranked, or in every bounce): the bake within 1e-4 of PLAIN."""
+    scene, want = traced
+    from lightbaker_amd.gpu import Pass
+    gpu.set_tuning(pass_=Pass.GATHER, gather_order=order)
+    got = bake(scene, gpu, form=N.FORM_TUNED, graph=True)
+    a, b = got.light.reshape(-1, 4), want.light.reshape(-1, 4)
+    err = np.abs(a - b) / (np.abs(b) + 1e-3)
+    assert err.max() < 1e-4, err.max()
+    assert same(got.validity, want.validity)
