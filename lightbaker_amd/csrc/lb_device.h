@@ -135,13 +135,14 @@ __device__ __forceinline__ float slab(f3 o, f3 inv, f3 mn, f3 mx, float tmax) {
 
 // slab() as its two parts: the entry max(lo, 0) and whether the box counts as hit (slab() is
 // the entry when hit, FAR when not)
+// With tmax > 0 (a ray's reach, or a hit's t > 1e-4), lo < tmax is l0 < tmax: when lo < 0 both
+// hold. The entry is taken in one chain of maxima from 0.
 __device__ __forceinline__ float slab_hit(f3 o, f3 inv, f3 mn, f3 mx, float tmax, bool *hit) {
     f3 a = mul(sub(mn, o), inv);
     f3 b = mul(sub(mx, o), inv);
-    float lo = vmax(vmax(vmin(a.x, b.x), vmin(a.y, b.y)), vmin(a.z, b.z));
+    float l0 = vmax(vmax(vmax(0.0f, vmin(a.x, b.x)), vmin(a.y, b.y)), vmin(a.z, b.z));
     float hi = vmin(vmin(vmax(a.x, b.x), vmax(a.y, b.y)), vmax(a.z, b.z));
-    float l0 = vmax(lo, 0.0f);
-    *hit = hi >= l0 && lo < tmax;
+    *hit = hi >= l0 && l0 < tmax;
     return l0;
 }
 

@@ -169,11 +169,11 @@ __device__ __forceinline__ void quant_slabs(const SceneView &S, const Ray &R, fl
           cy = (bitsf(q0.y & 0xffffff00u) - R.o.y) * R.inv.y,
           cz = (bitsf(q0.z & 0xffffff00u) - R.o.z) * R.inv.z;
     auto b = [](uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); };
+    // as slab_hit: the entry max(lo, 0) taken in one chain, and lo < t as l0 < t (t > 0)
     auto slab = [&](float x0, float x1, float y0, float y1, float z0, float z1, bool *hit) {
-        float lo = vmax(vmax(vmin(x0, x1), vmin(y0, y1)), vmin(z0, z1));
+        float l0 = vmax(vmax(vmax(0.0f, vmin(x0, x1)), vmin(y0, y1)), vmin(z0, z1));
         float hi = vmin(vmin(vmax(x0, x1), vmax(y0, y1)), vmax(z0, z1));
-        float l0 = vmax(lo, 0.0f);
-        *hit = hi >= l0 && lo < R.t;
+        *hit = hi >= l0 && l0 < R.t;
         return l0;
     };
     la = slab(__builtin_fmaf(b(q1.x, 0), kx, cx), __builtin_fmaf(b(q1.x, 3), kx, cx),
