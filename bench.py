@@ -371,7 +371,7 @@ def main() -> int:
     if form == N.FORM_PLAIN:
         kernel = "k_gather_plain<false>"
     elif tuning["schedule"] == 0:
-        kernel = f"k_gather_tuned<{layout}, false, {tuning['waves']}>"
+        kernel = f"k_gather_tuned<{layout}, 0, {tuning['waves']}>"
     else:
         kernel = f"k_pool<0, {layout}, false>"
     # the recorded counters are the map's gather launch
