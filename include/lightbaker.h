@@ -254,6 +254,11 @@ typedef struct lb_bvh lb_bvh;
  * left out, as are degenerate ones. */
 int lb_bvh_build(const float *positions, size_t n_tris, const uint32_t *material,
                  const uint32_t *chart, const uint8_t *casts, size_t n_materials, lb_bvh **out);
+/* As lb_bvh_build on up to `threads` host threads (0: the host's): the same nodes and
+ * triangles whatever the count (subtrees are built apart and numbered as the single loop would). */
+int lb_bvh_build_threads(const float *positions, size_t n_tris, const uint32_t *material,
+                         const uint32_t *chart, const uint8_t *casts, size_t n_materials,
+                         int threads, lb_bvh **out);
 size_t lb_bvh_nodes(const lb_bvh *bvh, const lb_node **nodes);
 size_t lb_bvh_tris(const lb_bvh *bvh, const lb_tri **tris);
 void lb_bvh_destroy(lb_bvh *bvh);
@@ -298,7 +303,8 @@ typedef struct lb_bake_cache {
 typedef struct lb_bake_options {
     int form;      /* LB_FORM_* of the ray passes */
     int graph;     /* nonzero: the bounce loop (gather + texel passes) replayed from a hipGraph */
-    int threads;   /* host threads (0: all); the tiles are rasterized on the device */
+    int threads;   /* host threads of the BVH build (0: all, at most 16); the tiles are
+                      rasterized on the device */
     const lb_bake_cache *cache; /* NULL: bake every tile and every probe */
     /* nonzero: the first bounce records what each gather ray meets and the later bounces replay
      * those records with the new light instead of tracing the same rays again (every bounce

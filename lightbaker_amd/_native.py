@@ -140,6 +140,7 @@ SIGNATURES = {
     "lb_timer_end": (_I, [_VP, C.POINTER(C.c_float)]),
     "lb_set_form": (_I, [_VP, _I]),
     "lb_bvh_build": (_I, [_VP, _SZ, _VP, _VP, _VP, _SZ, C.POINTER(_VP)]),
+    "lb_bvh_build_threads": (_I, [_VP, _SZ, _VP, _VP, _VP, _SZ, _I, C.POINTER(_VP)]),
     "lb_bvh_nodes": (_SZ, [_VP, C.POINTER(_VP)]),
     "lb_bvh_tris": (_SZ, [_VP, C.POINTER(_VP)]),
     "lb_bvh_destroy": (None, [_VP]),

@@ -150,7 +150,8 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
         for (int c = 0; c < 3; c++) B->mats[m * 4 + c] = sc.materials[m].albedo[c];
         B->mats[m * 4 + 3] = sc.materials[m].opacity;
     }
-    build_bvh(pos.data(), n_tris, tmat.data(), tchart.data(), casts.data(), B->nodes, B->tris);
+    build_bvh(pos.data(), n_tris, tmat.data(), tchart.data(), casts.data(), B->nodes, B->tris,
+              opt.threads);
     I.bvh_nodes = (uint32_t)B->nodes.size();
     I.bvh_tris = (uint32_t)B->tris.size();
     I.s_bvh = secs(t0);

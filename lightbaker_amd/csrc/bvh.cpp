@@ -2,11 +2,14 @@
 // 16-bin SAH over the casting, non-degenerate triangles, leaves of at most 4 (more where no split
 // beats a leaf), children adjacent. Scalar FP32 in glam's operation order (Vec3 ops are per
 // component; no FMA: built with -ffp-contract=off) so the node array matches the reference's.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <limits>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -47,23 +50,72 @@ float area(V3 mn, V3 mx) {
 
 const float INF = std::numeric_limits<float>::infinity();
 
+// A slice's reductions, on up to `threads` threads for large slices: minima, maxima and counts
+// are exact, so any grouping gives the single loop's values.
+struct Bins {
+    V3 cmin{INF, INF, INF}, cmax{-INF, -INF, -INF}, pmin{INF, INF, INF}, pmax{-INF, -INF, -INF};
+    V3 bmin[3][BINS], bmax[3][BINS];
+    uint32_t bn[3][BINS];
+};
+
+template <typename F>
+void parallel_chunks(size_t n, size_t threads, F f) {
+    size_t parts = n >= 65536 && threads > 1 ? threads : 1;
+    std::vector<std::thread> pool;
+    for (size_t k = 1; k < parts; k++) pool.emplace_back(f, k, n * k / parts, n * (k + 1) / parts);
+    f(0, 0, n / parts);
+    for (auto &t : pool) t.join();
+}
+
 // bvh.rs:119-151: the axis and plane of the cheapest binned split, if one beats a leaf
-bool best_split(const Prim *p, size_t n, int *axis_out, float *at_out) {
+bool best_split(const Prim *p, size_t n, int *axis_out, float *at_out, size_t threads) {
+    size_t parts = n >= 65536 && threads > 1 ? threads : 1;
+    std::vector<Bins> part(parts);
+    parallel_chunks(n, threads, [&](size_t k, size_t a, size_t b) {
+        Bins &q = part[k];
+        for (size_t i = a; i < b; i++) {
+            q.cmin = vmin3(q.cmin, p[i].centre);
+            q.cmax = vmax3(q.cmax, p[i].centre);
+            q.pmin = vmin3(q.pmin, p[i].min);
+            q.pmax = vmax3(q.pmax, p[i].max);
+        }
+    });
     V3 cmin{INF, INF, INF}, cmax{-INF, -INF, -INF}, pmin = cmin, pmax = cmax;
-    for (size_t i = 0; i < n; i++) {
-        cmin = vmin3(cmin, p[i].centre);
-        cmax = vmax3(cmax, p[i].centre);
+    for (const Bins &q : part) {
+        cmin = vmin3(cmin, q.cmin);
+        cmax = vmax3(cmax, q.cmax);
+        pmin = vmin3(pmin, q.pmin);
+        pmax = vmax3(pmax, q.pmax);
     }
-    for (size_t i = 0; i < n; i++) {
-        pmin = vmin3(pmin, p[i].min);
-        pmax = vmax3(pmax, p[i].max);
-    }
+    float extent3[3] = {cmax.x - cmin.x, cmax.y - cmin.y, cmax.z - cmin.z};
+    parallel_chunks(n, threads, [&](size_t k, size_t a, size_t b) {
+        Bins &q = part[k];
+        for (int axis = 0; axis < 3; axis++)
+            for (int j = 0; j < BINS; j++) {
+                q.bmin[axis][j] = {INF, INF, INF};
+                q.bmax[axis][j] = {-INF, -INF, -INF};
+                q.bn[axis][j] = 0;
+            }
+        for (int axis = 0; axis < 3; axis++) {
+            float extent = extent3[axis];
+            if (extent < 1e-6f) continue;
+            for (size_t i = a; i < b; i++) {
+                float f = (p[i].centre[axis] - cmin[axis]) / extent * (float)BINS;
+                // `as usize` saturates: negative and NaN become 0
+                size_t j = !(f > 0.0f) ? 0 : (f >= 4294967296.0f ? (size_t)BINS - 1 : (size_t)f);
+                if (j > BINS - 1) j = BINS - 1;
+                q.bmin[axis][j] = vmin3(q.bmin[axis][j], p[i].min);
+                q.bmax[axis][j] = vmax3(q.bmax[axis][j], p[i].max);
+                q.bn[axis][j]++;
+            }
+        }
+    });
     float leaf_cost = (float)n * area(pmin, pmax);
     bool have = false;
     float best_cost = 0.0f, best_at = 0.0f;
     int best_axis = 0;
     for (int axis = 0; axis < 3; axis++) {
-        float extent = cmax[axis] - cmin[axis];
+        float extent = extent3[axis];
         if (extent < 1e-6f) continue;
         V3 bmin[BINS], bmax[BINS];
         uint32_t bn[BINS];
@@ -71,15 +123,11 @@ bool best_split(const Prim *p, size_t n, int *axis_out, float *at_out) {
             bmin[b] = {INF, INF, INF};
             bmax[b] = {-INF, -INF, -INF};
             bn[b] = 0;
-        }
-        for (size_t i = 0; i < n; i++) {
-            float f = (p[i].centre[axis] - cmin[axis]) / extent * (float)BINS;
-            // `as usize` saturates: negative and NaN become 0
-            size_t b = !(f > 0.0f) ? 0 : (f >= 4294967296.0f ? (size_t)BINS - 1 : (size_t)f);
-            if (b > BINS - 1) b = BINS - 1;
-            bmin[b] = vmin3(bmin[b], p[i].min);
-            bmax[b] = vmax3(bmax[b], p[i].max);
-            bn[b]++;
+            for (const Bins &q : part) {
+                bmin[b] = vmin3(bmin[b], q.bmin[axis][b]);
+                bmax[b] = vmax3(bmax[b], q.bmax[axis][b]);
+                bn[b] += q.bn[axis][b];
+            }
         }
         for (int s = 1; s < BINS; s++) {
             uint32_t ln = 0, rn = 0;
@@ -115,7 +163,7 @@ bool best_split(const Prim *p, size_t n, int *axis_out, float *at_out) {
 // bvh.rs:50-106
 void build_bvh(const float *positions, size_t n_tris, const uint32_t *material,
                const uint32_t *chart, const uint8_t *casts, std::vector<lb_node> &nodes,
-               std::vector<lb_tri> &tris) {
+               std::vector<lb_tri> &tris, int max_threads) {
     std::vector<Prim> prims;
     prims.reserve(n_tris);
     for (size_t t = 0; t < n_tris; t++) {
@@ -143,27 +191,45 @@ void build_bvh(const float *positions, size_t n_tris, const uint32_t *material,
         tris.assign(1, lb_tri{});
         return;
     }
-    std::vector<std::tuple<size_t, size_t, size_t>> stack{{0, 0, prims.size()}};
-    auto set = [&](size_t ni, V3 bmin, V3 bmax, uint32_t first, uint32_t count) {
-        nodes[ni] = lb_node{{bmin.x, bmin.y, bmin.z}, first, {bmax.x, bmax.y, bmax.z}, count};
+    // The reference's build is one depth-first loop (bvh.rs:82-105): pop a node, split it or make
+    // it a leaf, and on a split append its two children to the node array and push them, the
+    // left one on top. Node numbers follow the order of those appends. Here the loop runs the
+    // same way down to subtrees of at most `deferred` primitives; those are built on their own
+    // threads with the same loop over their own slices (disjoint, so the partitions do not meet)
+    // and numbered locally, and the numbers are put together afterwards in the order the single
+    // loop would have appended them: a subtree's appends all come at the point it is popped.
+    struct Task {
+        size_t node, start, end;        // provisional node id, slice
+        std::vector<lb_node> local;     // its nodes, local 0 its root, children as local ids
     };
-    while (!stack.empty()) {
-        auto [ni, start, end] = stack.back();
-        stack.pop_back();
-        Prim *slice = prims.data() + start;
+    struct Event {
+        bool subtree;
+        size_t id;  // split: the provisional id of the left child (the right is id + 1); subtree: task
+    };
+    std::vector<lb_node> prov(1);  // the loop's own nodes, by provisional id
+    std::vector<Task> tasks;
+    std::vector<Event> events;
+    size_t threads = max_threads > 0 ? (size_t)max_threads : std::thread::hardware_concurrency();
+    if (threads < 1) threads = 1;
+    if (threads > 16) threads = 16;
+    size_t deferred = threads > 1 ? std::max<size_t>(4096, prims.size() / (threads * 8)) : 0;
+
+    // one node of the loop (bvh.rs:84-104) over slice [start, end): returns true on a split,
+    // with the slice partitioned at *mid
+    auto visit = [&](Prim *all, size_t start, size_t end, lb_node &out, size_t *mid,
+                     size_t nthreads) {
+        Prim *slice = all + start;
         size_t len = end - start;
         V3 bmin{INF, INF, INF}, bmax{-INF, -INF, -INF};
         for (size_t i = 0; i < len; i++) {
             bmin = vmin3(bmin, slice[i].min);
             bmax = vmax3(bmax, slice[i].max);
         }
+        out = lb_node{{bmin.x, bmin.y, bmin.z}, (uint32_t)start, {bmax.x, bmax.y, bmax.z},
+                      (uint32_t)len};
         int axis = 0;
         float at = 0.0f;
-        bool split = len > LEAF && best_split(slice, len, &axis, &at);
-        if (!split) {
-            set(ni, bmin, bmax, (uint32_t)start, (uint32_t)len);
-            continue;
-        }
+        if (!(len > LEAF && best_split(slice, len, &axis, &at, nthreads))) return false;
         // bvh.rs:153-162: swap the prims whose centre lies below `at` to the front
         size_t k = 0;
         for (size_t j = 0; j < len; j++) {
@@ -172,18 +238,104 @@ void build_bvh(const float *positions, size_t n_tris, const uint32_t *material,
                 k++;
             }
         }
-        size_t mid = start + k;
-        if (mid == start || mid == end) mid = start + len / 2;
-        if (mid - start == 0 || end - mid == 0) {
-            set(ni, bmin, bmax, (uint32_t)start, (uint32_t)len);
+        size_t m = start + k;
+        if (m == start || m == end) m = start + len / 2;
+        if (m - start == 0 || end - m == 0) return false;
+        *mid = m;
+        out.count = 0;
+        return true;
+    };
+    // the loop over one subtree, numbering its nodes from 0
+    auto build_local = [&](Task &t) {
+        t.local.assign(1, lb_node{});
+        std::vector<std::tuple<size_t, size_t, size_t>> st{{0, t.start, t.end}};
+        while (!st.empty()) {
+            auto [ni, start, end] = st.back();
+            st.pop_back();
+            size_t mid = 0;
+            lb_node nd;
+            if (!visit(prims.data(), start, end, nd, &mid, 1)) {
+                t.local[ni] = nd;
+                continue;
+            }
+            size_t left = t.local.size();
+            t.local.push_back(lb_node{});
+            t.local.push_back(lb_node{});
+            nd.first = (uint32_t)left;
+            t.local[ni] = nd;
+            st.emplace_back(left + 1, mid, end);
+            st.emplace_back(left, start, mid);
+        }
+    };
+
+    std::vector<std::tuple<size_t, size_t, size_t>> stack{{0, 0, prims.size()}};
+    while (!stack.empty()) {
+        auto [pn, start, end] = stack.back();
+        stack.pop_back();
+        if (end - start <= deferred && end - start > LEAF) {
+            events.push_back(Event{true, tasks.size()});
+            tasks.push_back(Task{pn, start, end, {}});
             continue;
         }
-        size_t left = nodes.size();
-        nodes.push_back(lb_node{});
-        nodes.push_back(lb_node{});
-        set(ni, bmin, bmax, (uint32_t)left, 0);
+        size_t mid = 0;
+        lb_node nd;
+        if (!visit(prims.data(), start, end, nd, &mid, threads)) {
+            prov[pn] = nd;
+            continue;
+        }
+        size_t left = prov.size();
+        prov.push_back(lb_node{});
+        prov.push_back(lb_node{});
+        nd.first = (uint32_t)left;  // provisional; renumbered below
+        prov[pn] = nd;
+        events.push_back(Event{false, left});
         stack.emplace_back(left + 1, mid, end);
         stack.emplace_back(left, start, mid);
+    }
+    // the subtrees, largest first, on the pool
+    if (!tasks.empty()) {
+        std::vector<size_t> order(tasks.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = i;
+        std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+            return tasks[a].end - tasks[a].start > tasks[b].end - tasks[b].start;
+        });
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t i; (i = next.fetch_add(1)) < order.size();) build_local(tasks[order[i]]);
+        };
+        std::vector<std::thread> pool;
+        for (size_t i = 1; i < std::min(threads, tasks.size()); i++) pool.emplace_back(work);
+        work();
+        for (auto &th : pool) th.join();
+    }
+    // the single loop's numbers: provisional ids and each subtree's local ids, in append order
+    std::vector<size_t> pmap(prov.size(), 0);
+    std::vector<size_t> tbase(tasks.size(), 0);  // a subtree's local id k > 0 is tbase + k - 1
+    size_t g = 1;
+    for (const Event &e : events) {
+        if (!e.subtree) {
+            pmap[e.id] = g;
+            pmap[e.id + 1] = g + 1;
+            g += 2;
+        } else {
+            tbase[e.id] = g;
+            g += tasks[e.id].local.size() - 1;
+        }
+    }
+    nodes.assign(g, lb_node{});
+    for (size_t pn = 0; pn < prov.size(); pn++) {
+        lb_node nd = prov[pn];
+        if (nd.count == 0) nd.first = (uint32_t)pmap[nd.first];
+        nodes[pmap[pn]] = nd;
+    }
+    for (size_t ti = 0; ti < tasks.size(); ti++) {
+        const Task &t = tasks[ti];
+        auto gid = [&](size_t k) { return k == 0 ? pmap[t.node] : tbase[ti] + k - 1; };
+        for (size_t k = 0; k < t.local.size(); k++) {
+            lb_node nd = t.local[k];
+            if (nd.count == 0) nd.first = (uint32_t)gid(nd.first);
+            nodes[gid(k)] = nd;
+        }
     }
     tris.reserve(prims.size());
     for (auto &p : prims) tris.push_back(p.tri);

@@ -20,7 +20,14 @@ extern "C" {
 
 int lb_bvh_build(const float *positions, size_t n_tris, const uint32_t *material,
                  const uint32_t *chart, const uint8_t *casts, size_t n_materials, lb_bvh **out) {
+    return lb_bvh_build_threads(positions, n_tris, material, chart, casts, n_materials, 0, out);
+}
+
+int lb_bvh_build_threads(const float *positions, size_t n_tris, const uint32_t *material,
+                         const uint32_t *chart, const uint8_t *casts, size_t n_materials,
+                         int threads, lb_bvh **out) {
     if (!out) return lbk::fail("lb_bvh_build: null handle");
+    if (threads < 0) return lbk::fail("lb_bvh_build: %d threads", threads);
     *out = nullptr;
     if (n_tris && (!positions || !material || !chart)) return lbk::fail("lb_bvh_build: null input");
     if (!casts && n_materials) return lbk::fail("lb_bvh_build: null casts");
@@ -30,7 +37,7 @@ int lb_bvh_build(const float *positions, size_t n_tris, const uint32_t *material
                              n_materials);
     auto *b = new (std::nothrow) lb_bvh();
     if (!b) return lbk::fail("lb_bvh_build: out of memory");
-    lbk::build_bvh(positions, n_tris, material, chart, casts, b->nodes, b->tris);
+    lbk::build_bvh(positions, n_tris, material, chart, casts, b->nodes, b->tris, threads);
     *out = b;
     return 0;
 }

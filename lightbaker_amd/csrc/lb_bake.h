@@ -37,10 +37,11 @@ inline bool try_normalize(V3 v, V3 *out) {
     return true;
 }
 
-// bvh.rs:50-106 over non-indexed triangles (9 floats each) with their material and chart.
+// bvh.rs:50-106 over non-indexed triangles (9 floats each) with their material and chart, on
+// up to `threads` host threads (0: the host's, at most 16); the same nodes whatever the count.
 void build_bvh(const float *positions, size_t n_tris, const uint32_t *material,
                const uint32_t *chart, const uint8_t *casts, std::vector<lb_node> &nodes,
-               std::vector<lb_tri> &tris);
+               std::vector<lb_tri> &tris, int threads = 0);
 
 // A scene as the bake reads it (scene.rs:17-128), borrowed from the caller.
 struct SceneRef {

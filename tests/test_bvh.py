@@ -64,3 +64,39 @@ def test_empty_scene_is_one_unreachable_leaf():
     bvh = Bvh.build(pos, np.zeros(1, np.uint32), np.zeros(1, np.uint32), np.array([0], np.uint8))
     assert len(bvh.nodes) == 1 and bvh.nodes[0]["count"] == 0 and len(bvh.tris) == 1
     assert bvh.nodes[0]["min"][0] == np.finfo(np.float32).max
+
+
+def _build_threads(scene, threads):
+    import ctypes as C
+    from lightbaker_amd import _native as N
+    pos = np.concatenate([m.positions.reshape(-1, 9) for m in scene.meshes]).astype(np.float32)
+    mat = np.concatenate([np.full(len(m.positions) // 3, m.material, np.uint32)
+                          for m in scene.meshes])
+    chart = np.arange(len(pos), dtype=np.uint32)
+    casts = np.array([m.casts for m in scene.materials], np.uint8)
+    lib = N.lib()
+    h = C.c_void_p()
+    N.check(lib.lb_bvh_build_threads(N.ptr(pos), len(pos), N.ptr(mat), N.ptr(chart), N.ptr(casts),
+                                     len(casts), threads, C.byref(h)))
+    try:
+        p = C.c_void_p()
+        n = lib.lb_bvh_nodes(h, C.byref(p))
+        nodes = np.frombuffer((C.c_ubyte * (n * 32)).from_address(p.value), np.uint8).copy()
+        n = lib.lb_bvh_tris(h, C.byref(p))
+        tris = np.frombuffer((C.c_ubyte * (n * 48)).from_address(p.value), np.uint8).copy()
+    finally:
+        lib.lb_bvh_destroy(h)
+    return nodes, tris
+
+
+@pytest.mark.parametrize("gen", [lambda: small_scene(1), lambda: map_scene(2)],
+                         ids=["small-scene", "map-scene"])
+def test_threads_build_the_same_tree(gen):
+    """Subtrees built on their own threads and numbered afterwards, and the large slices' bins
+    reduced in parts, give the single loop's nodes and triangle order byte for byte (the map:
+    204k triangles, its top slices past the parallel binning's 65 536)."""
+    scene = gen()
+    one = _build_threads(scene, 1)
+    for t in (2, 3, 8):
+        many = _build_threads(scene, t)
+        assert np.array_equal(one[0], many[0]) and np.array_equal(one[1], many[1]), t
