@@ -449,6 +449,7 @@ def main() -> int:
             "ms_direct": info["ms_direct"], "ms_direct_texel": info["ms_direct_texel"],
             "ms_bounces": info["ms_bounces"], "ms_volume": info["ms_volume"],
             "s_prepare_host": prep_s, "s_finish_host": finish_s,
+            "s_prepare_parts": {k: info[k] for k in ("s_layout", "s_bvh", "s_raster", "s_total")},
             "ms_finish_device": fin["ms_finish"],
             "finish_bytes": fin_bytes, "finish_gbps": fin_bytes / (fin["ms_finish"] / 1e3) / 1e9
             if fin["ms_finish"] > 0 else None,

@@ -13,9 +13,13 @@
 // the inner child's index), so no leaf visit reads another record.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lb_host.h"
@@ -138,9 +142,11 @@ int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const l
     std::vector<uint32_t> quant(n_nodes * 8, 0u);
     std::vector<uint32_t> leafinfo(n_nodes * 2, 0u);
     bool empty = is_empty_bvh(nodes, n_nodes);
-    for (size_t i = 0; i < n_nodes && !empty; i++) {
+    // every inner node's records on its own: the host's threads take ranges of nodes
+    std::atomic<size_t> too_large{SIZE_MAX};
+    auto one = [&](size_t i) -> bool {
         const lb_node &n = nodes[i];
-        if (n.count > 0) continue;
+        if (n.count > 0) return true;
         const lb_node &a = nodes[n.first], &b = nodes[n.first + 1];
         float *w = &wide[i * 16];
         const float vals[12] = {a.min[0], a.min[1], a.min[2], a.max[0], a.max[1], a.max[2],
@@ -170,13 +176,32 @@ int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const l
                     ex[ax] = eb;
                     break;
                 }
-                if (e > 120) return fail("scene: node %zu is too large to quantize", i);
+                if (e > 120) return false;
             }
             q[ax] = bits(origin) | ex[ax];
         }
         q[3] = ra;
         std::memcpy(&q[4], qb, 12);
         q[7] = rb;
+        return true;
+    };
+    if (!empty) {
+        size_t threads = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
+        if (n_nodes < 65536) threads = 1;
+        auto range = [&](size_t k) {
+            for (size_t i = n_nodes * k / threads; i < n_nodes * (k + 1) / threads; i++)
+                if (!one(i)) {
+                    size_t cur = too_large.load();
+                    while (i < cur && !too_large.compare_exchange_weak(cur, i)) {
+                    }
+                }
+        };
+        std::vector<std::thread> pool;
+        for (size_t k = 1; k < threads; k++) pool.emplace_back(range, k);
+        range(0);
+        for (auto &t : pool) t.join();
+        if (too_large.load() != SIZE_MAX)
+            return fail("scene: node %zu is too large to quantize", too_large.load());
     }
     // leaves: (first triangle, count) at the leaf's own index
     for (size_t i = 0; i < n_nodes; i++) {
