@@ -90,7 +90,8 @@ typedef struct lb_scene lb_scene;
 /* Gpu::new (gpu/mod.rs:74-127): opens device 0 and loads the kernels. Errors: "no GPU adapter"
  * when no HIP device exists, "no GPU device" when it cannot be opened. */
 int lb_open(lb_gpu **out);
-/* Gpu.adapter (gpu/mod.rs:62,126): "<device name> (HIP <arch>)". */
+/* Gpu.adapter (gpu/mod.rs:62,126): "<device name> (HIP <arch>, <n> CUs)", the device name
+ * "AMD GPU <pci domain:bus:device>" where the runtime reports none. */
 const char *lb_adapter_name(const lb_gpu *gpu);
 void lb_close(lb_gpu *gpu);
 const char *lb_last_error(void);

@@ -61,7 +61,16 @@ int lb_open(lb_gpu **out) {
         delete g;
         return fail("no GPU device");
     }
-    g->name = std::string(prop.name) + " (HIP " + prop.gcnArchName + ")";
+    // the marketing name where the runtime reports one, else the device's PCI id
+    std::string dev = prop.name;
+    if (dev.empty()) {
+        char id[64];
+        std::snprintf(id, sizeof id, "AMD GPU %04x:%02x:%02x", prop.pciDomainID, prop.pciBusID,
+                      prop.pciDeviceID);
+        dev = id;
+    }
+    g->name = dev + " (HIP " + prop.gcnArchName + ", " + std::to_string(prop.multiProcessorCount) +
+              " CUs)";
     g->cus = prop.multiProcessorCount;
     g->lds_per_block = prop.sharedMemPerBlock;
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
