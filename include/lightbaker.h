@@ -394,7 +394,14 @@ enum lb_bake_array {
     /* the atlases as they ship (output.rs:99-110, after a finish) */
     LB_BAKE_DIRECT_BC5 = 24,       /* BC5 blocks of the direct atlas (encode.rs:37-50), w * h bytes */
     LB_BAKE_INDIRECT_RGB9E5 = 25,  /* u32 per indirect texel (encode.rs:53-69) */
-    LB_BAKE_VOLUME_BYTES = 26      /* 16 bytes per probe (output.rs:165-181) */
+    LB_BAKE_VOLUME_BYTES = 26,     /* 16 bytes per probe (output.rs:165-181) */
+    /* the TUNED lock-step gather's ray order (after an execute that ranked it; else empty): u8
+     * per gather item and ray, item after item. No counterpart in the reference, whose gather
+     * traces each item's rays in index order. */
+    LB_BAKE_GATHER_COST = 27,  /* each ray's traversal steps (inner visits + triangle tests, 255
+                                * at most) in the first bounce, by azimuth slot */
+    LB_BAKE_GATHER_ORDER = 28  /* the azimuth slots ordered by (cost, slot): the later bounces'
+                                * slot s traces the ray of azimuth slot order[s] */
 };
 
 /* The bake's array `which`: its bytes, with *data pointing at them (owned by the bake, valid

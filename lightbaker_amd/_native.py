@@ -114,6 +114,7 @@ BAKE_NODES, BAKE_TRIS, BAKE_MATERIALS, BAKE_PARAMS, BAKE_TEXELS, BAKE_GATHER_IND
 BAKE_TILE_KNOWN, BAKE_TILE_STATE, BAKE_TILE_FIELD, BAKE_TILE_AO = 15, 16, 17, 18
 BAKE_TILE_DIRECT, BAKE_TILE_INDIRECT, BAKE_PROBES, BAKE_BAKED_TILES, BAKE_LIGHT0 = 19, 20, 21, 22, 23
 BAKE_DIRECT_BC5, BAKE_INDIRECT_RGB9E5, BAKE_VOLUME_BYTES = 24, 25, 26
+BAKE_GATHER_COST, BAKE_GATHER_ORDER = 27, 28
 
 # Every symbol include/lightbaker.h declares, with its ctypes signature.
 _VP, _SZ, _U32, _I = C.c_void_p, C.c_size_t, C.c_uint32, C.c_int

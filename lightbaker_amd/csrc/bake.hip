@@ -536,6 +536,7 @@ int bounce(lb_bake *B, lb_stats *acc, uint32_t k) {
             hipError_t e = rank_slots((const uint8_t *)B->d_cost.ptr, (uint32_t)B->ng,
                                       B->P.counts[2], (uint8_t *)B->d_perm.ptr, st);
             if (e != hipSuccess) return hip_fail(e, "rank_slots");
+            B->ranked = true;
         }
     } else {
         TRY(pass_stats(B, LB_PASS_GATHER, &B->P, (const lb_texel *)B->d_gitems.ptr, B->ng,
@@ -571,6 +572,7 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
     lb_stats *sv = per_pass ? &per_pass[2] : nullptr;
     auto t_exec = clk::now();
     B->held = 0;  // the host copies of device arrays are stale from here
+    B->ranked = false;
 
     // the prepared state (bake.rs:204-205: soft 0, ao 1, hard 0, no state; valid = covered)
     TRYH(hipEventRecord(B->ev[0], st));
@@ -971,6 +973,12 @@ size_t lb_bake_array(const lb_bake *b, int which, const void **data) {
     case LB_BAKE_DIRECT_BC5: lazy(b->h_bc5, b->d_bc5, b->direct.empty() ? 0 : na); break;
     case LB_BAKE_INDIRECT_RGB9E5: lazy(b->h_rgb, b->d_rgb, ni); break;
     case LB_BAKE_VOLUME_BYTES: lazy(b->h_vbytes, b->d_vbytes, b->volume.empty() ? 0 : b->np * 16); break;
+    case LB_BAKE_GATHER_COST:
+        lazy(b->h_cost, b->d_cost, b->ranked ? b->ng * b->P.counts[2] : 0);
+        break;
+    case LB_BAKE_GATHER_ORDER:
+        lazy(b->h_perm, b->d_perm, b->ranked ? b->ng * b->P.counts[2] : 0);
+        break;
     case LB_BAKE_GATHER_INDEX:
         if (b->gat.size() != b->ng) {
             auto *m = const_cast<lb_bake *>(b);

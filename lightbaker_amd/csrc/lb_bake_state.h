@@ -35,7 +35,8 @@ struct lb_bake {
     lbk::AtlasMap amap{};
     // host copies of device arrays, read back when lb_bake_array asks (bit k: array k is held)
     mutable uint64_t held = 0;
-    mutable std::vector<uint8_t> h_tl_known, h_tl_state, h_bc5, h_vbytes;
+    mutable std::vector<uint8_t> h_tl_known, h_tl_state, h_bc5, h_vbytes, h_cost, h_perm;
+    bool ranked = false;  // the last execute ranked the gather's rays (d_cost, d_perm hold them)
     mutable std::vector<float> h_tl_field, h_tl_ao, h_tl_direct, h_tl_indirect, h_light, h_light0;
     mutable std::vector<uint32_t> h_rgb;
     mutable std::vector<lb_texel> h_texels;

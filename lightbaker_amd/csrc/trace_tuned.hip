@@ -744,34 +744,66 @@ static hipError_t tuned_grid(lb_gpu *gpu, int pass, int mode, const lb_params &P
     return hipSuccess;
 }
 
-// Each item's azimuth slots sorted by (recorded cost, slot): a bitonic sort of 256 keys in LDS,
-// one workgroup an item. Slot s of an ordered gather then takes the s-th cheapest ray, so the
+// Each item's azimuth slots ordered by (recorded cost, slot): a stable counting sort on the 8-bit
+// cost, one wave an item. Slot s of an ordered gather then takes the s-th cheapest ray, so the
 // lanes of one round trace rays of like length.
-// Items a workgroup at a time over the grid (a launch holds under 2^32 work-items).
-__global__ __launch_bounds__(256) void k_rank_slots(const uint8_t *__restrict__ cost,
-                                                    uint32_t n_items, uint32_t rays,
-                                                    uint8_t *__restrict__ perm) {
-    __shared__ uint32_t key[256];
-    uint32_t t = threadIdx.x;
-    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+//
+// Lane l holds slots j * 64 + l (j < 4). Chunk by chunk in slot order, a lane's place among the
+// slots of its cost is the count of that cost in the earlier chunks (the wave's 256 bins in LDS)
+// plus the lanes below it in this chunk with the same cost (eight ballots, one a cost bit); the
+// chunk's highest lane of each cost then adds the chunk's count to the bin. An exclusive scan of
+// the bins gives each cost's first place. The wave's LDS requests complete in the order it
+// issues them, so it needs no barrier: every wave keeps to its own bins.
+constexpr uint32_t RANK_WAVES = 4;
+__global__ __launch_bounds__(RANK_WAVES * 64) void k_rank_slots(const uint8_t *__restrict__ cost,
+                                                                uint32_t n_items, uint32_t rays,
+                                                                uint8_t *__restrict__ perm) {
+    __shared__ uint32_t bins[RANK_WAVES][256];
+    __shared__ uint8_t order[RANK_WAVES][256];
+    uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
+    volatile uint32_t *bin = bins[w];
+    volatile uint8_t *ord = order[w];
+    uint64_t below = (1ull << l) - 1ull;
+    for (uint32_t item = blockIdx.x * RANK_WAVES + w; item < n_items;
+         item += gridDim.x * RANK_WAVES) {
         size_t base = (size_t)item * rays;
-        key[t] = t < rays ? ((uint32_t)cost[base + t] << 16) | t : 0xffffffffu;
-        __syncthreads();
-        for (uint32_t size = 2; size <= 256u; size <<= 1)
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                uint32_t j = t ^ stride;
-                if (j > t) {
-                    bool up = (t & size) == 0u;
-                    uint32_t a = key[t], b = key[j];
-                    if ((a > b) == up) {
-                        key[t] = b;
-                        key[j] = a;
-                    }
-                }
-                __syncthreads();
+        for (uint32_t k = l; k < 256u; k += 64u) bin[k] = 0u;
+        uint32_t c[4], place[4];
+        for (uint32_t j = 0; j < 4u; j++) {
+            uint32_t s = j * 64u + l;
+            bool v = s < rays;
+            uint32_t x = v ? (uint32_t)cost[base + s] : 0u;
+            uint64_t same = __ballot(v);
+            for (uint32_t b = 0; b < 8u; b++) {
+                uint64_t m = __ballot((x >> b) & 1u);
+                same &= ((x >> b) & 1u) ? m : ~m;
             }
-        if (t < rays) perm[base + t] = (uint8_t)(key[t] & 0xffffu);
-        __syncthreads();
+            uint32_t prior = v ? bin[x] : 0u;
+            c[j] = x;
+            place[j] = prior + (uint32_t)__popcll(same & below);
+            if (v && (same >> l) == 1ull) bin[x] = prior + (uint32_t)__popcll(same);
+        }
+        // each lane's four bins, then the wave's inclusive scan of their sums
+        uint32_t h0 = bin[4u * l], h1 = bin[4u * l + 1u], h2 = bin[4u * l + 2u],
+                 h3 = bin[4u * l + 3u];
+        uint32_t sum = h0 + h1 + h2 + h3, inc = sum;
+        for (uint32_t d = 1; d < 64u; d <<= 1) {
+            uint32_t up = (uint32_t)__shfl_up((int)inc, d);
+            if (l >= d) inc += up;
+        }
+        uint32_t first = inc - sum;
+        bin[4u * l] = first;
+        bin[4u * l + 1u] = first + h0;
+        bin[4u * l + 2u] = first + h0 + h1;
+        bin[4u * l + 3u] = first + h0 + h1 + h2;
+        for (uint32_t j = 0; j < 4u; j++) {
+            uint32_t s = j * 64u + l;
+            if (s < rays) ord[bin[c[j]] + place[j]] = (uint8_t)s;
+        }
+        for (uint32_t j = 0; j < 4u; j++) {
+            uint32_t s = j * 64u + l;
+            if (s < rays) perm[base + s] = ord[s];
+        }
     }
 }
 
@@ -779,8 +811,10 @@ hipError_t rank_slots(const uint8_t *cost, uint32_t n_items, uint32_t rays, uint
                       hipStream_t stream) {
     if (!n_items) return hipSuccess;
     if (rays > 256u) return hipErrorInvalidValue;
-    uint32_t grid = n_items < (1u << 20) ? n_items : (1u << 20);
-    hipLaunchKernelGGL(k_rank_slots, dim3(grid), dim3(256), 0, stream, cost, n_items, rays, perm);
+    uint32_t blocks = (n_items + RANK_WAVES - 1u) / RANK_WAVES;
+    uint32_t grid = blocks < 8192u ? blocks : 8192u;
+    hipLaunchKernelGGL(k_rank_slots, dim3(grid), dim3(RANK_WAVES * 64), 0, stream, cost, n_items,
+                       rays, perm);
     return hipGetLastError();
 }
 

@@ -52,6 +52,8 @@ def lib() -> C.CDLL:
         L.orc_trace_brute.restype = None
         L.orc64_run.argtypes = [I, VP, SZ, VP, VP, VP, VP, VP, SZ, U32, VP, VP]
         L.orc64_run.restype = I
+        L.orc_gather_steps.argtypes = [VP, SZ, VP, VP, VP, VP, SZ, U32, VP]
+        L.orc_gather_steps.restype = I
         _lib = L
     return _lib
 
@@ -101,6 +103,19 @@ def run64(pass_: int, nodes, tris, materials, charts, params, items, key_base: i
                                                                              dtype=np.float32)
     assert L.orc64_run(pass_, _p(nodes), len(nodes), _p(tris), _p(mats), _p(ch), _p(params),
                        _p(items), n, key_base, _p(lt), _p(out)) == 0
+    return out
+
+
+def gather_steps(nodes, tris, materials, params, items, key_base: int = 0) -> np.ndarray:
+    """orc_gather_steps: (items, rays) u8, each gather ray's inner visits + triangle tests."""
+    L = lib()
+    items = np.ascontiguousarray(items)
+    nodes, tris = np.ascontiguousarray(nodes), np.ascontiguousarray(tris)
+    params = np.ascontiguousarray(params)
+    mats = np.ascontiguousarray(materials, dtype=np.float32)
+    out = np.zeros((len(items), int(params["counts"][0][2])), np.uint8)
+    assert L.orc_gather_steps(_p(nodes), len(nodes), _p(tris), _p(mats), _p(params), _p(items),
+                              len(items), key_base, _p(out)) == 0
     return out
 
 

@@ -37,6 +37,9 @@ int orc_run(int pass, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
 int orc_gather_records(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
                        const float *materials, const float *charts, const lb_params *params,
                        const lb_texel *items, size_t n, uint32_t key_base, uint32_t *out);
+int orc_gather_steps(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+                     const float *materials, const lb_params *params, const lb_texel *items,
+                     size_t n, uint32_t key_base, uint8_t *out);
 void orc_smooth64(int w, int h, const uint32_t *chart, const float *normal, const double *v,
                   int C, const uint8_t *valid, int r, double *out);
 void orc_fill64(int w, int h, const uint32_t *chart, double *v, int C, uint8_t *valid, int rings);

@@ -540,6 +540,48 @@ int orc_gather_records(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
     return 0;
 }
 
+/* Each gather ray's traversal steps, inner-node visits plus triangle tests (trace.wgsl:83-130,
+ * capped at 255), ray r of item i at i * rays + r: the rays of trace.wgsl:248-262 as gather_item
+ * makes them, each through trace() with counters of its own. The TUNED gather's first bounce
+ * records the same counts to rank each item's rays by; the reference has no such output. */
+int orc_gather_steps(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+                     const float *materials, const lb_params *params, const lb_texel *items,
+                     size_t n, uint32_t key_base, uint8_t *out) {
+    lb_node tmp;
+    ctx_t c0;
+    memset(&c0, 0, sizeof c0);
+    c0.nodes = empty_fix(nodes, n_nodes, &tmp); c0.tris = tris; c0.materials = materials;
+    c0.P = params;
+    uint32_t rays = params->counts[2];
+    long nn = (long)n;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < nn; i++) {
+        ctx_t c = c0;
+        orc_counters cnt;
+        c.cnt = &cnt;
+        const lb_texel *tx = &items[i];
+        const lb_params *P = c.P;
+        v3 nrm = ld3(tx->normal);
+        frame_t f = frame(nrm);
+        uint32_t key = (key_base + (uint32_t)i) * 7919u;
+        float s1 = rnd(&c, key, 11u), s2 = rnd(&c, key, 13u);
+        v3 pos = ld3(tx->pos), dx = ld3(tx->dx), dy = ld3(tx->dy);
+        for (uint32_t r = 0u; r < rays; r++) {
+            float jx = rnd(&c, key + r, 19u) - 0.5f;
+            float jy = rnd(&c, key + r, 23u) - 0.5f;
+            v3 o = add(add(add(pos, scl(dx, jx)), scl(dy, jy)), scl(nrm, P->reach[2]));
+            float u0 = ((float)r + 0.5f) / (float)rays + s1;
+            float v0 = orc_radical(r) + s2;
+            v3 d = cosine(u0 - floorf(u0), v0 - floorf(v0), &f);
+            memset(&cnt, 0, sizeof cnt);
+            trace(&c, o, d, P->reach[1], 0, key + r + 4000u);
+            uint64_t steps = cnt.inner_visits + cnt.tri_tests;
+            out[(size_t)i * rays + r] = (uint8_t)(steps < 255u ? steps : 255u);
+        }
+    }
+    return 0;
+}
+
 /* One ray through trace() (trace.wgsl:83-130): t, triangle, back. */
 void orc_trace(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, const float *materials,
                const lb_params *params, const float o[3], const float d[3], float tmax, int any,
