@@ -752,58 +752,66 @@ static hipError_t tuned_grid(lb_gpu *gpu, int pass, int mode, const lb_params &P
 // slots of its cost is the count of that cost in the earlier chunks (the wave's 256 bins in LDS)
 // plus the lanes below it in this chunk with the same cost (eight ballots, one a cost bit); the
 // chunk's highest lane of each cost then adds the chunk's count to the bin. An exclusive scan of
-// the bins gives each cost's first place. The wave's LDS requests complete in the order it
-// issues them, so it needs no barrier: every wave keeps to its own bins.
+// the bins gives each cost's first place. A wave's LDS requests complete in the order it issues
+// them and every wave keeps to its own bins, so no barrier is needed, only the compiler's
+// (`order_lds`) between the phases that read what other lanes wrote.
 constexpr uint32_t RANK_WAVES = 4;
+__device__ __forceinline__ void order_lds() { asm volatile("" ::: "memory"); }
 __global__ __launch_bounds__(RANK_WAVES * 64) void k_rank_slots(const uint8_t *__restrict__ cost,
                                                                 uint32_t n_items, uint32_t rays,
                                                                 uint8_t *__restrict__ perm) {
-    __shared__ uint32_t bins[RANK_WAVES][256];
-    __shared__ uint8_t order[RANK_WAVES][256];
+    __shared__ uint32_t bin[RANK_WAVES][256];
+    __shared__ uint8_t ord[RANK_WAVES][256];
     uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
-    volatile uint32_t *bin = bins[w];
-    volatile uint8_t *ord = order[w];
     uint64_t below = (1ull << l) - 1ull;
     for (uint32_t item = blockIdx.x * RANK_WAVES + w; item < n_items;
          item += gridDim.x * RANK_WAVES) {
         size_t base = (size_t)item * rays;
-        for (uint32_t k = l; k < 256u; k += 64u) bin[k] = 0u;
         uint32_t c[4], place[4];
         for (uint32_t j = 0; j < 4u; j++) {
             uint32_t s = j * 64u + l;
-            bool v = s < rays;
-            uint32_t x = v ? (uint32_t)cost[base + s] : 0u;
+            c[j] = s < rays ? (uint32_t)cost[base + s] : 0u;
+        }
+        order_lds();
+        for (uint32_t k = l; k < 256u; k += 64u) bin[w][k] = 0u;
+        order_lds();
+        for (uint32_t j = 0; j < 4u; j++) {
+            bool v = j * 64u + l < rays;
+            uint32_t x = c[j];
             uint64_t same = __ballot(v);
             for (uint32_t b = 0; b < 8u; b++) {
                 uint64_t m = __ballot((x >> b) & 1u);
                 same &= ((x >> b) & 1u) ? m : ~m;
             }
-            uint32_t prior = v ? bin[x] : 0u;
-            c[j] = x;
+            uint32_t prior = v ? bin[w][x] : 0u;
             place[j] = prior + (uint32_t)__popcll(same & below);
-            if (v && (same >> l) == 1ull) bin[x] = prior + (uint32_t)__popcll(same);
+            if (v && (same >> l) == 1ull) bin[w][x] = prior + (uint32_t)__popcll(same);
+            order_lds();
         }
         // each lane's four bins, then the wave's inclusive scan of their sums
-        uint32_t h0 = bin[4u * l], h1 = bin[4u * l + 1u], h2 = bin[4u * l + 2u],
-                 h3 = bin[4u * l + 3u];
+        uint32_t h0 = bin[w][4u * l], h1 = bin[w][4u * l + 1u], h2 = bin[w][4u * l + 2u],
+                 h3 = bin[w][4u * l + 3u];
         uint32_t sum = h0 + h1 + h2 + h3, inc = sum;
         for (uint32_t d = 1; d < 64u; d <<= 1) {
             uint32_t up = (uint32_t)__shfl_up((int)inc, d);
             if (l >= d) inc += up;
         }
         uint32_t first = inc - sum;
-        bin[4u * l] = first;
-        bin[4u * l + 1u] = first + h0;
-        bin[4u * l + 2u] = first + h0 + h1;
-        bin[4u * l + 3u] = first + h0 + h1 + h2;
+        bin[w][4u * l] = first;
+        bin[w][4u * l + 1u] = first + h0;
+        bin[w][4u * l + 2u] = first + h0 + h1;
+        bin[w][4u * l + 3u] = first + h0 + h1 + h2;
+        order_lds();
         for (uint32_t j = 0; j < 4u; j++) {
             uint32_t s = j * 64u + l;
-            if (s < rays) ord[bin[c[j]] + place[j]] = (uint8_t)s;
+            if (s < rays) ord[w][bin[w][c[j]] + place[j]] = (uint8_t)s;
         }
+        order_lds();
         for (uint32_t j = 0; j < 4u; j++) {
             uint32_t s = j * 64u + l;
-            if (s < rays) perm[base + s] = ord[s];
+            if (s < rays) perm[base + s] = ord[w][s];
         }
+        order_lds();
     }
 }
 
@@ -812,7 +820,7 @@ hipError_t rank_slots(const uint8_t *cost, uint32_t n_items, uint32_t rays, uint
     if (!n_items) return hipSuccess;
     if (rays > 256u) return hipErrorInvalidValue;
     uint32_t blocks = (n_items + RANK_WAVES - 1u) / RANK_WAVES;
-    uint32_t grid = blocks < 8192u ? blocks : 8192u;
+    uint32_t grid = blocks < 65536u ? blocks : 65536u;
     hipLaunchKernelGGL(k_rank_slots, dim3(grid), dim3(RANK_WAVES * 64), 0, stream, cost, n_items,
                        rays, perm);
     return hipGetLastError();
