@@ -464,6 +464,21 @@ void fill(lb_bake *B, int channels, float *v, uint32_t rings, const uint8_t *val
     if (final_mask) tp_copy_u8(ma, final_mask, T, st);
 }
 
+// the direct phase's three fills (bake.rs:253-255): the occlusion, the sun field and the direct
+// light fill from the same mask over the same rings, so each ring is one pass over all three
+void fill_direct(lb_bake *B, uint32_t rings, const uint8_t *valid0, uint8_t *final_mask) {
+    hipStream_t st = B->gpu->stream;
+    TexelView &V = B->V;
+    uint32_t T = (uint32_t)B->T;
+    uint8_t *ma = (uint8_t *)B->d_m1.ptr, *mb = (uint8_t *)B->d_m2.ptr;
+    tp_copy_u8(valid0, ma, T, st);
+    for (uint32_t r = 0; r < rings; r++) {
+        tp_fill_ring_direct(V, V.ao, V.field, (float *)V.direct, ma, mb, st);
+        std::swap(ma, mb);
+    }
+    tp_copy_u8(ma, final_mask, T, st);
+}
+
 int pass_stats(lb_bake *B, int pass, const lb_params *P, const lb_texel *items, size_t n,
                const float *light, size_t n_light, float *out, lb_stats *acc) {
     if (!acc) return run_pass(B->gpu, pass, B->opt.form, B->scene, P, items, n, 0, light, n_light,
@@ -607,10 +622,7 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
     tp_direct_light(V, (const float *)B->d_f1.ptr,
                     make_float3(sc.sun_irradiance[0], sc.sun_irradiance[1], sc.sun_irradiance[2]),
                     make_float3(B->sun.x, B->sun.y, B->sun.z), st);
-    fill(B, 1, V.ao, B->rings, V.valid, nullptr);
-    fill(B, 1, V.field, B->rings, V.valid, nullptr);
-    fill(B, 3, (float *)V.direct, B->rings, V.valid,
-         (uint8_t *)B->d_filled.ptr);
+    fill_direct(B, B->rings, V.valid, (uint8_t *)B->d_filled.ptr);
     tp_known(V, (const uint8_t *)B->d_filled.ptr, st);
     tp_write_light(V, (float4 *)B->d_light.ptr, B->aw, st);  // bake.rs:277-281
     TRYH(hipGetLastError());

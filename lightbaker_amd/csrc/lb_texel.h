@@ -73,6 +73,9 @@ void tp_sun_field(const TexelView &V, const uint8_t *covered, uint32_t g, float 
 void tp_direct_light(const TexelView &V, const float *soft, float3 irr, float3 sun, hipStream_t s);
 void tp_fill_ring(const TexelView &V, int channels, float *v, const uint8_t *was, uint8_t *now,
                   hipStream_t s);
+// one ring of the occlusion (1 channel), sun field (1) and direct light (3, stride 4) together
+void tp_fill_ring_direct(const TexelView &V, float *ao, float *field, float *direct,
+                         const uint8_t *was, uint8_t *now, hipStream_t s);
 void tp_known(const TexelView &V, const uint8_t *filled, hipStream_t s);
 void tp_scatter(const float4 *res, const uint32_t *idx, uint32_t n_items, float4 *e, hipStream_t s);
 void tp_write_light(const TexelView &V, float4 *light, uint32_t atlas_w, hipStream_t s);

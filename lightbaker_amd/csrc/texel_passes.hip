@@ -138,20 +138,52 @@ __global__ void k_direct_light(TexelView V, const float *__restrict__ soft, floa
     V.direct[g] = make_float4(irr.x * c * s, irr.y * c * s, irr.z * c * s, 0.0f);
 }
 
-// filter.rs:58-94, one ring: texels without a value take the mean of their 3 x 3 neighbours that
-// had one (their own chart's first; an empty texel any chart's). C channels in `v` (stride S,
-// first C used), updated in place: a ring writes only texels without a value, and reads only
-// texels that had one before it (`was`), whose values no ring changes; `now` out.
+// One array of a fill: C channels at stride S (C = 0: no array), with the two running sums.
 template <int C, int S>
-__global__ void k_fill_ring(TexelView V, float *v, const uint8_t *__restrict__ was,
-                            uint8_t *__restrict__ now, uint32_t n) {
+struct FillArr {
+    float *v;
+    float own[C > 0 ? C : 1], any[C > 0 ? C : 1];
+    __device__ __forceinline__ void clear() {
+        for (int c = 0; c < C; c++) own[c] = any[c] = 0.0f;
+    }
+    __device__ __forceinline__ void add_own(uint32_t j) {
+        for (int c = 0; c < C; c++) own[c] += v[j * S + c];
+    }
+    __device__ __forceinline__ void add_any(uint32_t j) {
+        for (int c = 0; c < C; c++) any[c] += v[j * S + c];
+    }
+    __device__ __forceinline__ void put_own(uint32_t g, int no) {
+        for (int c = 0; c < C; c++) v[g * S + c] = own[c] / (float)no;
+    }
+    __device__ __forceinline__ void put_any(uint32_t g, int na) {
+        for (int c = 0; c < C; c++) v[g * S + c] = any[c] / (float)na;
+    }
+};
+
+// filter.rs:58-94, one ring: texels without a value take the mean of their 3 x 3 neighbours that
+// had one (their own chart's first; an empty texel any chart's). Up to three arrays that fill from
+// the same mask (the direct phase's occlusion, sun field and light) take one pass over the
+// neighbours, each channel summed in the same order as alone. Updated in place: a ring writes only
+// texels without a value, and reads only texels that had one before it (`was`), whose values no
+// ring changes; `now` out.
+template <int C0, int S0, int C1, int S1, int C2, int S2>
+__global__ void k_fill_ring(TexelView V, float *v0, float *v1, float *v2,
+                            const uint8_t *__restrict__ was, uint8_t *__restrict__ now,
+                            uint32_t n) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     uint8_t known = was[g];
     if (!known) {
         Where p = where(V, g);
-        float own[C], any[C];
-        for (int c = 0; c < C; c++) own[c] = any[c] = 0.0f;
+        FillArr<C0, S0> a0;
+        FillArr<C1, S1> a1;
+        FillArr<C2, S2> a2;
+        a0.v = v0;
+        a1.v = v1;
+        a2.v = v2;
+        a0.clear();
+        a1.clear();
+        a2.clear();
         int no = 0, na = 0;
         uint32_t ci = V.chart[g];
         int y0 = max((int)p.y - 1, 0), y1 = min((int)p.y + 2, (int)p.h);
@@ -161,18 +193,26 @@ __global__ void k_fill_ring(TexelView V, float *v, const uint8_t *__restrict__ w
                 uint32_t j = p.off + (uint32_t)yy * p.w + (uint32_t)xx;
                 if (!was[j]) continue;
                 if (ci != NONE && V.chart[j] == ci) {
-                    for (int c = 0; c < C; c++) own[c] += v[j * S + c];
+                    a0.add_own(j);
+                    a1.add_own(j);
+                    a2.add_own(j);
                     no++;
                 } else {
-                    for (int c = 0; c < C; c++) any[c] += v[j * S + c];
+                    a0.add_any(j);
+                    a1.add_any(j);
+                    a2.add_any(j);
                     na++;
                 }
             }
         if (no > 0) {
-            for (int c = 0; c < C; c++) v[g * S + c] = own[c] / (float)no;
+            a0.put_own(g, no);
+            a1.put_own(g, no);
+            a2.put_own(g, no);
             known = 1;
         } else if (na > 0 && ci == NONE) {
-            for (int c = 0; c < C; c++) v[g * S + c] = any[c] / (float)na;
+            a0.put_any(g, na);
+            a1.put_any(g, na);
+            a2.put_any(g, na);
             known = 1;
         }
     }
@@ -357,10 +397,19 @@ void tp_direct_light(const TexelView &V, const float *soft, float3 irr, float3 s
 void tp_fill_ring(const TexelView &V, int channels, float *v, const uint8_t *was, uint8_t *now,
                   hipStream_t s) {
     if (!V.n) return;
+    float *no_array = nullptr;
     if (channels == 1)
-        hipLaunchKernelGGL((k_fill_ring<1, 1>), grid_for(V.n), dim3(256), 0, s, V, v, was, now, V.n);
+        hipLaunchKernelGGL((k_fill_ring<1, 1, 0, 1, 0, 1>), grid_for(V.n), dim3(256), 0, s, V, v,
+                           no_array, no_array, was, now, V.n);
     else
-        hipLaunchKernelGGL((k_fill_ring<3, 4>), grid_for(V.n), dim3(256), 0, s, V, v, was, now, V.n);
+        hipLaunchKernelGGL((k_fill_ring<3, 4, 0, 1, 0, 1>), grid_for(V.n), dim3(256), 0, s, V, v,
+                           no_array, no_array, was, now, V.n);
+}
+void tp_fill_ring_direct(const TexelView &V, float *ao, float *field, float *direct,
+                         const uint8_t *was, uint8_t *now, hipStream_t s) {
+    if (V.n)
+        hipLaunchKernelGGL((k_fill_ring<1, 1, 1, 1, 3, 4>), grid_for(V.n), dim3(256), 0, s, V, ao,
+                           field, direct, was, now, V.n);
 }
 void tp_known(const TexelView &V, const uint8_t *filled, hipStream_t s) {
     if (V.n) hipLaunchKernelGGL(k_known, grid_for(V.n), dim3(256), 0, s, V, filled, V.n);
