@@ -558,12 +558,14 @@ int bounce(lb_bake *B, lb_stats *acc, uint32_t k) {
                        (const float *)B->d_light.ptr, (size_t)B->aw * B->ah,
                        (float *)B->d_gres.ptr, acc));
     }
-    tp_fill_f32((float *)V.indirect, 0.0f, T * 4u, st);
+    // the gathered irradiance at its texels (zero elsewhere) in d_f4, smoothed into the
+    // indirect light
+    float4 *irr = (float4 *)B->d_f4.ptr;
+    tp_fill_f32((float *)irr, 0.0f, T * 4u, st);
     tp_scatter((const float4 *)B->d_gres.ptr, (const uint32_t *)B->d_gat.ptr, (uint32_t)B->ng,
-               V.indirect, st);
-    tp_smooth(V, 4, (const float *)V.indirect, (float *)B->d_f4.ptr,
-              (const uint8_t *)B->d_gmask.ptr, (int)B->rg, (const float *)B->d_wg.ptr, st);
-    tp_copy_f32((const float *)B->d_f4.ptr, (float *)V.indirect, T * 4u, st);
+               irr, st);
+    tp_smooth(V, 4, (const float *)irr, (float *)V.indirect, (const uint8_t *)B->d_gmask.ptr,
+              (int)B->rg, (const float *)B->d_wg.ptr, st);
     fill(B, 3, (float *)V.indirect, B->grings,
          (const uint8_t *)B->d_gmask.ptr, nullptr);
     tp_write_light(V, (float4 *)B->d_light.ptr, B->aw, st);

@@ -111,9 +111,10 @@ __global__ void k_sun_field(TexelView V, const uint8_t *__restrict__ covered, ui
     for (int yy = y0; yy < y1; yy++)
         for (int xx = x0; xx < x1; xx++) {
             uint32_t j = p.off + (uint32_t)yy * p.w + (uint32_t)xx;
-            if (!covered[j] || !same_surface(V, gi, j)) continue;
-            // the samples of the opposite state
+            // the samples of the opposite state, tested first: most neighbours have none, and
+            // then their chart and normal are not read
             uint32_t opp = (lit ? ~V.hard[j] : V.hard[j]) & mask;
+            if (!opp || !covered[j] || !same_surface(V, gi, j)) continue;
             while (opp) {
                 uint32_t c = (uint32_t)__builtin_ctz(opp);
                 opp &= opp - 1u;
