@@ -134,3 +134,15 @@ def test_ranked_gather_order(gpu, gen, rays, nodes):
     assert steps.min() > 0 and len(np.unique(steps)) > 4
     assert np.array_equal(order, np.argsort(cost, axis=1, kind="stable"))
     sess.close()
+
+
+def test_gather_order_empty_unless_ranked(gpu):
+    """A PLAIN bake, and a TUNED one with a single bounce, rank nothing: both arrays are empty."""
+    for form, bounces in ((N.FORM_PLAIN, 2), (N.FORM_TUNED, 1)):
+        scene = small_scene(1)
+        scene.settings.gi_bounces = bounces
+        sess = BakeSession(scene, gpu, form=form, graph=form == N.FORM_TUNED)
+        sess.execute()
+        assert len(sess.array(N.BAKE_GATHER_COST, np.uint8)) == 0, form
+        assert len(sess.array(N.BAKE_GATHER_ORDER, np.uint8)) == 0, form
+        sess.close()
