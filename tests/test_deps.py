@@ -104,3 +104,22 @@ def test_settings_json_layout():
         ["0.5", "2.0", "12.0", "0.8", "0.25", "1.0", "100.0", "-1.0", "0.0"]
     assert _f32(0.1) == "0.1" and _f32(1 / 3) == "0.33333334"
     assert _json({"a": [1.5, 2], "b": True, "c": "x"}) == '{"a":[1.5,2],"b":true,"c":"x"}'
+
+
+def test_settings_f32_shortest_round_trip():
+    """What the f32 layout must hold whatever the value (ryu's contract): the digits read back as
+    the same f32, and no fewer significant digits do; integers keep a ".0"."""
+    from lightbaker_amd.deps import _f32
+    rng = np.random.default_rng(11)
+    bits = rng.integers(0, 0x7f800000, 20000, dtype=np.uint32)
+    vals = np.concatenate([bits.view(np.float32), np.float32([1, 0.1, 0.25, 3e-7, 1e16, 123456.0])])
+    for x in vals:
+        s = _f32(x)
+        assert np.float32(float(s)) == x, (x, s)
+        mant = s.split("e")[0].replace(".", "").replace("-", "").lstrip("0").rstrip("0") or "0"
+        n = len(mant)
+        if n > 1:
+            shorter = np.float32(float(f"{float(x):.{n - 2}e}"))
+            assert shorter != x, (x, s)
+        if "e" not in s:
+            assert "." in s, s
