@@ -146,3 +146,24 @@ def test_gather_order_empty_unless_ranked(gpu):
         assert len(sess.array(N.BAKE_GATHER_COST, np.uint8)) == 0, form
         assert len(sess.array(N.BAKE_GATHER_ORDER, np.uint8)) == 0, form
         sess.close()
+
+
+def test_unranked_gather_beyond_256_rays(gpu):
+    """More than 256 gather rays an item (a slot no longer fits a byte): the TUNED lock-step
+    bake traces every bounce unranked, within 1e-4 of the PLAIN bake, and ranks nothing."""
+    def scene():
+        s = small_scene(1)
+        s.settings.gi_rays, s.settings.gi_bounces = 320, 2
+        return s
+    print("unranked gather: 320 rays", flush=True)
+    want = BakeSession(scene(), gpu, form=N.FORM_PLAIN)
+    want.execute()
+    wl = want.array(N.BAKE_LIGHT, np.float32).reshape(-1, 4)
+    want.close()
+    sess = BakeSession(scene(), gpu, form=N.FORM_TUNED, graph=True)
+    sess.execute()
+    light = sess.array(N.BAKE_LIGHT, np.float32).reshape(-1, 4)
+    assert len(sess.array(N.BAKE_GATHER_ORDER, np.uint8)) == 0
+    sess.close()
+    err = np.abs(light - wl) / (np.abs(wl) + 1e-3)
+    assert err.max() < 1e-4, err.max()
