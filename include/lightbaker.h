@@ -11,6 +11,11 @@
  * Handles are owned by the library; input arrays are borrowed for the duration of a call;
  * output arrays are allocated by the caller. Calls on one lb_gpu are serialized by the caller,
  * as the reference's single-threaded driver does (bake.rs:151,302, volume.rs:77).
+ *
+ * Lifetimes: handles may be released in any order. A bake (lb_bake_*) holds a reference to its
+ * lb_gpu and an adjoint holds one to its bake, so lb_close and lb_bake_destroy only drop the
+ * caller's reference; the handle is freed with the last one. An lb_scene keeps only its device,
+ * never the lb_gpu. Buffers from lb_buffer_alloc are freed through an open lb_gpu.
  */
 #ifndef LIGHTBAKER_H
 #define LIGHTBAKER_H
@@ -360,6 +365,10 @@ typedef struct lb_bake_info {
     double s_layout, s_bvh, s_raster, s_gather_set, s_execute, s_finish, s_total;
     float volume_scale;  /* output.rs:108: the brightest L0 channel, what a volume byte of 255 is */
     float ms_finish;     /* device time of finish's kernels, ms */
+    /* bounce graphs captured so far: a graph is captured again when the gather item count, the
+     * gather's tuning or a buffer its kernels read (the TUNED form's GPU-wide spill area and
+     * pooled hits, which another launch may reallocate) changed since the last capture */
+    uint32_t graph_captures;
 } lb_bake_info;
 
 int lb_bake_info_get(const lb_bake *bake, lb_bake_info *info);

@@ -5,7 +5,9 @@ for the HIP kernels.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -97,7 +99,7 @@ class BakeInfo(C.Structure):
         ("ms_volume", C.c_float), ("s_layout", C.c_double), ("s_bvh", C.c_double),
         ("s_raster", C.c_double), ("s_gather_set", C.c_double), ("s_execute", C.c_double),
         ("s_finish", C.c_double), ("s_total", C.c_double), ("volume_scale", C.c_float),
-        ("ms_finish", C.c_float),
+        ("ms_finish", C.c_float), ("graph_captures", C.c_uint32),
     ]
 
 
@@ -213,3 +215,31 @@ def ptr(a: np.ndarray | None) -> int | None:
         return None
     assert a.flags["C_CONTIGUOUS"]
     return a.ctypes.data
+
+
+# Objects holding native handles, by the order they are closed in: an adjoint before its bake,
+# bakes and scenes and device buffers before their Gpu. Gpu.close closes its own first; at exit
+# every one left open is closed here, while the HIP runtime is still up (Python's exit handlers
+# run before the C library's), so no handle is released from a finalizer during teardown.
+ADJOINT, BAKE, SCENE, ARRAY, GPU = range(5)
+_OWNED = [weakref.WeakSet() for _ in range(5)]
+
+
+def own(obj, rank: int) -> None:
+    """Registers an object with a close() method and a `gpu` attribute (a Gpu: itself)."""
+    _OWNED[rank].add(obj)
+
+
+def close_owned(gpu=None) -> None:
+    """Closes the registered objects of `gpu` (every object when None), in rank order."""
+    for rank, objs in enumerate(_OWNED):
+        for obj in list(objs):
+            if gpu is not None and (rank == GPU or getattr(obj, "gpu", None) is not gpu):
+                continue
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+atexit.register(close_owned)

@@ -28,9 +28,11 @@ class Adjoint:
     def __init__(self, sess: BakeSession):
         self.lib = N.lib()
         self.sess = sess
+        self.gpu = sess.gpu
         h = C.c_void_p()
         N.check(self.lib.lb_adjoint_create(sess.handle, C.byref(h)))
         self.handle = h
+        N.own(self, N.ADJOINT)
         atlas, params, items, rays = (C.c_size_t() for _ in range(4))
         N.check(self.lib.lb_adjoint_info(h, C.byref(atlas), C.byref(params), C.byref(items),
                                          C.byref(rays)))

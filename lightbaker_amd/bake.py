@@ -211,6 +211,7 @@ class BakeSession:
         N.check(self.lib.lb_bake_prepare(gpu.handle, C.byref(self._desc), C.byref(opt),
                                          C.byref(h)))
         self.handle = h
+        N.own(self, N.BAKE)
 
     def info(self) -> dict:
         inf = N.BakeInfo()

@@ -609,13 +609,17 @@ int lb_adjoint_create(lb_bake *bake, lb_adjoint **out) {
         delete a;
         return 1;
     }
+    bake->refs.fetch_add(1);  // the bake lives until this adjoint is destroyed
     *out = a;
     return 0;
 }
 
 void lb_adjoint_destroy(lb_adjoint *adj) {
-    if (adj && adj->B && adj->B->gpu) hipStreamSynchronize(adj->B->gpu->stream);
+    if (!adj) return;
+    lb_bake *B = adj->B;
+    if (B && B->gpu) hipStreamSynchronize(B->gpu->stream);
     delete adj;
+    lb_bake_destroy(B);  // drops the adjoint's reference
 }
 
 int lb_adjoint_info(const lb_adjoint *adj, size_t *atlas_texels, size_t *n_params, size_t *items,

@@ -82,6 +82,7 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     hipStream_t st = gpu->stream;
     hipSetDevice(gpu->device);
     B->gpu = gpu;
+    gpu_retain(gpu);  // released by ~lb_bake
     B->opt = opt;
     B->sc = sc;
     lb_bake_info &I = B->info;
@@ -646,26 +647,33 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
     // the bounces (bake.rs:300-330)
     TRYH(hipEventRecord(B->ev[3], st));
     if (sc.gi_bounces > 0 && B->ng > 0) {
-        // the ordered gather's step counts and ranks, one byte a ray
+        // the ordered gather's step counts and ranks, one byte a ray (not with replayed
+        // bounces: those trace one bounce only)
         size_t nr = B->ng * (size_t)B->P.counts[2];
         if (B->opt.form == LB_FORM_TUNED && B->P.counts[2] <= 256u && sc.gi_bounces > 1 &&
-            (B->d_cost.reserve(nr) || B->d_perm.reserve(nr)))
-            return 1;
+            !B->opt.replay) {
+            if (nr > B->d_perm.cap) B->perm_ng = 0;  // a new area holds no ranking
+            if (B->d_cost.reserve(nr) || B->d_perm.reserve(nr)) return 1;
+        }
         if (B->opt.replay && sc.gi_bounces > 1 && B->d_rec.reserve(nr * 8)) return 1;
         if (B->opt.graph && !per_pass) {
-            // the bounces captured once and replayed (same buffers each time); the item count is
-            // part of the capture, so a new count captures again
-            const Tuning &tn = gpu->tuning[LB_PASS_GATHER];
-            const Tuning &tg = B->graph_tuning;
-            bool same = tn.workgroup == tg.workgroup && tn.rays_per_lane == tg.rays_per_lane &&
-                        tn.lds_stack == tg.lds_stack && tn.nodes == tg.nodes &&
-                        tn.refill == tg.refill && tn.schedule == tg.schedule &&
-                        tn.chunk == tg.chunk && tn.leaf_batch == tg.leaf_batch &&
-                        tn.waves == tg.waves;
-            if (!B->graph || B->graph_ng != B->ng || !same) {
-                if (B->opt.form == LB_FORM_TUNED &&
-                    tuned_prepare(gpu, LB_PASS_GATHER, B->P, (uint32_t)B->ng) != hipSuccess)
-                    return fail("GPU: tuned spill area");
+            // the bounces captured once and replayed. The capture holds the item count, the
+            // gather's tuning and every buffer's address by value: any change captures again.
+            // The spill area is sized for this tuning before the key is taken, so the key sees
+            // the buffers the kernels will use.
+            if (B->opt.form == LB_FORM_TUNED &&
+                tuned_prepare(gpu, LB_PASS_GATHER, B->P, (uint32_t)B->ng) != hipSuccess)
+                return fail("GPU: tuned spill area");
+            GraphKey key;
+            key.ng = B->ng;
+            key.tuning = gpu->tuning[LB_PASS_GATHER];
+            const DevBuf *bufs[6] = {&gpu->spill, &gpu->stream_hits, &gpu->stream_aux,
+                                     &B->d_cost, &B->d_perm, &B->d_rec};
+            for (int i = 0; i < 6; i++) {
+                key.ptr[i] = bufs[i]->ptr;
+                key.cap[i] = bufs[i]->cap;
+            }
+            if (!B->graph || !(key == B->graph_key)) {
                 if (B->graph) hipGraphExecDestroy(B->graph);
                 B->graph = nullptr;
                 hipGraph_t gr;
@@ -678,13 +686,17 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
                 hipError_t ie = hipGraphInstantiate(&B->graph, gr, nullptr, nullptr, 0);
                 hipGraphDestroy(gr);
                 if (ie != hipSuccess) return hip_fail(ie, "graph instantiate");
-                B->graph_ng = B->ng;
-                B->graph_tuning = tn;
+                B->graph_key = key;
+                B->graph_ranks = B->ranked;
+                B->graph_captures++;
             }
             TRYH(hipGraphLaunch(B->graph, st));
+            B->ranked = B->graph_ranks;
+            I.graph_captures = B->graph_captures;
         } else {
             for (uint32_t b = 0; b < sc.gi_bounces; b++) TRY(bounce(B, sg, b));
         }
+        if (B->ranked) B->perm_ng = B->ng;
     }
     TRYH(hipEventRecord(B->ev[4], st));
 
@@ -859,7 +871,9 @@ int bake_time_pass(lb_bake *B, int pass, int reps, float *ms_avg) {
     if (reps < 1) return fail("bake: reps %d", reps);
     // the gather as the bake's later bounces run it (its rays ranked by the first bounce's cost)
     GatherOrder o;
-    o.perm = pass == LB_PASS_GATHER && ordered_gather(B) ? (const uint8_t *)B->d_perm.ptr : nullptr;
+    o.perm = pass == LB_PASS_GATHER && ordered_gather(B) && B->perm_ng == B->ng
+                 ? (const uint8_t *)B->d_perm.ptr
+                 : nullptr;
     TRYH(hipEventRecord(B->ev[6], st));
     for (int r = 0; r < reps; r++)
         TRY(run_pass(gpu, pass, B->opt.form, B->scene, P, items, n, 0,
@@ -923,7 +937,9 @@ int lb_bake_run(lb_gpu *gpu, const lb_scene_desc *scene, const lb_bake_options *
     return 0;
 }
 
-void lb_bake_destroy(lb_bake *bake) { delete bake; }
+void lb_bake_destroy(lb_bake *bake) {
+    if (bake && bake->refs.fetch_sub(1) == 1) delete bake;
+}
 
 int lb_bake_info_get(const lb_bake *bake, lb_bake_info *info) {
     if (!bake || !info) return fail("lb_bake_info_get: null handle");

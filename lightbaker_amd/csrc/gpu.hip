@@ -87,8 +87,18 @@ int lb_open(lb_gpu **out) {
 
 const char *lb_adapter_name(const lb_gpu *gpu) { return gpu ? gpu->name.c_str() : ""; }
 
-void lb_close(lb_gpu *gpu) {
-    if (!gpu) return;
+void lb_close(lb_gpu *gpu) { gpu_release(gpu); }
+
+}  // extern "C"
+
+namespace lbk {
+
+void gpu_retain(lb_gpu *gpu) {
+    if (gpu) gpu->refs.fetch_add(1);
+}
+
+void gpu_release(lb_gpu *gpu) {
+    if (!gpu || gpu->refs.fetch_sub(1) != 1) return;
     hipSetDevice(gpu->device);
     hipStreamSynchronize(gpu->stream);
     gpu->run_items.release();
@@ -106,6 +116,10 @@ void lb_close(lb_gpu *gpu) {
     hipStreamDestroy(gpu->stream);
     delete gpu;
 }
+
+}  // namespace lbk
+
+extern "C" {
 
 int lb_scene_create(lb_gpu *gpu, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
                     size_t n_tris, const float *materials, size_t n_materials, const float *charts,

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <vector>
 
 #include "lb_atlas.h"
@@ -11,7 +12,29 @@
 #include "lb_host.h"
 #include "lb_texel.h"
 
+// What a captured bounce graph was recorded with. Its kernels take these by value, so the graph
+// replays only while all of them still hold: the gather item count and the gather's tuning, and
+// the GPU-wide buffers the TUNED form's launches read (a later launch with another tuning may
+// grow, that is free and reallocate, the spill area or the pooled schedule's hits).
+// Each buffer is keyed by address and capacity: a regrown buffer may come back at the same
+// address, and the capacity tells the two apart.
+struct GraphKey {
+    size_t ng = 0;
+    lbk::Tuning tuning;
+    const void *ptr[6] = {};  // spill, pooled hits, pooled aux, cost, perm, records
+    size_t cap[6] = {};
+    bool operator==(const GraphKey &o) const {
+        if (ng != o.ng || !(tuning == o.tuning)) return false;
+        for (int i = 0; i < 6; i++)
+            if (ptr[i] != o.ptr[i] || cap[i] != o.cap[i]) return false;
+        return true;
+    }
+};
+
+// A bake holds a reference to its lb_gpu (released when the bake is deleted); an adjoint holds one
+// to its bake, so lb_bake_destroy defers the delete until the last adjoint over it is gone.
 struct lb_bake {
+    std::atomic<int> refs{1};
     lb_gpu *gpu = nullptr;
     lb_bake_options opt{};
     lb_scene_desc sc{};  // valid while the caller's scene lives (prepare .. finish)
@@ -37,6 +60,9 @@ struct lb_bake {
     mutable uint64_t held = 0;
     mutable std::vector<uint8_t> h_tl_known, h_tl_state, h_bc5, h_vbytes, h_cost, h_perm;
     bool ranked = false;  // the last execute ranked the gather's rays (d_cost, d_perm hold them)
+    // the gather item count d_perm holds a ranking for (0: none yet); every execute traces the
+    // same gather rays, so a ranking stays a valid permutation of each item's rays
+    size_t perm_ng = 0;
     mutable std::vector<float> h_tl_field, h_tl_ao, h_tl_direct, h_tl_indirect, h_light, h_light0;
     mutable std::vector<uint32_t> h_rgb;
     mutable std::vector<lb_texel> h_texels;
@@ -53,8 +79,9 @@ struct lb_bake {
     lbk::TexelView V{};
     hipEvent_t ev[10] = {};
     hipGraphExec_t graph = nullptr;
-    size_t graph_ng = 0;  // the gather item count the graph was captured with
-    lbk::Tuning graph_tuning;  // and the tuning
+    GraphKey graph_key;        // what the graph was captured with
+    bool graph_ranks = false;  // the graph's first bounce counts and ranks the gather's rays
+    uint32_t graph_captures = 0;
     // results
     std::vector<float> direct, indirect, volume, charts;
     std::vector<uint8_t> validity;
@@ -70,6 +97,7 @@ struct lb_bake {
         for (auto &e : ev)
             if (e) hipEventDestroy(e);
         if (scene) lb_scene_destroy(scene);
+        lbk::gpu_release(gpu);
     }
 };
 

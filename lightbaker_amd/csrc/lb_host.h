@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string>
 
 #include "lb_kernels.h"
@@ -27,7 +28,11 @@ struct DevBuf {
 
 }  // namespace lbk
 
+// An lb_gpu is shared by its bakes and adjoints: each holds a reference (gpu_retain) until it is
+// destroyed, and lb_close drops the caller's, so the handle outlives every child whatever order the
+// caller closes them in.
 struct lb_gpu {
+    std::atomic<int> refs{1};
     int device = 0;
     int cus = 0;
     hipStream_t stream = nullptr;
@@ -59,6 +64,10 @@ struct lb_scene {
 extern "C" int lb_set_form(lb_gpu *gpu, int form);
 
 namespace lbk {
+
+void gpu_retain(lb_gpu *gpu);
+// drops one reference; the last one synchronizes the stream and frees the handle
+void gpu_release(lb_gpu *gpu);
 
 bool validate_bvh(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, size_t n_tris,
                   size_t n_materials, size_t n_charts, std::string *err);

@@ -19,7 +19,15 @@ struct Tuning {
     int waves = 6;           // lock-step: residency target (0: the compiler's; 6; 8)
     int xcd_split = 0;       // lock-step: items split over the 8 chiplets' queues (0: one queue)
     int gather_order = 0;    // lock-step gather: 0 azimuth, 1 elevation bands then ranked, 2 bands
+    bool operator==(const Tuning &o) const {
+        return workgroup == o.workgroup && rays_per_lane == o.rays_per_lane &&
+               lds_stack == o.lds_stack && nodes == o.nodes && refill == o.refill &&
+               schedule == o.schedule && chunk == o.chunk && leaf_batch == o.leaf_batch &&
+               waves == o.waves && xcd_split == o.xcd_split && gather_order == o.gather_order;
+    }
 };
+// every field of Tuning is an int and operator== names each one
+static_assert(sizeof(Tuning) == 11 * sizeof(int), "a new Tuning field must join operator==");
 
 // LDS words between two lanes' traversal stacks of `depth` entries: odd, so a wave's lanes
 // spread over all banks.

@@ -60,6 +60,7 @@ class SceneBuffers:
     def __init__(self, gpu: "Gpu", handle: C.c_void_p):
         self.gpu = gpu
         self.handle = handle
+        N.own(self, N.SCENE)
 
     def close(self) -> None:
         if self.handle:
@@ -82,6 +83,7 @@ class DeviceArray:
         p = C.c_void_p()
         N.check(N.lib().lb_buffer_alloc(gpu.handle, max(nbytes, 16), C.byref(p)))
         self.ptr = p
+        N.own(self, N.ARRAY)
 
     @staticmethod
     def of(gpu: "Gpu", a: np.ndarray) -> "DeviceArray":
@@ -97,9 +99,11 @@ class DeviceArray:
         return out
 
     def free(self) -> None:
-        if self.ptr:
+        if self.ptr and self.gpu.handle:
             N.lib().lb_buffer_free(self.gpu.handle, self.ptr)
-            self.ptr = None
+        self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:
@@ -115,9 +119,13 @@ class Gpu:
         N.check(self._lib.lb_open(C.byref(h)))
         self.handle = h
         self.adapter = self._lib.lb_adapter_name(h).decode()
+        self.gpu = self
+        N.own(self, N.GPU)
 
     def close(self) -> None:
+        """Closes this Gpu's adjoints, bake sessions, scenes and buffers, then the Gpu."""
         if self.handle:
+            N.close_owned(self)
             self._lib.lb_close(self.handle)
             self.handle = None
 

@@ -34,8 +34,9 @@ def setup(gpu):
     params = kit.params_for(scene)
     sb = gpu.scene(bvh, mats, charts)
     probes = kit.probe_items(scene, 600)
-    return dict(scene=scene, bvh=bvh, items=items, charts=charts, light=light, mats=mats,
-                params=params, sb=sb, probes=probes)
+    yield dict(scene=scene, bvh=bvh, items=items, charts=charts, light=light, mats=mats,
+               params=params, sb=sb, probes=probes)
+    sb.close()
 
 
 def _oracle(s, pass_, items):
