@@ -135,6 +135,22 @@ int lb_run_device(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const 
                   const lb_texel *items_dev, size_t n, uint32_t key_base, const float *light_dev,
                   size_t n_light, float *out_dev);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. As lb_run_device, but
+ * the TUNED form's lock-step launch takes its work-queue heads and traversal spill area from a
+ * static per-device scratch of 64 MiB (1 KiB of queue heads, the rest spill) instead of buffers
+ * the library grows, and no call allocates or frees device memory. The caller still passes every
+ * input and output. The PLAIN and WATERTIGHT forms use no scratch and run as lb_run_device.
+ * Errors: the pooled schedule (its hits need memory sized by the launch), and a launch whose
+ * spill area (resident lanes x stack entries beyond the LDS part x 4 B) exceeds the scratch;
+ * 16 or more entries in LDS fit every lane a 256-CU device keeps resident. Every call on a device
+ * shares that device's scratch: calls that overlap on one device (two lb_gpu handles' streams at
+ * once) race. */
+int lb_run_device_noalloc(lb_gpu *gpu, int pass, int form, const lb_scene *scene,
+                          const lb_params *params, const lb_texel *items_dev, size_t n,
+                          uint32_t key_base, const float *light_dev, size_t n_light,
+                          float *out_dev);
+
 /* Tuning of the TUNED form. */
 typedef struct lb_tuning {
     int workgroup;      /* threads a workgroup: 64..1024, a multiple of 64 */

@@ -15,6 +15,15 @@ using namespace lbk;
 
 namespace {
 thread_local std::string g_error;
+
+// lb_run_device_noalloc's scratch: the TUNED lock-step launch's work-queue heads (the first
+// 1 KiB), then its traversal spill area. 64 MiB holds the spill of every lane a 256-CU device
+// keeps resident (32 waves a CU) with 16 or more of the 48 stack entries in LDS. Each device
+// holds its own copy for the life of the process; host code reaches it only through the address
+// hipGetSymbolAddress gives on that device (lb_gpu.scratch), never through this name.
+constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+constexpr size_t SCRATCH_QUEUE_BYTES = 1024;
+static __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];
 }
 
 namespace lbk {
@@ -80,6 +89,12 @@ int lb_open(lb_gpu **out) {
         hipMemset(g->zero.ptr, 0, 64) != hipSuccess || hipMemset(g->queue, 0, 1024) != hipSuccess) {
         delete g;
         return fail("no GPU device");
+    }
+    // this device's copy of the no-allocation entry point's scratch (device 0 is current)
+    hipError_t se = hipGetSymbolAddress(reinterpret_cast<void **>(&g->scratch), HIP_SYMBOL(scratch));
+    if (se != hipSuccess || !g->scratch) {
+        delete g;
+        return fail("GPU: the static scratch's device address: %s", hipGetErrorString(se));
     }
     *out = g;
     return 0;
@@ -222,6 +237,38 @@ int lb_run_device(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const 
                   size_t n_light, float *out_dev) {
     return run_pass(gpu, pass, form, scene, params, items_dev, n, key_base, light_dev, n_light,
                     out_dev, false);
+}
+
+int lb_run_device_noalloc(lb_gpu *gpu, int pass, int form, const lb_scene *scene,
+                          const lb_params *params, const lb_texel *items_dev, size_t n,
+                          uint32_t key_base, const float *light_dev, size_t n_light,
+                          float *out_dev) {
+    if (!gpu || !scene || !params) return fail("lb_run_device_noalloc: null handle");
+    if (pass < LB_PASS_DIRECT || pass > LB_PASS_VOLUME)
+        return fail("lb_run_device_noalloc: no pass %d", pass);
+    if (form != LB_FORM_TUNED)  // the other forms' launches use no scratch
+        return run_pass(gpu, pass, form, scene, params, items_dev, n, key_base, light_dev,
+                        n_light, out_dev, false);
+    if (gpu->tuning[pass].schedule != 0)
+        return fail("lb_run_device_noalloc: the pooled schedule keeps every ray's hit in memory "
+                    "sized by the launch; set schedule 0 (lock-step) for this pass");
+    if (n == 0) return 0;
+    if (n > 0xffffffffu) return fail("lb_run_device_noalloc: %zu items exceed 32-bit indices", n);
+    hipSetDevice(gpu->device);
+    size_t need = 0;
+    HIP_TRY(tuned_spill_bytes(gpu, pass, *params, (uint32_t)n, &need), "occupancy");
+    const size_t have = SCRATCH_BYTES - SCRATCH_QUEUE_BYTES;
+    if (need > have)
+        return fail("lb_run_device_noalloc: this launch spills %zu bytes of traversal stack, the "
+                    "static scratch holds %zu: keep more stack entries in LDS (lds_stack) or use "
+                    "a smaller workgroup",
+                    need, have);
+    FixedScratch fx;
+    fx.queue = reinterpret_cast<uint32_t *>(gpu->scratch);
+    fx.spill = reinterpret_cast<uint32_t *>(gpu->scratch + SCRATCH_QUEUE_BYTES);
+    fx.spill_bytes = have;
+    return run_pass(gpu, pass, form, scene, params, items_dev, n, key_base, light_dev, n_light,
+                    out_dev, false, nullptr, &fx);
 }
 
 int lb_run_stats(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_params *params,
@@ -454,7 +501,8 @@ int check_params(const lb_scene *scene, const lb_params *P, int pass, size_t n_l
 
 int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_params *params,
              const lb_texel *items, size_t n, uint32_t key_base, const float *light,
-             size_t n_light, float *out, bool stats, const GatherOrder *order) {
+             size_t n_light, float *out, bool stats, const GatherOrder *order,
+             const FixedScratch *fixed) {
     if (!gpu || !scene || !params) return fail("run: null handle");
     if (pass < LB_PASS_DIRECT || pass > LB_PASS_VOLUME) return fail("run: no pass %d", pass);
     if (form != LB_FORM_PLAIN && form != LB_FORM_TUNED && form != LB_FORM_WATERTIGHT)
@@ -480,7 +528,7 @@ int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_pa
                          key_base, reinterpret_cast<float4 *>(out), gpu->counters, gpu->stream);
     else
         e = launch_tuned(gpu, pass, stats, S, P, items, (uint32_t)n, key_base,
-                         reinterpret_cast<float4 *>(out), order);
+                         reinterpret_cast<float4 *>(out), order, fixed);
     if (e != hipSuccess) return hip_fail(e, "launch");
     return 0;
 }

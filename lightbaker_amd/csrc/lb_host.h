@@ -49,6 +49,7 @@ struct lb_gpu {
     lbk::DevBuf stream_aux;                     // TUNED, pooled: picks, item lists, counts
     int form = LB_FORM_PLAIN;
     lbk::Tuning tuning[3];  // TUNED: the knobs of each pass (LB_PASS_*)
+    uint8_t *scratch = nullptr;  // this device's copy of the static scratch (lb_run_device_noalloc)
 };
 
 struct lb_scene {
@@ -83,9 +84,21 @@ struct GatherOrder {
     const uint8_t *perm = nullptr;
     uint2 *rec = nullptr;
 };
+// Device memory a TUNED lock-step launch takes in place of the gpu's own queue and spill area
+// (which it grows on demand): the launch then allocates nothing, and fails if the spill area it
+// needs exceeds `spill_bytes`.
+struct FixedScratch {
+    uint32_t *queue = nullptr;  // the chiplets' queue heads, 1 KiB
+    uint32_t *spill = nullptr;
+    size_t spill_bytes = 0;
+};
 int run_pass(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const lb_params *params,
              const lb_texel *items, size_t n, uint32_t key_base, const float *light,
-             size_t n_light, float *out, bool stats, const GatherOrder *order = nullptr);
+             size_t n_light, float *out, bool stats, const GatherOrder *order = nullptr,
+             const FixedScratch *fixed = nullptr);
+// the spill area a TUNED lock-step launch of `pass` over n items needs with the gpu's tuning
+hipError_t tuned_spill_bytes(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n,
+                             size_t *bytes);
 // each item's azimuth slots ranked by recorded cost, into perm (rays <= 256)
 hipError_t rank_slots(const uint8_t *cost, uint32_t n_items, uint32_t rays, uint8_t *perm,
                       hipStream_t stream);
@@ -98,6 +111,7 @@ hipError_t launch_stream(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                          float4 *out);
 hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                         const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
-                        float4 *out, const GatherOrder *order = nullptr);
+                        float4 *out, const GatherOrder *order = nullptr,
+                        const FixedScratch *fixed = nullptr);
 
 }  // namespace lbk

@@ -718,9 +718,10 @@ struct TunedGrid {
     size_t lds;
 };
 
-// mode: 0 plain, 1 the gather's counting launch, 2 with the traversal counters
+// mode: 0 plain, 1 the gather's counting launch, 2 with the traversal counters. With `spill`
+// the launch's spill area in bytes is returned and nothing is reserved.
 static hipError_t tuned_grid(lb_gpu *gpu, int pass, int mode, const lb_params &P, uint32_t n,
-                             TunedGrid *out) {
+                             TunedGrid *out, size_t *spill = nullptr) {
     const Tuning &T = gpu->tuning[pass];
     out->fn = pick_kernel(pass, T.nodes, mode, T.waves);
     out->block = (uint32_t)T.workgroup;
@@ -740,8 +741,18 @@ static hipError_t tuned_grid(lb_gpu *gpu, int pass, int mode, const lb_params &P
     out->grid = grid;
     size_t spill_bytes = (size_t)grid * out->block * (size_t)(STACK - T.lds_stack) * 4u;
     if (spill_bytes < 16) spill_bytes = 16;
+    if (spill) {
+        *spill = spill_bytes;
+        return hipSuccess;
+    }
     if (gpu->spill.reserve(spill_bytes)) return hipErrorOutOfMemory;
     return hipSuccess;
+}
+
+hipError_t tuned_spill_bytes(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n,
+                             size_t *bytes) {
+    TunedGrid tg;
+    return tuned_grid(gpu, pass, 0, P, n, &tg, bytes);
 }
 
 // Each item's azimuth slots ordered by (recorded cost, slot): a stable counting sort on the 8-bit
@@ -838,15 +849,22 @@ hipError_t tuned_prepare(lb_gpu *gpu, int pass, const lb_params &P, uint32_t n) 
 
 hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
                         const lb_params &P, const lb_texel *items, uint32_t n, uint32_t key_base,
-                        float4 *out, const GatherOrder *order) {
-    if (gpu->tuning[pass].schedule == 1) return launch_stream(gpu, pass, stats, S, P, items, n, key_base, out);
+                        float4 *out, const GatherOrder *order, const FixedScratch *fixed) {
+    if (gpu->tuning[pass].schedule == 1) {
+        // the pooled schedule keeps every ray's hit in memory sized by the launch
+        if (fixed) return hipErrorNotSupported;
+        return launch_stream(gpu, pass, stats, S, P, items, n, key_base, out);
+    }
     // the gather's counting launch records each ray's steps and counts nothing else
     bool ordered = order && pass == LB_PASS_GATHER && P.counts[2] <= 256u;
     uint8_t *cost = ordered ? order->cost : nullptr;
     int mode = stats ? 2 : (cost ? 1 : 0);
     TunedGrid tg;
-    hipError_t e = tuned_grid(gpu, pass, mode, P, n, &tg);
+    size_t need = 0;
+    hipError_t e = tuned_grid(gpu, pass, mode, P, n, &tg, fixed ? &need : nullptr);
     if (e != hipSuccess) return e;
+    if (fixed && need > fixed->spill_bytes) return hipErrorOutOfMemory;
+    uint32_t *queue = fixed ? fixed->queue : gpu->queue;
     TunedArgs A;
     A.S = S;
     A.P = P;
@@ -855,8 +873,8 @@ hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
     A.key_base = key_base;
     A.out = out;
     A.stats = gpu->counters;
-    A.queue = gpu->queue;
-    A.spill = static_cast<uint32_t *>(gpu->spill.ptr);
+    A.queue = queue;
+    A.spill = fixed ? fixed->spill : static_cast<uint32_t *>(gpu->spill.ptr);
     A.lanes_per_item = tg.lanes;
     A.depth = (uint32_t)gpu->tuning[pass].lds_stack;
     A.leaf_batch = gpu->tuning[pass].leaf_batch;
@@ -867,7 +885,7 @@ hipError_t launch_tuned(lb_gpu *gpu, int pass, bool stats, const SceneView &S,
     uint32_t rays = P.counts[2];
     A.el_order = pass == LB_PASS_GATHER && !A.perm && gpu->tuning[pass].gather_order >= 1 &&
                  rays >= 64u && rays <= 65536u && (rays & (rays - 1u)) == 0u;
-    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, gpu->stream, gpu->queue);
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, gpu->stream, queue);
     hipLaunchKernelGGL(tg.fn, dim3(tg.grid), dim3(tg.block), tg.lds, gpu->stream, A);
     return hipGetLastError();
 }
