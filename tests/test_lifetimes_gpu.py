@@ -19,6 +19,13 @@ from lightbaker_amd.scenes import small_scene
 pytestmark = pytest.mark.gpu
 
 
+def scene2():
+    """The small scene with two bounces: the second takes the rays ranked by the first."""
+    s = small_scene(1)
+    s.settings.gi_bounces = 2
+    return s
+
+
 def _light(sess):
     return sess.array(N.BAKE_LIGHT, np.float32).reshape(-1, 4).copy()
 
@@ -33,7 +40,7 @@ def test_graph_recaptured_when_spill_area_regrows(gpu):
     next execute's direct launch; the bounce graph, which spills into that area, is captured
     again instead of replayed, and the light stays within the lock-step tolerance."""
     print("graph key: spill area regrown between executes", flush=True)
-    sess = BakeSession(small_scene(1), gpu, form=N.FORM_TUNED, graph=True)
+    sess = BakeSession(scene2(), gpu, form=N.FORM_TUNED, graph=True)
     sess.execute()
     first = _light(sess)
     assert sess.info()["graph_captures"] == 1
@@ -51,7 +58,7 @@ def test_graph_recaptured_on_every_gather_knob(gpu):
     """Each gather knob a TUNED bake's graph depends on (the chiplet split and the ray order
     included) captures the graph again when it changes."""
     print("graph key: gather knobs", flush=True)
-    sess = BakeSession(small_scene(1), gpu, form=N.FORM_TUNED, graph=True)
+    sess = BakeSession(scene2(), gpu, form=N.FORM_TUNED, graph=True)
     sess.execute()
     first = _light(sess)
     n = sess.info()["graph_captures"]
@@ -69,7 +76,7 @@ def test_ranking_survives_graph_replays(gpu):
     """The gather's step counts and their order are read back after the second execute of a
     graph-replayed bake too (the replay re-runs the ranking), equal to the first's."""
     print("ranking after a graph replay", flush=True)
-    sess = BakeSession(small_scene(1), gpu, form=N.FORM_TUNED, graph=True)
+    sess = BakeSession(scene2(), gpu, form=N.FORM_TUNED, graph=True)
     sess.execute()
     cost = sess.array(N.BAKE_GATHER_COST, np.uint8).copy()
     order = sess.array(N.BAKE_GATHER_ORDER, np.uint8).copy()
@@ -85,7 +92,7 @@ def test_time_pass_after_replayed_bounces(gpu):
     """A bake whose later bounces replay ray records ranks nothing, so timing its gather takes
     the rays in azimuth order (no ranking is read) and leaves the bake's results as they were."""
     print("time_pass after a replayed bake", flush=True)
-    sess = BakeSession(small_scene(1), gpu, form=N.FORM_TUNED, graph=True, replay=True)
+    sess = BakeSession(scene2(), gpu, form=N.FORM_TUNED, graph=True, replay=True)
     sess.execute()
     assert len(sess.array(N.BAKE_GATHER_ORDER, np.uint8)) == 0
     light = _light(sess)
@@ -100,7 +107,7 @@ def test_stats_execute_keeps_ranking_valid(gpu):
     """An execute with the traversal counters traces every bounce unranked; a later time_pass
     then still reads the ranking of the last ranked execute, a permutation of each item's rays."""
     print("stats execute between ranked executes", flush=True)
-    sess = BakeSession(small_scene(1), gpu, form=N.FORM_TUNED, graph=True)
+    sess = BakeSession(scene2(), gpu, form=N.FORM_TUNED, graph=True)
     sess.execute()
     order = sess.array(N.BAKE_GATHER_ORDER, np.uint8).copy()
     rays = small_scene(1).settings.gi_rays
