@@ -67,6 +67,51 @@ def offset(q):
     return {5: (F(-0.45), F(0)), 6: (F(0.45), F(0)), 7: (F(0), F(-0.45))}.get(q, (F(0), F(0.45)))
 
 
+def ao_tone(occ, strength, contrast, floor):
+    """bake.rs:238: max(floor, max(0, 1 - strength * occ) ^ contrast), FP32."""
+    v = max(F(1.0) - F(strength) * F(occ), F(0.0))
+    if contrast != 1.0:
+        v = F(v ** F(contrast))
+    return max(v, F(floor))
+
+
+def decode(r, texel, settings):
+    """bake.rs:207-241 for one covered texel: its direct-pass result `r` (vec4) and its record
+    (moved in place to the point the pass sampled). Returns None when the texel lies inside a
+    solid (invalid), else (soft, hard bits, validity state, occlusion tone)."""
+    if r[3] > F(settings.inside_share):
+        return None
+    bits = int(np.asarray(r[1:2], np.float32).view(np.uint32)[0])
+    pick = bits >> 24
+    state = VALIDITY_CENTRE if pick == 0 else (VALIDITY_PUSHED if pick == 9 else VALIDITY_OFFSET)
+    if pick == 9:
+        texel["pos"][:3] = texel["pos"][:3] + texel["normal"][:3] * PUSH
+    elif pick > 0:
+        cx, cy = offset(pick)
+        texel["pos"][:3] = texel["pos"][:3] + (texel["dx"][:3] * cx + texel["dy"][:3] * cy)
+    ao = ao_tone(r[2], settings.ao_strength, settings.ao_contrast, settings.ao_floor)
+    return F(r[0]), bits & 0x00FFFFFF, state, ao
+
+
+def gather_set(tw, size, valid, chart, ds):
+    """bake.rs:286-299 over one tile of width tw: the valid texels on the ds grid, then the first
+    valid texel (row-major) of every (ds-block, chart) pair the grid left uncovered. Returns a
+    boolean mask over the tile's `size` texels."""
+    bw = -(-tw // ds)
+    gathered = np.zeros(size, bool)
+    seen = set()
+    for i in range(size):
+        if valid[i] and (i % tw) % ds == 0 and (i // tw) % ds == 0:
+            gathered[i] = True
+            seen.add((((i // tw) // ds) * bw + (i % tw) // ds, int(chart[i])))
+    for i in range(size):
+        key = (((i // tw) // ds) * bw + (i % tw) // ds, int(chart[i]))
+        if valid[i] and key not in seen:
+            seen.add(key)
+            gathered[i] = True
+    return gathered
+
+
 def bake(inp: dict) -> dict:
     """inp: texels (T,) TEXEL of the dirty tiles, tile after tile; tiles (nt, 6) of the dirty
     tiles; light0 (aw * ah, 4) the clean tiles' light or None; aw, ah; nodes, tris, mats, charts,
@@ -96,24 +141,11 @@ def bake(inp: dict) -> dict:
     ao = np.ones(T, np.float32)
     hard = np.zeros(T, np.uint32)
     for k, gi in enumerate(cov):
-        r = res[k]
-        if r[3] > F(st.inside_share):
+        d = decode(res[k], tex[gi], st)
+        if d is None:
             valid[gi] = False
             continue
-        soft[gi] = r[0]
-        bits = int(r[1:2].view(np.uint32)[0])
-        hard[gi] = bits & 0x00FFFFFF
-        pick = bits >> 24
-        state[gi] = VALIDITY_CENTRE if pick == 0 else (VALIDITY_PUSHED if pick == 9 else VALIDITY_OFFSET)
-        if pick == 9:
-            tex["pos"][gi, :3] = tex["pos"][gi, :3] + tex["normal"][gi, :3] * PUSH
-        elif pick > 0:
-            cx, cy = offset(pick)
-            tex["pos"][gi, :3] = tex["pos"][gi, :3] + (tex["dx"][gi, :3] * cx + tex["dy"][gi, :3] * cy)
-        v = max(F(1.0) - F(st.ao_strength) * r[2], F(0.0))
-        if st.ao_contrast != 1.0:
-            v = F(v ** F(st.ao_contrast))
-        ao[gi] = max(v, F(st.ao_floor))
+        soft[gi], hard[gi], state[gi], ao[gi] = d
     known = np.zeros(T, np.uint8)
     field = np.zeros(T, np.float32)
     direct = np.zeros((T, 3), np.float32)
@@ -164,20 +196,8 @@ def bake(inp: dict) -> dict:
     ds = st.indirect_downscale
     gathered = np.zeros(T, bool)
     for t in range(len(tiles)):
-        tw = int(tiles[t, 2])
-        o = int(offs[t])
-        size = int(offs[t + 1] - offs[t])
-        bw = -(-tw // ds)
-        seen = set()
-        for i in range(size):
-            if valid[o + i] and (i % tw) % ds == 0 and (i // tw) % ds == 0:
-                gathered[o + i] = True
-                seen.add((((i // tw) // ds) * bw + (i % tw) // ds, int(chart[o + i])))
-        for i in range(size):
-            key = (((i // tw) // ds) * bw + (i % tw) // ds, int(chart[o + i]))
-            if valid[o + i] and key not in seen:
-                seen.add(key)
-                gathered[o + i] = True
+        o, e = int(offs[t]), int(offs[t + 1])
+        gathered[o:e] = gather_set(int(tiles[t, 2]), e - o, valid[o:e], chart[o:e], ds)
     gat = np.nonzero(gathered)[0]
     grings = st.padding + 2 * ds + 2
     for _ in range(st.gi_bounces):

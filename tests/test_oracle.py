@@ -13,6 +13,8 @@ swapped mix, a dropped cosine factor, a wrong shift or constant) fails:
                          occ = 1 - a^2 + 2 a^2 ln a, a = h/R
   volume                 constant radiance L: L0 = 0.282095 * 4 pi * L, L1 = 0, sun seen
   direct sun bits        the analytic shadow of a box on the ground
+The FP64 tracer (oracle/fp64_ref.c), the yardstick of the FP32 passes' error, is held to the same
+closed forms (the tests parametrized over "f32" and "f64").
 """
 from __future__ import annotations
 
@@ -157,39 +159,55 @@ def _ground_texel(x=0.0, z=0.0):
     return it
 
 
-def _run(scene, pass_, items, light=None):
+def _run(scene, pass_, items, light=None, prec="f32"):
     pos, mesh, mat = scene.flat()
     bvh = Bvh.build(pos, mat, np.full(len(mat), 0xffffffff, np.uint32), scene.casts())
     P = make_params(scene, 4, 4)
-    return oracle.run(pass_, bvh.nodes, bvh.tris, scene.materials_array(), None, P, items, 0,
-                      light if light is not None else np.zeros((16, 4), np.float32))
+    fn = oracle.run if prec == "f32" else oracle.run64
+    return fn(pass_, bvh.nodes, bvh.tris, scene.materials_array(), None, P, items, 0,
+              light if light is not None else np.zeros((16, 4), np.float32))
 
 
-def test_gather_open_plane_gradient_sky():
+def _word(out, k, prec):
+    """The direct pass's hard bits and pick word of item k (FP32: the float's bits; the FP64
+    tracer stores the integer's value)."""
+    if prec == "f32":
+        return int(np.float32(out[k, 1]).view(np.uint32))
+    return int(out[k, 1])
+
+
+PREC = pytest.mark.parametrize("prec", ["f32", "f64"])
+
+
+@PREC
+def test_gather_open_plane_gradient_sky(prec):
     scene = _plane_scene()
-    out = _run(scene, 1, _ground_texel())
+    out = _run(scene, 1, _ground_texel(), prec=prec)
     z, h = 0.8, 0.2
     want = math.pi * (h / 3 + 2 * z / 3)
     assert abs(out[0, 0] - want) < 2e-3 * want, (out[0], want)
     assert out[0, 3] == 0.0
 
 
+@PREC
 @pytest.mark.parametrize("h", [0.4, 0.2])
-def test_ao_under_infinite_ceiling(h):
+def test_ao_under_infinite_ceiling(h, prec):
     scene = _plane_scene(ceiling_h=h)
-    out = _run(scene, 0, _ground_texel())
+    out = _run(scene, 0, _ground_texel(), prec=prec)
     a = (h - 0.01) / 0.8  # rays leave 0.01 m above the surface (bake.rs:24 LIFT)
     want = 1 - a * a + 2 * a * a * math.log(a)
     assert abs(out[0, 2] - want) < 3e-3, (out[0], want)
 
 
-def test_ao_open_plane_is_zero():
-    out = _run(_plane_scene(), 0, _ground_texel())
+@PREC
+def test_ao_open_plane_is_zero(prec):
+    out = _run(_plane_scene(), 0, _ground_texel(), prec=prec)
     assert out[0, 2] == 0.0 and out[0, 0] == 1.0
-    assert out[0, 1].view(np.uint32) == 0xFFFF  # 16 hard bits lit, pick 0
+    assert _word(out, 0, prec) == 0xFFFF  # 16 hard bits lit, pick 0
 
 
-def test_volume_constant_radiance():
+@PREC
+def test_volume_constant_radiance(prec):
     far = Mesh("far", 0, np.array(quad((500, 0, 500), (500, 0, 501), (501, 0, 501), (501, 0, 500)),
                                   np.float32))
     scene = _plane_scene()
@@ -199,14 +217,15 @@ def test_volume_constant_radiance():
     probe = np.zeros(1, dtype=N.TEXEL)
     probe["pos"] = [0, 5, 0, 1]
     probe["normal"] = [0, 1, 0, np.uint32(3).view(np.float32)]
-    out = _run(scene, 2, probe).reshape(4, 4)
+    out = _run(scene, 2, probe, prec=prec).reshape(4, 4)
     want = 0.282095 * 4 * math.pi * 0.5
     assert abs(out[0, 0] - want) < 2e-5 * want
     assert abs(out[1:, :3]).max() < 2e-3
     assert out[0, 3] == 1.0 and out[1, 3] == 0.0
 
 
-def test_sun_bits_match_the_analytic_shadow():
+@PREC
+def test_sun_bits_match_the_analytic_shadow(prec):
     box = Mesh("box", 0, cube((0.0, 0.0, 0.0), (2.0, 2.0, 2.0)))
     scene = _plane_scene(extra=[box])
     scene.sun_direction = (-1.0, 0.8, 0.3)
@@ -215,7 +234,7 @@ def test_sun_bits_match_the_analytic_shadow():
     xs = np.arange(2.2, 6.0, 0.25) + 0.125
     zs = np.arange(-1.0, 3.0, 0.25) + 0.125
     items = np.concatenate([_ground_texel(x, z) for x in xs for z in zs])
-    out = _run(scene, 0, items)
+    out = _run(scene, 0, items, prec=prec)
 
     def shadowed(p):
         # the ray p + t sun against the box [0,2]^3 (slab test in double)
@@ -236,25 +255,26 @@ def test_sun_bits_match_the_analytic_shadow():
         states = {shadowed(c) for c in corners} | {shadowed(p)}
         if len(states) != 1:
             continue  # near the shadow's edge
-        bits = int(out[k, 1].view(np.uint32)) & 0xFFFF
+        bits = _word(out, k, prec) & 0xFFFF
         assert bits == (0 if states.pop() else 0xFFFF), (it["pos"], hex(bits))
         checked += 1
     assert checked > 100
 
 
-def test_texel_inside_a_slab_is_retried():
+@PREC
+def test_texel_inside_a_slab_is_retried(prec):
     # a slab 0.2 m thick on the ground: from under it, the rays meet its top face from behind
     # within the inside reach (min(0.3, aoReach), bake.rs:115), so the texel lies inside a solid
     slab = Mesh("slab", 0, cube((-1.0, 0.0, -1.0), (1.0, 0.2, 1.0)))
     scene = _plane_scene(extra=[slab])
-    out = _run(scene, 0, _ground_texel(0.0, 0.0))
-    pick = int(out[0, 1].view(np.uint32)) >> 24
+    out = _run(scene, 0, _ground_texel(0.0, 0.0), prec=prec)
+    pick = _word(out, 0, prec) >> 24
     # every point across it (and pushed 0.03 m up) is inside: no pick; the AO rays' back share is
     # the share of cosine rays meeting the top (0.19 m up) within 0.3 m: 1 - (0.19 / 0.3)^2
     assert pick == 0
     assert abs(out[0, 3] - (1 - (0.19 / 0.3) ** 2)) < 0.02, out[0]
     # centre 0.05 m inside the slab's edge: the first quarter point (+0.3, -0.3 texel,
     # trace.wgsl:153) lies 0.025 m outside, so it is picked and the sun cells shrink round it
-    edge = _run(scene, 0, _ground_texel(0.95, 0.0))
-    assert int(edge[0, 1].view(np.uint32)) >> 24 == 1, edge[0]
+    edge = _run(scene, 0, _ground_texel(0.95, 0.0), prec=prec)
+    assert _word(edge, 0, prec) >> 24 == 1, edge[0]
     assert edge[0, 3] == 0.0
