@@ -187,6 +187,18 @@ int lb_get_tuning(const lb_gpu *gpu, lb_tuning *tuning);
 int lb_set_pass_tuning(lb_gpu *gpu, int pass, const lb_tuning *tuning);
 int lb_get_pass_tuning(const lb_gpu *gpu, int pass, lb_tuning *tuning);
 
+/* No counterpart in the reference (its nodes are the 32 B records alone). The TUNED form's node
+ * records lb_scene_create derives from a BVH, built on the host with no device: per node index,
+ * the QUANT record (8 u32: for each axis the origin's FP32 bits with the axis's step exponent
+ * (step 2^(e - 127)) in the low byte; child a's reference; 12 corner bytes, child a's min xyz and
+ * max xyz then child b's, each corner origin + byte * step; child b's reference) and the WIDE
+ * record (16 f32: both children's boxes, then their references as bits). Leaves' records are
+ * zero. A reference is an inner child's node index, or bit 31 with a leaf's count in bits 26..30
+ * and first triangle in bits 0..25 (count 31: the leaf's node index instead). Either output may
+ * be NULL; each holds n_nodes records. For callers that cache or inspect the records. */
+int lb_tuned_nodes(const lb_node *nodes, size_t n_nodes, uint32_t *quant /* [n][8] */,
+                   float *wide /* [n][16] */);
+
 /* Traversal counters of the last lb_run_stats call: rays traced, inner-node visits (child-box
  * pairs tested), triangle tests, material loads, bounced() light gathers and leaf visits. */
 typedef struct lb_stats {

@@ -135,6 +135,7 @@ SIGNATURES = {
     "lb_sync": (_I, [_VP]),
     "lb_run_device": (_I, [_VP, _I, _I, _VP, _VP, _VP, _SZ, _U32, _VP, _SZ, _VP]),
     "lb_run_device_noalloc": (_I, [_VP, _I, _I, _VP, _VP, _VP, _SZ, _U32, _VP, _SZ, _VP]),
+    "lb_tuned_nodes": (_I, [_VP, _SZ, _VP, _VP]),
     "lb_set_tuning": (_I, [_VP, C.POINTER(Tuning)]),
     "lb_set_pass_tuning": (_I, [_VP, C.c_int, C.POINTER(Tuning)]),
     "lb_get_pass_tuning": (_I, [_VP, C.c_int, C.POINTER(Tuning)]),

@@ -135,12 +135,12 @@ static uint32_t leaf_ref(const lb_node &leaf, size_t index) {
     return 0x80000000u | (31u << 26) | (uint32_t)index;
 }
 
-int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
-                      size_t n_tris, const float *materials) {
+// Both layouts' records of every inner node (zero for leaves), on the host's threads.
+static int tuned_records(const lb_node *nodes, size_t n_nodes, std::vector<float> &wide,
+                         std::vector<uint32_t> &quant) {
     if (n_nodes >= (1u << 26)) return fail("scene: %zu nodes exceed 26-bit node indices", n_nodes);
-    std::vector<float> wide(n_nodes * 16, 0.0f);
-    std::vector<uint32_t> quant(n_nodes * 8, 0u);
-    std::vector<uint32_t> leafinfo(n_nodes * 2, 0u);
+    wide.assign(n_nodes * 16, 0.0f);
+    quant.assign(n_nodes * 8, 0u);
     bool empty = is_empty_bvh(nodes, n_nodes);
     // every inner node's records on its own: the host's threads take ranges of nodes
     std::atomic<size_t> too_large{SIZE_MAX};
@@ -203,6 +203,16 @@ int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const l
         if (too_large.load() != SIZE_MAX)
             return fail("scene: node %zu is too large to quantize", too_large.load());
     }
+    return 0;
+}
+
+int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+                      size_t n_tris, const float *materials) {
+    std::vector<float> wide;
+    std::vector<uint32_t> quant;
+    if (tuned_records(nodes, n_nodes, wide, quant)) return 1;
+    std::vector<uint32_t> leafinfo(n_nodes * 2, 0u);
+    bool empty = is_empty_bvh(nodes, n_nodes);
     // leaves: (first triangle, count) at the leaf's own index
     for (size_t i = 0; i < n_nodes; i++) {
         leafinfo[i * 2] = nodes[i].first;
@@ -231,3 +241,14 @@ int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const l
 }
 
 }  // namespace lbk
+
+extern "C" int lb_tuned_nodes(const lb_node *nodes, size_t n_nodes, uint32_t *quant,
+                              float *wide) {
+    if (!nodes || n_nodes == 0) return lbk::fail("lb_tuned_nodes: a BVH has at least one node");
+    std::vector<float> w;
+    std::vector<uint32_t> q;
+    if (lbk::tuned_records(nodes, n_nodes, w, q)) return 1;
+    if (quant) std::memcpy(quant, q.data(), q.size() * 4);
+    if (wide) std::memcpy(wide, w.data(), w.size() * 4);
+    return 0;
+}
