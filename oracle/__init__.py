@@ -54,6 +54,8 @@ def lib() -> C.CDLL:
         L.orc64_run.restype = I
         L.orc_gather_steps.argtypes = [VP, SZ, VP, VP, VP, VP, SZ, U32, VP]
         L.orc_gather_steps.restype = I
+        L.orc_gather_terms.argtypes = [VP, SZ, VP, VP, VP, VP, VP, SZ, U32, VP, VP]
+        L.orc_gather_terms.restype = I
         _lib = L
     return _lib
 
@@ -142,3 +144,21 @@ def trace_brute(tris, materials, params, o, d, tmax, ray):
     L.orc_trace_brute(_p(tris), len(tris), _p(mats), _p(params), _p(o), _p(d), tmax, ray,
                       C.byref(t), C.byref(tri), C.byref(back))
     return t.value, tri.value, bool(back.value)
+
+
+def gather_terms(nodes, tris, materials, charts, params, items, key_base: int = 0,
+                 light=None) -> np.ndarray:
+    """orc_gather_terms: (items, rays, 3) float32, each gather ray's term as the sum adds it."""
+    L = lib()
+    items = np.ascontiguousarray(items)
+    nodes, tris = np.ascontiguousarray(nodes), np.ascontiguousarray(tris)
+    params = np.ascontiguousarray(params)
+    mats = np.ascontiguousarray(materials, dtype=np.float32)
+    ch = None if charts is None or len(charts) == 0 else np.ascontiguousarray(charts,
+                                                                               dtype=np.float32)
+    lt = None if light is None or len(light) == 0 else np.ascontiguousarray(light,
+                                                                             dtype=np.float32)
+    out = np.zeros((len(items), int(params["counts"][0][2]), 3), np.float32)
+    assert L.orc_gather_terms(_p(nodes), len(nodes), _p(tris), _p(mats), _p(ch), _p(params),
+                              _p(items), len(items), key_base, _p(lt), _p(out)) == 0
+    return out

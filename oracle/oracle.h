@@ -37,6 +37,10 @@ int orc_run(int pass, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
 int orc_gather_records(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
                        const float *materials, const float *charts, const lb_params *params,
                        const lb_texel *items, size_t n, uint32_t key_base, uint32_t *out);
+int orc_gather_terms(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+                     const float *materials, const float *charts, const lb_params *params,
+                     const lb_texel *items, size_t n, uint32_t key_base, const float *light,
+                     float *out);
 int orc_gather_steps(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
                      const float *materials, const lb_params *params, const lb_texel *items,
                      size_t n, uint32_t key_base, uint8_t *out);

@@ -540,6 +540,51 @@ int orc_gather_records(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
     return 0;
 }
 
+/* Each gather ray's FP32 term as gather_item (trace.wgsl:258-268) adds it: sky(d) on a miss,
+ * zero on a back face, bounced() on a front hit; ray r of item i at (i * rays + r) * 3. The exact
+ * sum of these terms is what every form's FP32 sum approximates: the yardstick of summation
+ * error alone (the hits being equal). */
+int orc_gather_terms(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+                     const float *materials, const float *charts, const lb_params *params,
+                     const lb_texel *items, size_t n, uint32_t key_base, const float *light,
+                     float *out) {
+    static const float zero_light[4] = {0, 0, 0, 0};
+    static const float zero_chart[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    lb_node tmp;
+    ctx_t c0;
+    memset(&c0, 0, sizeof c0);
+    c0.nodes = empty_fix(nodes, n_nodes, &tmp); c0.tris = tris; c0.materials = materials;
+    c0.charts = charts ? charts : zero_chart;
+    c0.light = light ? light : zero_light;
+    c0.P = params;
+    uint32_t rays = params->counts[2];
+    long nn = (long)n;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < nn; i++) {
+        const ctx_t *c = &c0;
+        const lb_texel *tx = &items[i];
+        const lb_params *P = c->P;
+        v3 nrm = ld3(tx->normal);
+        frame_t f = frame(nrm);
+        uint32_t key = (key_base + (uint32_t)i) * 7919u;
+        float s1 = rnd(c, key, 11u), s2 = rnd(c, key, 13u);
+        v3 pos = ld3(tx->pos), dx = ld3(tx->dx), dy = ld3(tx->dy);
+        for (uint32_t r = 0u; r < rays; r++) {
+            float jx = rnd(c, key + r, 19u) - 0.5f;
+            float jy = rnd(c, key + r, 23u) - 0.5f;
+            v3 o = add(add(add(pos, scl(dx, jx)), scl(dy, jy)), scl(nrm, P->reach[2]));
+            float u0 = ((float)r + 0.5f) / (float)rays + s1;
+            float v0 = orc_radical(r) + s2;
+            v3 d = cosine(u0 - floorf(u0), v0 - floorf(v0), &f);
+            hit_t h = trace(c, o, d, P->reach[1], 0, key + r + 4000u);
+            v3 t = h.tri == NONE_U ? sky(c, d) : (h.back ? mk(0.0f, 0.0f, 0.0f) : bounced(c, o, d, h));
+            float *q = out + ((size_t)i * rays + r) * 3u;
+            q[0] = t.x; q[1] = t.y; q[2] = t.z;
+        }
+    }
+    return 0;
+}
+
 /* Each gather ray's traversal steps, inner-node visits plus triangle tests (trace.wgsl:83-130,
  * capped at 255), ray r of item i at i * rays + r: the rays of trace.wgsl:248-262 as gather_item
  * makes them, each through trace() with counters of its own. The TUNED gather's first bounce

@@ -2,8 +2,9 @@
 of 14 to 400 sides, tall and thin walls, many boxes sharing planes): the TUNED form with 8-bit
 outward-rounded child bounds gives the same direct and gather results as with FP32 child bounds
 and the same as the oracle's PLAIN restatement on identical rays. The quantized boxes contain the
-FP32 ones, so culling stays conservative; a ray can only meet another node order where two
-entry distances tie, so at most 0.5 % of items may differ, in counts (observed: none).
+FP32 ones, so culling stays conservative and every ray meets the same triangles; another node
+order could only decide an exact tie of two triangles' distances, and the block holds none, so
+both node layouts are held to the same bar against the oracle.
 """
 from __future__ import annotations
 
@@ -59,8 +60,8 @@ def test_quantized_nodes_keep_the_hits(gpu, city, pass_):
             assert np.array_equal(got[:, [0, 3]], want[:, [0, 3]])
             assert np.allclose(got[:, 2], want[:, 2], rtol=2e-6, atol=1e-6)
     else:
-        differ = np.nonzero((q[:, 3] != w[:, 3]) |
-                            ~np.isclose(q[:, :3], w[:, :3], rtol=2e-6, atol=1e-6).all(axis=1))[0]
-        assert len(differ) <= 0.005 * len(q), f"{len(differ)} items differ"
-        assert np.array_equal(w[:, 3], want[:, 3])
-        assert np.allclose(w[:, :3], want[:, :3], rtol=2e-6, atol=1e-6)
+        # the 8-bit boxes contain the FP32 ones: the same hits, so the same bar as FP32 nodes
+        # (only an exact tie of two triangles' distances could tell the entry orders apart)
+        for got in (q, w):
+            assert np.array_equal(got[:, 3], want[:, 3])
+            assert np.allclose(got[:, :3], want[:, :3], rtol=2e-6, atol=1e-6)

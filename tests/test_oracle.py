@@ -278,3 +278,24 @@ def test_texel_inside_a_slab_is_retried(prec):
     edge = _run(scene, 0, _ground_texel(0.95, 0.0), prec=prec)
     assert _word(edge, 0, prec) >> 24 == 1, edge[0]
     assert edge[0, 3] == 0.0
+
+
+def test_gather_terms_sum_to_the_gather():
+    """oracle.gather_terms gives each gather ray's term as gather_item adds it: summed in ray
+    order in FP32, times pi, over the ray count, they are the gather's output bit for bit."""
+    from lightbaker_amd.scenes import small_scene
+    scene = small_scene(1)
+    bvh, pos, mesh = kit.scene_bvh(scene)
+    rng = np.random.default_rng(17)
+    items = kit.sample_items(pos, mesh, 200, rng)
+    charts = kit.charts_for(scene)
+    light = kit.random_light(rng)
+    P = kit.params_for(scene)
+    mats = scene.materials_array()
+    want = oracle.run(1, bvh.nodes, bvh.tris, mats, charts, P, items, 0, light)
+    terms = oracle.gather_terms(bvh.nodes, bvh.tris, mats, charts, P, items, 0, light)
+    s = np.zeros((len(items), 3), np.float32)
+    for r in range(terms.shape[1]):
+        s = s + terms[:, r]
+    got = s * np.float32(math.pi) / np.float32(terms.shape[1])
+    assert np.array_equal(got.view(np.uint32), want[:, :3].view(np.uint32))

@@ -75,12 +75,12 @@ def _compare_tuned(pass_, got, want, quant):
     tol = 2e-6 * np.abs(want[:, sum_cols]) + 1e-6
     sum_bad = (np.abs(got[:, sum_cols] - want[:, sum_cols]) > tol).any(axis=1)
     bad = exact_bad | sum_bad
-    if quant:
-        assert bad.sum() <= max(1, int(0.005 * n)), f"{bad.sum()} of {n} items differ"
-    else:
-        idx = np.nonzero(bad)[0]
-        assert len(idx) == 0, f"{len(idx)} of {n} differ, first {idx[:5]}: " \
-                              f"{got[idx[:3]]} vs {want[idx[:3]]}"
+    # 8-bit node boxes contain their FP32 boxes, so every ray meets the same triangles; only the
+    # order two children are entered in may change, which decides nothing but an exact tie of two
+    # triangles' distances. The test scene holds none: the QUANT form is held to the same bar.
+    idx = np.nonzero(bad)[0]
+    assert len(idx) == 0, f"{len(idx)} of {n} differ (quant {quant}), first {idx[:5]}: " \
+                          f"{got[idx[:3]]} vs {want[idx[:3]]}"
 
 
 @pytest.mark.parametrize("nodes", [N.NODES_WIDE, N.NODES_QUANT])
