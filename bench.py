@@ -12,7 +12,8 @@ step on the same map, and the oracle (the C restatement of the passes) timed on 
 on a bounded sample of the gather pass.
 
 `--workload rebake` measures a rebake instead: lightbaker_amd.scenes.rebake_scene(3), a 200 x
-200 m map in 64 tiles of 25 m whose texel counts differ about fourfold, baked once whole (its
+200 m map in 64 tiles of 25 m, each laid out at its own density of 4-16 texels/m (texel counts
+more than tenfold apart), baked once whole (its
 tiles' light is the cache), then a seeded subset of its tiles and their probe chunks rebaked; a
 step is the device part of that rebake, every dirty tile's items in one list per pass.
 
@@ -416,7 +417,7 @@ def main() -> int:
                     "64 GI rays, 3 bounces, 256 tiles")
     else:
         metric = "Grays/s, rebake of a seeded share of a 200x200 m map's 64 tiles"
-        workload = (f"rebake_scene(3): 200x200 m, 4 texels/m, 64 GI rays, 1 bounce, "
+        workload = (f"rebake_scene(3): 200x200 m, 4-16 texels/m by tile, 64 GI rays, 1 bounce, "
                     f"{info['tiles_baked']} of {info['tiles']} tiles dirty, "
                     f"{info['probes_baked']} of {info['probes']} probes")
     out = {

@@ -241,6 +241,14 @@ typedef struct lb_mesh_desc {
     const float *positions;
 } lb_mesh_desc;
 
+/* A tile's own texel density: tiles whose ground-plan key (layout.rs:347-349: floor of x and z
+ * over tileSize) is (x, z) lay their charts out at texels_per_meter. No counterpart in the
+ * reference, whose layout has the one density (scene.rs:84). */
+typedef struct lb_tile_density {
+    int32_t x, z;
+    float texels_per_meter;
+} lb_tile_density;
+
 /* scene.rs:17-128 `SceneHeader` (sun, sky, materials, meshes, lightmap, bake, volume) */
 typedef struct lb_scene_desc {
     float sun_direction[3];
@@ -279,6 +287,11 @@ typedef struct lb_scene_desc {
     float volume_max[3];
     float volume_step;
     uint32_t volume_rays;
+    /* per-tile texel densities (n_tile_density entries, distinct keys, each positive); tiles not
+     * listed take texels_per_meter. NULL / 0: the reference's layout, byte for byte. A tile's
+     * density joins its dependency hash (lb_deps_tile_hashes) only when it is listed. */
+    const lb_tile_density *tile_density;
+    size_t n_tile_density;
 } lb_scene_desc;
 
 typedef struct lb_bvh lb_bvh;

@@ -55,6 +55,11 @@ class MeshDesc(C.Structure):
                 ("positions", C.POINTER(C.c_float))]
 
 
+class TileDensity(C.Structure):
+    """lb_tile_density: a tile's own texels per metre."""
+    _fields_ = [("x", C.c_int32), ("z", C.c_int32), ("texels_per_meter", C.c_float)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("sun_direction", C.c_float * 3), ("sun_radius_deg", C.c_float),
@@ -69,6 +74,7 @@ class SceneDesc(C.Structure):
         ("gi_reach", C.c_float), ("inside_share", C.c_float), ("gi_filter", C.c_uint32),
         ("has_volume", C.c_uint32), ("volume_min", C.c_float * 3), ("volume_max", C.c_float * 3),
         ("volume_step", C.c_float), ("volume_rays", C.c_uint32),
+        ("tile_density", C.POINTER(TileDensity)), ("n_tile_density", C.c_size_t),
     ]
 
 

@@ -128,6 +128,13 @@ def scene_desc(scene: Scene):
         d.volume_max[:] = list(scene.volume.max)
         d.volume_step = scene.volume.step
         d.volume_rays = scene.volume.rays
+    if scene.tile_density:
+        dens = (N.TileDensity * len(scene.tile_density))()
+        for i, ((x, z), v) in enumerate(sorted(scene.tile_density.items())):
+            dens[i].x, dens[i].z, dens[i].texels_per_meter = int(x), int(z), float(v)
+        d.tile_density = dens
+        d.n_tile_density = len(scene.tile_density)
+        keep.append(dens)
     keep += [mats, meshes]
     return d, keep
 

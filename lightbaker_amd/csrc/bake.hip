@@ -125,7 +125,7 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     size_t n_charts = L.charts.size();
     B->charts.assign(n_charts * 8, 0.0f);
     for (size_t c = 0; c < n_charts; c++)
-        chart_affine(L.charts[c], L.tiles[L.chart_tile[c]], L.density, L.padding, &B->charts[c * 8]);
+        chart_affine(L.charts[c], L.tiles[L.chart_tile[c]], L.padding, &B->charts[c * 8]);
     t0 = clk::now();
     size_t n_tris = 0;
     for (size_t m = 0; m < sc.n_meshes; m++) n_tris += sc.meshes[m].triangles;

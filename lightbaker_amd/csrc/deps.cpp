@@ -177,6 +177,11 @@ int lb_deps_tile_hashes(lb_gpu *gpu, const lb_scene_desc *scene, const lb_layout
                 h.update(p, 36);
             }
         }
+        // a tile laid out at a density of its own (no counterpart in the reference): the
+        // density's bytes end its seed, so changing it rebakes the tile
+        for (size_t k = 0; k < s.n_tile_density; k++)
+            if (s.tile_density[k].x == tl.kx && s.tile_density[k].z == tl.kz)
+                h.update(&s.tile_density[k].texels_per_meter, 4);
         uint8_t seed[32];
         h.finalize(seed);
         seeds[t].assign((const char *)seed, 32);

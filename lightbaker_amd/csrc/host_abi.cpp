@@ -102,8 +102,7 @@ int lb_layout_get(const lb_layout *l, float *affines, uint32_t *chart_tiles, int
     const lbk::Layout &L = l->L;
     for (size_t c = 0; c < L.charts.size(); c++) {
         if (affines)
-            lbk::chart_affine(L.charts[c], L.tiles[L.chart_tile[c]], L.density, L.padding,
-                              affines + c * 8);
+            lbk::chart_affine(L.charts[c], L.tiles[L.chart_tile[c]], L.padding, affines + c * 8);
         if (chart_tiles) chart_tiles[c] = L.chart_tile[c];
     }
     if (tiles)

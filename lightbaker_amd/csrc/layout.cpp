@@ -97,7 +97,8 @@ std::vector<std::vector<uint32_t>> grow(const lb_scene_desc &s, uint32_t mesh, f
 
 // layout.rs:173-196
 Chart make_chart(const lb_scene_desc &s, uint32_t mesh, std::vector<uint32_t> tris, float tile,
-                 float density, uint32_t padding) {
+                 const std::map<std::pair<int32_t, int32_t>, float> &densities, float density,
+                 uint32_t padding) {
     V3 sum{0, 0, 0};
     for (uint32_t t : tris) {
         V3 p0 = vert(s, mesh, t, 0), p1 = vert(s, mesh, t, 1), p2 = vert(s, mesh, t, 2);
@@ -117,8 +118,11 @@ Chart make_chart(const lb_scene_desc &s, uint32_t mesh, std::vector<uint32_t> tr
             mn = {std::fmin(mn.x, qx), std::fmin(mn.y, qy)};
             mx = {std::fmax(mx.x, qx), std::fmax(mx.y, qy)};
         }
-    auto size = [&](float e) { return (uint32_t)std::ceil(e * density) + 1 + 2 * padding; };
     V3 c = box_centre(s, mesh, tris[0]);
+    int32_t kx = key_of(c.x, tile), kz = key_of(c.z, tile);
+    auto d = densities.find({kx, kz});
+    if (d != densities.end()) density = d->second;
+    auto size = [&](float e) { return (uint32_t)std::ceil(e * density) + 1 + 2 * padding; };
     Chart ch;
     ch.mesh = mesh;
     ch.normal = n;
@@ -126,8 +130,9 @@ Chart make_chart(const lb_scene_desc &s, uint32_t mesh, std::vector<uint32_t> tr
     ch.v = v;
     ch.min = mn;
     ch.max = mx;
-    ch.tile_x = key_of(c.x, tile);
-    ch.tile_z = key_of(c.z, tile);
+    ch.tile_x = kx;
+    ch.tile_z = kz;
+    ch.density = density;
     ch.w = size(mx.x - mn.x);
     ch.h = size(mx.y - mn.y);
     ch.tris = std::move(tris);
@@ -264,14 +269,16 @@ V3 face_normal(const lb_scene_desc &s, uint32_t mesh, uint32_t t) {
     return n;
 }
 
-V2 chart_local(const Chart &c, V3 p, float density, uint32_t padding) {
+V2 chart_local(const Chart &c, V3 p, uint32_t padding) {
+    float density = c.density;
     V2 a{(dot(p, c.u) - c.min.x) * density, (dot(p, c.v) - c.min.y) * density};
     if (c.turned) a = {a.y, a.x};
     float off = (float)padding + 0.5f;
     return {((float)c.x + off) + a.x, ((float)c.y + off) + a.y};
 }
 
-void chart_affine(const Chart &c, const Tile &t, float density, uint32_t padding, float out[8]) {
+void chart_affine(const Chart &c, const Tile &t, uint32_t padding, float out[8]) {
+    float density = c.density;
     float off = (float)padding + 0.5f;
     V2 o{(float)(t.x + c.x) + off, (float)(t.y + c.y) + off};
     V3 ax = c.turned ? c.v : c.u, ay = c.turned ? c.u : c.v;
@@ -281,7 +288,35 @@ void chart_affine(const Chart &c, const Tile &t, float density, uint32_t padding
     out[4] = cy.x; out[5] = cy.y; out[6] = cy.z; out[7] = o.y - my * density;
 }
 
+bool check_tile_density(const lb_scene_desc &s, std::string *err) {
+    if (s.n_tile_density && !s.tile_density) {
+        *err = "lightmap: per-tile densities: null array";
+        return false;
+    }
+    std::map<std::pair<int32_t, int32_t>, float> seen;
+    for (size_t i = 0; i < s.n_tile_density; i++) {
+        const lb_tile_density &d = s.tile_density[i];
+        char buf[160];
+        if (!(d.texels_per_meter > 0.0f) || !std::isfinite(d.texels_per_meter)) {
+            snprintf(buf, sizeof buf, "lightmap: tile (%d, %d): texelsPerMeter %g is not positive",
+                     d.x, d.z, (double)d.texels_per_meter);
+            *err = buf;
+            return false;
+        }
+        if (!seen.emplace(std::make_pair(d.x, d.z), d.texels_per_meter).second) {
+            snprintf(buf, sizeof buf, "lightmap: tile (%d, %d) has two densities", d.x, d.z);
+            *err = buf;
+            return false;
+        }
+    }
+    return true;
+}
+
 bool build_layout(const lb_scene_desc &s, Layout &L, std::string *err) {
+    if (!check_tile_density(s, err)) return false;
+    std::map<std::pair<int32_t, int32_t>, float> densities;
+    for (size_t i = 0; i < s.n_tile_density; i++)
+        densities[{s.tile_density[i].x, s.tile_density[i].z}] = s.tile_density[i].texels_per_meter;
     float density = s.texels_per_meter;
     uint32_t padding = s.padding;
     float cosang = std::cos(s.chart_angle_deg * (3.14159265358979323846f / 180.0f));
@@ -293,7 +328,8 @@ bool build_layout(const lb_scene_desc &s, Layout &L, std::string *err) {
         L.tri_chart[mi].assign(s.meshes[mi].triangles, 0xffffffffu);
         if (!s.materials[s.meshes[mi].material].receives) continue;
         for (auto &group : grow(s, (uint32_t)mi, cosang, s.tile_size)) {
-            Chart c = make_chart(s, (uint32_t)mi, std::move(group), s.tile_size, density, padding);
+            Chart c = make_chart(s, (uint32_t)mi, std::move(group), s.tile_size, densities,
+                                 density, padding);
             for (uint32_t t : c.tris) L.tri_chart[mi][t] = (uint32_t)L.charts.size();
             L.charts.push_back(std::move(c));
         }

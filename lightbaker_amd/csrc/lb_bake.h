@@ -59,6 +59,7 @@ struct Chart {
     V3 normal, u, v;
     V2 min, max;
     int32_t tile_x, tile_z;
+    float density = 0.0f;  // texels a metre: its tile's (lb_scene_desc.tile_density), else the scene's
     bool turned = false;
     uint32_t w = 0, h = 0, x = 0, y = 0;
 };
@@ -73,7 +74,7 @@ struct Layout {
     std::vector<Chart> charts;
     std::vector<Tile> tiles;  // in key order
     uint32_t width = 0, height = 0;
-    float density = 0.0f;
+    float density = 0.0f;  // the scene's texels_per_meter (each chart keeps its tile's)
     uint32_t padding = 0;
     std::vector<std::vector<uint32_t>> tri_chart;  // each mesh's triangles' chart (~0u: none)
     std::vector<uint32_t> chart_tile;              // each chart's tile
@@ -81,10 +82,12 @@ struct Layout {
 
 // layout.rs:89-115; false with a message on a tile wider than the atlas (layout.rs:247-249)
 bool build_layout(const lb_scene_desc &s, Layout &out, std::string *err);
-// layout.rs:70-74
-V2 chart_local(const Chart &c, V3 p, float density, uint32_t padding);
-// layout.rs:77-85: two rows of (coefficients, constant)
-void chart_affine(const Chart &c, const Tile &t, float density, uint32_t padding, float out[8]);
+// layout.rs:70-74, at the chart's density
+V2 chart_local(const Chart &c, V3 p, uint32_t padding);
+// layout.rs:77-85: two rows of (coefficients, constant), at the chart's density
+void chart_affine(const Chart &c, const Tile &t, uint32_t padding, float out[8]);
+// the scene's per-tile densities: false with a message on a bad or repeated entry
+bool check_tile_density(const lb_scene_desc &s, std::string *err);
 V3 face_normal(const lb_scene_desc &s, uint32_t mesh, uint32_t t);
 // raster.rs:39-100: the tile's texels, row by row in its w x h rectangle
 void rasterize(const lb_scene_desc &s, const Layout &l, const Tile &t, std::vector<lb_texel> &out);

@@ -62,9 +62,9 @@ void rasterize(const lb_scene_desc &s, const Layout &L, const Tile &tile, std::v
             for (uint32_t t : c.tris) {
                 const float *pp = s.meshes[c.mesh].positions + (size_t)t * 9;
                 V3 pa{pp[0], pp[1], pp[2]}, pb{pp[3], pp[4], pp[5]}, pc{pp[6], pp[7], pp[8]};
-                V2 ua = chart_local(c, pa, L.density, L.padding);
-                V2 ub = chart_local(c, pb, L.density, L.padding);
-                V2 uc = chart_local(c, pc, L.density, L.padding);
+                V2 ua = chart_local(c, pa, L.padding);
+                V2 ub = chart_local(c, pb, L.padding);
+                V2 uc = chart_local(c, pc, L.padding);
                 float det = (ub.y - uc.y) * (ua.x - uc.x) + (uc.x - ub.x) * (ua.y - uc.y);
                 if (std::fabs(det) < 1e-12f) continue;
                 auto bary = [&](V2 q, float *l1, float *l2, float *l3) {
@@ -119,9 +119,9 @@ void raster_records(const lb_scene_desc &s, const Layout &L, const std::vector<u
             for (uint32_t t : c.tris) {
                 const float *pp = s.meshes[c.mesh].positions + (size_t)t * 9;
                 V3 pa{pp[0], pp[1], pp[2]}, pb{pp[3], pp[4], pp[5]}, pc{pp[6], pp[7], pp[8]};
-                V2 ua = chart_local(c, pa, L.density, L.padding);
-                V2 ub = chart_local(c, pb, L.density, L.padding);
-                V2 uc = chart_local(c, pc, L.density, L.padding);
+                V2 ua = chart_local(c, pa, L.padding);
+                V2 ub = chart_local(c, pb, L.padding);
+                V2 uc = chart_local(c, pc, L.padding);
                 float det = (ub.y - uc.y) * (ua.x - uc.x) + (uc.x - ub.x) * (ua.y - uc.y);
                 if (std::fabs(det) < 1e-12f) continue;
                 auto at = [&](V2 q) {

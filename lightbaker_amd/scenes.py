@@ -81,6 +81,9 @@ class Scene:
     meshes: list
     settings: Settings = field(default_factory=Settings)
     volume: Volume | None = None
+    # per-tile texels per metre, {(key x, key z): density}; tiles not listed take
+    # settings.texels_per_meter (no counterpart in the reference's one density, scene.rs:84)
+    tile_density: dict | None = None
 
     def triangle_count(self) -> int:
         return sum(len(m.positions) // 3 for m in self.meshes)
@@ -226,11 +229,12 @@ def fit_scene(seed: int = 5) -> Scene:
         materials=mats, meshes=meshes, settings=st, volume=None)
 
 
-def rebake_scene(seed: int = 3, size: float = 200.0) -> Scene:
-    """A size x size m map built like small_scene (yawed boxes, edges U[0.5, 4] m, on a ground of
-    5 m quads, grey albedo 0.5, constant sky 0.5, sun at 45 degrees), in 25 m tiles whose box
-    counts run from 10 to 150 (50 per 400 m^2 on average, small_scene's density), so the tiles'
-    texel counts differ about fourfold."""
+def rebake_scene(seed: int = 3, size: float = 200.0, densities: tuple | None = (4, 16)) -> Scene:
+    """A size x size m map from small_scene's generator at its density (yawed boxes, edges
+    U[0.5, 4] m, 50 per 400 m^2 on average, on a ground of 5 m quads, grey albedo 0.5,
+    constant sky 0.5, sun at 45 degrees), in 25 m tiles, each laid out at its own texel density
+    drawn uniformly from the integers in `densities` (texels per metre; None: 4 everywhere), so
+    the tiles' texel counts differ up to (16 / 4)^2 = 16-fold through chart density."""
     rng = np.random.default_rng(seed)
     cells = int(size // 5)
     ground = []
@@ -242,9 +246,10 @@ def rebake_scene(seed: int = 3, size: float = 200.0) -> Scene:
     boxes = []
     for tx in range(tiles):
         for tz in range(tiles):
-            for _ in range(int(rng.integers(10, 151))):
+            # 50 boxes per 400 m^2 on average, 3 m from the tile's edges: every face of a box
+            # falls in its tile, so the map has exactly tiles^2 tiles
+            for _ in range(int(rng.poisson(625.0 / 400.0 * 50))):
                 sx, sy, sz = rng.uniform(0.5, 4.0, 3)
-                # 3 m from the tile's edges: every face of a box falls in its tile
                 cx = tx * 25.0 + rng.uniform(3.0, 22.0)
                 cz = tz * 25.0 + rng.uniform(3.0, 22.0)
                 boxes.append(yawed_box(cx, cz, 0.0, sx, sy, sz, rng.uniform(0, 2 * math.pi)))
@@ -253,13 +258,20 @@ def rebake_scene(seed: int = 3, size: float = 200.0) -> Scene:
     sun = (math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az))
     st = Settings(texels_per_meter=4.0, tile_size=25.0, atlas_width=4096, sun_samples=16,
                   ao_rays=64, gi_rays=64, gi_bounces=1)
+    dens = None
+    if densities is not None:
+        drng = np.random.default_rng(seed + 1000)
+        lo, hi = densities
+        dens = {(tx, tz): float(drng.integers(lo, hi + 1))
+                for tx in range(tiles) for tz in range(tiles)}
     return Scene(
         sun_direction=sun, sun_radius_deg=0.5, sun_irradiance=(3.0, 3.0, 3.0),
         zenith=(0.5, 0.5, 0.5), horizon=(0.5, 0.5, 0.5), ground=(0.5, 0.5, 0.5),
         materials=[Material("grey", (0.5, 0.5, 0.5))],
         meshes=[Mesh("ground", 0, np.array(ground, np.float32)),
                 Mesh("boxes", 0, np.concatenate(boxes))],
-        settings=st, volume=Volume((0.0, 0.0, 0.0), (size, 8.0, size), 1.0, 64))
+        settings=st, volume=Volume((0.0, 0.0, 0.0), (size, 8.0, size), 1.0, 64),
+        tile_density=dens)
 
 
 def building(cx, cz, radius, height, sides, yaw) -> np.ndarray:

@@ -12,7 +12,8 @@ import ctypes as C
 
 import numpy as np
 
-from . import lib, run
+from . import lib, run_quant
+from . import run as run_fp32
 
 VALIDITY_NONE, VALIDITY_CENTRE, VALIDITY_OFFSET = 0, 1, 2
 VALIDITY_PUSHED, VALIDITY_FILLED, VALIDITY_VOLUME = 3, 4, 5
@@ -30,6 +31,8 @@ def _filters():
         L.orc_fill.restype = None
         L.orc_sun_field.argtypes = [I, I, VP, VP, VP, VP, I, C.c_float, VP]
         L.orc_sun_field.restype = None
+        L.orc_powf_n.argtypes = [VP, C.c_size_t, C.c_float, VP]
+        L.orc_powf_n.restype = None
         L._filters_bound = True
     return L
 
@@ -67,11 +70,19 @@ def offset(q):
     return {5: (F(-0.45), F(0)), 6: (F(0.45), F(0)), 7: (F(0), F(-0.45))}.get(q, (F(0), F(0.45)))
 
 
+def powf(x, y):
+    """The host's powf elementwise (glibc's, as Rust's f32::powf), float32 in and out."""
+    x = np.ascontiguousarray(np.atleast_1d(x), np.float32)
+    out = np.empty_like(x)
+    _filters().orc_powf_n(x.ctypes.data, len(x), F(y), out.ctypes.data)
+    return out
+
+
 def ao_tone(occ, strength, contrast, floor):
     """bake.rs:238: max(floor, max(0, 1 - strength * occ) ^ contrast), FP32."""
     v = max(F(1.0) - F(strength) * F(occ), F(0.0))
     if contrast != 1.0:
-        v = F(v ** F(contrast))
+        v = powf(v, contrast)[0]
     return max(v, F(floor))
 
 
@@ -112,12 +123,60 @@ def gather_set(tw, size, valid, chart, ds):
     return gathered
 
 
+def decode_all(res, tex, cov, settings):
+    """decode() over every covered texel at once, elementwise in the same FP32 operations:
+    `cov` the texels' indices in `tex` (moved in place), `res` their direct-pass results in that
+    order. Returns (inside, soft, hard bits, state, occlusion tone) over `cov`."""
+    res = np.asarray(res, np.float32)
+    inside = res[:, 3] > F(settings.inside_share)
+    bits = res[:, 1].view(np.uint32)
+    pick = bits >> 24
+    state = np.where(pick == 0, VALIDITY_CENTRE,
+                     np.where(pick == 9, VALIDITY_PUSHED, VALIDITY_OFFSET)).astype(np.uint8)
+    ok = ~inside
+    pos = tex["pos"][cov, :3]
+    m = ok & (pick == 9)
+    pos[m] = pos[m] + tex["normal"][cov[m], :3] * PUSH
+    for q in range(1, 9):
+        m = ok & (pick == q)
+        cx, cy = offset(q)
+        pos[m] = pos[m] + (tex["dx"][cov[m], :3] * cx + tex["dy"][cov[m], :3] * cy)
+    tex["pos"][cov, :3] = pos
+    v = np.maximum(F(1.0) - F(settings.ao_strength) * res[:, 2], F(0.0))
+    if settings.ao_contrast != 1.0:
+        v = powf(v, settings.ao_contrast)
+    ao = np.maximum(v, F(settings.ao_floor))
+    return inside, res[:, 0].copy(), bits & 0x00FFFFFF, state, ao
+
+
+def gather_set_all(tw, size, valid, chart, ds):
+    """gather_set() over a tile with array operations: the grid texels, then of the other valid
+    texels whose (block, chart) pair no grid texel holds, the first of each pair."""
+    i = np.arange(size)
+    x, y = i % tw, i // tw
+    valid = np.asarray(valid, bool)
+    key = ((y // ds) * (-(-tw // ds)) + x // ds).astype(np.int64) << 32 | np.asarray(chart, np.int64)
+    grid = valid & (x % ds == 0) & (y % ds == 0)
+    cand = valid & ~grid & ~np.isin(key, key[grid])
+    _, first = np.unique(key[cand], return_index=True)
+    out = grid.copy()
+    out[np.nonzero(cand)[0][first]] = True
+    return out
+
+
 def bake(inp: dict) -> dict:
     """inp: texels (T,) TEXEL of the dirty tiles, tile after tile; tiles (nt, 6) of the dirty
     tiles; light0 (aw * ah, 4) the clean tiles' light or None; aw, ah; nodes, tris, mats, charts,
     params; settings (lightmap/bake fields); sun (3,), irr (3,); probes (np,) TEXEL of the dirty
-    probes or None, volume_params."""
+    probes or None, volume_params; quant (optional): the 8-bit node records (quant_records), whose
+    boxes the passes then traverse as the device's quantized form does."""
     st = inp["settings"]
+    q = inp.get("quant")
+
+    def run(pass_, nodes, tris, mats, charts, P, items, key_base, light):
+        if q is None:
+            return run_fp32(pass_, nodes, tris, mats, charts, P, items, key_base, light)
+        return run_quant(pass_, nodes, q, tris, mats, charts, P, items, key_base, light)
     tex = inp["texels"].copy()
     tiles = inp["tiles"]
     aw, ah = inp["aw"], inp["ah"]
@@ -140,12 +199,10 @@ def bake(inp: dict) -> dict:
     soft = np.zeros(T, np.float32)
     ao = np.ones(T, np.float32)
     hard = np.zeros(T, np.uint32)
-    for k, gi in enumerate(cov):
-        d = decode(res[k], tex[gi], st)
-        if d is None:
-            valid[gi] = False
-            continue
-        soft[gi], hard[gi], state[gi], ao[gi] = d
+    inside, sf, hb, stt, tone = decode_all(res, tex, cov, st)
+    ok = cov[~inside]
+    valid[cov[inside]] = False
+    soft[ok], hard[ok], state[ok], ao[ok] = sf[~inside], hb[~inside], stt[~inside], tone[~inside]
     known = np.zeros(T, np.uint8)
     field = np.zeros(T, np.float32)
     direct = np.zeros((T, 3), np.float32)
@@ -197,7 +254,7 @@ def bake(inp: dict) -> dict:
     gathered = np.zeros(T, bool)
     for t in range(len(tiles)):
         o, e = int(offs[t]), int(offs[t + 1])
-        gathered[o:e] = gather_set(int(tiles[t, 2]), e - o, valid[o:e], chart[o:e], ds)
+        gathered[o:e] = gather_set_all(int(tiles[t, 2]), e - o, valid[o:e], chart[o:e], ds)
     gat = np.nonzero(gathered)[0]
     grings = st.padding + 2 * ds + 2
     for _ in range(st.gi_bounces):

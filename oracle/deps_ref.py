@@ -68,12 +68,15 @@ def hash_reach_stream(global_: bytes, seed: bytes, meshes, row_bits, first) -> b
     return bytes(out)
 
 
-def tile_seed_stream(key, charts, meshes) -> bytes:
-    """bake.rs:461-477's bytes: charts = [(mesh, [its triangles in order])]."""
+def tile_seed_stream(key, charts, meshes, density=None) -> bytes:
+    """bake.rs:461-477's bytes: charts = [(mesh, [its triangles in order])]; a tile given a texel
+    density of its own (no counterpart in the reference) ends with the density's f32 bytes."""
     out = bytearray(i32(key[0]) + i32(key[1]))
     for m, tris in charts:
         pos, material, _ = meshes[m]
         out += u32(m) + u32(material)
         for t in tris:
             out += u32(t) + np.ascontiguousarray(pos[t], "<f4").tobytes()
+    if density is not None:
+        out += np.float32(density).tobytes()
     return bytes(out)

@@ -124,3 +124,9 @@ void orc_sun_field(int w, int h, const uint32_t *chart, const float *normal, con
             out[i] = 0.5f + 0.5f * (lit ? d : -d) / radius;
         }
 }
+
+/* bake.rs:238's `.powf(b.ao_contrast)` over n values: the host's powf (glibc's on Linux, as Rust's
+ * f32::powf calls it), so the array and the scalar restatement round alike */
+void orc_powf_n(const float *x, size_t n, float y, float *out) {
+    for (size_t i = 0; i < n; i++) out[i] = powf(x[i], y);
+}

@@ -34,6 +34,15 @@ int orc_run(int pass, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
             const float *materials, const float *charts, const lb_params *params,
             const lb_texel *items, size_t n, uint32_t key_base, const float *light, float *out,
             orc_counters *cnt);
+int orc_run_quant(int pass, const lb_node *nodes, size_t n_nodes, const uint32_t *quant,
+                  const lb_tri *tris, const float *materials, const float *charts,
+                  const lb_params *params, const lb_texel *items, size_t n, uint32_t key_base,
+                  const float *light, float *out, orc_counters *cnt);
+int orc_quant_compare(int pass, const lb_node *nodes, size_t n_nodes, const uint32_t *quant,
+                      const lb_tri *tris, const float *materials, const float *charts,
+                      const lb_params *params, const lb_texel *items, size_t n,
+                      uint32_t key_base, const float *light, float *out,
+                      unsigned long long counts[3]);
 int orc_gather_records(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
                        const float *materials, const float *charts, const lb_params *params,
                        const lb_texel *items, size_t n, uint32_t key_base, uint32_t *out);

@@ -95,6 +95,14 @@ typedef struct {
     const float *light;
     const lb_params *P;
     orc_counters *cnt;
+    const uint32_t *quant; /* the TUNED form's 8-bit node records (lb_tuned_nodes), or NULL */
+    /* with `cmp` (and `cmp_quant`): every ray is traced over both node forms, the reference's
+     * result is kept, and cmp counts rays, rays whose results differ, and those among them that
+     * are no near-tie (a hit against none, another any-hit outcome, or distances more than
+     * 2^-20 t apart); an any-hit ray counts as differing only when one form is blocked and the
+     * other not */
+    unsigned long long *cmp;
+    const uint32_t *cmp_quant;
 } ctx_t;
 
 /* trace.wgsl:46 */
@@ -171,8 +179,76 @@ static float boxt(v3 o, v3 inv, const float *mn, const float *mx, float tmax) {
 
 typedef struct { float t; uint32_t tri; int back; } hit_t;
 
+/* Not the reference: one inner-node visit over the TUNED form's 8-bit child boxes (a record of
+ * lb_tuned_nodes: per axis the origin's bits with the step's exponent in the low byte, the 12
+ * corner bytes), with the device's arithmetic (lb_traverse.h quant_slabs): the plane of byte q
+ * at t = fma(q, 2^e * inv, (origin - o) * inv), the entry max(0, ...) and the exit min, a child
+ * hit when exit >= entry and entry < t, the near child the hit one of the lower entry (a on a tie),
+ * the far one pushed when both are hit. Moves to the near child and returns 1, else returns 0. The
+ * boxes contain the reference's, so the same triangles can be met; the order and the entries
+ * differ, which decides between two hits only when their distances lie within a few ulps. */
+static int quant_visit(const uint32_t *q, v3 o, v3 inv, float tmax, uint32_t first, uint32_t *n,
+                       uint32_t *stack, uint32_t *sp) {
+    float kx = bitsf((q[0] & 0xffu) << 23) * inv.x, ky = bitsf((q[1] & 0xffu) << 23) * inv.y,
+          kz = bitsf((q[2] & 0xffu) << 23) * inv.z;
+    float cx = (bitsf(q[0] & 0xffffff00u) - o.x) * inv.x;
+    float cy = (bitsf(q[1] & 0xffffff00u) - o.y) * inv.y;
+    float cz = (bitsf(q[2] & 0xffffff00u) - o.z) * inv.z;
+    const uint8_t *b = (const uint8_t *)(q + 4); /* a min xyz, a max xyz, b min xyz, b max xyz */
+    float x0 = fmaf((float)b[0], kx, cx), y0 = fmaf((float)b[1], ky, cy), z0 = fmaf((float)b[2], kz, cz);
+    float x1 = fmaf((float)b[3], kx, cx), y1 = fmaf((float)b[4], ky, cy), z1 = fmaf((float)b[5], kz, cz);
+    float la = fmaxv(fmaxv(fmaxv(0.0f, fminv(x0, x1)), fminv(y0, y1)), fminv(z0, z1));
+    float ea = fminv(fminv(fmaxv(x0, x1), fmaxv(y0, y1)), fmaxv(z0, z1));
+    x0 = fmaf((float)b[6], kx, cx); y0 = fmaf((float)b[7], ky, cy); z0 = fmaf((float)b[8], kz, cz);
+    x1 = fmaf((float)b[9], kx, cx); y1 = fmaf((float)b[10], ky, cy); z1 = fmaf((float)b[11], kz, cz);
+    float lb = fmaxv(fmaxv(fmaxv(0.0f, fminv(x0, x1)), fminv(y0, y1)), fminv(z0, z1));
+    float eb = fminv(fminv(fmaxv(x0, x1), fmaxv(y0, y1)), fmaxv(z0, z1));
+    int ha = ea >= la && la < tmax, hb = eb >= lb && lb < tmax;
+    if (!(ha || hb)) return 0;
+    int an = ha && (!hb || la <= lb);
+    if (ha && hb && *sp < STACK) stack[(*sp)++] = an ? first + 1u : first;
+    *n = an ? first : first + 1u;
+    return 1;
+}
+
 /* trace.wgsl:83-130 */
+static hit_t trace_nodes(const ctx_t *c, v3 o, v3 d, float tmax, int any, uint32_t ray);
+
+/* the first rays orc_quant_compare finds differing beyond a near-tie: o, d, tmax, any, the two
+ * forms' t and triangles (bits), the ray key (bits) */
+#define CMP_EXAMPLES 16
+static float g_cmp_examples[CMP_EXAMPLES][13];
+void orc_quant_examples(float out[CMP_EXAMPLES][13]) { memcpy(out, g_cmp_examples, sizeof g_cmp_examples); }
+
 static hit_t trace(const ctx_t *c, v3 o, v3 d, float tmax, int any, uint32_t ray) {
+    if (!c->cmp) return trace_nodes(c, o, d, tmax, any, ray);
+    hit_t hf = trace_nodes(c, o, d, tmax, any, ray);
+    ctx_t cq = *c;
+    cq.quant = c->cmp_quant;
+    cq.cnt = NULL;
+    hit_t hq = trace_nodes(&cq, o, d, tmax, any, ray);
+    __atomic_fetch_add(&c->cmp[0], 1ull, __ATOMIC_RELAXED);
+    /* an any-hit ray's result is whether it was blocked; a closest-hit ray's, its hit */
+    int differ = any ? (hf.tri == NONE_U) != (hq.tri == NONE_U)
+                     : (hf.tri != hq.tri || hf.t != hq.t || hf.back != hq.back);
+    if (differ) {
+        __atomic_fetch_add(&c->cmp[1], 1ull, __ATOMIC_RELAXED);
+        int near_tie = !any && hf.tri != NONE_U && hq.tri != NONE_U &&
+                       fabsf(hf.t - hq.t) <= ldexpf(fmaxv(hf.t, hq.t), -20);
+        if (!near_tie) {
+            unsigned long long k = __atomic_fetch_add(&c->cmp[2], 1ull, __ATOMIC_RELAXED);
+            if (k < CMP_EXAMPLES) {
+                float *e = g_cmp_examples[k];
+                e[0] = o.x; e[1] = o.y; e[2] = o.z; e[3] = d.x; e[4] = d.y; e[5] = d.z;
+                e[6] = tmax; e[7] = (float)any; e[8] = hf.t; e[9] = hq.t;
+                e[10] = bitsf(hf.tri); e[11] = bitsf(hq.tri); e[12] = bitsf(ray);
+            }
+        }
+    }
+    return hf;
+}
+
+static hit_t trace_nodes(const ctx_t *c, v3 o, v3 d, float tmax, int any, uint32_t ray) {
     hit_t hit = {tmax, NONE_U, 0};
     v3 dd = mk(fabsf(d.x) < 1e-12f ? 1e-12f : d.x, fabsf(d.y) < 1e-12f ? 1e-12f : d.y,
                fabsf(d.z) < 1e-12f ? 1e-12f : d.z);
@@ -204,6 +280,10 @@ static hit_t trace(const ctx_t *c, v3 o, v3 d, float tmax, int any, uint32_t ray
                     if (any) return hit;
                 }
             }
+        } else if (c->quant) {
+            if (c->cnt) c->cnt->inner_visits++;
+            if (quant_visit(c->quant + (size_t)n * 8u, o, inv, hit.t, node->first, &n, stack, &sp))
+                continue;
         } else {
             const lb_node *a = &c->nodes[node->first];
             const lb_node *b = &c->nodes[node->first + 1u];
@@ -445,10 +525,11 @@ static const lb_node *empty_fix(const lb_node *nodes, size_t n_nodes, lb_node *t
     return nodes;
 }
 
-int orc_run(int pass, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
-            const float *materials, const float *charts, const lb_params *params,
-            const lb_texel *items, size_t n, uint32_t key_base, const float *light, float *out,
-            orc_counters *cnt) {
+static int run_ctx(int pass, const lb_node *nodes, size_t n_nodes, const uint32_t *quant,
+                   const lb_tri *tris, const float *materials, const float *charts,
+                   const lb_params *params, const lb_texel *items, size_t n, uint32_t key_base,
+                   const float *light, float *out, orc_counters *cnt, unsigned long long *cmp,
+                   const uint32_t *cmp_quant) {
     static const float zero_light[4] = {0, 0, 0, 0};
     static const float zero_chart[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     lb_node tmp;
@@ -456,7 +537,8 @@ int orc_run(int pass, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
     c0.nodes = empty_fix(nodes, n_nodes, &tmp); c0.tris = tris; c0.materials = materials;
     c0.charts = charts ? charts : zero_chart;
     c0.light = light ? light : zero_light;
-    c0.P = params; c0.cnt = cnt;
+    c0.P = params; c0.cnt = cnt; c0.quant = quant;
+    c0.cmp = cmp; c0.cmp_quant = cmp_quant;
     if (pass < 0 || pass > 2) return 1;
     long nn = (long)n;
     if (cnt) {
@@ -476,6 +558,41 @@ int orc_run(int pass, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
     return 0;
 }
 
+int orc_run(int pass, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
+            const float *materials, const float *charts, const lb_params *params,
+            const lb_texel *items, size_t n, uint32_t key_base, const float *light, float *out,
+            orc_counters *cnt) {
+    return run_ctx(pass, nodes, n_nodes, NULL, tris, materials, charts, params, items, n,
+                   key_base, light, out, cnt, NULL, NULL);
+}
+
+/* Not the reference: orc_run over the TUNED form's 8-bit node records (`quant`, n_nodes x 8
+ * words from lb_tuned_nodes) in place of the reference's boxes, visited as the device visits
+ * them. The yardstick of the quantized form's hits; where it differs from orc_run it differs by
+ * a near-tie between two faces (their distances a few ulps apart). */
+int orc_run_quant(int pass, const lb_node *nodes, size_t n_nodes, const uint32_t *quant,
+                  const lb_tri *tris, const float *materials, const float *charts,
+                  const lb_params *params, const lb_texel *items, size_t n, uint32_t key_base,
+                  const float *light, float *out, orc_counters *cnt) {
+    if (!quant) return 1;
+    return run_ctx(pass, nodes, n_nodes, quant, tris, materials, charts, params, items, n,
+                   key_base, light, out, cnt, NULL, NULL);
+}
+
+/* Not the reference: orc_run (its results in `out`) with every ray also traced over `quant`;
+ * counts[0..2]: rays, rays the two node forms end differently, and those among them that are no
+ * near-tie (see ctx_t.cmp). */
+int orc_quant_compare(int pass, const lb_node *nodes, size_t n_nodes, const uint32_t *quant,
+                      const lb_tri *tris, const float *materials, const float *charts,
+                      const lb_params *params, const lb_texel *items, size_t n,
+                      uint32_t key_base, const float *light, float *out,
+                      unsigned long long counts[3]) {
+    if (!quant || !counts) return 1;
+    counts[0] = counts[1] = counts[2] = 0;
+    return run_ctx(pass, nodes, n_nodes, NULL, tris, materials, charts, params, items, n,
+                   key_base, light, out, NULL, counts, quant);
+}
+
 /* What each gather ray's light depends on (trace.wgsl:248-268), for the adjoint's checker: per
  * item i and ray r, at (i * rays + r) * 2: a word (kind 0: the atlas texel bounced() reads,
  * trace.wgsl:137-145; kind 3: the bits of the direction's y; else 0) and a tag (kind << 30 |
@@ -489,7 +606,7 @@ int orc_gather_records(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
     ctx_t c0;
     c0.nodes = empty_fix(nodes, n_nodes, &tmp); c0.tris = tris; c0.materials = materials;
     c0.charts = charts ? charts : zero_chart;
-    c0.light = NULL;
+    c0.light = NULL; c0.quant = NULL; c0.cmp = NULL; c0.cmp_quant = NULL;
     c0.P = params; c0.cnt = NULL;
     uint32_t rays = params->counts[2];
     long nn = (long)n;

@@ -167,3 +167,24 @@ def test_unranked_gather_beyond_256_rays(gpu):
     sess.close()
     err = np.abs(light - wl) / (np.abs(wl) + 1e-3)
     assert err.max() < 1e-4, err.max()
+
+
+def test_occlusion_contrast_against_oracle(gpu):
+    """aoContrast 2.2 (bake.rs:238's powf): the device's powf and the host's (glibc's, as Rust's
+    f32::powf) may round one value a unit in the last place apart, so the occlusion is held
+    within 1e-6 relative; the light, which the occlusion does not enter, bit for bit."""
+    scene = reference_test_scene(8.0)
+    scene.settings.ao_contrast = 2.2
+    print("bake parity: occlusion contrast 2.2", flush=True)
+    sess = BakeSession(scene, gpu, form=N.FORM_PLAIN, graph=False)
+    sess.execute()
+    sess.finish()
+    want = ref.bake(oracle_inputs(scene, sess))
+    light = sess.array(N.BAKE_LIGHT, np.float32).reshape(-1, 4)
+    assert np.array_equal(light.view(np.uint32), want["light"].reshape(-1, 4).view(np.uint32))
+    ao = sess.tile_cache().ao
+    own = want["known"] != 0
+    got, w = ao[: len(own)][own], want["ao"][own]
+    assert np.abs(got - w).max() <= 1e-6 * np.abs(w).max(), np.abs(got - w).max()
+    assert (w < 0.999).sum() > 100  # the contrast is exercised
+    sess.close()

@@ -85,12 +85,18 @@ def tile_streams(scene):
             v = pos[g].reshape(-1, 3)
             lo, hi = np.minimum(lo, v.min(0)), np.maximum(hi, v.max(0))
         key = (int(L.tiles[t, 0]), int(L.tiles[t, 1]))
-        out.append((np.concatenate([lo, hi]), D.tile_seed_stream(key, charts, meshes)))
+        dens = (scene.tile_density or {}).get(key)
+        out.append((np.concatenate([lo, hi]), D.tile_seed_stream(key, charts, meshes, dens)))
     return out, meshes
 
 
-def test_tile_hashes_match_the_streams(gpu):
+@pytest.mark.parametrize("own_density", [False, True], ids=["one-density", "tile-density"])
+def test_tile_hashes_match_the_streams(gpu, own_density):
     scene = reference_test_scene(8.0)
+    if own_density:
+        # one tile laid out at a density of its own: its seed ends with the density
+        L = layout(scene)
+        scene.tile_density = {(int(L.tiles[1, 0]), int(L.tiles[1, 1])): 6.0}
     s = scene.settings
     g = deps.global_hash(scene)
     got = deps.tile_hashes(gpu, scene, g)
@@ -174,3 +180,20 @@ def test_rebake_from_the_dirty_flags(gpu):
               probe_dirty=pd)
     assert b1.info["tiles_baked"] == d.sum() and b1.info["probes_baked"] == pd.sum()
     assert np.array_equal(b1.cache.probes[~pd], b0.cache.probes[~pd])
+
+
+def test_tile_density_change_dirties_its_tile_only(gpu):
+    """A tile given a density of its own is rebaked (its hash changes); the others keep theirs:
+    the density moves no casting triangle."""
+    a = reference_test_scene(8.0)
+    h0 = deps.tile_hashes(gpu, a)
+    L = layout(a)
+    key = (int(L.tiles[2, 0]), int(L.tiles[2, 1]))
+    b = reference_test_scene(8.0)
+    b.tile_density = {key: 5.0}
+    d = deps.dirty(h0, deps.tile_hashes(gpu, b))
+    assert list(np.nonzero(d)[0]) == [2]
+    # a listed density equal to the scene's is still a change of the tile's inputs
+    c = reference_test_scene(8.0)
+    c.tile_density = {key: float(c.settings.texels_per_meter)}
+    assert list(np.nonzero(deps.dirty(h0, deps.tile_hashes(gpu, c)))[0]) == [2]

@@ -10,9 +10,21 @@ from lightbaker_amd.bake import layout
 from lightbaker_amd.scenes import map_scene, reference_test_scene, small_scene
 
 
-@pytest.fixture(scope="module", params=["ref", "small"])
+def dense_small():
+    """The small scene with per-tile densities of 2, 8 and 16 texels/m beside its own 4."""
+    s = small_scene(1)
+    s.tile_density = {(-1, -1): 2.0, (0, -1): 8.0, (-1, 0): 16.0}
+    return s
+
+
+def density_of(s, kx, kz):
+    return (s.tile_density or {}).get((int(kx), int(kz)), s.settings.texels_per_meter)
+
+
+@pytest.fixture(scope="module", params=["ref", "small", "tile-density"])
 def laid(request):
-    s = reference_test_scene(8.0) if request.param == "ref" else small_scene(1)
+    s = {"ref": lambda: reference_test_scene(8.0), "small": lambda: small_scene(1),
+         "tile-density": dense_small}[request.param]()
     return s, layout(s)
 
 
@@ -71,8 +83,8 @@ def test_texels_lie_on_their_chart_and_map_back_to_themselves(laid):
             assert abs(u - (x0 + x + 0.5)) <= 0.5 + 1e-3 and abs(v - (y0 + y + 0.5)) <= 0.5 + 1e-3
             n = tx["normal"][:3]
             assert abs(np.linalg.norm(n) - 1.0) < 1e-5
-            # dx, dy step one texel: 1 / density metres, in the face's plane
-            assert abs(np.linalg.norm(tx["dx"][:3]) - 1.0 / s.settings.texels_per_meter) < 1e-3
+            # dx, dy step one texel: 1 / density metres (the tile's), in the face's plane
+            assert abs(np.linalg.norm(tx["dx"][:3]) - 1.0 / density_of(s, kx, kz)) < 1e-3
             assert abs(float(np.dot(tx["dx"][:3], n))) < 1e-4
 
 
@@ -100,3 +112,47 @@ def test_map_scene_layout():
     L = layout(s)
     assert len(L.tiles) == 16  # 100 m in tiles of 25 m
     assert L.width == 4096 and 0 < L.height <= 4096
+
+
+def test_tile_density_default_is_the_reference_layout():
+    """No per-tile density, an empty one, or every tile listed at the scene's own density: the
+    same layout byte for byte (the reference's single density, layout.rs:89-115)."""
+    base = layout(small_scene(1))
+    s = small_scene(1)
+    s.tile_density = {}
+    same = layout(s)
+    assert np.array_equal(same.affines, base.affines) and np.array_equal(same.tiles, base.tiles)
+    s.tile_density = {(int(t[0]), int(t[1])): 4.0 for t in base.tiles}
+    same = layout(s)
+    assert np.array_equal(same.affines, base.affines) and np.array_equal(same.tiles, base.tiles)
+    assert np.array_equal(same.tri_charts, base.tri_charts)
+
+
+def test_tile_density_scales_its_tile_only():
+    """A tile at 16 texels/m against the scene's 4: each chart's affine rows (a texel a step of
+    1 / density metres) have the tile's density as their length, the charts being the same; its
+    texel count grows about 16-fold; a tile left at the scene's density is laid out as before
+    (the packing may turn a chart another way, swapping its rows, when the sizes change)."""
+    base = layout(small_scene(1))
+    s = dense_small()
+    L = layout(s)
+    assert np.array_equal(L.tri_charts, base.tri_charts)  # the same charts, grown the same way
+    for c in range(len(L.affines)):
+        kx, kz = L.tiles[L.chart_tiles[c]][:2]
+        norms = np.linalg.norm(L.affines[c][:, :3].astype(np.float64), axis=1)
+        assert np.allclose(norms, density_of(s, kx, kz), rtol=1e-5)
+        if (int(kx), int(kz)) == (0, 0):
+            assert np.array_equal(L.affines[c][:, :3], base.affines[c][:, :3])
+    tex = {tuple(t[:2]): int(t[2]) * int(t[3]) for t in L.tiles}
+    tex0 = {tuple(t[:2]): int(t[2]) * int(t[3]) for t in base.tiles}
+    assert 8 < tex[(-1, 0)] / tex0[(-1, 0)] < 20 and 0.1 < tex[(-1, -1)] / tex0[(-1, -1)] < 0.5
+    assert tex[(0, 0)] == tex0[(0, 0)]
+
+
+@pytest.mark.parametrize("bad,msg", [({(0, 0): 0.0}, "not positive"),
+                                     ({(0, 0): float("nan")}, "not positive")])
+def test_tile_density_rejects_bad_values(bad, msg):
+    s = small_scene(1)
+    s.tile_density = bad
+    with pytest.raises(N.NativeError, match=msg):
+        layout(s)

@@ -5,9 +5,11 @@ against the oracle, on the bake's own scene records and its final light atlas.
 Bar: on sampled direct and gather items and volume probes, the PLAIN form equals the oracle
 (oracle/trace_ref.c) bit for bit; the TUNED form with 8-bit quantized nodes and the settings the
 benchmark's autotuner picks on this map gives every hit-derived quantity exactly and every sum
-within 2e-6 relative. The TUNED form's Kahan-compensated gather sums are at least as close to the
-exact sum of the same per-ray terms (oracle.gather_terms, summed in float64) as the PLAIN form's
-naive FP32 sums: summation error alone, the hits being equal.
+within 2e-6 relative, against the oracle traversing the same 8-bit node records (run_quant).
+The TUNED form's Kahan-compensated gather sums (over FP32 node records, so its hits are the
+reference's) are at least as close to the exact sum of the same per-ray terms
+(oracle.gather_terms, summed in float64) as the PLAIN form's naive FP32 sums: summation error
+alone, the hits being equal.
 """
 from __future__ import annotations
 
@@ -62,6 +64,7 @@ def baked(gpu):
     probes = probe_items(scene)
     s["probes"] = probes[np.sort(rng.choice(len(probes), 800, replace=False))]
     s["sb"] = gpu.scene(Bvh(s["nodes"], s["tris"]), s["mats"], s["charts"])
+    s["quant"] = oracle.quant_records(s["nodes"])
     print(f"map: {len(cov)} covered texels, {len(s['gathered'])} gather items, "
           f"{len(probes)} probes", flush=True)
     yield s
@@ -76,7 +79,10 @@ def _params(s, pass_):
     return s["vparams"] if pass_ == Pass.VOLUME else s["params"]
 
 
-def _oracle(s, pass_):
+def _oracle(s, pass_, quant=False):
+    if quant:
+        return oracle.run_quant(int(pass_), s["nodes"], s["quant"], s["tris"], s["mats"],
+                                s["charts"], _params(s, pass_), _items(s, pass_), 0, s["light"])
     return oracle.run(int(pass_), s["nodes"], s["tris"], s["mats"], s["charts"], _params(s, pass_),
                       _items(s, pass_), 0, s["light"])
 
@@ -99,7 +105,7 @@ def test_map_tuned_quantized_bench_settings(gpu, baked, pass_):
     gpu.set_tuning(pass_=pass_, **BENCH_TUNING[pass_])
     gpu.set_form(N.FORM_TUNED)
     got = gpu.run(pass_, baked["sb"], _params(baked, pass_), items, baked["light"])
-    _compare_tuned(pass_, got, _oracle(baked, pass_), True)
+    _compare_tuned(pass_, got, _oracle(baked, pass_, quant=True), True)
 
 
 def test_map_kahan_sums_no_worse_than_naive(gpu, baked):
@@ -107,7 +113,7 @@ def test_map_kahan_sums_no_worse_than_naive(gpu, baked):
     print(f"map gather sums: {len(items)} items x 256 rays", flush=True)
     gpu.set_form(N.FORM_PLAIN)
     plain = gpu.run(Pass.GATHER, baked["sb"], baked["params"], items, baked["light"])[:, :3]
-    gpu.set_tuning(pass_=Pass.GATHER, **BENCH_TUNING[Pass.GATHER])
+    gpu.set_tuning(pass_=Pass.GATHER, **dict(BENCH_TUNING[Pass.GATHER], nodes=N.NODES_WIDE))
     gpu.set_form(N.FORM_TUNED)
     tuned = gpu.run(Pass.GATHER, baked["sb"], baked["params"], items, baked["light"])[:, :3]
     terms = oracle.gather_terms(baked["nodes"], baked["tris"], baked["mats"], baked["charts"],

@@ -403,3 +403,42 @@ def test_gather_set_partial_block_and_ds1():
     # ds = 1: every valid texel
     valid[7] = False
     assert np.array_equal(ref.gather_set(tw, tw * th, valid, ch, 1), valid)
+
+
+# ---- the whole-bake oracle's array forms equal the pinned scalar ones ---------------------------
+
+@pytest.mark.parametrize("contrast", [1.0, 2.2])
+def test_decode_all_equals_decode(contrast):
+    rng = np.random.default_rng(31)
+    n = 4000
+    st = Settings(ao_contrast=contrast, ao_strength=1.3)
+    tex = np.zeros(n, N.TEXEL)
+    for f in ("pos", "normal", "dx", "dy"):
+        tex[f][:, :3] = rng.normal(size=(n, 3)).astype(np.float32)
+    pick = rng.integers(0, 10, n).astype(np.uint32)
+    hard = rng.integers(0, 1 << 16, n).astype(np.uint32)
+    res = np.stack([rng.random(n), (hard | pick << 24).view(np.float32), rng.random(n),
+                    rng.choice([0.0, st.inside_share, 0.3, 0.9], n)], 1).astype(np.float32)
+    cov = np.sort(rng.choice(n + 50, n, replace=False))  # a sparse subset of a longer list
+    big = np.zeros(n + 50, N.TEXEL)
+    big[cov] = tex
+    a, b = big.copy(), big.copy()
+    inside, soft, bits, state, ao = ref.decode_all(res, a, cov, st)
+    for k, gi in enumerate(cov):
+        d = ref.decode(res[k], b[gi], st)
+        assert (d is None) == inside[k]
+        if d is not None:
+            assert (d[0], d[1], d[2]) == (soft[k], bits[k], state[k]) and d[3] == ao[k], k
+    assert np.array_equal(a["pos"].view(np.uint32), b["pos"].view(np.uint32))
+
+
+@pytest.mark.parametrize("ds", [1, 2, 4])
+def test_gather_set_all_equals_gather_set(ds):
+    rng = np.random.default_rng(ds)
+    for tw, th in ((13, 9), (16, 16), (5, 11)):
+        size = tw * th
+        valid = rng.random(size) > 0.3
+        chart = rng.integers(0, 4, size).astype(np.uint32)
+        chart[rng.random(size) < 0.1] = NONE
+        assert np.array_equal(ref.gather_set_all(tw, size, valid, chart, ds),
+                              ref.gather_set(tw, size, valid, chart, ds))

@@ -77,3 +77,72 @@ def test_quantized_boxes_contain_their_fp32_boxes(gen):
     assert not quant[nodes["count"] > 0].any()
     print(f"{len(inner)} inner records, slack {float((slack / step).max()):.2f} steps at most",
           flush=True)
+
+
+# ---- how the two node layouts' hits differ, on the CPU ------------------------------------------
+
+def exact_hit(tri, o, d):
+    """The ray o + t d against the stored triangle in float64: (hit, t)."""
+    v0, e1, e2 = (tri[k][:3].astype(np.float64) for k in ("v0", "e1", "e2"))
+    a = np.stack([-d, e1, e2], 1)
+    if abs(np.linalg.det(a)) < 1e-300:
+        return False, np.inf
+    t, u, v = np.linalg.solve(a, o - v0)
+    return bool(u >= 0 and v >= 0 and u + v <= 1 and t > 0), float(t)
+
+
+def _scene_items(scene, tiles=None):
+    from lightbaker_amd.bake import layout
+    from lightbaker_amd.gpu import Bvh
+    from lightbaker_amd.scenes import make_params
+    L = layout(scene)
+    pos, _, mat = scene.flat()
+    bvh = Bvh.build(pos, mat, L.tri_charts, scene.casts())
+    tiles = range(len(L.tiles)) if tiles is None else tiles
+    tex = np.concatenate([L.rasterize(t).reshape(-1) for t in tiles])
+    items = np.ascontiguousarray(tex[tex["pos"][:, 3] > 0.5])
+    light = np.random.default_rng(1).random((L.width * L.height, 4)).astype(np.float32)
+    light[:, 3] = 1.0
+    return bvh, items, make_params(scene, L.width, L.height), L.affines.reshape(-1, 8), light
+
+
+@pytest.mark.parametrize("which", ["small", "rebake-2-tiles"])
+def test_quantized_hits_differ_only_by_near_ties_and_grazing_false_positives(which):
+    """Every ray of the direct and gather passes traced over both node layouts
+    (oracle.quant_compare). The 8-bit boxes contain the FP32 boxes, but the traversal visits
+    children in the order of their entry distances and prunes a child whose entry lies beyond
+    the nearest hit so far, and both read different boxes; so where a ray's two candidate hits
+    lie a few ulps apart (a box's bottom face on the ground), the layouts may keep different
+    ones (a near-tie, distances within 2^-20 t). Beyond near-ties, a ray may differ only where
+    the larger boxes visit a leaf the FP32 boxes cull and the FP32 Moller-Trumbore test
+    (trace.wgsl:94-104) accepts a ray that misses the triangle exactly (a grazing ray, its
+    determinant near zero): the hard difference must be such a false positive, never a hit the
+    FP32 traversal finds and the 8-bit one loses."""
+    import oracle
+    from lightbaker_amd.scenes import rebake_scene
+    if which == "small":
+        scene, tiles = small_scene(1), None
+    else:
+        scene, tiles = rebake_scene(3), [51, 52]
+    bvh, items, P, charts, light = _scene_items(scene, tiles)
+    q = oracle.quant_records(bvh.nodes)
+    mats = scene.materials_array()
+    total = [0, 0, 0]
+    for pass_, it in ((0, items), (1, items[::4])):
+        _, cnt = oracle.quant_compare(pass_, bvh.nodes, q, bvh.tris, mats, charts, P, it, 0, light)
+        total = [a + b for a, b in zip(total, cnt)]
+        ex = oracle.quant_examples()[: min(cnt[2], 16)]
+        assert cnt[2] <= 16, f"pass {pass_}: {cnt[2]} hard differences"
+        for row in ex:
+            o, d = row[0:3].astype(np.float64), row[3:6].astype(np.float64)
+            tf, tq = float(row[8]), float(row[9])
+            trf, trq = (int(v) for v in row[10:12].view(np.uint32))
+            # the hit one layout kept nearer than anything the other kept
+            extra = trq if tq < tf else trf
+            assert extra != 0xFFFFFFFF, row
+            hit, _ = exact_hit(bvh.tris[extra], o, d)
+            assert not hit, f"pass {pass_}: triangle {extra} is a true hit one layout lost: {row}"
+    rays, differ, hard = total
+    print(f"{which}: {rays} rays, {differ} differ ({differ / rays:.2e}), {hard} beyond a near-tie",
+          flush=True)
+    assert differ <= 1e-4 * rays
