@@ -51,27 +51,62 @@ def node_bytes(stats: dict, form: int, nodes_layout: int) -> int:
     return nb + stats["tri_tests"] * 48 + stats["material_loads"] * 16 + stats["light_gathers"] * 48
 
 
-def measured_traffic(kernel: str) -> dict | None:
-    """HBM bytes a launch of `kernel` moves, as measured by rocprofv3 (FETCH_SIZE and WRITE_SIZE,
-    scaled by what the counters read for a copy of known size) and recorded with the command
-    that measured them in profiles/gather_map_scene_traffic.json; None when the record is for
-    another kernel."""
+def kernel_record(kernel: str) -> dict | None:
+    """What rocprofv3 measured for a launch of `kernel` on the map's gather pass, recorded with
+    the commands that measured it in profiles/gather_map_scene_roofline.json: HBM bytes
+    (FETCH_SIZE and WRITE_SIZE, scaled by what the counters read for a copy of known size), VALU
+    and SALU wave instructions, the launch's GPU cycles (GRBM_GUI_ACTIVE per XCD) and SIMDs; None
+    when no record names the kernel."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "gather_map_scene_traffic.json")
+                        "gather_map_scene_roofline.json")
     try:
         with open(path) as f:
-            rec = json.load(f)
-    except (OSError, ValueError):
+            recs = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
         return None
-    return rec if rec.get("kernel") == kernel else None
+    return recs.get(kernel)
 
 
-def cpu_baseline(sess, gpu, target_s: float = 12.0, max_items: int = 500000) -> dict:
+def roofline(kernel: str, ms: float, algorithmic_bytes: float, record: dict | None) -> dict:
+    """The roofline position of one launch of the dominant kernel.
+
+    HBM: the bytes rocprofv3 measured a launch moving (FETCH + WRITE, calibrated), over the launch
+    time measured here with HIP events, against the 8 TB/s peak. Instruction issue: the VALU wave
+    instructions the counters measured, over the launch's cycles times the SIMDs (each SIMD
+    issues at most one VALU wave instruction a cycle), from the profiled run alone. The
+    traversal's algorithmic bytes (every node, triangle, material and light read a ray makes, by
+    the kernel's own counters) are reported beside them: most are served by L2 and the MALL, so
+    their rate is no HBM figure."""
+    out = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": None, "traffic": None, "kernel": kernel, "ms_per_launch": ms,
+           "algorithmic": {"bytes_per_launch": algorithmic_bytes,
+                           "gbs": algorithmic_bytes / (ms / 1e3) / 1e9,
+                           "note": "demand on the memory system, mostly L2 and MALL hits"},
+           "issue": None, "binding": None, "source": None}
+    if record:
+        hbm = record["hbm_bytes"]
+        out["traffic"] = hbm
+        out["achieved"] = hbm / (ms / 1e3) / 1e9
+        out["frac"] = out["achieved"] / HBM_PEAK_GBS
+        cycles = record["gui_active_cycles"]
+        issue = record["valu_insts"] / (cycles * record["simds"])
+        out["issue"] = {"unit": "VALU wave instructions a SIMD cycle", "achieved": issue,
+                        "peak": 1.0, "frac": issue,
+                        "salu_per_valu": record["salu_insts"] / record["valu_insts"],
+                        "lanes_active": record.get("lanes_active"),
+                        "ms_profiled": record.get("ms_profiled")}
+        out["binding"] = "instruction issue" if issue > out["frac"] else "hbm"
+        out["source"] = "profiles/gather_map_scene_roofline.md"
+    return out
+
+
+def cpu_baseline(sess, gpu, volume_rays: int, target_s: float = 12.0,
+                 max_items: int = 500000) -> dict:
     """The CPU leg: the oracle (oracle/trace_ref.c, FP32, OpenMP over items) timed on a sample of
     the gather pass, and the error of the PLAIN form and of the TUNED form with each node layout
     against the oracle's FP64 tracer on identical rays (gather: the gather items, a random
-    `max_items` of them where there are more; direct: 200 000 random covered texels), with the
-    final light atlas."""
+    `max_items` of them where there are more; direct: 200 000 random covered texels; volume:
+    20 000 random probes), with the final light atlas."""
     import oracle
     from lightbaker_amd import _native as N
     from lightbaker_amd.gpu import Bvh, Pass
@@ -110,17 +145,17 @@ def cpu_baseline(sess, gpu, target_s: float = 12.0, max_items: int = 500000) -> 
     # error against FP64 on identical rays; the key of an item is its index in the pass
     sb = gpu.scene(Bvh(nodes, tris), mats, charts)
 
-    def forms(pass_, its, lt):
+    def forms(pass_, its, lt, prm=params):
         out = {}
         gpu.set_form(N.FORM_PLAIN)
-        out["plain"] = gpu.run(pass_, sb, params, its, lt)
+        out["plain"] = gpu.run(pass_, sb, prm, its, lt)
         gpu.set_form(N.FORM_WATERTIGHT)
-        out["plain_watertight"] = gpu.run(pass_, sb, params, its, lt)
+        out["plain_watertight"] = gpu.run(pass_, sb, prm, its, lt)
         gpu.set_form(N.FORM_TUNED)
         keep = gpu.tuning(pass_)["nodes"]
         for name, layout in (("tuned_quant", N.NODES_QUANT), ("tuned_wide", N.NODES_WIDE)):
             gpu.set_tuning(pass_=pass_, nodes=layout)
-            out[name] = gpu.run(pass_, sb, params, its, lt)
+            out[name] = gpu.run(pass_, sb, prm, its, lt)
         gpu.set_tuning(pass_=pass_, nodes=keep)
         gpu.set_form(N.FORM_PLAIN)
         return out
@@ -159,6 +194,29 @@ def cpu_baseline(sess, gpu, target_s: float = 12.0, max_items: int = 500000) -> 
     ref = oracle.run64(0, nodes, tris, mats, charts, params, d_items, 0, None)
     acc["direct_occlusion"] = dict(errors(got, ref, [2], True), items=len(d_items))
     acc["direct_soft_sun"] = dict(errors(got, ref, [0], True), items=len(d_items))
+    # the volume (trace.wgsl:272-324): a sample of probes, each keyed by its index in the grid
+    # (x fastest, then z, then y: volume.rs:39), its L0 and L1 radiance sums and its sun share
+    inf = sess.info()
+    nx, ny, nz = inf["volume_size"]
+    if nx * ny * nz:
+        k = np.sort(rng.permutation(nx * ny * nz)[:20000])
+        f32 = np.float32
+        o, step = np.asarray(inf["volume_origin"], f32), f32(inf["volume_step"])
+        p_items = np.zeros(len(k), N.TEXEL)
+        p_items["pos"][:, 0] = o[0] + (k % nx).astype(f32) * step
+        p_items["pos"][:, 1] = o[1] + (k // (nx * nz)).astype(f32) * step
+        p_items["pos"][:, 2] = o[2] + ((k // nx) % nz).astype(f32) * step
+        p_items["pos"][:, 3] = 1.0
+        p_items["normal"][:, 1] = 1.0
+        p_items["normal"][:, 3] = k.astype(np.uint32).view(f32)
+        pv = params.copy()
+        pv["counts"][0][2] = volume_rays
+        got = {n: v.reshape(-1, 16) for n, v in forms(Pass.VOLUME, p_items, light, pv).items()}
+        ref = oracle.run64(2, nodes, tris, mats, charts, pv, p_items, 0, light).reshape(-1, 16)
+        acc["volume_l0"] = dict(errors(got, ref, [0, 1, 2]), items=len(p_items))
+        acc["volume_l1"] = dict(errors(got, ref, [4, 5, 6, 8, 9, 10, 12, 13, 14]),
+                                items=len(p_items))
+        acc["volume_sun"] = dict(errors(got, ref, [3]), items=len(p_items))
     sb.close()
     base["error_vs_fp64"] = acc
 
@@ -367,28 +425,41 @@ def main() -> int:
     g = stats[1]
     per_launch = {k: v / bounces for k, v in g.items()}
     bytes_launch = node_bytes(per_launch, form, layout) + info["gather_items"] * (64 + 16)
-    achieved = bytes_launch / (ms_gather / 1e3) / 1e9
     tuning = gpu.tuning()
     if form == N.FORM_PLAIN:
-        kernel = "k_gather_plain<false>"
+        kernel = "k_gather_plain<false, false>"
     elif tuning["schedule"] == 0:
         kernel = f"k_gather_tuned<{layout}, 0, {tuning['waves']}>"
     else:
         kernel = f"k_pool<0, {layout}, false>"
     # the recorded counters are the map's gather launch
-    traffic = measured_traffic(kernel) if args.workload == "map" else None
+    roof = roofline(kernel, ms_gather, bytes_launch,
+                    kernel_record(kernel) if args.workload == "map" else None)
     print(f"# gather launch {ms_gather:.2f} ms, {bytes_launch / 1e9:.3f} GB algorithmic, "
           f"{per_launch['rays'] / ms_gather / 1e6:.2f} Grays/s", file=sys.stderr, flush=True)
 
-    plain_ms = None
+    # the PLAIN form on the same bake: the median of 5 steps after 2 warm-ups, and its gather
+    # launch's roofline
+    plain_ms = plain_roof = plain_steps = None
     if not args.no_plain and form == N.FORM_TUNED:
         ps = BakeSession(scene, gpu, form=N.FORM_PLAIN, graph=True, **rebake)
-        ps.execute()
-        gpu.sync()
-        t0 = time.perf_counter()
-        ps.execute()
-        gpu.sync()
-        plain_ms = (time.perf_counter() - t0) * 1000.0
+        pstats = ps.execute(stats=True)
+        for _ in range(2):
+            ps.execute()
+        plain_steps = []
+        for _ in range(5):
+            gpu.sync()
+            t0 = time.perf_counter()
+            ps.execute()
+            gpu.sync()
+            plain_steps.append((time.perf_counter() - t0) * 1000.0)
+        plain_ms = float(np.median(plain_steps))
+        pms = ps.time_pass(Pass.GATHER, reps=5)
+        pl = {k: v / bounces for k, v in pstats[1].items()}
+        pbytes = node_bytes(pl, N.FORM_PLAIN, layout) + info["gather_items"] * (64 + 16)
+        pk = "k_gather_plain<false, false>"
+        plain_roof = roofline(pk, pms, pbytes,
+                              kernel_record(pk) if args.workload == "map" else None)
         ps.close()
 
     # the same bake with the later bounces replaying the first bounce's ray records (the same
@@ -406,7 +477,8 @@ def main() -> int:
         rs.close()
         print(f"# replayed bounces: {replay_ms:.2f} ms a step", file=sys.stderr, flush=True)
 
-    cpu = None if args.no_cpu else cpu_baseline(sess, gpu)
+    cpu = None if args.no_cpu else cpu_baseline(
+        sess, gpu, scene.volume.rays if scene.volume is not None else 1)
     dep = deps_timing(gpu, scene)
     if args.workload == "map":
         metric = "Grays/s, 100x100 m 3-bounce bake (direct + 3 gather bounces + volume)"
@@ -458,6 +530,8 @@ def main() -> int:
             "tiles_baked": info["tiles_baked"],
             "ms_per_tile": ms_step / max(info["tiles_baked"], 1),
             "plain_ms_per_step": plain_ms,
+            "plain_ms_steps": plain_steps,
+            "plain_roofline": plain_roof,
             "tuned_vs_plain": (plain_ms / ms_step) if plain_ms else None,
             "replay_ms_per_step": replay_ms,
             "replay_rays_traced_per_step": (stats[0]["rays"] + stats[1]["rays"] // bounces
@@ -465,12 +539,7 @@ def main() -> int:
             "replay_vs_plain": (plain_ms / replay_ms) if plain_ms and replay_ms else None,
             "deps": dep,
         },
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic["hbm_bytes"] if traffic else None,
-                     "kernel": kernel, "ms_per_launch": ms_gather,
-                     "bytes_per_launch": bytes_launch,
-                     "traffic_source": traffic["source"] if traffic else None},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
