@@ -109,6 +109,14 @@ int lb_scene_create(lb_gpu *gpu, const lb_node *nodes, size_t n_nodes, const lb_
                     size_t n_materials, const float *charts /* [n_charts][2][4] */,
                     size_t n_charts, lb_scene **out);
 void lb_scene_destroy(lb_scene *scene);
+/* No counterpart in the reference (its sky is zenith, horizon and ground, trace.wgsl:63-66): an
+ * L1 spherical-harmonic term added to the sky the passes see where a ray escapes, per channel
+ * 0.282095 c0 + 0.488603 ((c1 d.x + c2 d.y) + c3 d.z), the coefficients as the volume's probes
+ * order them (L0, L1 x, y, z): sh[12] = c0 rgb, c1 rgb, c2 rgb, c3 rgb; NULL or all zero: the
+ * reference's sky, bit for bit. The term is not clamped (the caller keeps it non-negative). The
+ * fallback irradiance in lb_params (bounced() off the charts) is the caller's: the bake adds the
+ * term's irradiance on an upward face, pi 0.282095 c0 + 2 pi / 3 0.488603 c2. */
+int lb_scene_set_sky(lb_scene *scene, const float *sh);
 
 /* Gpu::run (gpu/mod.rs:140-196): runs `pass` over `n` items (texels, or probes for the volume)
  * reading `light` (rgba per atlas texel; none becomes one zero texel), writes n * width vec4s in
@@ -292,6 +300,8 @@ typedef struct lb_scene_desc {
      * density joins its dependency hash (lb_deps_tile_hashes) only when it is listed. */
     const lb_tile_density *tile_density;
     size_t n_tile_density;
+    /* the sky's L1 term (lb_scene_set_sky): c0 rgb, c1 rgb, c2 rgb, c3 rgb; zero: the reference */
+    float sky_sh[12];
 } lb_scene_desc;
 
 typedef struct lb_bvh lb_bvh;
@@ -508,9 +518,11 @@ int lb_deps_probe_hashes(lb_gpu *gpu, const lb_scene_desc *scene, const uint8_t 
  * scene's light to a target lightmap (inverse lighting: albedos, sky and sun from a bake). The
  * light atlas after the bounces (bake.rs:300-330: the atlas the last bounce writes, direct +
  * indirect per known texel) is a function of the materials' albedo, the sky (trace.wgsl:63-66:
- * zenith, horizon, ground; the fallback irradiance pi (zenith + horizon) / 2 follows them) and
- * the sun's irradiance (bake.rs:247-252), with visibility held fixed: every gather ray's hit is
- * recorded once, and each forward replays those records through the bounces. */
+ * zenith, horizon, ground, plus the L1 spherical-harmonic term of lb_scene_set_sky; the fallback
+ * irradiance pi (zenith + horizon) / 2 + pi 0.282095 c0 + 2 pi / 3 0.488603 c2 follows them)
+ * and the sun's irradiance (bake.rs:247-252), with visibility held fixed: every gather ray's hit
+ * is recorded once, and each forward replays those records through the bounces. Every sum the
+ * backward takes is FP32 with exact compensation (no FP64 on the device). */
 
 typedef struct lb_adjoint lb_adjoint;
 
@@ -519,6 +531,7 @@ typedef struct lb_light_params {
     size_t n_materials;    /* the scene's */
     float zenith[3], horizon[3], ground[3];
     float sun[3];          /* irradiance */
+    float sky_sh[12];      /* the sky's L1 term (lb_scene_set_sky): c0 rgb, c1 rgb, c2 rgb, c3 rgb */
 } lb_light_params;
 
 /* After lb_bake_execute: records the gather rays' hits (the bake must outlive the adjoint).
@@ -528,7 +541,7 @@ typedef struct lb_light_params {
  * face, 3 the sky). */
 int lb_adjoint_create(lb_bake *bake, lb_adjoint **out);
 void lb_adjoint_destroy(lb_adjoint *adj);
-/* atlas texels, parameters (n_materials * 3 + 12), gather items, rays per item */
+/* atlas texels, parameters (n_materials * 3 + 24), gather items, rays per item */
 int lb_adjoint_info(const lb_adjoint *adj, size_t *atlas_texels, size_t *n_params, size_t *items,
                     size_t *rays);
 /* The records, copied to `out` (host, cap u32 words). */
@@ -539,7 +552,8 @@ int lb_adjoint_records(const lb_adjoint *adj, uint32_t *out, size_t cap);
 int lb_adjoint_forward(lb_adjoint *adj, const lb_light_params *params, float *light);
 /* The gradient of a loss through the last forward: `dlight` (device, atlas_texels x 4 f32, the
  * loss's derivative by each atlas texel's rgb; w unused) in, `grad` (device, n_params f32) out:
- * albedo rgb per material, then zenith, horizon, ground and sun rgb. */
+ * albedo rgb per material, then zenith, horizon, ground rgb, the sky's L1 coefficients c0..c3
+ * rgb (as lb_light_params.sky_sh), and sun rgb. */
 int lb_adjoint_backward(lb_adjoint *adj, const float *dlight, float *grad);
 
 #ifdef __cplusplus

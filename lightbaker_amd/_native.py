@@ -75,6 +75,7 @@ class SceneDesc(C.Structure):
         ("has_volume", C.c_uint32), ("volume_min", C.c_float * 3), ("volume_max", C.c_float * 3),
         ("volume_step", C.c_float), ("volume_rays", C.c_uint32),
         ("tile_density", C.POINTER(TileDensity)), ("n_tile_density", C.c_size_t),
+        ("sky_sh", C.c_float * 12),
     ]
 
 
@@ -113,7 +114,7 @@ class LightParams(C.Structure):
     """lb_light_params: the parameters the adjoint differentiates by."""
     _fields_ = [("albedo", C.c_void_p), ("n_materials", C.c_size_t),
                 ("zenith", C.c_float * 3), ("horizon", C.c_float * 3), ("ground", C.c_float * 3),
-                ("sun", C.c_float * 3)]
+                ("sun", C.c_float * 3), ("sky_sh", C.c_float * 12)]
 
 
 BAKE_DIRECT, BAKE_INDIRECT, BAKE_VALIDITY, BAKE_VOLUME, BAKE_CHARTS = 0, 1, 2, 3, 4
@@ -133,6 +134,7 @@ SIGNATURES = {
     "lb_last_error": (C.c_char_p, []),
     "lb_scene_create": (_I, [_VP, _VP, _SZ, _VP, _SZ, _VP, _SZ, _VP, _SZ, C.POINTER(_VP)]),
     "lb_scene_destroy": (None, [_VP]),
+    "lb_scene_set_sky": (_I, [_VP, _VP]),
     "lb_run": (_I, [_VP, _I, _VP, _VP, _VP, _SZ, _VP, _SZ, _VP, _SZ]),
     "lb_buffer_alloc": (_I, [_VP, _SZ, C.POINTER(_VP)]),
     "lb_buffer_free": (_I, [_VP, _VP]),

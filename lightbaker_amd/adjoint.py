@@ -41,7 +41,7 @@ class Adjoint:
         assert self.shape[0] * self.shape[1] == atlas.value
         self.n_params = params.value
         self.items, self.rays = items.value, rays.value
-        self.materials = (self.n_params - 12) // 3
+        self.materials = (self.n_params - 24) // 3
         self.forwards = 0  # forwards run; the backward differentiates the last
 
     def records(self) -> np.ndarray:
@@ -50,7 +50,7 @@ class Adjoint:
         N.check(self.lib.lb_adjoint_records(self.handle, out.ctypes.data, out.size))
         return out
 
-    def _params(self, albedo, sky, sun):
+    def _params(self, albedo, sky, sun, sky_sh=None):
         alb = np.ascontiguousarray(np.asarray(albedo, np.float32).reshape(-1, 3))
         sky = np.asarray(sky, np.float32).reshape(3, 3)
         p = N.LightParams()
@@ -60,12 +60,14 @@ class Adjoint:
         p.horizon[:] = [float(v) for v in sky[1]]
         p.ground[:] = [float(v) for v in sky[2]]
         p.sun[:] = [float(v) for v in np.asarray(sun, np.float32).reshape(3)]
+        if sky_sh is not None:
+            p.sky_sh[:] = [float(v) for v in np.asarray(sky_sh, np.float32).reshape(12)]
         return p, alb
 
-    def forward(self, albedo, sky, sun) -> torch.Tensor:
+    def forward(self, albedo, sky, sun, sky_sh=None) -> torch.Tensor:
         """The final light atlas, (ah, aw, 4) float32 on the GPU; sky rows zenith, horizon,
-        ground."""
-        p, keep = self._params(albedo, sky, sun)
+        ground; sky_sh (4, 3) the sky's L1 coefficients c0..c3 (None: zero)."""
+        p, keep = self._params(albedo, sky, sun, sky_sh)
         light = torch.empty(self.shape, dtype=torch.float32, device="cuda")
         torch.cuda.synchronize()
         N.check(self.lib.lb_adjoint_forward(self.handle, C.byref(p), C.c_void_p(light.data_ptr())))
@@ -74,8 +76,9 @@ class Adjoint:
         return light
 
     def backward(self, dlight: torch.Tensor) -> torch.Tensor:
-        """The gradient by (albedo rgb per material, zenith, horizon, ground, sun) of a loss whose
-        derivative by the last forward's atlas is `dlight` ((ah, aw, 4), rgb used)."""
+        """The gradient by (albedo rgb per material, zenith, horizon, ground, the sky's L1
+        coefficients c0..c3, sun; rgb each) of a loss whose derivative by the last forward's atlas
+        is `dlight` ((ah, aw, 4), rgb used)."""
         d = dlight.detach().to(device="cuda", dtype=torch.float32).contiguous()
         if tuple(d.shape) != self.shape:
             raise ValueError(f"dlight shape {tuple(d.shape)}, the atlas is {self.shape}")
@@ -98,43 +101,48 @@ class Adjoint:
 
 
 class BakedLight(torch.autograd.Function):
-    """light = bake(albedo (M, 3), sky (3, 3), sun (3,)) with visibility fixed."""
+    """light = bake(albedo (M, 3), sky (3, 3), sun (3,), sky_sh (4, 3)) with visibility fixed."""
 
     @staticmethod
-    def forward(ctx, adj: Adjoint, albedo, sky, sun):
-        light = adj.forward(albedo.detach().cpu().numpy(), sky.detach().cpu().numpy(),
-                            sun.detach().cpu().numpy())
+    def forward(ctx, adj: Adjoint, albedo, sky, sun, sky_sh):
+        args = [t.detach().cpu().numpy() for t in (albedo, sky, sun, sky_sh)]
+        light = adj.forward(*args)
         ctx.adj = adj
         ctx.forward_id = adj.forwards
-        ctx.save_for_backward(albedo.detach().clone(), sky.detach().clone(), sun.detach().clone())
+        ctx.save_for_backward(*(t.detach().clone() for t in (albedo, sky, sun, sky_sh)))
         return light
 
     @staticmethod
     def backward(ctx, grad_light):
         adj: Adjoint = ctx.adj
-        albedo, sky, sun = ctx.saved_tensors
+        albedo, sky, sun, sky_sh = ctx.saved_tensors
         if adj.forwards != ctx.forward_id:
             # another forward ran since: restore this one's bounces before differentiating
-            adj.forward(albedo.cpu().numpy(), sky.cpu().numpy(), sun.cpu().numpy())
+            adj.forward(*(t.cpu().numpy() for t in (albedo, sky, sun, sky_sh)))
             ctx.forward_id = adj.forwards
         g = adj.backward(grad_light)
         M = albedo.shape[0]
         ga = g[:M * 3].view(M, 3).to(albedo.device)
         gs = g[M * 3:M * 3 + 9].view(3, 3).to(sky.device)
-        gu = g[M * 3 + 9:].view(3).to(sun.device)
-        return None, ga, gs, gu
+        gh = g[M * 3 + 9:M * 3 + 21].view(4, 3).to(sky_sh.device)
+        gu = g[M * 3 + 21:].view(3).to(sun.device)
+        return None, ga, gs, gu, gh
 
 
-def baked_light(adj: Adjoint, albedo: torch.Tensor, sky: torch.Tensor,
-                sun: torch.Tensor) -> torch.Tensor:
-    return BakedLight.apply(adj, albedo, sky, sun)
+def baked_light(adj: Adjoint, albedo: torch.Tensor, sky: torch.Tensor, sun: torch.Tensor,
+                sky_sh: torch.Tensor | None = None) -> torch.Tensor:
+    if sky_sh is None:
+        sky_sh = torch.zeros(4, 3, dtype=torch.float32)
+    return BakedLight.apply(adj, albedo, sky, sun, sky_sh)
 
 
 def scene_light_params(scene):
-    """A scene's own (albedo (M, 3), sky (3, 3), sun (3,)) as float32 arrays."""
+    """A scene's own (albedo (M, 3), sky (3, 3), sun (3,), sky_sh (4, 3)) as float32 arrays."""
     alb = np.array([m.albedo for m in scene.materials], np.float32)
     sky = np.array([scene.zenith, scene.horizon, scene.ground], np.float32)
-    return alb, sky, np.asarray(scene.sun_irradiance, np.float32)
+    sh = (np.zeros((4, 3), np.float32) if scene.sky_sh is None
+          else np.asarray(scene.sky_sh, np.float32).reshape(4, 3))
+    return alb, sky, np.asarray(scene.sun_irradiance, np.float32), sh
 
 
 def l2_loss(light: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
@@ -143,18 +151,25 @@ def l2_loss(light: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
     return (((light[..., :3] - target[..., :3]) ** 2) * w).sum()
 
 
-def fit(adj: Adjoint, target: torch.Tensor, albedo0, sky0, sun0, steps: int = 50,
-        lr: float = 0.02, fit_sun: bool = False, log=None) -> dict:
-    """Adam on the L2 loss to `target` over albedo and sky (and the sun's irradiance when
-    `fit_sun`), visibility fixed; albedo kept in [0, 1], sky and sun non-negative."""
+def fit(adj: Adjoint, target: torch.Tensor, albedo0, sky0, sun0, sky_sh0=None, steps: int = 50,
+        lr: float = 0.02, fit_sun: bool = False, fit_sky: str = "gradient", log=None) -> dict:
+    """Adam on the L2 loss to `target`, visibility fixed, over the albedo (kept in [0, 1]) and the
+    sky: `fit_sky` "gradient" (zenith, horizon, ground, kept non-negative), "sh" (the L1
+    coefficients c0..c3) or "both"; and the sun's irradiance when `fit_sun`."""
     albedo = torch.tensor(np.asarray(albedo0, np.float32), requires_grad=True)
-    sky = torch.tensor(np.asarray(sky0, np.float32), requires_grad=True)
+    grad_sky = fit_sky in ("gradient", "both")
+    sh_sky = fit_sky in ("sh", "both")
+    sky = torch.tensor(np.asarray(sky0, np.float32), requires_grad=grad_sky)
+    sh0 = np.zeros((4, 3), np.float32) if sky_sh0 is None else np.asarray(sky_sh0, np.float32)
+    sky_sh = torch.tensor(sh0.reshape(4, 3), requires_grad=sh_sky)
     sun = torch.tensor(np.asarray(sun0, np.float32), requires_grad=fit_sun)
-    opt = torch.optim.Adam([albedo, sky] + ([sun] if fit_sun else []), lr=lr)
+    params = ([albedo] + ([sky] if grad_sky else []) + ([sky_sh] if sh_sky else [])
+              + ([sun] if fit_sun else []))
+    opt = torch.optim.Adam(params, lr=lr)
     history = []
     for k in range(steps):
         opt.zero_grad()
-        loss = l2_loss(baked_light(adj, albedo, sky, sun), target)
+        loss = l2_loss(baked_light(adj, albedo, sky, sun, sky_sh), target)
         loss.backward()
         opt.step()
         with torch.no_grad():
@@ -165,4 +180,4 @@ def fit(adj: Adjoint, target: torch.Tensor, albedo0, sky0, sun0, steps: int = 50
         if log:
             log(f"step {k}: loss {history[-1]:.6g}")
     return {"albedo": albedo.detach().numpy(), "sky": sky.detach().numpy(),
-            "sun": sun.detach().numpy(), "loss": history}
+            "sky_sh": sky_sh.detach().numpy(), "sun": sun.detach().numpy(), "loss": history}

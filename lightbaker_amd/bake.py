@@ -128,6 +128,8 @@ def scene_desc(scene: Scene):
         d.volume_max[:] = list(scene.volume.max)
         d.volume_step = scene.volume.step
         d.volume_rays = scene.volume.rays
+    if scene.sky_sh is not None:
+        d.sky_sh[:] = [float(v) for v in np.asarray(scene.sky_sh, np.float32).reshape(12)]
     if scene.tile_density:
         dens = (N.TileDensity * len(scene.tile_density))()
         for i, ((x, z), v) in enumerate(sorted(scene.tile_density.items())):

@@ -14,8 +14,12 @@
 // and the sun's share through the direct light (linear in the irradiance: its derivative is the
 // direct pass's filled cos x sun share).
 //
-// Sums of the backward are in FP64 where they run over many terms (atlas and parameter
-// accumulators); the forward replay repeats the gather's FP32 arithmetic in the reference's order.
+// The backward's sums run over many terms (an atlas texel's adjoint gathers every ray that read
+// it; a parameter's, every ray and texel) and are FP32 with exact compensation: each sum is a
+// pair (hi, lo), every addition into hi by an atomic that returns the value it added to, so the
+// addition's rounding error is known exactly (Knuth's TwoSum) and goes into lo; block partials and
+// the final reduction add pairs the same way. No FP64 on the device. The forward replay repeats the
+// gather's FP32 arithmetic in the reference's order.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -76,8 +80,7 @@ __global__ __launch_bounds__(BLOCK) void k_replay(const uint2 *__restrict__ rec,
         uint2 q = rec[(size_t)r * n + g];
         uint32_t kind = q.y >> 30;
         if (kind == REC_MISS) {
-            float y = bitsf(q.x);
-            sum = add(sum, y < 0.0f ? K.ground : add(scl(K.horizon, 1.0f - y), scl(K.zenith, y)));
+            sum = add(sum, replay_sky(K, q.x, g, r, rays));
             continue;
         }
         if (kind == REC_BACK) {
@@ -225,10 +228,36 @@ __global__ void k_smooth_t(TexelView V, const uint8_t *__restrict__ valid, int r
     dout[j] = make_float4(acc.x, acc.y, acc.z, 0.0f);
 }
 
+// The rounding error of a + b when s = fl(a + b) (Knuth's TwoSum: exact in round-to-nearest)
+__device__ __forceinline__ float two_sum_err(float a, float b, float s) {
+    float bb = s - a;
+    return (a - (s - bb)) + (b - bb);
+}
+
+// Adds x to the compensated sum (hi, lo) at one address: the atomic returns the value it added
+// to, so the addition's rounding error is known exactly and joins lo.
+__device__ __forceinline__ void comp_add(float *hi, float *lo, float x) {
+    if (x == 0.0f) return;
+    float old = atomicAdd(hi, x);
+    float e = two_sum_err(old, x, old + x);
+    if (e != 0.0f) atomicAdd(lo, e);
+}
+
+// Neumaier's running sum in a register pair
+struct Comp {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float x) {
+        float t = s + x;
+        c += two_sum_err(s, x, t);
+        s = t;
+    }
+};
+
 // write_light's transpose (bake.rs:435-444): a known texel's direct and indirect each get the
-// adjoint of its atlas texel; `di` (when given) is set, `dd` accumulates.
-__global__ void k_light_t(TexelView V, const double *__restrict__ dl, uint32_t aw,
-                          float4 *__restrict__ di, float4 *__restrict__ dd, uint32_t n) {
+// adjoint of its atlas texel (the pair hi + lo); `di` (when given) is set, `dd` accumulates.
+__global__ void k_light_t(TexelView V, const float *__restrict__ dl_hi,
+                          const float *__restrict__ dl_lo, uint32_t aw, float4 *__restrict__ di,
+                          float4 *__restrict__ dd, uint32_t n) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     f3 v = mk(0.0f, 0.0f, 0.0f);
@@ -236,25 +265,29 @@ __global__ void k_light_t(TexelView V, const double *__restrict__ dl, uint32_t a
         Where p = where(V, g);
         uint32_t corner = V.tiles[p.tile].w;
         size_t a = (size_t)((corner >> 16) + p.y) * aw + (corner & 0xffffu) + p.x;
-        v = mk((float)dl[a * 3], (float)dl[a * 3 + 1], (float)dl[a * 3 + 2]);
+        v = mk(dl_hi[a * 3] + dl_lo[a * 3], dl_hi[a * 3 + 1] + dl_lo[a * 3 + 1],
+               dl_hi[a * 3 + 2] + dl_lo[a * 3 + 2]);
     }
     if (di) di[g] = make_float4(v.x, v.y, v.z, 0.0f);
     float4 o = dd[g];
     dd[g] = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, 0.0f);
 }
 
-__global__ void k_to_f64(const float4 *__restrict__ src, double *__restrict__ dst, uint32_t n) {
+// the loss's adjoint by the atlas as a compensated pair: hi its rgb, lo zero
+__global__ void k_seed(const float4 *__restrict__ src, float *__restrict__ hi,
+                       float *__restrict__ lo, uint32_t n) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
     float4 v = src[a];
-    dst[(size_t)a * 3] = v.x;
-    dst[(size_t)a * 3 + 1] = v.y;
-    dst[(size_t)a * 3 + 2] = v.z;
+    hi[(size_t)a * 3] = v.x;
+    hi[(size_t)a * 3 + 1] = v.y;
+    hi[(size_t)a * 3 + 2] = v.z;
+    lo[(size_t)a * 3] = lo[(size_t)a * 3 + 1] = lo[(size_t)a * 3 + 2] = 0.0f;
 }
 
-__global__ void k_zero_f64(double *p, size_t n) {
+__global__ void k_zero_f32(float *p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0;
+    if (i < n) p[i] = 0.0f;
 }
 
 __global__ void k_zero_f4(float4 *p, uint32_t n) {
@@ -264,38 +297,54 @@ __global__ void k_zero_f4(float4 *p, uint32_t n) {
 
 // The gather's transpose for one bounce. Item g's result E = pi / rays * sum over its rays; with
 // dE its adjoint, each ray hands on
-//   sky:        pi / rays * dE * (y, 1 - y) to zenith and horizon (y >= 0), to ground (y < 0)
+//   sky:        pi / rays * dE * (y, 1 - y) to zenith and horizon (y >= 0), to ground (y < 0), and
+//               pi / rays * dE * (0.282095, 0.488603 d) to the sky's L1 coefficients (c0..c3)
 //   atlas hit:  dE * light / rays to its material's albedo, dE * albedo / rays to the texel
-//   fallback:   dE * fallback / rays to the albedo, dE * albedo / rays * pi / 2 to zenith and
-//               horizon (fallback = pi (zenith + horizon) / 2)
-// Parameter sums go to this block's row of `part` (albedo rgb per material, then zenith, horizon,
-// ground): LDS accumulators in FP64, added to the row, so the rows of several bounces add up.
+//   fallback:   dE * fallback / rays to the albedo, dE * albedo / rays * dfallback to zenith and
+//               horizon (pi / 2 each: fallback = pi (zenith + horizon) / 2) and to c0 and c2 (the
+//               L1 term's irradiance on an upward face)
+// Parameter sums go to this block's row of `part` as (hi, lo) pairs, slots albedo rgb per
+// material, then zenith, horizon, ground, c0..c3 rgb (width slots); the rows of several bounces
+// add up. LDS holds the block's pairs; each item's sky and fallback terms are summed in
+// registers first (Neumaier), the albedo and atlas terms go straight to compensated atomics.
+constexpr uint32_t SKY_SLOTS = 21;  // zenith, horizon, ground, c0, c1, c2, c3 (rgb each)
 __global__ __launch_bounds__(BLOCK) void k_gather_t(
     const uint2 *__restrict__ rec, uint32_t n, uint32_t rays, const uint32_t *__restrict__ gat,
     const float4 *__restrict__ de, const float4 *__restrict__ albedo,
-    const float4 *__restrict__ light, SkyP K, double *__restrict__ dl, double *__restrict__ part,
-    uint32_t width) {
-    extern __shared__ double acc[];
-    for (uint32_t k = threadIdx.x; k < width; k += BLOCK) acc[k] = 0.0;
+    const float4 *__restrict__ light, SkyP K, float *__restrict__ dl_hi,
+    float *__restrict__ dl_lo, float *__restrict__ part, uint32_t width) {
+    extern __shared__ float acc[];  // width hi, then width lo
+    for (uint32_t k = threadIdx.x; k < 2u * width; k += BLOCK) acc[k] = 0.0f;
     __syncthreads();
+    float *ahi = acc, *alo = acc + width;
     uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
     if (g < n) {
         f3 dE = f3of(de[gat[g]]);
         float inv = 1.0f / (float)rays;
         float c = PI / (float)rays;
-        f3 gz = mk(0, 0, 0), gh = mk(0, 0, 0), gg = mk(0, 0, 0);
-        uint32_t M3 = width - 9u;
+        Comp sk[SKY_SLOTS];
+        uint32_t M3 = width - SKY_SLOTS;
         for (uint32_t r = 0; r < rays; r++) {
             uint2 q = rec[(size_t)r * n + g];
             uint32_t kind = q.y >> 30;
             if (kind == REC_BACK) continue;
             if (kind == REC_MISS) {
                 float y = bitsf(q.x);
-                if (y < 0.0f) {
-                    gg = add(gg, scl(dE, c));
-                } else {
-                    gz = add(gz, scl(dE, c * y));
-                    gh = add(gh, scl(dE, c * (1.0f - y)));
+                float dv[3] = {dE.x * c, dE.y * c, dE.z * c};
+                for (int k = 0; k < 3; k++) {
+                    if (y < 0.0f) {
+                        sk[6 + k].add(dv[k]);
+                    } else {
+                        sk[k].add(dv[k] * y);
+                        sk[3 + k].add(dv[k] * (1.0f - y));
+                    }
+                }
+                {  // the L1 coefficients' share (their derivative at any value, 0 included)
+                    f3 d = gather_dir(K.items, g, r, rays, K.seed);
+                    float basis[4] = {0.282095f, 0.488603f * d.x, 0.488603f * d.y,
+                                      0.488603f * d.z};
+                    for (int j = 0; j < 4; j++)
+                        for (int k = 0; k < 3; k++) sk[9 + 3 * j + k].add(dv[k] * basis[j]);
                 }
                 continue;
             }
@@ -303,78 +352,94 @@ __global__ __launch_bounds__(BLOCK) void k_gather_t(
             f3 a = f3of(albedo[m]);
             f3 da = scl(dE, inv);  // dE / rays
             bool lit = false;
+            f3 ga;
             if (kind == REC_ATLAS) {
                 float4 l = light[q.x];
                 if (l.w > 0.5f) {
                     lit = true;
-                    f3 e = f3of(l);
-                    f3 ga = mul(da, e);
-                    atomicAdd(&acc[m * 3u], (double)ga.x);
-                    atomicAdd(&acc[m * 3u + 1u], (double)ga.y);
-                    atomicAdd(&acc[m * 3u + 2u], (double)ga.z);
+                    ga = mul(da, f3of(l));
                     f3 gl = mul(da, a);
-                    unsafeAtomicAdd(&dl[(size_t)q.x * 3], (double)gl.x);
-                    unsafeAtomicAdd(&dl[(size_t)q.x * 3 + 1], (double)gl.y);
-                    unsafeAtomicAdd(&dl[(size_t)q.x * 3 + 2], (double)gl.z);
+                    comp_add(&dl_hi[(size_t)q.x * 3], &dl_lo[(size_t)q.x * 3], gl.x);
+                    comp_add(&dl_hi[(size_t)q.x * 3 + 1], &dl_lo[(size_t)q.x * 3 + 1], gl.y);
+                    comp_add(&dl_hi[(size_t)q.x * 3 + 2], &dl_lo[(size_t)q.x * 3 + 2], gl.z);
                 }
             }
             if (!lit) {
-                f3 ga = mul(da, K.fallback);
-                atomicAdd(&acc[m * 3u], (double)ga.x);
-                atomicAdd(&acc[m * 3u + 1u], (double)ga.y);
-                atomicAdd(&acc[m * 3u + 2u], (double)ga.z);
-                f3 gf = scl(mul(da, a), 0.5f * PI);
-                gz = add(gz, gf);
-                gh = add(gh, gf);
+                ga = mul(da, K.fallback);
+                f3 w = mul(da, a);
+                float wv[3] = {w.x, w.y, w.z};
+                for (int k = 0; k < 3; k++) {
+                    sk[k].add(wv[k] * (0.5f * PI));
+                    sk[3 + k].add(wv[k] * (0.5f * PI));
+                    sk[9 + k].add(wv[k] * SH_FALLBACK_C0);
+                    sk[15 + k].add(wv[k] * SH_FALLBACK_C2);
+                }
             }
+            comp_add(&ahi[m * 3u], &alo[m * 3u], ga.x);
+            comp_add(&ahi[m * 3u + 1u], &alo[m * 3u + 1u], ga.y);
+            comp_add(&ahi[m * 3u + 2u], &alo[m * 3u + 2u], ga.z);
         }
-        for (int k = 0; k < 3; k++) {
-            float z = k == 0 ? gz.x : (k == 1 ? gz.y : gz.z);
-            float h = k == 0 ? gh.x : (k == 1 ? gh.y : gh.z);
-            float q = k == 0 ? gg.x : (k == 1 ? gg.y : gg.z);
-            atomicAdd(&acc[M3 + k], (double)z);
-            atomicAdd(&acc[M3 + 3 + k], (double)h);
-            atomicAdd(&acc[M3 + 6 + k], (double)q);
+        for (uint32_t k = 0; k < SKY_SLOTS; k++) {
+            comp_add(&ahi[M3 + k], &alo[M3 + k], sk[k].s);
+            comp_add(&ahi[M3 + k], &alo[M3 + k], sk[k].c);
         }
     }
     __syncthreads();
-    double *row = part + (size_t)blockIdx.x * width;
-    for (uint32_t k = threadIdx.x; k < width; k += BLOCK) row[k] += acc[k];
+    // the block's pairs into its row (this kernel alone writes the row, bounce after bounce)
+    float *row = part + (size_t)blockIdx.x * width * 2u;
+    for (uint32_t k = threadIdx.x; k < width; k += BLOCK) {
+        float h = row[2u * k], s = h + ahi[k];
+        row[2u * k] = s;
+        row[2u * k + 1u] += two_sum_err(h, ahi[k], s) + alo[k];
+    }
 }
 
-// The sun's share: sum over texels of dD . G (G the filled cos x sun share), a block's partial
-// sums of the three channels in its row of `part` (a fixed-order tree in FP64).
+// The sun's share: sum over texels of dD . G (G the filled cos x sun share), each block's three
+// channels as (hi, lo) pairs in its row of `part`: a tree of compensated pair additions.
 __global__ __launch_bounds__(BLOCK) void k_sun_t(const float4 *__restrict__ dd,
                                                  const float4 *__restrict__ G, uint32_t n,
-                                                 double *__restrict__ part) {
-    __shared__ double s[3][BLOCK];
+                                                 float *__restrict__ part) {
+    __shared__ float hs[3][BLOCK], ls[3][BLOCK];
     uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
-    double v[3] = {0.0, 0.0, 0.0};
+    float v[3] = {0.0f, 0.0f, 0.0f};
     if (g < n) {
         float4 d = dd[g];
-        double k = G[g].x;
+        float k = G[g].x;
         v[0] = d.x * k;
         v[1] = d.y * k;
         v[2] = d.z * k;
     }
-    for (int c = 0; c < 3; c++) s[c][threadIdx.x] = v[c];
+    for (int c = 0; c < 3; c++) {
+        hs[c][threadIdx.x] = v[c];
+        ls[c][threadIdx.x] = 0.0f;
+    }
     __syncthreads();
     for (uint32_t h = BLOCK / 2; h > 0; h >>= 1) {
         if (threadIdx.x < h)
-            for (int c = 0; c < 3; c++) s[c][threadIdx.x] += s[c][threadIdx.x + h];
+            for (int c = 0; c < 3; c++) {
+                float a = hs[c][threadIdx.x], b = hs[c][threadIdx.x + h], s = a + b;
+                hs[c][threadIdx.x] = s;
+                ls[c][threadIdx.x] += ls[c][threadIdx.x + h] + two_sum_err(a, b, s);
+            }
         __syncthreads();
     }
-    if (threadIdx.x < 3) part[(size_t)blockIdx.x * 3 + threadIdx.x] = s[threadIdx.x][0];
+    if (threadIdx.x < 3) {
+        part[((size_t)blockIdx.x * 3 + threadIdx.x) * 2] = hs[threadIdx.x][0];
+        part[((size_t)blockIdx.x * 3 + threadIdx.x) * 2 + 1] = ls[threadIdx.x][0];
+    }
 }
 
-// out[k] = sum over rows of part[row * width + k], rows in order
-__global__ void k_rows(const double *__restrict__ part, uint32_t rows, uint32_t width,
+// out[k] = the sum over rows of the pairs part[row][k], rows in order, compensated
+__global__ void k_rows(const float *__restrict__ part, uint32_t rows, uint32_t width,
                        float *__restrict__ out) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= width) return;
-    double s = 0.0;
-    for (uint32_t b = 0; b < rows; b++) s += part[(size_t)b * width + k];
-    out[k] = (float)s;
+    Comp s;
+    for (uint32_t b = 0; b < rows; b++) {
+        s.add(part[((size_t)b * width + k) * 2]);
+        s.c += part[((size_t)b * width + k) * 2 + 1];
+    }
+    out[k] = s.s + s.c;
 }
 
 inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
@@ -429,8 +494,8 @@ int adjoint_create(lb_bake *B, lb_adjoint *A) {
     A->T = (uint32_t)B->T;
     A->atlas = (size_t)B->aw * B->ah;
     if (B->grings >= (uint32_t)LEVEL_NEVER) return fail("lb_adjoint_create: %u fill rings", B->grings);
-    uint32_t width = A->M * 3u + 9u;
-    if ((size_t)width * sizeof(double) > 64 * 1024)
+    uint32_t width = A->M * 3u + SKY_SLOTS;
+    if ((size_t)width * 2u * sizeof(float) > 64 * 1024)
         return fail("lb_adjoint_create: %u materials exceed the LDS accumulators", A->M);
     uint64_t nrays = (uint64_t)A->ng * A->rays;
     if (nrays >= (1ull << 40)) return fail("lb_adjoint_create: %llu rays", (unsigned long long)nrays);
@@ -446,7 +511,7 @@ int adjoint_create(lb_bake *B, lb_adjoint *A) {
     A->blocks_g = (uint32_t)((Gn + BLOCK - 1) / BLOCK);
     A->blocks_s = (uint32_t)((Tn + BLOCK - 1) / BLOCK);
     if (A->part_g.reserve((size_t)A->blocks_g * width * 8) || A->part_s.reserve((size_t)A->blocks_s * 24))
-        return 1;
+        return 1;  // (hi, lo) FP32 pairs: 8 B a slot
     const TexelView &V = B->V;
 
     // the records (the gather items are the bake's, resident since its execute)
@@ -493,10 +558,17 @@ int adjoint_forward(lb_adjoint *A, const lb_light_params &p, float *light) {
     K.zenith = f3{p.zenith[0], p.zenith[1], p.zenith[2]};
     K.horizon = f3{p.horizon[0], p.horizon[1], p.horizon[2]};
     K.ground = f3{p.ground[0], p.ground[1], p.ground[2]};
-    // bake.rs:106-116: fallback = (zenith + horizon) * 0.5 * pi, in that order
-    K.fallback = f3{(p.zenith[0] + p.horizon[0]) * 0.5f * PIF,
-                    (p.zenith[1] + p.horizon[1]) * 0.5f * PIF,
-                    (p.zenith[2] + p.horizon[2]) * 0.5f * PIF};
+    // bake.rs:106-116: fallback = (zenith + horizon) * 0.5 * pi, in that order; then the sky's L1
+    // term's irradiance on an upward face, as the bake's prepare adds it
+    float fb[3];
+    for (int c = 0; c < 3; c++) {
+        fb[c] = (p.zenith[c] + p.horizon[c]) * 0.5f * PIF;
+        fb[c] += SH_FALLBACK_C0 * p.sky_sh[c] + SH_FALLBACK_C2 * p.sky_sh[6 + c];
+    }
+    K.fallback = f3{fb[0], fb[1], fb[2]};
+    K.sh = sky_sh_of(p.sky_sh);
+    K.items = (const lb_texel *)B->d_gitems.ptr;
+    std::memcpy(&K.seed, &B->P.reach[3], 4);
     const TexelView &V0 = B->V;
     TexelView V = view_with(A, (float4 *)A->D.ptr, (float4 *)A->I.ptr);
     uint32_t T = A->T;
@@ -547,19 +619,19 @@ int adjoint_backward(lb_adjoint *A, const float *dlight, float *grad) {
     hipStream_t st = B->gpu->stream;
     hipSetDevice(B->gpu->device);
     const TexelView &V = B->V;
-    uint32_t T = A->T, width = A->M * 3u + 9u;
-    double *dl = (double *)A->dl.ptr;
+    uint32_t T = A->T, width = A->M * 3u + SKY_SLOTS;
+    float *dl_hi = (float *)A->dl.ptr, *dl_lo = dl_hi + A->atlas * 3;
     float4 *dI = (float4 *)A->dI.ptr, *dI2 = (float4 *)A->dI2.ptr, *dD = (float4 *)A->dD.ptr;
-    size_t pg = (size_t)A->blocks_g * width, ps = (size_t)A->blocks_s * 3;
-    hipLaunchKernelGGL(k_zero_f64, grid_for(pg), dim3(BLOCK), 0, st, (double *)A->part_g.ptr, pg);
+    size_t pg = (size_t)A->blocks_g * width * 2, ps = (size_t)A->blocks_s * 3 * 2;
+    hipLaunchKernelGGL(k_zero_f32, grid_for(pg), dim3(BLOCK), 0, st, (float *)A->part_g.ptr, pg);
     if (T) hipLaunchKernelGGL(k_zero_f4, grid_for(T), dim3(BLOCK), 0, st, dD, T);
-    hipLaunchKernelGGL(k_to_f64, grid_for(A->atlas), dim3(BLOCK), 0, st, (const float4 *)dlight,
-                       dl, (uint32_t)A->atlas);
+    hipLaunchKernelGGL(k_seed, grid_for(A->atlas), dim3(BLOCK), 0, st, (const float4 *)dlight,
+                       dl_hi, dl_lo, (uint32_t)A->atlas);
     uint32_t last = A->ng ? A->bounces : 0;
     for (uint32_t k = last; k >= 1; k--) {
         if (T) {
-            hipLaunchKernelGGL(k_light_t, grid_for(T), dim3(BLOCK), 0, st, V, (const double *)dl,
-                               B->aw, dI, dD, T);
+            hipLaunchKernelGGL(k_light_t, grid_for(T), dim3(BLOCK), 0, st, V,
+                               (const float *)dl_hi, (const float *)dl_lo, B->aw, dI, dD, T);
             for (int r = (int)B->grings - 1; r >= 0; r--) {
                 hipLaunchKernelGGL(k_fill_t, grid_for(T), dim3(BLOCK), 0, st, V,
                                    (const int8_t *)A->level.ptr, (const uint8_t *)A->count.ptr, r,
@@ -572,24 +644,25 @@ int adjoint_backward(lb_adjoint *A, const float *dlight, float *grad) {
                                (const float4 *)dI, dI2, T);
             std::swap(dI, dI2);
         }
-        hipLaunchKernelGGL(k_zero_f64, grid_for(A->atlas * 3), dim3(BLOCK), 0, st, dl, A->atlas * 3);
-        hipLaunchKernelGGL(k_gather_t, dim3(A->blocks_g), dim3(BLOCK), width * sizeof(double), st,
-                           (const uint2 *)A->rec.ptr, A->ng, A->rays,
-                           (const uint32_t *)B->d_gat.ptr, (const float4 *)dI,
+        hipLaunchKernelGGL(k_zero_f32, grid_for(A->atlas * 6), dim3(BLOCK), 0, st, dl_hi,
+                           A->atlas * 6);
+        hipLaunchKernelGGL(k_gather_t, dim3(A->blocks_g), dim3(BLOCK),
+                           width * 2u * sizeof(float), st, (const uint2 *)A->rec.ptr, A->ng,
+                           A->rays, (const uint32_t *)B->d_gat.ptr, (const float4 *)dI,
                            (const float4 *)A->albedo.ptr, (const float4 *)A->ck(k - 1), A->sky,
-                           dl, (double *)A->part_g.ptr, width);
+                           dl_hi, dl_lo, (float *)A->part_g.ptr, width);
     }
     if (T) {
-        hipLaunchKernelGGL(k_light_t, grid_for(T), dim3(BLOCK), 0, st, V, (const double *)dl,
-                           B->aw, (float4 *)nullptr, dD, T);
+        hipLaunchKernelGGL(k_light_t, grid_for(T), dim3(BLOCK), 0, st, V, (const float *)dl_hi,
+                           (const float *)dl_lo, B->aw, (float4 *)nullptr, dD, T);
         hipLaunchKernelGGL(k_sun_t, dim3(A->blocks_s), dim3(BLOCK), 0, st, (const float4 *)dD,
-                           (const float4 *)A->G.ptr, T, (double *)A->part_s.ptr);
+                           (const float4 *)A->G.ptr, T, (float *)A->part_s.ptr);
     } else {
-        hipLaunchKernelGGL(k_zero_f64, grid_for(ps), dim3(BLOCK), 0, st, (double *)A->part_s.ptr, ps);
+        hipLaunchKernelGGL(k_zero_f32, grid_for(ps), dim3(BLOCK), 0, st, (float *)A->part_s.ptr, ps);
     }
-    hipLaunchKernelGGL(k_rows, grid_for(width), dim3(BLOCK), 0, st, (const double *)A->part_g.ptr,
+    hipLaunchKernelGGL(k_rows, grid_for(width), dim3(BLOCK), 0, st, (const float *)A->part_g.ptr,
                        A->blocks_g, width, grad);
-    hipLaunchKernelGGL(k_rows, dim3(1), dim3(BLOCK), 0, st, (const double *)A->part_s.ptr,
+    hipLaunchKernelGGL(k_rows, dim3(1), dim3(BLOCK), 0, st, (const float *)A->part_s.ptr,
                        T ? A->blocks_s : 1u, 3u, grad + width);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -626,7 +699,7 @@ int lb_adjoint_info(const lb_adjoint *adj, size_t *atlas_texels, size_t *n_param
                     size_t *rays) {
     if (!adj) return fail("lb_adjoint_info: null handle");
     if (atlas_texels) *atlas_texels = adj->atlas;
-    if (n_params) *n_params = (size_t)adj->M * 3 + 12;
+    if (n_params) *n_params = (size_t)adj->M * 3 + lbk::SKY_SLOTS + 3;
     if (items) *items = adj->ng;
     if (rays) *rays = adj->rays;
     return 0;

@@ -158,7 +158,8 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     I.s_bvh = secs(t0);
     if (lb_scene_create(gpu, B->nodes.data(), B->nodes.size(), B->tris.data(), B->tris.size(),
                         B->mats.data(), sc.n_materials, n_charts ? B->charts.data() : nullptr,
-                        n_charts, &B->scene))
+                        n_charts, &B->scene) ||
+        lb_scene_set_sky(B->scene, sc.sky_sh))
         return 1;
 
     // Params (bake.rs:105-120)
@@ -178,6 +179,9 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
         P.horizon[c] = sc.horizon[c];
         P.ground[c] = sc.ground[c];
         P.fallback[c] = (sc.zenith[c] + sc.horizon[c]) * 0.5f * PIF;
+        // the sky's L1 term (no counterpart in the reference): its irradiance on an upward face,
+        // pi a0 c0 + 2 pi / 3 a1 c2 (the volume's cosine lobe, volume.rs:125-147); 0 without one
+        P.fallback[c] += SH_FALLBACK_C0 * sc.sky_sh[c] + SH_FALLBACK_C2 * sc.sky_sh[6 + c];
     }
     P.ground[3] = std::fmin(INSIDE_REACH, sc.ao_reach);
     P.counts[0] = g;
@@ -518,7 +522,9 @@ int bounce(lb_bake *B, lb_stats *acc, uint32_t k) {
         // the rays of the first bounce, met again: their records with this bounce's atlas
         hipError_t e = replay_gather((const uint2 *)B->d_rec.ptr, (uint32_t)B->ng,
                                      B->P.counts[2], (const float4 *)B->scene->materials.ptr,
-                                     (const float4 *)B->d_light.ptr, sky_of(B->P),
+                                     (const float4 *)B->d_light.ptr,
+                                     sky_of(B->P, sky_sh_of(B->scene->sky_sh),
+                                            (const lb_texel *)B->d_gitems.ptr),
                                      (float4 *)B->d_gres.ptr, st);
         if (e != hipSuccess) return hip_fail(e, "replay_gather");
     } else if (replay) {

@@ -2,6 +2,7 @@
 // on one HIP device, plus the device-resident extensions.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -183,6 +184,16 @@ int lb_scene_create(lb_gpu *gpu, const lb_node *nodes, size_t n_nodes, const lb_
         return 1;
     }
     *out = s;
+    return 0;
+}
+
+int lb_scene_set_sky(lb_scene *scene, const float *sh) {
+    if (!scene) return fail("lb_scene_set_sky: null handle");
+    for (int k = 0; k < 12; k++) {
+        float v = sh ? sh[k] : 0.0f;
+        if (!std::isfinite(v)) return fail("lb_scene_set_sky: coefficient %d is not finite", k);
+        scene->sky_sh[k] = v;
+    }
     return 0;
 }
 
@@ -465,6 +476,16 @@ void DevBuf::release() {
     cap = 0;
 }
 
+SkySH sky_sh_of(const float c[12]) {
+    SkySH K;
+    K.r = make_float4(c[0], c[3], c[6], c[9]);
+    K.g = make_float4(c[1], c[4], c[7], c[10]);
+    K.b = make_float4(c[2], c[5], c[8], c[11]);
+    K.on = 0;
+    for (int k = 0; k < 12; k++) K.on |= c[k] != 0.0f;
+    return K;
+}
+
 SceneView scene_view(const lb_scene *s, const float *light) {
     SceneView v;
     v.nodes = static_cast<const float4 *>(s->nodes.ptr);
@@ -479,6 +500,7 @@ SceneView scene_view(const lb_scene *s, const float *light) {
     v.n_nodes = s->n_nodes;
     v.root = s->root_ref;
     v.n_tris = s->n_tris;
+    v.sh = sky_sh_of(s->sky_sh);
     return v;
 }
 

@@ -115,12 +115,31 @@ __device__ __forceinline__ f3 cosine(float u, float v, const frame_t &f) {
     return normalize3(fmul(f, mk(r * ca, r * sa, sqrtf(vmax(0.0f, 1.0f - u)))));
 }
 
-// trace.wgsl:63-66
-__device__ __forceinline__ f3 sky(const lb_params &P, f3 d) {
-    if (d.y < 0.0f) return mk(P.ground[0], P.ground[1], P.ground[2]);
-    float w = 1.0f - d.y;
-    return add(scl(mk(P.horizon[0], P.horizon[1], P.horizon[2]), w),
-               scl(mk(P.zenith[0], P.zenith[1], P.zenith[2]), d.y));
+// The sky's L1 spherical-harmonic radiance, added to the reference's gradient sky (no
+// counterpart in the reference, whose sky is the gradient alone): per channel
+// 0.282095 c0 + 0.488603 ((c1 d.x + c2 d.y) + c3 d.z), the coefficients in the order of the
+// volume's probes (L0, then L1 along x, y, z; volume.rs:25-31). With `on` clear (every
+// coefficient zero) the sky is the reference's, bit for bit.
+struct SkySH {
+    float4 r, g, b;  // (c0, c1, c2, c3) of each channel
+    uint32_t on;
+};
+__device__ __forceinline__ float sh_channel(float4 c, f3 d) {
+    return 0.282095f * c.x + 0.488603f * ((c.y * d.x + c.z * d.y) + c.w * d.z);
+}
+
+// trace.wgsl:63-66, plus the L1 term
+__device__ __forceinline__ f3 sky(const SkySH &K, const lb_params &P, f3 d) {
+    f3 s;
+    if (d.y < 0.0f) {
+        s = mk(P.ground[0], P.ground[1], P.ground[2]);
+    } else {
+        float w = 1.0f - d.y;
+        s = add(scl(mk(P.horizon[0], P.horizon[1], P.horizon[2]), w),
+                scl(mk(P.zenith[0], P.zenith[1], P.zenith[2]), d.y));
+    }
+    if (K.on) s = add(s, mk(sh_channel(K.r, d), sh_channel(K.g, d), sh_channel(K.b, d)));
+    return s;
 }
 
 // trace.wgsl:74-80 over a box given by its corners
@@ -172,6 +191,7 @@ struct SceneView {
     const float4 *tris_t;     // TUNED: the triangles with their material's opacity in e2.w
     uint32_t n_nodes, n_tris;
     uint32_t root;            // TUNED: the root as a child reference (bit 31: a leaf)
+    SkySH sh;                 // the sky's L1 term (lb_scene_set_sky)
 };
 
 // trace.wgsl:69-72

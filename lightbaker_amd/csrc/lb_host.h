@@ -60,6 +60,7 @@ struct lb_scene {
     lbk::DevBuf tris_t;               // TUNED triangles: opacity in e2.w
     uint32_t n_nodes = 0, n_tris = 0, n_materials = 0, n_charts = 0, n_inner = 0;
     uint32_t root_ref = 0;  // the root as a TUNED child reference
+    float sky_sh[12] = {};  // the sky's L1 term, c0 rgb, c1 rgb, c2 rgb, c3 rgb (all 0: none)
 };
 
 extern "C" int lb_set_form(lb_gpu *gpu, int form);
@@ -75,6 +76,8 @@ bool validate_bvh(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, size
 int build_tuned_nodes(lb_scene *s, const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
                       size_t n_tris, const float *materials);
 SceneView scene_view(const lb_scene *s, const float *light);
+// the kernels' form of 12 sky coefficients (c0 rgb, c1 rgb, c2 rgb, c3 rgb)
+SkySH sky_sh_of(const float c[12]);
 // The order a TUNED lock-step gather takes each item's rays in (one byte a ray, item-major): with
 // `cost`, the launch records each ray's traversal steps by its azimuth slot (a counting launch);
 // with `perm`, slot s of item i traces azimuth slot perm[i * rays + s]. Either needs rays <= 256.

@@ -132,7 +132,7 @@ __global__ __launch_bounds__(64) void k_gather_plain(SceneView S, lb_params P,
         f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
         Hit h = trace_plain<false, STATS, WT>(S, P, o, d, P.reach[1], key + r + 4000u, cnt);
         if (h.tri == NONE) {
-            sum = add(sum, sky(P, d));
+            sum = add(sum, sky(S.sh, P, d));
             continue;
         }
         if (h.back) {
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(64) void k_volume_plain(SceneView S, lb_params P,
         sincos_lb(a, &sa, &ca);
         f3 d = mk(rr * ca, z, rr * sa);
         Hit h = trace_plain<false, STATS, WT>(S, P, o, d, P.reach[1], key + r, cnt);
-        f3 l = sky(P, d);
+        f3 l = sky(S.sh, P, d);
         if (h.tri != NONE) {
             if (h.back) back += 1.0f;
             if (STATS && !h.back) {

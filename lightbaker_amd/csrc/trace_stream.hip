@@ -323,7 +323,7 @@ __global__ void k_shade_gather(SceneView S, lb_params P, const lb_texel *items, 
         f3 o, d;
         gather_ray(P, tx, key, r, &o, &d);
         if (h.tri == NONE) {
-            sum = add(sum, sky(P, d));
+            sum = add(sum, sky(S.sh, P, d));
             continue;
         }
         if (STATS) {
@@ -353,7 +353,7 @@ __global__ void k_shade_volume(SceneView S, lb_params P, const lb_texel *items, 
     for (uint32_t r = 0u; r < nr; r++) {
         Hit h = hit_of(hits[(size_t)r * nc + li]);
         f3 d = volume_ray(P, key, r);
-        f3 l = sky(P, d);
+        f3 l = sky(S.sh, P, d);
         if (h.tri != NONE) {
             if (h.back) back += 1.0f;
             if (STATS && !h.back) {

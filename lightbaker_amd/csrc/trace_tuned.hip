@@ -535,7 +535,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
                 A.cost[(size_t)i * rays + cost_slot] = (uint8_t)(steps < 255u ? steps : 255u);
             }
             if (R.tri == NONE) {
-                f3 l = sky(P, R.d);
+                f3 l = sky(A.S.sh, P, R.d);
                 sr.add(l.x);
                 sg.add(l.y);
                 sb.add(l.z);
@@ -620,7 +620,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_vo
         auto shade = [&](uint32_t slot, const Ray &R) {
             if (slot < rays) {
                 f3 d = R.d;
-                f3 l = sky(P, d);
+                f3 l = sky(A.S.sh, P, d);
                 if (R.tri != NONE) {
                     if (R.back) {
                         back += 1.0f;

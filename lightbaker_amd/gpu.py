@@ -135,7 +135,9 @@ class Gpu:
     def __exit__(self, *exc):
         self.close()
 
-    def scene(self, bvh: Bvh, materials: np.ndarray, charts: np.ndarray) -> SceneBuffers:
+    def scene(self, bvh: Bvh, materials: np.ndarray, charts: np.ndarray,
+              sky_sh: np.ndarray | None = None) -> SceneBuffers:
+        """Gpu::scene; `sky_sh` (4 x 3: c0..c3 rgb) the sky's L1 term (lb_scene_set_sky)."""
         mats = np.ascontiguousarray(materials, dtype=np.float32).reshape(-1, 4)
         ch = np.ascontiguousarray(charts, dtype=np.float32).reshape(-1, 8)
         nodes = np.ascontiguousarray(bvh.nodes)
@@ -144,7 +146,11 @@ class Gpu:
         N.check(self._lib.lb_scene_create(self.handle, N.ptr(nodes), len(nodes), N.ptr(tris),
                                           len(tris), N.ptr(mats), len(mats),
                                           N.ptr(ch) if len(ch) else None, len(ch), C.byref(h)))
-        return SceneBuffers(self, h)
+        sb = SceneBuffers(self, h)
+        if sky_sh is not None:
+            sh = np.ascontiguousarray(np.asarray(sky_sh, np.float32).reshape(12))
+            N.check(self._lib.lb_scene_set_sky(h, N.ptr(sh)))
+        return sb
 
     def set_form(self, form: int) -> None:
         N.check(self._lib.lb_set_form(self.handle, int(form)))

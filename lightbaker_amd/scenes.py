@@ -84,6 +84,9 @@ class Scene:
     # per-tile texels per metre, {(key x, key z): density}; tiles not listed take
     # settings.texels_per_meter (no counterpart in the reference's one density, scene.rs:84)
     tile_density: dict | None = None
+    # the sky's L1 spherical-harmonic term, (4, 3): c0, c1, c2, c3 rgb, added to the gradient sky
+    # where a ray escapes (no counterpart in the reference; None: the reference's sky)
+    sky_sh: np.ndarray | None = None
 
     def triangle_count(self) -> int:
         return sum(len(m.positions) // 3 for m in self.meshes)
@@ -107,6 +110,11 @@ class Scene:
         return copy.deepcopy(self)
 
 
+# the sky's L1 term's irradiance on an upward face per unit c0 and c2, FP32
+SH_FALLBACK_C0 = np.float32(math.pi * 0.282095)
+SH_FALLBACK_C2 = np.float32(2.0 * math.pi / 3.0 * 0.488603)
+
+
 def make_params(scene: Scene, atlas_w: int, atlas_h: int) -> np.ndarray:
     """Params as bake.rs:105-120 assembles them (LIFT 0.01, INSIDE_REACH 0.3, seed 0)."""
     s = scene.settings
@@ -116,6 +124,10 @@ def make_params(scene: Scene, atlas_w: int, atlas_h: int) -> np.ndarray:
     z = np.asarray(scene.zenith, np.float32)
     h = np.asarray(scene.horizon, np.float32)
     sky_avg = (z + h) * np.float32(0.5) * np.float32(math.pi)
+    if scene.sky_sh is not None:
+        # the sky's L1 term's irradiance on an upward face: pi 0.282095 c0 + 2 pi / 3 0.488603 c2
+        sh = np.asarray(scene.sky_sh, np.float32).reshape(4, 3)
+        sky_avg = sky_avg + (SH_FALLBACK_C0 * sh[0] + SH_FALLBACK_C2 * sh[2])
     p = np.zeros(1, dtype=N.PARAMS)
     p["sun_dir"] = [sun[0], sun[1], sun[2], np.float32(math.tan(math.radians(scene.sun_radius_deg)))]
     p["sun_irr"] = [*scene.sun_irradiance, 0.0]
@@ -206,7 +218,8 @@ def fit_scene(seed: int = 5) -> Scene:
     """small_scene's layout (a 20 x 20 m ground and 50 yawed boxes) with 16 materials (albedo
     U[0.2, 0.8] per channel; the ground is material 0, each box one of the 16; material 15
     receives no light map, so rays that meet it take the fallback irradiance), a sky graded from
-    horizon to zenith over a darker ground, and 3 bounces: the scene light is fitted on."""
+    horizon to zenith over a darker ground plus an L1 spherical-harmonic term (brighter above and
+    towards +x), and 3 bounces: the scene light is fitted on."""
     rng = np.random.default_rng(seed)
     ground = np.array(quad((-10, 0, -10), (-10, 0, 10), (10, 0, 10), (10, 0, -10)), np.float32)
     meshes = [Mesh("ground", 0, ground)]
@@ -226,7 +239,9 @@ def fit_scene(seed: int = 5) -> Scene:
     return Scene(
         sun_direction=sun, sun_radius_deg=0.5, sun_irradiance=(3.0, 3.0, 3.0),
         zenith=(0.35, 0.45, 0.6), horizon=(0.5, 0.5, 0.5), ground=(0.25, 0.22, 0.2),
-        materials=mats, meshes=meshes, settings=st, volume=None)
+        materials=mats, meshes=meshes, settings=st, volume=None,
+        sky_sh=np.array([[0.40, 0.42, 0.45], [0.05, 0.04, 0.03], [0.12, 0.10, 0.08],
+                         [-0.03, -0.02, -0.01]], np.float32))
 
 
 def rebake_scene(seed: int = 3, size: float = 200.0, densities: tuple | None = (4, 16)) -> Scene:

@@ -64,6 +64,10 @@ def lib() -> C.CDLL:
         L.orc_gather_steps.restype = I
         L.orc_gather_terms.argtypes = [VP, SZ, VP, VP, VP, VP, VP, SZ, U32, VP, VP]
         L.orc_gather_terms.restype = I
+        L.orc_set_sky_sh.argtypes = [VP]
+        L.orc_set_sky_sh.restype = None
+        L.orc_gather_dirs.argtypes = [VP, VP, SZ, U32, VP]
+        L.orc_gather_dirs.restype = I
         _lib = L
     return _lib
 
@@ -235,4 +239,24 @@ def gather_terms(nodes, tris, materials, charts, params, items, key_base: int = 
     out = np.zeros((len(items), int(params["counts"][0][2]), 3), np.float32)
     assert L.orc_gather_terms(_p(nodes), len(nodes), _p(tris), _p(mats), _p(ch), _p(params),
                               _p(items), len(items), key_base, _p(lt), _p(out)) == 0
+    return out
+
+
+def set_sky_sh(sh) -> None:
+    """The sky's L1 term every later pass adds (12 values: c0 rgb, c1 rgb, c2 rgb, c3 rgb, or a
+    (4, 3) array; None: the reference's sky)."""
+    L = lib()
+    if sh is None:
+        L.orc_set_sky_sh(None)
+        return
+    a = np.ascontiguousarray(np.asarray(sh, np.float32).reshape(12))
+    L.orc_set_sky_sh(a.ctypes.data)
+
+
+def gather_dirs(params, items, key_base: int = 0) -> np.ndarray:
+    """orc_gather_dirs: (items, rays, 3) float32, each gather ray's direction."""
+    items = np.ascontiguousarray(items)
+    params = np.ascontiguousarray(params)
+    out = np.zeros((len(items), int(params["counts"][0][2]), 3), np.float32)
+    assert lib().orc_gather_dirs(_p(params), _p(items), len(items), key_base, _p(out)) == 0
     return out

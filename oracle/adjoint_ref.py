@@ -9,7 +9,8 @@ independent of the package's reverse-mode kernels: a wrong transpose, index or s
 another gradient than the contraction of these tangents.
 
 Parameters, in the order the package's gradient uses: albedo rgb per material, zenith, horizon,
-ground, sun irradiance (rgb each).
+ground, the sky's L1 coefficients c0..c3 (the term 0.282095 c0 + 0.488603 (c1, c2, c3) . d added
+where a ray escapes, no counterpart in the reference), sun irradiance (rgb each).
 """
 from __future__ import annotations
 
@@ -18,7 +19,7 @@ import math
 
 import numpy as np
 
-from . import lib
+from . import gather_dirs, lib
 
 PI = math.pi
 
@@ -108,22 +109,30 @@ def setup(inp: dict, out: dict) -> dict:
     l0 = inp.get("light0")
     light0 = (np.zeros((aw * ah, 4)) if l0 is None or len(l0) == 0
               else np.asarray(l0, np.float64).reshape(-1, 4))
-    return {"tiles": tiles, "offs": offs, "T": T, "chart": chart, "normal": normal,
+    dirs = gather_dirs(inp["params"], out["texels"][gat]).astype(np.float64)
+    return {"tiles": tiles, "offs": offs, "T": T, "chart": chart, "normal": normal, "dirs": dirs,
             "known": out["known"], "gat": gat, "gmask": gmask, "G": G, "atlas": atlas, "inv": inv,
             "rec": rec, "light0": light0, "aw": aw, "ah": ah, "settings": st,
             "materials": len(inp["mats"])}
 
 
-def forward(S: dict, albedo, zenith, horizon, ground, sun, tangents: bool = True) -> np.ndarray:
+SH0, SH1 = 0.282095, 0.488603
+FB0, FB2 = PI * SH0, 2.0 * PI / 3.0 * SH1  # the L1 term's irradiance on an upward face
+
+
+def forward(S: dict, albedo, zenith, horizon, ground, sun, sky_sh=None,
+            tangents: bool = True) -> np.ndarray:
     """The dirty texels' light after the bounces, (T, 3, K): channel 0 the value, channel 1 + p
     its derivative by parameter p (K = 1 without tangents)."""
     st = S["settings"]
     M = S["materials"]
     albedo = np.asarray(albedo, np.float64).reshape(M, 3)
     z, hz, gr, sn = (np.asarray(v, np.float64) for v in (zenith, horizon, ground, sun))
-    P = 3 * M + 12
+    sh = (np.zeros((4, 3)) if sky_sh is None
+          else np.asarray(sky_sh, np.float64).reshape(4, 3))
+    P = 3 * M + 24
     K = 1 + P if tangents else 1
-    iz, ih, ig, isun = 1 + 3 * M, 4 + 3 * M, 7 + 3 * M, 10 + 3 * M
+    iz, ih, ig, ish, isun = 1 + 3 * M, 4 + 3 * M, 7 + 3 * M, 10 + 3 * M, 22 + 3 * M
     T, known, G = S["T"], S["known"], S["G"]
     lit_t = known >= 1
     D = np.zeros((T, 3, K))
@@ -136,7 +145,8 @@ def forward(S: dict, albedo, zenith, horizon, ground, sun, tangents: bool = True
     n, R = rec.shape[0], rec.shape[1]
     word, tag = rec[:, :, 0], rec[:, :, 1]
     kind, mat = tag >> 30, (tag & 0x3FFFFFFF).astype(np.int64)
-    fb = (z + hz) * 0.5 * PI
+    fb = (z + hz) * 0.5 * PI + FB0 * sh[0] + FB2 * sh[2]
+    dirs = S["dirs"]
     ds = st.indirect_downscale
     rg, grings = st.gi_filter * ds, st.padding + 2 * ds + 2
     tiles, offs = S["tiles"], S["offs"]
@@ -148,13 +158,19 @@ def forward(S: dict, albedo, zenith, horizon, ground, sun, tangents: bool = True
             y = wd.view(np.float32).astype(np.float64)
             up = (kd == 3) & (y >= 0)
             dn = (kd == 3) & (y < 0)
+            sk = kd == 3
+            d = dirs[:, r]
+            basis = [np.full(n, SH0), SH1 * d[:, 0], SH1 * d[:, 1], SH1 * d[:, 2]]
             for c in range(3):
                 E[up, c, 0] += hz[c] * (1.0 - y[up]) + z[c] * y[up]
                 E[dn, c, 0] += gr[c]
+                E[sk, c, 0] += sum(basis[j][sk] * sh[j, c] for j in range(4))
                 if tangents:
                     E[up, c, iz + c] += y[up]
                     E[up, c, ih + c] += 1.0 - y[up]
                     E[dn, c, ig + c] += 1.0
+                    for j in range(4):
+                        E[sk, c, ish + 3 * j + c] += basis[j][sk]
             hit = kd <= 1
             ev = np.zeros((n, 3, K))
             lit = np.zeros(n, bool)
@@ -173,6 +189,8 @@ def forward(S: dict, albedo, zenith, horizon, ground, sun, tangents: bool = True
                 if tangents:
                     ev[fbr, c, iz + c] = PI / 2
                     ev[fbr, c, ih + c] = PI / 2
+                    ev[fbr, c, ish + c] = FB0
+                    ev[fbr, c, ish + 6 + c] = FB2
             a = albedo[m]
             E[hit] += (a[hit] / PI)[:, :, None] * ev[hit]
             if tangents:

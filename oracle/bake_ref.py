@@ -12,7 +12,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import lib, run_quant
+from . import lib, run_quant, set_sky_sh
 from . import run as run_fp32
 
 VALIDITY_NONE, VALIDITY_CENTRE, VALIDITY_OFFSET = 0, 1, 2
@@ -169,7 +169,16 @@ def bake(inp: dict) -> dict:
     tiles; light0 (aw * ah, 4) the clean tiles' light or None; aw, ah; nodes, tris, mats, charts,
     params; settings (lightmap/bake fields); sun (3,), irr (3,); probes (np,) TEXEL of the dirty
     probes or None, volume_params; quant (optional): the 8-bit node records (quant_records), whose
-    boxes the passes then traverse as the device's quantized form does."""
+    boxes the passes then traverse as the device's quantized form does; sky_sh (optional): the
+    sky's L1 term (4 x 3)."""
+    set_sky_sh(inp.get("sky_sh"))
+    try:
+        return _bake(inp)
+    finally:
+        set_sky_sh(None)
+
+
+def _bake(inp: dict) -> dict:
     st = inp["settings"]
     q = inp.get("quant")
 

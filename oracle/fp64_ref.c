@@ -75,9 +75,16 @@ static d3 cosine(double u, double v, const frame_t *f) {
     d3 l = mk(r * cos(a), r * sin(a), sqrt(fmax(0.0, 1.0 - u)));
     return normalize3(add(add(scl(f->t, l.x), scl(f->b, l.y)), scl(f->n, l.z)));
 }
+extern float orc_sky_sh[12]; /* trace_ref.c: the sky's L1 term */
 static d3 sky(const ctx_t *c, d3 d) {
-    if (d.y < 0.0) return ld3(c->P->ground);
-    return add(scl(ld3(c->P->horizon), 1.0 - d.y), scl(ld3(c->P->zenith), d.y));
+    d3 s = d.y < 0.0 ? ld3(c->P->ground)
+                     : add(scl(ld3(c->P->horizon), 1.0 - d.y), scl(ld3(c->P->zenith), d.y));
+    double v[3];
+    for (int k = 0; k < 3; k++) {
+        const float *q = orc_sky_sh;
+        v[k] = 0.282095 * q[k] + 0.488603 * (q[3 + k] * d.x + q[6 + k] * d.y + q[9 + k] * d.z);
+    }
+    return add(s, mk(v[0], v[1], v[2]));
 }
 static int stops(const ctx_t *c, uint32_t tri, uint32_t ray) {
     double op = c->materials[fbits(c->tris[tri].v0[3]) * 4 + 3];

@@ -43,6 +43,9 @@ int orc_quant_compare(int pass, const lb_node *nodes, size_t n_nodes, const uint
                       const lb_params *params, const lb_texel *items, size_t n,
                       uint32_t key_base, const float *light, float *out,
                       unsigned long long counts[3]);
+void orc_set_sky_sh(const float *sh);
+int orc_gather_dirs(const lb_params *params, const lb_texel *items, size_t n, uint32_t key_base,
+                    float *out);
 int orc_gather_records(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
                        const float *materials, const float *charts, const lb_params *params,
                        const lb_texel *items, size_t n, uint32_t key_base, uint32_t *out);

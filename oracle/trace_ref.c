@@ -153,11 +153,36 @@ void orc_cosine(float u, float v, const float n[3], float out[3]) {
 }
 
 /* trace.wgsl:63-66 */
+/* Not the reference: the sky's L1 spherical-harmonic term (orc_set_sky_sh), added where a ray
+ * escapes, per channel 0.282095 c0 + 0.488603 ((c1 d.x + c2 d.y) + c3 d.z); all zero: none. */
+float orc_sky_sh[12];
+int orc_sky_on;
+void orc_set_sky_sh(const float *sh) {
+    orc_sky_on = 0;
+    for (int k = 0; k < 12; k++) {
+        orc_sky_sh[k] = sh ? sh[k] : 0.0f;
+        orc_sky_on |= orc_sky_sh[k] != 0.0f;
+    }
+}
+
 static v3 sky(const ctx_t *c, v3 d) {
     const lb_params *P = c->P;
-    if (d.y < 0.0f) return ld3(P->ground);
-    float w = 1.0f - d.y;
-    return add(scl(ld3(P->horizon), w), scl(ld3(P->zenith), d.y));
+    v3 s;
+    if (d.y < 0.0f) {
+        s = ld3(P->ground);
+    } else {
+        float w = 1.0f - d.y;
+        s = add(scl(ld3(P->horizon), w), scl(ld3(P->zenith), d.y));
+    }
+    if (orc_sky_on) {
+        float v[3];
+        for (int k = 0; k < 3; k++) {
+            const float *q = orc_sky_sh;
+            v[k] = 0.282095f * q[k] + 0.488603f * ((q[3 + k] * d.x + q[6 + k] * d.y) + q[9 + k] * d.z);
+        }
+        s = add(s, mk(v[0], v[1], v[2]));
+    }
+    return s;
 }
 
 /* trace.wgsl:69-72 */
@@ -652,6 +677,31 @@ int orc_gather_records(const lb_node *nodes, size_t n_nodes, const lb_tri *tris,
                     q[1] = m;
                 }
             }
+        }
+    }
+    return 0;
+}
+
+/* Each gather ray's direction (trace.wgsl:262), ray r of item i at (i * rays + r) * 3: what the
+ * sky's L1 term of a ray that escapes reads (a record keeps only d.y). */
+int orc_gather_dirs(const lb_params *params, const lb_texel *items, size_t n, uint32_t key_base,
+                    float *out) {
+    ctx_t c0;
+    memset(&c0, 0, sizeof c0);
+    c0.P = params;
+    uint32_t rays = params->counts[2];
+    for (size_t i = 0; i < n; i++) {
+        const ctx_t *c = &c0;
+        v3 nrm = ld3(items[i].normal);
+        frame_t f = frame(nrm);
+        uint32_t key = (key_base + (uint32_t)i) * 7919u;
+        float s1 = rnd(c, key, 11u), s2 = rnd(c, key, 13u);
+        for (uint32_t r = 0u; r < rays; r++) {
+            float u0 = ((float)r + 0.5f) / (float)rays + s1;
+            float v0 = orc_radical(r) + s2;
+            v3 d = cosine(u0 - floorf(u0), v0 - floorf(v0), &f);
+            float *q = out + (i * rays + r) * 3u;
+            q[0] = d.x; q[1] = d.y; q[2] = d.z;
         }
     }
     return 0;

@@ -104,4 +104,4 @@ def host_bake_inputs(scene: Scene) -> dict:
             "ah": L.height, "nodes": bvh.nodes, "tris": bvh.tris, "mats": mats,
             "charts": L.affines.reshape(-1, 8).astype(np.float32), "params": P,
             "sun": P["sun_dir"][0][:3].copy(), "irr": np.asarray(scene.sun_irradiance, np.float32),
-            "probes": None, "volume_params": None}
+            "probes": None, "volume_params": None, "sky_sh": scene.sky_sh}

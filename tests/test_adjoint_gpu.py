@@ -4,10 +4,13 @@ Pins, on the 16-material fitting scene (3 bounces):
   * the gather rays' records equal the oracle's (oracle/trace_ref.c orc_gather_records), bit for
     bit;
   * a forward with the scene's own parameters reproduces the PLAIN bake's light atlas bit for bit;
-  * the gradient of an L2 loss against a target baked from perturbed albedos and sky equals the
-    FP64 forward-mode gradient (oracle/adjoint_ref.py) within 1e-3 relative, and central finite
-    differences (h = 1e-3) of the GPU's own forward within 1e-3 relative;
-  * the torch.autograd.Function gives the same gradient, and 50 Adam steps fit the target.
+  * the gradient of an L2 loss against a target baked from perturbed albedos (+-0.1), sky
+    gradient colours and the sky's L1 coefficients equals the FP64 forward-mode gradient
+    (oracle/adjoint_ref.py) within 1e-3 relative, and central finite differences (h = 1e-3) of
+    the GPU's own forward within 1e-3 relative, every parameter (3 M + 24) included;
+  * the torch.autograd.Function gives the same gradient, and 50 Adam steps fit the target, the
+    sky's L1 coefficients among the fitted parameters.
+The backward accumulates in FP32 with compensated (two-sum) atomics and reductions, no FP64.
 Tolerance: per parameter, |g - g_ref| <= 1e-3 * max(|g_ref|, 1e-2 * max |g_ref|) (parameters
 whose gradient is under 1 % of the largest are held to 1e-5 of the largest).
 """
@@ -38,17 +41,21 @@ def session_inputs(scene, sess):
         "mats": sess.array(N.BAKE_MATERIALS, np.float32).reshape(-1, 4),
         "charts": sess.array(N.BAKE_CHARTS, np.float32).reshape(-1, 8), "params": P,
         "sun": P["sun_dir"][0][:3].copy(), "irr": np.asarray(scene.sun_irradiance, np.float32),
-        "probes": None, "volume_params": None,
+        "probes": None, "volume_params": None, "sky_sh": scene.sky_sh,
     }
 
 
 def perturbed(scene, seed=55):
+    """The scene's own light parameters, and a target's: albedo +-0.1, the sky's gradient
+    colours and L1 coefficients scaled by 0.8..1.2."""
     from lightbaker_amd.adjoint import scene_light_params
-    alb, sky, sun = scene_light_params(scene)
+    alb, sky, sun, sh = scene_light_params(scene)
+    assert np.abs(sh).max() > 0
     rng = np.random.default_rng(seed)
     alb_t = np.clip(alb + rng.uniform(-0.1, 0.1, alb.shape), 0.05, 0.95).astype(np.float32)
     sky_t = (sky * rng.uniform(0.8, 1.2, sky.shape)).astype(np.float32)
-    return (alb, sky, sun), (alb_t, sky_t, sun)
+    sh_t = (sh * rng.uniform(0.8, 1.2, sh.shape)).astype(np.float32)
+    return (alb, sky, sun, sh), (alb_t, sky_t, sun, sh_t)
 
 
 def check_close(g, ref, what):
@@ -109,12 +116,13 @@ def loss_and_dlight(light, target):
 
 def test_gradient_matches_fp64(fitted):
     scene, sess, adj, inp, out, S = fitted
-    (alb, sky, sun), (alb_t, sky_t, _) = perturbed(scene)
-    target = adj.forward(alb_t, sky_t, sun).cpu().numpy().reshape(-1, 4).astype(np.float64)
-    light = adj.forward(alb, sky, sun).cpu().numpy().reshape(-1, 4)
+    (alb, sky, sun, sh), target_params = perturbed(scene)
+    target = adj.forward(*target_params).cpu().numpy().reshape(-1, 4).astype(np.float64)
+    light = adj.forward(alb, sky, sun, sh).cpu().numpy().reshape(-1, 4)
     loss, dl = loss_and_dlight(light, target)
     g = adj.backward(torch.tensor(dl, dtype=torch.float32).reshape(adj.shape)).cpu().numpy()
-    Lt = aref.forward(S, alb, sky[0], sky[1], sky[2], sun)
+    assert len(g) == 3 * len(alb) + 24
+    Lt = aref.forward(S, alb, sky[0], sky[1], sky[2], sun, sh)
     light64 = aref.atlas_light(S, Lt)
     loss64, dl64 = loss_and_dlight(light64, target)
     g64 = aref.gradient(S, Lt, dl64)
@@ -125,20 +133,21 @@ def test_gradient_matches_fp64(fitted):
 
 def test_gradient_matches_finite_differences(fitted):
     scene, sess, adj, *_ = fitted
-    (alb, sky, sun), (alb_t, sky_t, _) = perturbed(scene)
-    target = adj.forward(alb_t, sky_t, sun).cpu().numpy().reshape(-1, 4).astype(np.float64)
-    light = adj.forward(alb, sky, sun).cpu().numpy().reshape(-1, 4)
+    (alb, sky, sun, sh), target_params = perturbed(scene)
+    target = adj.forward(*target_params).cpu().numpy().reshape(-1, 4).astype(np.float64)
+    light = adj.forward(alb, sky, sun, sh).cpu().numpy().reshape(-1, 4)
     _, dl = loss_and_dlight(light, target)
     g = adj.backward(torch.tensor(dl, dtype=torch.float32).reshape(adj.shape)).cpu().numpy()
-    theta = np.concatenate([alb.reshape(-1), sky.reshape(-1), sun]).astype(np.float64)
+    theta = np.concatenate([alb.reshape(-1), sky.reshape(-1), sh.reshape(-1), sun]).astype(np.float64)
     M = len(alb)
     h = 1e-3
 
     def loss_at(th):
         a = th[:M * 3].reshape(M, 3).astype(np.float32)
         s = th[M * 3:M * 3 + 9].reshape(3, 3).astype(np.float32)
-        u = th[M * 3 + 9:].astype(np.float32)
-        lt = adj.forward(a, s, u).cpu().numpy().reshape(-1, 4)
+        c = th[M * 3 + 9:M * 3 + 21].reshape(4, 3).astype(np.float32)
+        u = th[M * 3 + 21:].astype(np.float32)
+        lt = adj.forward(a, s, u, c).cpu().numpy().reshape(-1, 4)
         return loss_and_dlight(lt, target)[0]
 
     fd = np.zeros_like(theta)
@@ -152,32 +161,37 @@ def test_gradient_matches_finite_differences(fitted):
 def test_autograd_function(fitted):
     from lightbaker_amd.adjoint import baked_light, l2_loss
     scene, sess, adj, *_ = fitted
-    (alb, sky, sun), (alb_t, sky_t, _) = perturbed(scene)
-    target = adj.forward(alb_t, sky_t, sun)
+    (alb, sky, sun, sh), target_params = perturbed(scene)
+    target = adj.forward(*target_params)
     a = torch.tensor(alb, requires_grad=True)
     s = torch.tensor(sky, requires_grad=True)
     u = torch.tensor(sun, requires_grad=True)
-    loss = l2_loss(baked_light(adj, a, s, u), target)
+    c = torch.tensor(sh, requires_grad=True)
+    loss = l2_loss(baked_light(adj, a, s, u, c), target)
     loss.backward()
-    light = adj.forward(alb, sky, sun)
+    light = adj.forward(alb, sky, sun, sh)
     w = (target[..., 3:] > 0.5).float()
     dl = torch.zeros_like(light)
     dl[..., :3] = 2.0 * (light[..., :3] - target[..., :3]) * w
     g = adj.backward(dl).cpu().numpy()
     M = len(alb)
-    got = np.concatenate([a.grad.numpy().reshape(-1), s.grad.numpy().reshape(-1), u.grad.numpy()])
+    got = np.concatenate([a.grad.numpy().reshape(-1), s.grad.numpy().reshape(-1),
+                          c.grad.numpy().reshape(-1), u.grad.numpy()])
     assert np.allclose(got, g, rtol=1e-5, atol=1e-6 * np.abs(g).max())
     assert a.grad.shape == (M, 3) and s.grad.shape == (3, 3) and u.grad.shape == (3,)
+    assert c.grad.shape == (4, 3)
 
 
 def test_adam_fits_the_target(fitted):
     from lightbaker_amd.adjoint import fit
     scene, sess, adj, *_ = fitted
-    (alb, sky, sun), (alb_t, sky_t, _) = perturbed(scene)
-    target = adj.forward(alb_t, sky_t, sun)
-    res = fit(adj, target, alb, sky, sun, steps=50, lr=0.02)
+    (alb, sky, sun, sh), (alb_t, sky_t, _, sh_t) = perturbed(scene)
+    target = adj.forward(alb_t, sky_t, sun, sh_t)
+    res = fit(adj, target, alb, sky, sun, sh, steps=50, lr=0.02, fit_sky="both")
     first, last = res["loss"][0], res["loss"][-1]
     print(f"Adam: loss {first:.4g} -> {last:.4g}; albedo error "
-          f"{np.abs(alb - alb_t).mean():.4f} -> {np.abs(res['albedo'] - alb_t).mean():.4f}")
+          f"{np.abs(alb - alb_t).mean():.4f} -> {np.abs(res['albedo'] - alb_t).mean():.4f}; "
+          f"L1 sky error {np.abs(sh - sh_t).mean():.4f} -> "
+          f"{np.abs(res['sky_sh'] - sh_t).mean():.4f}")
     assert last < 0.1 * first
     assert np.abs(res["sky"] - sky_t).mean() < np.abs(sky - sky_t).mean()

@@ -4,6 +4,8 @@ bake.rs:149-330 and volume.rs:69-117 over the oracle's C passes), on the same pr
 Bar: the PLAIN form, and the TUNED form's pooled schedule with FP32 node records (it shades in
 the reference's order), give the final light atlas (direct + last bounce per texel) bit for bit.
 The TUNED form's lock-step schedule (compensated sums in another order) within 1e-4 relative.
+The small scene is baked again under its sky plus an L1 spherical-harmonic term, to the same bar
+(the term in every escaping ray's sky and in the fallback irradiance).
 """
 from __future__ import annotations
 
@@ -48,12 +50,21 @@ def oracle_inputs(scene, sess):
         "mats": sess.array(N.BAKE_MATERIALS, np.float32).reshape(-1, 4),
         "charts": sess.array(N.BAKE_CHARTS, np.float32).reshape(-1, 8), "params": P,
         "sun": P["sun_dir"][0][:3].copy(), "irr": np.asarray(scene.sun_irradiance, np.float32),
-        "probes": probe_items(scene), "volume_params": PV,
+        "probes": probe_items(scene), "volume_params": PV, "sky_sh": scene.sky_sh,
     }
 
 
-@pytest.mark.parametrize("gen", [lambda: reference_test_scene(8.0), lambda: small_scene(1)],
-                         ids=["reference-test-scene", "small-scene"])
+def small_scene_l1_sky():
+    """The small scene under its gradient sky plus an L1 term (lb_scene_desc.sky_sh)."""
+    from .test_sky_sh import SH
+    scene = small_scene(1)
+    scene.sky_sh = SH
+    return scene
+
+
+@pytest.mark.parametrize("gen", [lambda: reference_test_scene(8.0), lambda: small_scene(1),
+                                 small_scene_l1_sky],
+                         ids=["reference-test-scene", "small-scene", "small-scene-l1-sky"])
 @pytest.mark.parametrize("form,schedule", [(N.FORM_PLAIN, 0), (N.FORM_TUNED, 0),
                                            (N.FORM_TUNED, 1)], ids=["plain", "tuned", "pooled"])
 def test_bake_matches_oracle(gpu, gen, form, schedule):

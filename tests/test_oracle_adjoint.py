@@ -3,7 +3,7 @@
     (oracle/bake_ref.py, which traces every ray) within FP32 rounding: the records hold what the
     gather's light depends on and the replay combines them as trace.wgsl:256-269 does;
   * its forward-mode derivatives equal FP64 central differences of its own forward, for albedo,
-    sky and sun parameters (a dropped or misplaced tangent fails);
+    sky (gradient and L1 coefficients) and sun parameters (a dropped or misplaced tangent fails);
   * the light is linear in the sun's irradiance and in the sky at fixed albedo (closed forms of
     the forward's structure)."""
 from __future__ import annotations
@@ -28,12 +28,13 @@ def setup():
     alb = np.array([m.albedo for m in scene.materials])
     sky = np.array([scene.zenith, scene.horizon, scene.ground])
     sun = np.array(scene.sun_irradiance)
-    return scene, out, S, alb, sky, sun
+    sh = np.asarray(scene.sky_sh, np.float64)
+    return scene, out, S, alb, sky, sun, sh
 
 
 def test_replay_matches_traced_bake(setup):
-    scene, out, S, alb, sky, sun = setup
-    Lt = aref.forward(S, alb, sky[0], sky[1], sky[2], sun, tangents=False)
+    scene, out, S, alb, sky, sun, sh = setup
+    Lt = aref.forward(S, alb, sky[0], sky[1], sky[2], sun, sh, tangents=False)
     got = aref.atlas_light(S, Lt)
     want = out["light"].reshape(-1, 4).astype(np.float64)
     assert np.array_equal(got[:, 3], want[:, 3])
@@ -42,35 +43,43 @@ def test_replay_matches_traced_bake(setup):
 
 
 def test_tangents_match_central_differences(setup):
-    scene, out, S, alb, sky, sun = setup
+    scene, out, S, alb, sky, sun, sh = setup
     M = len(alb)
-    Lt = aref.forward(S, alb, sky[0], sky[1], sky[2], sun)
+    Lt = aref.forward(S, alb, sky[0], sky[1], sky[2], sun, sh)
     h = 1e-4
 
-    def value(a, s, u):
-        return aref.forward(S, a, s[0], s[1], s[2], u, tangents=False)[:, :, 0]
+    def value(a, s, u, c):
+        return aref.forward(S, a, s[0], s[1], s[2], u, c, tangents=False)[:, :, 0]
 
-    for p in (0, 3 * 3 + 1, 3 * 7 + 2, 3 * M + 2, 3 * M + 3, 3 * M + 7, 3 * M + 11):
-        th = [alb.copy(), sky.copy(), sun.copy()]
-        lo = [alb.copy(), sky.copy(), sun.copy()]
-        if p < 3 * M:
+    # albedo, zenith, horizon, ground, the L1 coefficients c0..c3, the sun
+    base = 3 * M
+    for p in (0, 3 * 3 + 1, 3 * 7 + 2, base + 2, base + 3, base + 7, base + 9, base + 13,
+              base + 17, base + 19, base + 22):
+        th = [alb.copy(), sky.copy(), sun.copy(), sh.copy()]
+        lo = [alb.copy(), sky.copy(), sun.copy(), sh.copy()]
+        if p < base:
             th[0].flat[p] += h
             lo[0].flat[p] -= h
-        elif p < 3 * M + 9:
-            th[1].flat[p - 3 * M] += h
-            lo[1].flat[p - 3 * M] -= h
+        elif p < base + 9:
+            th[1].flat[p - base] += h
+            lo[1].flat[p - base] -= h
+        elif p < base + 21:
+            th[3].flat[p - base - 9] += h
+            lo[3].flat[p - base - 9] -= h
         else:
-            th[2][p - 3 * M - 9] += h
-            lo[2][p - 3 * M - 9] -= h
+            th[2][p - base - 21] += h
+            lo[2][p - base - 21] -= h
         fd = (value(*th) - value(*lo)) / (2 * h)
         tg = Lt[:, :, 1 + p]
         assert np.abs(fd - tg).max() <= 1e-6 * max(1.0, np.abs(tg).max()), p
+        assert np.abs(tg).max() > 0, p  # every parameter reaches the light
 
 
 def test_linear_in_sun_and_sky(setup):
-    scene, out, S, alb, sky, sun = setup
-    a = aref.forward(S, alb, sky[0], sky[1], sky[2], sun, tangents=False)[:, :, 0]
-    b = aref.forward(S, alb, 2 * sky[0], 2 * sky[1], 2 * sky[2], 2 * sun, tangents=False)[:, :, 0]
+    scene, out, S, alb, sky, sun, sh = setup
+    a = aref.forward(S, alb, sky[0], sky[1], sky[2], sun, sh, tangents=False)[:, :, 0]
+    b = aref.forward(S, alb, 2 * sky[0], 2 * sky[1], 2 * sky[2], 2 * sun, 2 * sh,
+                     tangents=False)[:, :, 0]
     assert np.allclose(b, 2 * a, rtol=1e-12, atol=1e-12)
-    z = aref.forward(S, alb, 0 * sky[0], 0 * sky[1], 0 * sky[2], 0 * sun, tangents=False)
+    z = aref.forward(S, alb, 0 * sky[0], 0 * sky[1], 0 * sky[2], 0 * sun, 0 * sh, tangents=False)
     assert np.abs(z).max() == 0.0
