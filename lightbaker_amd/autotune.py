@@ -6,9 +6,11 @@ atlas.
 
 Search: coordinate descent from the current setting, one knob at a time over its range, at most
 `rounds` rounds, each candidate timed as the median of `reps` launches of its pass. A knob that
-the candidate's schedule ignores is skipped (rays per lane and waves for the pooled schedule,
-refill for the lock-step one); a candidate the device cannot launch (an LDS stack larger
-than a workgroup's LDS) is skipped too.
+the candidate's schedule ignores is skipped (rays per lane, waves and the chiplet split for the
+pooled schedule, refill for the lock-step one); a candidate the device cannot launch (an LDS stack larger than a workgroup's
+LDS) is skipped too. The node layout stays as set unless `tune_nodes`: the 8-bit and FP32 boxes
+may keep different ones of two triangles a few ulps apart along a ray, which moves a texel's
+light by more than rounding, so a change of layout cannot pass the final check.
 """
 from __future__ import annotations
 
@@ -38,7 +40,8 @@ def _skip(dim: str, cand: dict) -> bool:
     return dim == "rays_per_lane" and cand["schedule"] == 1
 
 
-def _tune_pass(sess, pass_: Pass, reps: int, rounds: int, log) -> tuple[dict, float, int]:
+def _tune_pass(sess, pass_: Pass, reps: int, rounds: int, log,
+               tune_nodes: bool = False) -> tuple[dict, float, int]:
     gpu = sess.gpu
 
     def measure(c):
@@ -54,6 +57,8 @@ def _tune_pass(sess, pass_: Pass, reps: int, rounds: int, log) -> tuple[dict, fl
     for _ in range(rounds):
         changed = False
         for dim, values in RANGES.items():
+            if dim == "nodes" and not tune_nodes:
+                continue
             for v in values:
                 cand = dict(cur, **{dim: v})
                 if v == cur[dim] or _skip(dim, cand):
@@ -70,7 +75,7 @@ def _tune_pass(sess, pass_: Pass, reps: int, rounds: int, log) -> tuple[dict, fl
     return cur, best, tried
 
 
-def autotune(sess, reps: int = 3, rounds: int = 2, log=None) -> dict:
+def autotune(sess, reps: int = 3, rounds: int = 2, log=None, tune_nodes: bool = False) -> dict:
     """Tunes the session's GPU in place, pass by pass (the session must be TUNED and executed
     once). Returns each pass's pick and time; reverts every pass if the picks change the light
     atlas by more than the compensated sums' rounding (1e-4 relative)."""
@@ -81,7 +86,8 @@ def autotune(sess, reps: int = 3, rounds: int = 2, log=None) -> dict:
     ref_light = sess.array(N.BAKE_LIGHT, np.float32).copy()
     picks, ms, tried = {}, {}, 0
     for p in (Pass.GATHER, Pass.DIRECT, Pass.VOLUME):
-        picks[p.name.lower()], ms[p.name.lower()], n = _tune_pass(sess, p, reps, rounds, log)
+        picks[p.name.lower()], ms[p.name.lower()], n = _tune_pass(sess, p, reps, rounds, log,
+                                                                  tune_nodes)
         tried += n
     sess.execute()
     sess.finish()

@@ -52,44 +52,20 @@ __device__ __forceinline__ StackView make_stack(uint32_t *spill, uint32_t depth)
     return st;
 }
 
-// The children of inner node n: boxes, and each child's reference.
-template <int LAYOUT>
+// The children of inner node n from its WIDE record: boxes, and each child's reference.
 __device__ __forceinline__ void children(const SceneView &S, uint32_t n, f3 &amn, f3 &amx,
                                          f3 &bmn, f3 &bmx, uint32_t &ra, uint32_t &rb) {
-    if (LAYOUT == LB_NODES_WIDE) {
-        const float4 *w = S.wide + n * 4u;
-        float4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
-        // the references' load goes out with the boxes' (one round trip a visit): without this
-        // the compiler defers it past the box tests
-        asm volatile("" ::"v"(w3.x), "v"(w3.y));
-        amn = mk(w0.x, w0.y, w0.z);
-        amx = mk(w0.w, w1.x, w1.y);
-        bmn = mk(w1.z, w1.w, w2.x);
-        bmx = mk(w2.y, w2.z, w2.w);
-        ra = fbits(w3.x);
-        rb = fbits(w3.y);
-    } else {
-        const uint4 *q = S.quant + n * 2u;
-        uint4 q0 = q[0], q1 = q[1];
-        asm volatile("" ::"v"(q1.w));
-        // origin: the word's top 24 bits; step 2^e: its low byte as a biased exponent
-        float ox = bitsf(q0.x & 0xffffff00u), oy = bitsf(q0.y & 0xffffff00u),
-              oz = bitsf(q0.z & 0xffffff00u);
-        float sx = bitsf((q0.x & 0xffu) << 23), sy = bitsf((q0.y & 0xffu) << 23),
-              sz = bitsf((q0.z & 0xffu) << 23);
-        auto b = [](uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); };
-        // q * 2^e is exact, so the fused form rounds once, as origin + q * 2^e does
-        amn = mk(__builtin_fmaf(b(q1.x, 0), sx, ox), __builtin_fmaf(b(q1.x, 1), sy, oy),
-                 __builtin_fmaf(b(q1.x, 2), sz, oz));
-        amx = mk(__builtin_fmaf(b(q1.x, 3), sx, ox), __builtin_fmaf(b(q1.y, 0), sy, oy),
-                 __builtin_fmaf(b(q1.y, 1), sz, oz));
-        bmn = mk(__builtin_fmaf(b(q1.y, 2), sx, ox), __builtin_fmaf(b(q1.y, 3), sy, oy),
-                 __builtin_fmaf(b(q1.z, 0), sz, oz));
-        bmx = mk(__builtin_fmaf(b(q1.z, 1), sx, ox), __builtin_fmaf(b(q1.z, 2), sy, oy),
-                 __builtin_fmaf(b(q1.z, 3), sz, oz));
-        ra = q0.w;
-        rb = q1.w;
-    }
+    const float4 *w = S.wide + n * 4u;
+    float4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+    // the references' load goes out with the boxes' (one round trip a visit): without this the
+    // compiler defers it past the box tests
+    asm volatile("" ::"v"(w3.x), "v"(w3.y));
+    amn = mk(w0.x, w0.y, w0.z);
+    amx = mk(w0.w, w1.x, w1.y);
+    bmn = mk(w1.z, w1.w, w2.x);
+    bmx = mk(w2.y, w2.z, w2.w);
+    ra = fbits(w3.x);
+    rb = fbits(w3.y);
 }
 
 // A ray in flight: trace.wgsl:83-130's loop state.
@@ -100,6 +76,7 @@ struct Ray {
     uint32_t n;    // the node to visit next
     uint32_t sp;   // stack entries
     uint32_t key;  // stops()'s ray key
+    uint32_t qsel; // quant_slabs' byte selector of the planes the ray meets first
     bool back, any;
 };
 
@@ -115,6 +92,10 @@ __device__ __forceinline__ void ray_start(Ray &R, uint32_t root, f3 o, f3 d, flo
     R.sp = 0;
     R.key = key;
     R.any = any;
+    // on an axis where inv >= 0 the planes' t rise with the byte, so the lower corner's byte
+    // gives the nearer plane; else the upper's (byte indices of child a's corners, quant_slabs)
+    R.qsel = (R.inv.x >= 0.0f ? 0u : 3u) | (R.inv.y >= 0.0f ? 1u : 4u) << 8 |
+             (R.inv.z >= 0.0f ? 2u : 5u) << 16 | 0x0c000000u;
 }
 
 // The triangles of leaf R.n (trace.wgsl:91-104); true when an any-hit ray accepted one.
@@ -153,35 +134,48 @@ __device__ __forceinline__ bool leaf_tests(const SceneView &S, const lb_params &
 
 // Inner node R.n (trace.wgsl:105-123): true when the ray moved on to its near child (the far one
 // pushed if it was hit too), false when it missed both.
-// A QUANT record's two slab tests without decoding the corners: the plane of byte q on axis a
-// lies at t = q * (2^e_a * inv_a) + (origin_a - o_a) * inv_a, one fused multiply-add a plane
-// after six products a node. The planes' t round differently from (corner - o) * inv on the
-// decoded corners, by a few ulps; the encoder keeps every corner outside its FP32 box in exact
-// arithmetic as well as after rounding, so the boxes stay conservative.
+// A QUANT record's two slab tests without decoding the corners. Per axis the record's word holds
+// the origin (a float at or below both children's boxes, its low 9 bits zero) with the step's
+// biased exponent in bits 0-7, so the word shifted left by 23 reads as the step 2^e; the plane of
+// byte q then lies at
+// t = q * (2^e * inv) + (origin - o) * inv, one fused multiply-add a plane after six products a
+// node. The planes' t round differently from (corner - o) * inv on the decoded corners, by a few
+// ulps; the encoder keeps every corner outside its FP32 box in exact arithmetic as well as after
+// rounding, so the boxes stay conservative.
+// On each axis the nearer plane is known from the sign of inv (t is monotone in q, and so after
+// rounding), so a ray's selector (Ray::qsel) gathers the near and far bytes of each box with one
+// byte permute each: the slab test's per-axis min and max are exact selections.
+// Record: q0 = (origin words x, y, z, reference a), q1 = 12 corner bytes (a min xyz, a max xyz,
+// b min xyz, b max xyz), reference b.
 __device__ __forceinline__ void quant_slabs(const SceneView &S, const Ray &R, float &la, bool &ha,
                                             float &lb, bool &hb, uint32_t &ra, uint32_t &rb) {
     const uint4 *q = S.quant + R.n * 2u;
     uint4 q0 = q[0], q1 = q[1];
     asm volatile("" ::"v"(q1.w));
-    float kx = bitsf((q0.x & 0xffu) << 23) * R.inv.x, ky = bitsf((q0.y & 0xffu) << 23) * R.inv.y,
-          kz = bitsf((q0.z & 0xffu) << 23) * R.inv.z;
+    float kx = bitsf(q0.x << 23) * R.inv.x, ky = bitsf(q0.y << 23) * R.inv.y,
+          kz = bitsf(q0.z << 23) * R.inv.z;
     float cx = (bitsf(q0.x & 0xffffff00u) - R.o.x) * R.inv.x,
           cy = (bitsf(q0.y & 0xffffff00u) - R.o.y) * R.inv.y,
           cz = (bitsf(q0.z & 0xffffff00u) - R.o.z) * R.inv.z;
+    // child a's bytes are q1.x bytes 0-3 and q1.y bytes 0-1 (selector values 0-5 of q1.y:q1.x);
+    // child b's, q1.y bytes 2-3 and q1.z (values 2-7 of q1.z:q1.y): its selector is a's + 2
+    uint32_t sna = R.qsel, sfa = sna ^ 0x00070503u;
+    uint32_t na = __builtin_amdgcn_perm(q1.y, q1.x, sna), fa = __builtin_amdgcn_perm(q1.y, q1.x, sfa);
+    uint32_t nb = __builtin_amdgcn_perm(q1.z, q1.y, sna + 0x00020202u),
+             fb = __builtin_amdgcn_perm(q1.z, q1.y, sfa + 0x00020202u);
     auto b = [](uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); };
-    // as slab_hit: the entry max(lo, 0) taken in one chain, and lo < t as l0 < t (t > 0)
-    auto slab = [&](float x0, float x1, float y0, float y1, float z0, float z1, bool *hit) {
-        float l0 = vmax(vmax(vmax(0.0f, vmin(x0, x1)), vmin(y0, y1)), vmin(z0, z1));
-        float hi = vmin(vmin(vmax(x0, x1), vmax(y0, y1)), vmax(z0, z1));
+    // as slab_hit: the entry max(0, nearest planes) in one chain, and lo < t as l0 < t (t > 0)
+    auto slab = [&](uint32_t wn, uint32_t wf, bool *hit) {
+        float l0 = vmax(vmax(vmax(0.0f, __builtin_fmaf(b(wn, 0), kx, cx)),
+                             __builtin_fmaf(b(wn, 1), ky, cy)),
+                        __builtin_fmaf(b(wn, 2), kz, cz));
+        float hi = vmin(vmin(__builtin_fmaf(b(wf, 0), kx, cx), __builtin_fmaf(b(wf, 1), ky, cy)),
+                        __builtin_fmaf(b(wf, 2), kz, cz));
         *hit = hi >= l0 && l0 < R.t;
         return l0;
     };
-    la = slab(__builtin_fmaf(b(q1.x, 0), kx, cx), __builtin_fmaf(b(q1.x, 3), kx, cx),
-              __builtin_fmaf(b(q1.x, 1), ky, cy), __builtin_fmaf(b(q1.y, 0), ky, cy),
-              __builtin_fmaf(b(q1.x, 2), kz, cz), __builtin_fmaf(b(q1.y, 1), kz, cz), &ha);
-    lb = slab(__builtin_fmaf(b(q1.y, 2), kx, cx), __builtin_fmaf(b(q1.z, 1), kx, cx),
-              __builtin_fmaf(b(q1.y, 3), ky, cy), __builtin_fmaf(b(q1.z, 2), ky, cy),
-              __builtin_fmaf(b(q1.z, 0), kz, cz), __builtin_fmaf(b(q1.z, 3), kz, cz), &hb);
+    la = slab(na, fa, &ha);
+    lb = slab(nb, fb, &hb);
     ra = q0.w;
     rb = q1.w;
 }
@@ -201,7 +195,7 @@ __device__ __forceinline__ bool inner_visit(const SceneView &S, Ray &R, const St
         quant_slabs(S, R, la, ha, lb, hb, ra, rb);
     } else {
         f3 amn, amx, bmn, bmx;
-        children<LAYOUT>(S, R.n, amn, amx, bmn, bmx, ra, rb);
+        children(S, R.n, amn, amx, bmn, bmx, ra, rb);
         la = slab_hit(R.o, R.inv, amn, amx, R.t, &ha);
         lb = slab_hit(R.o, R.inv, bmn, bmx, R.t, &hb);
     }

@@ -4,11 +4,13 @@
 // its two children (first, first + 1) so one fetch serves the pair the reference reads as three
 // 32 B records (trace.wgsl:90,111-112):
 //   WIDE   4 x 16 B: a.min xyz a.max.x | a.max.yz b.min.xy | b.min.z b.max xyz | a ref, b ref
-//   QUANT  2 x 16 B: origin | exponent per axis, a ref | 12 bytes of children's boxes, b ref
+//   QUANT  2 x 16 B: an origin word per axis, a ref | 12 bytes of children's boxes, b ref
 // QUANT boxes are 8-bit multiples of a power of two per axis above an origin at or below the
-// pair's min corner (an FP32 value whose low 8 bits are zero, so the axis's exponent byte fits
-// beside it), each rounded outward and checked against the device's own decode (origin + q * 2^e
-// in FP32: the product is exact, so one rounding), so every decoded box contains its FP32 box and
+// pair's min corner: an FP32 value whose low 9 bits are zero (a box face at a round coordinate
+// keeps it exactly), so the axis's word holds the step's biased exponent in its low 8 bits beside
+// a clear bit 8, and the word shifted left by 23 reads as the step 2^e. Corners are rounded
+// outward and checked against the device's own decode (origin + q * 2^e in FP32: the product is
+// exact, so one rounding) and in exact arithmetic, so every decoded box contains its FP32 box and
 // culling stays conservative. Both layouts carry each child's reference (a leaf's triangles, or
 // the inner child's index), so no leaf visit reads another record.
 #include <hip/hip_runtime.h>
@@ -97,12 +99,12 @@ bool validate_bvh(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, size
 
 // The power-of-two step of a quantized axis and the outward-rounded 8-bit corners of a child.
 // Returns false if 255 steps do not reach (the caller widens the step).
-// The largest FP32 value with its low 8 bits zero that is at most x (x finite).
+// The largest FP32 value with its low 9 bits zero that is at most x (x finite).
 static float origin_below(float x) {
     uint32_t u = bits(x);
-    if (!(u & 0xffu)) return x;
-    if (x >= 0.0f) return fl(u & 0xffffff00u);
-    return fl((u & 0xffffff00u) + 0x100u);  // a larger magnitude: further below
+    if (!(u & 0x1ffu)) return x;
+    if (x >= 0.0f) return fl(u & ~0x1ffu);
+    return fl((u & ~0x1ffu) + 0x200u);  // a larger magnitude: further below
 }
 
 static bool quantize_axis(float origin, uint32_t ebits, float lo, float hi, uint8_t *qlo,
@@ -165,10 +167,11 @@ static int tuned_records(const lb_node *nodes, size_t n_nodes, std::vector<float
         for (int ax = 0; ax < 3; ax++) {
             float lo = std::fmin(a.min[ax], b.min[ax]);
             float hi = std::fmax(a.max[ax], b.max[ax]);
-            float origin = origin_below(lo);
             float ext = hi - lo;
             int e = ext > 0.0f ? (int)std::ceil(std::log2((double)ext / 250.0)) : -100;
             if (e < -100) e = -100;
+            float origin = origin_below(lo);
+            if (!std::isfinite(origin)) return false;
             for (;; e++) {
                 uint32_t eb = (uint32_t)(e + 127);
                 if (quantize_axis(origin, eb, a.min[ax], a.max[ax], &qb[ax], &qb[3 + ax]) &&

@@ -205,8 +205,9 @@ static float boxt(v3 o, v3 inv, const float *mn, const float *mx, float tmax) {
 typedef struct { float t; uint32_t tri; int back; } hit_t;
 
 /* Not the reference: one inner-node visit over the TUNED form's 8-bit child boxes (a record of
- * lb_tuned_nodes: per axis the origin's bits with the step's exponent in the low byte, the 12
- * corner bytes), with the device's arithmetic (lb_traverse.h quant_slabs): the plane of byte q
+ * lb_tuned_nodes: per axis the origin's bits, their low 9 bits zero, with the step's biased
+ * exponent in the low 8, so the word shifted left by 23 reads as the step 2^e; the 12 corner
+ * bytes), with the device's arithmetic (lb_traverse.h quant_slabs): the plane of byte q
  * at t = fma(q, 2^e * inv, (origin - o) * inv), the entry max(0, ...) and the exit min, a child
  * hit when exit >= entry and entry < t, the near child the hit one of the lower entry (a on a tie),
  * the far one pushed when both are hit. Moves to the near child and returns 1, else returns 0. The
@@ -214,8 +215,8 @@ typedef struct { float t; uint32_t tri; int back; } hit_t;
  * differ, which decides between two hits only when their distances lie within a few ulps. */
 static int quant_visit(const uint32_t *q, v3 o, v3 inv, float tmax, uint32_t first, uint32_t *n,
                        uint32_t *stack, uint32_t *sp) {
-    float kx = bitsf((q[0] & 0xffu) << 23) * inv.x, ky = bitsf((q[1] & 0xffu) << 23) * inv.y,
-          kz = bitsf((q[2] & 0xffu) << 23) * inv.z;
+    float kx = bitsf(q[0] << 23) * inv.x, ky = bitsf(q[1] << 23) * inv.y,
+          kz = bitsf(q[2] << 23) * inv.z;
     float cx = (bitsf(q[0] & 0xffffff00u) - o.x) * inv.x;
     float cy = (bitsf(q[1] & 0xffffff00u) - o.y) * inv.y;
     float cz = (bitsf(q[2] & 0xffffff00u) - o.z) * inv.z;

@@ -48,10 +48,18 @@ def test_quantized_boxes_contain_their_fp32_boxes(gen):
     first = nodes["first"][inner].astype(np.int64)
     a, b = nodes[first], nodes[first + 1]
     q = quant[inner]
-    origin = (q[:, :3] & 0xFFFFFF00).view(np.float32).astype(np.float64)
+    # an axis's word: the origin's bits (low 9 zero) with the step's biased exponent in bits
+    # 0-7, so the word shifted left by 23 reads as the step
+    origin = (q[:, :3] & 0xFFFFFE00).view(np.float32).astype(np.float64)
     e = (q[:, :3] & 0xFF).astype(np.int64)
     assert (e > 0).all() and (e < 255).all()
+    assert not (q[:, :3] & 0x100).any()
     step = np.ldexp(1.0, e - 127)
+    shifted = (q[:, :3].astype(np.uint64) << np.uint64(23)) & np.uint64(0xFFFFFFFF)
+    assert np.array_equal(shifted.astype(np.uint32).view(np.float32).astype(np.float64), step)
+    # the origin lies at or below both children's boxes
+    lo = np.minimum(a["min"], b["min"]).astype(np.float64)
+    assert (origin <= lo).all()
     corner = q[:, 4:7].view(np.uint8).reshape(-1, 12).astype(np.float64)
     exact = origin[:, [0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2]] + corner * step[:, [0, 1, 2] * 4]
     # the sums are exact in double: 24-bit origins plus 8-bit multiples of a power of two a few
