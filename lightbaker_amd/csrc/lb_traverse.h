@@ -76,7 +76,7 @@ struct Ray {
     uint32_t n;    // the node to visit next
     uint32_t sp;   // stack entries
     uint32_t key;  // stops()'s ray key
-    uint32_t qsel; // quant_slabs' byte selector of the planes the ray meets first
+    uint32_t qsel[3];  // quant_slabs' byte selectors: near x y, near z far x, far y z
     bool back, any;
 };
 
@@ -93,9 +93,12 @@ __device__ __forceinline__ void ray_start(Ray &R, uint32_t root, f3 o, f3 d, flo
     R.key = key;
     R.any = any;
     // on an axis where inv >= 0 the planes' t rise with the byte, so the lower corner's byte
-    // gives the nearer plane; else the upper's (byte indices of child a's corners, quant_slabs)
-    R.qsel = (R.inv.x >= 0.0f ? 0u : 3u) | (R.inv.y >= 0.0f ? 1u : 4u) << 8 |
-             (R.inv.z >= 0.0f ? 2u : 5u) << 16 | 0x0c000000u;
+    // gives the nearer plane; else the upper's. Byte indices of child a's corners (quant_slabs),
+    // each in the low byte of a 16-bit half whose high byte selector 12 clears
+    bool px = R.inv.x >= 0.0f, py = R.inv.y >= 0.0f, pz = R.inv.z >= 0.0f;
+    R.qsel[0] = (px ? 0u : 3u) | (py ? 1u : 4u) << 16 | 0x0c000c00u;
+    R.qsel[1] = (pz ? 2u : 5u) | (px ? 3u : 0u) << 16 | 0x0c000c00u;
+    R.qsel[2] = (py ? 4u : 1u) | (pz ? 5u : 2u) << 16 | 0x0c000c00u;
 }
 
 // The triangles of leaf R.n (trace.wgsl:91-104); true when an any-hit ray accepted one.
@@ -135,18 +138,27 @@ __device__ __forceinline__ bool leaf_tests(const SceneView &S, const lb_params &
 // Inner node R.n (trace.wgsl:105-123): true when the ray moved on to its near child (the far one
 // pushed if it was hit too), false when it missed both.
 // A QUANT record's two slab tests without decoding the corners. Per axis the record's word holds
-// the origin (a float at or below both children's boxes, its low 9 bits zero) with the step's
-// biased exponent in bits 0-7, so the word shifted left by 23 reads as the step 2^e; the plane of
-// byte q then lies at
-// t = q * (2^e * inv) + (origin - o) * inv, one fused multiply-add a plane after six products a
-// node. The planes' t round differently from (corner - o) * inv on the decoded corners, by a few
-// ulps; the encoder keeps every corner outside its FP32 box in exact arithmetic as well as after
-// rounding, so the boxes stay conservative.
+// the origin (a float at or below both children's boxes, its low 9 bits zero) with a biased
+// exponent in bits 0-7, so the word shifted left by 23 reads as 2^24 times the step 2^e; the
+// plane of byte q lies at t = q * (2^e * inv) + (origin - o) * inv, one fused multiply-add a
+// plane after six products a node. The planes' t round differently from (corner - o) * inv on the
+// decoded corners, by a few ulps; the encoder keeps every corner outside its FP32 box in exact
+// arithmetic as well as after rounding, so the boxes stay conservative.
 // On each axis the nearer plane is known from the sign of inv (t is monotone in q, and so after
-// rounding), so a ray's selector (Ray::qsel) gathers the near and far bytes of each box with one
-// byte permute each: the slab test's per-axis min and max are exact selections.
+// rounding), so the ray's selectors (Ray::qsel) gather each box's near and far bytes, two to a
+// word, each in the low byte of a 16-bit half: read as an FP16 value that half is q * 2^-24
+// (a subnormal, exact), and the mixed-precision fused multiply-add q * 2^-24 * (2^(e+24) * inv)
+// + c reads it straight from the word and gives the same rounding as q * (2^e * inv) + c, the
+// scaling by 2^24 being exact. The slab test's per-axis min and max are exact selections.
 // Record: q0 = (origin words x, y, z, reference a), q1 = 12 corner bytes (a min xyz, a max xyz,
 // b min xyz, b max xyz), reference b.
+__device__ __forceinline__ float half_lo(uint32_t w) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+}
+__device__ __forceinline__ float half_hi(uint32_t w) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+}
+
 __device__ __forceinline__ void quant_slabs(const SceneView &S, const Ray &R, float &la, bool &ha,
                                             float &lb, bool &hb, uint32_t &ra, uint32_t &rb) {
     const uint4 *q = S.quant + R.n * 2u;
@@ -157,25 +169,24 @@ __device__ __forceinline__ void quant_slabs(const SceneView &S, const Ray &R, fl
     float cx = (bitsf(q0.x & 0xffffff00u) - R.o.x) * R.inv.x,
           cy = (bitsf(q0.y & 0xffffff00u) - R.o.y) * R.inv.y,
           cz = (bitsf(q0.z & 0xffffff00u) - R.o.z) * R.inv.z;
-    // child a's bytes are q1.x bytes 0-3 and q1.y bytes 0-1 (selector values 0-5 of q1.y:q1.x);
-    // child b's, q1.y bytes 2-3 and q1.z (values 2-7 of q1.z:q1.y): its selector is a's + 2
-    uint32_t sna = R.qsel, sfa = sna ^ 0x00070503u;
-    uint32_t na = __builtin_amdgcn_perm(q1.y, q1.x, sna), fa = __builtin_amdgcn_perm(q1.y, q1.x, sfa);
-    uint32_t nb = __builtin_amdgcn_perm(q1.z, q1.y, sna + 0x00020202u),
-             fb = __builtin_amdgcn_perm(q1.z, q1.y, sfa + 0x00020202u);
-    auto b = [](uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); };
-    // as slab_hit: the entry max(0, nearest planes) in one chain, and lo < t as l0 < t (t > 0)
-    auto slab = [&](uint32_t wn, uint32_t wf, bool *hit) {
-        float l0 = vmax(vmax(vmax(0.0f, __builtin_fmaf(b(wn, 0), kx, cx)),
-                             __builtin_fmaf(b(wn, 1), ky, cy)),
-                        __builtin_fmaf(b(wn, 2), kz, cz));
-        float hi = vmin(vmin(__builtin_fmaf(b(wf, 0), kx, cx), __builtin_fmaf(b(wf, 1), ky, cy)),
-                        __builtin_fmaf(b(wf, 2), kz, cz));
+    // as slab_hit: the entry max(0, nearest planes) in one chain, and lo < t as l0 < t (t > 0).
+    // Child a's bytes are selector values 0-5 of q1.y:q1.x; child b's, 2-7 of q1.z:q1.y: its
+    // selectors are a's with 2 added to both data bytes.
+    auto slab = [&](uint32_t hi_w, uint32_t lo_w, uint32_t add, bool *hit) {
+        uint32_t w1 = __builtin_amdgcn_perm(hi_w, lo_w, R.qsel[0] + add);
+        uint32_t w2 = __builtin_amdgcn_perm(hi_w, lo_w, R.qsel[1] + add);
+        uint32_t w3 = __builtin_amdgcn_perm(hi_w, lo_w, R.qsel[2] + add);
+        float l0 = vmax(vmax(vmax(0.0f, __builtin_fmaf(half_lo(w1), kx, cx)),
+                             __builtin_fmaf(half_hi(w1), ky, cy)),
+                        __builtin_fmaf(half_lo(w2), kz, cz));
+        float hi = vmin(vmin(__builtin_fmaf(half_hi(w2), kx, cx),
+                             __builtin_fmaf(half_lo(w3), ky, cy)),
+                        __builtin_fmaf(half_hi(w3), kz, cz));
         *hit = hi >= l0 && l0 < R.t;
         return l0;
     };
-    la = slab(na, fa, &ha);
-    lb = slab(nb, fb, &hb);
+    la = slab(q1.y, q1.x, 0u, &ha);
+    lb = slab(q1.z, q1.y, 0x00020002u, &hb);
     ra = q0.w;
     rb = q1.w;
 }

@@ -7,8 +7,9 @@
 //   QUANT  2 x 16 B: an origin word per axis, a ref | 12 bytes of children's boxes, b ref
 // QUANT boxes are 8-bit multiples of a power of two per axis above an origin at or below the
 // pair's min corner: an FP32 value whose low 9 bits are zero (a box face at a round coordinate
-// keeps it exactly), so the axis's word holds the step's biased exponent in its low 8 bits beside
-// a clear bit 8, and the word shifted left by 23 reads as the step 2^e. Corners are rounded
+// keeps it exactly), so the axis's word holds e + 151 in its low 8 bits beside a clear bit 8, and
+// the word shifted left by 23 reads as 2^24 times the step 2^e (the kernels read the bytes as
+// FP16 subnormals, q * 2^-24). Corners are rounded
 // outward and checked against the device's own decode (origin + q * 2^e in FP32: the product is
 // exact, so one rounding) and in exact arithmetic, so every decoded box contains its FP32 box and
 // culling stays conservative. Both layouts carry each child's reference (a leaf's triangles, or
@@ -179,9 +180,9 @@ static int tuned_records(const lb_node *nodes, size_t n_nodes, std::vector<float
                     ex[ax] = eb;
                     break;
                 }
-                if (e > 120) return false;
+                if (e >= 103) return false;  // e + 151 must fit the byte
             }
-            q[ax] = bits(origin) | ex[ax];
+            q[ax] = bits(origin) | (ex[ax] + 24u);
         }
         q[3] = ra;
         std::memcpy(&q[4], qb, 12);

@@ -205,10 +205,11 @@ static float boxt(v3 o, v3 inv, const float *mn, const float *mx, float tmax) {
 typedef struct { float t; uint32_t tri; int back; } hit_t;
 
 /* Not the reference: one inner-node visit over the TUNED form's 8-bit child boxes (a record of
- * lb_tuned_nodes: per axis the origin's bits, their low 9 bits zero, with the step's biased
- * exponent in the low 8, so the word shifted left by 23 reads as the step 2^e; the 12 corner
- * bytes), with the device's arithmetic (lb_traverse.h quant_slabs): the plane of byte q
- * at t = fma(q, 2^e * inv, (origin - o) * inv), the entry max(0, ...) and the exit min, a child
+ * lb_tuned_nodes: per axis the origin's bits, their low 9 bits zero, with e + 151 in the low 8,
+ * so the word shifted left by 23 reads as 2^24 times the step 2^e; the 12 corner bytes), with
+ * the device's arithmetic (lb_traverse.h quant_slabs): the plane of byte q at
+ * t = fma(q 2^-24, 2^(e+24) * inv, (origin - o) * inv) (the same rounding as
+ * fma(q, 2^e * inv, ...): the scalings are exact), the entry max(0, ...) and the exit min, a child
  * hit when exit >= entry and entry < t, the near child the hit one of the lower entry (a on a tie),
  * the far one pushed when both are hit. Moves to the near child and returns 1, else returns 0. The
  * boxes contain the reference's, so the same triangles can be met; the order and the entries
@@ -220,13 +221,15 @@ static int quant_visit(const uint32_t *q, v3 o, v3 inv, float tmax, uint32_t fir
     float cx = (bitsf(q[0] & 0xffffff00u) - o.x) * inv.x;
     float cy = (bitsf(q[1] & 0xffffff00u) - o.y) * inv.y;
     float cz = (bitsf(q[2] & 0xffffff00u) - o.z) * inv.z;
-    const uint8_t *b = (const uint8_t *)(q + 4); /* a min xyz, a max xyz, b min xyz, b max xyz */
-    float x0 = fmaf((float)b[0], kx, cx), y0 = fmaf((float)b[1], ky, cy), z0 = fmaf((float)b[2], kz, cz);
-    float x1 = fmaf((float)b[3], kx, cx), y1 = fmaf((float)b[4], ky, cy), z1 = fmaf((float)b[5], kz, cz);
+    const uint8_t *b8 = (const uint8_t *)(q + 4); /* a min xyz, a max xyz, b min xyz, b max xyz */
+    float b[12];
+    for (int k = 0; k < 12; k++) b[k] = (float)b8[k] * 0x1p-24f;
+    float x0 = fmaf(b[0], kx, cx), y0 = fmaf(b[1], ky, cy), z0 = fmaf(b[2], kz, cz);
+    float x1 = fmaf(b[3], kx, cx), y1 = fmaf(b[4], ky, cy), z1 = fmaf(b[5], kz, cz);
     float la = fmaxv(fmaxv(fmaxv(0.0f, fminv(x0, x1)), fminv(y0, y1)), fminv(z0, z1));
     float ea = fminv(fminv(fmaxv(x0, x1), fmaxv(y0, y1)), fmaxv(z0, z1));
-    x0 = fmaf((float)b[6], kx, cx); y0 = fmaf((float)b[7], ky, cy); z0 = fmaf((float)b[8], kz, cz);
-    x1 = fmaf((float)b[9], kx, cx); y1 = fmaf((float)b[10], ky, cy); z1 = fmaf((float)b[11], kz, cz);
+    x0 = fmaf(b[6], kx, cx); y0 = fmaf(b[7], ky, cy); z0 = fmaf(b[8], kz, cz);
+    x1 = fmaf(b[9], kx, cx); y1 = fmaf(b[10], ky, cy); z1 = fmaf(b[11], kz, cz);
     float lb = fmaxv(fmaxv(fmaxv(0.0f, fminv(x0, x1)), fminv(y0, y1)), fminv(z0, z1));
     float eb = fminv(fminv(fmaxv(x0, x1), fmaxv(y0, y1)), fmaxv(z0, z1));
     int ha = ea >= la && la < tmax, hb = eb >= lb && lb < tmax;

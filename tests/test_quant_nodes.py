@@ -48,15 +48,16 @@ def test_quantized_boxes_contain_their_fp32_boxes(gen):
     first = nodes["first"][inner].astype(np.int64)
     a, b = nodes[first], nodes[first + 1]
     q = quant[inner]
-    # an axis's word: the origin's bits (low 9 zero) with the step's biased exponent in bits
-    # 0-7, so the word shifted left by 23 reads as the step
+    # an axis's word: the origin's bits (low 9 zero) with e + 151 in bits 0-7 (the step 2^e), so
+    # the word shifted left by 23 reads as 2^24 times the step
     origin = (q[:, :3] & 0xFFFFFE00).view(np.float32).astype(np.float64)
-    e = (q[:, :3] & 0xFF).astype(np.int64)
-    assert (e > 0).all() and (e < 255).all()
+    e = (q[:, :3] & 0xFF).astype(np.int64) - 151
+    assert (e > -151).all() and (e < 104).all()
     assert not (q[:, :3] & 0x100).any()
-    step = np.ldexp(1.0, e - 127)
+    step = np.ldexp(1.0, e)
     shifted = (q[:, :3].astype(np.uint64) << np.uint64(23)) & np.uint64(0xFFFFFFFF)
-    assert np.array_equal(shifted.astype(np.uint32).view(np.float32).astype(np.float64), step)
+    assert np.array_equal(shifted.astype(np.uint32).view(np.float32).astype(np.float64),
+                          step * 2.0 ** 24)
     # the origin lies at or below both children's boxes
     lo = np.minimum(a["min"], b["min"]).astype(np.float64)
     assert (origin <= lo).all()
