@@ -195,3 +195,20 @@ def test_adam_fits_the_target(fitted):
           f"{np.abs(res['sky_sh'] - sh_t).mean():.4f}")
     assert last < 0.1 * first
     assert np.abs(res["sky"] - sky_t).mean() < np.abs(sky - sky_t).mean()
+
+
+def test_adam_recovers_the_l1_sky(fitted):
+    """The gradient sky (a constant and a term linear in y above the horizon) and the L1 term
+    overlap, so a fit of both recovers the light, not each parameter. With the gradient sky
+    held at the target's, the L1 coefficients are determined: Adam on albedo and L1 sky brings
+    the coefficients towards the target's."""
+    from lightbaker_amd.adjoint import fit
+    scene, sess, adj, *_ = fitted
+    (alb, sky, sun, sh), (alb_t, _, _, sh_t) = perturbed(scene)
+    target = adj.forward(alb_t, sky, sun, sh_t)
+    res = fit(adj, target, alb, sky, sun, sh, steps=60, lr=0.01, fit_sky="sh")
+    first, last = res["loss"][0], res["loss"][-1]
+    e0, e1 = np.abs(sh - sh_t).mean(), np.abs(res["sky_sh"] - sh_t).mean()
+    print(f"Adam (L1 sky): loss {first:.4g} -> {last:.4g}; L1 sky error {e0:.4f} -> {e1:.4f}")
+    assert last < 0.1 * first
+    assert e1 < 0.5 * e0
