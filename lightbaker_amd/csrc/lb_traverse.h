@@ -241,4 +241,70 @@ __device__ __forceinline__ bool ray_step(const SceneView &S, const lb_params &P,
     return pop(R, st);
 }
 
+// ---- where a short ray's traversal can start ---------------------------------------------------
+//
+// Rays that stay within a known box (the occlusion and inside-test rays of a texel: their origins
+// within the texel's footprint, lengths within their reach) meet every node's boxes the same way
+// near the root: while one child's box holds the whole region with margin m and the other's lies
+// apart from it by m, each ray's slab test hits the first (its planes' t lie behind the origin or
+// past the reach) and misses the second (past the reach, or behind), so the reference's loop goes
+// to the first child and pushes nothing. Starting such a ray at the deepest node reached that way,
+// with an empty stack, it then takes exactly the steps it would take from the root. The margin
+// covers the slab arithmetic's rounding: the planes' t err by less than 2^-22 (|q step| +
+// |origin - o|) |inv|, m |inv| with m of a centimetre and more for large coordinates.
+__device__ __forceinline__ bool region_inside(f3 lo, f3 hi, f3 mn, f3 mx, float m) {
+    return lo.x >= mn.x + m && lo.y >= mn.y + m && lo.z >= mn.z + m && hi.x <= mx.x - m &&
+           hi.y <= mx.y - m && hi.z <= mx.z - m;
+}
+__device__ __forceinline__ bool region_apart(f3 lo, f3 hi, f3 mn, f3 mx, float m) {
+    return hi.x < mn.x - m || hi.y < mn.y - m || hi.z < mn.z - m || lo.x > mx.x + m ||
+           lo.y > mx.y + m || lo.z > mx.z + m;
+}
+
+// The children's boxes of inner node n as the TUNED form's records hold them: QUANT corners
+// origin + q * 2^e in FP32 (the product exact; within m of what the slab planes stand for),
+// WIDE boxes as stored.
+template <int LAYOUT>
+__device__ __forceinline__ void record_boxes(const SceneView &S, uint32_t n, f3 &amn, f3 &amx,
+                                             f3 &bmn, f3 &bmx, uint32_t &ra, uint32_t &rb) {
+    if constexpr (LAYOUT == LB_NODES_QUANT) {
+        const uint4 *q = S.quant + n * 2u;
+        uint4 q0 = q[0], q1 = q[1];
+        float ox = bitsf(q0.x & 0xffffff00u), oy = bitsf(q0.y & 0xffffff00u),
+              oz = bitsf(q0.z & 0xffffff00u);
+        float sx = bitsf(q0.x << 23) * 0x1p-24f, sy = bitsf(q0.y << 23) * 0x1p-24f,
+              sz = bitsf(q0.z << 23) * 0x1p-24f;
+        auto b = [](uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); };
+        amn = mk(ox + b(q1.x, 0) * sx, oy + b(q1.x, 1) * sy, oz + b(q1.x, 2) * sz);
+        amx = mk(ox + b(q1.x, 3) * sx, oy + b(q1.y, 0) * sy, oz + b(q1.y, 1) * sz);
+        bmn = mk(ox + b(q1.y, 2) * sx, oy + b(q1.y, 3) * sy, oz + b(q1.z, 0) * sz);
+        bmx = mk(ox + b(q1.z, 1) * sx, oy + b(q1.z, 2) * sy, oz + b(q1.z, 3) * sz);
+        ra = q0.w;
+        rb = q1.w;
+    } else {
+        children(S, n, amn, amx, bmn, bmx, ra, rb);
+    }
+}
+
+// The node (a child reference) rays within [lo, hi] may start at.
+template <int LAYOUT>
+__device__ __forceinline__ uint32_t entry_node(const SceneView &S, f3 lo, f3 hi) {
+    float big = vmax(vmax(vmax(fabsf(lo.x), fabsf(lo.y)), vmax(fabsf(lo.z), fabsf(hi.x))),
+                     vmax(fabsf(hi.y), fabsf(hi.z)));
+    float m = 0.01f + 2e-6f * big;
+    uint32_t n = S.root;
+    while (!(n & LEAF_BIT)) {
+        f3 amn, amx, bmn, bmx;
+        uint32_t ra, rb;
+        record_boxes<LAYOUT>(S, n, amn, amx, bmn, bmx, ra, rb);
+        if (region_inside(lo, hi, amn, amx, m) && region_apart(lo, hi, bmn, bmx, m))
+            n = ra;
+        else if (region_inside(lo, hi, bmn, bmx, m) && region_apart(lo, hi, amn, amx, m))
+            n = rb;
+        else
+            break;
+    }
+    return n;
+}
+
 }  // namespace lbk

@@ -241,7 +241,7 @@ __device__ __forceinline__ bool inside_g(const TunedArgs &A, const StackView &st
             kq = key + 331u;
         }
         f3 d = cosine(((float)r + 0.5f) / 16.0f, radical(r), f);
-        Hit h = trace_t<LAYOUT, STATS>(A.S, A.P, A.S.root, add(p, lift), d, A.P.ground[3], false,
+        Hit h = trace_t<LAYOUT, STATS>(A.S, A.P, slot[35], add(p, lift), d, A.P.ground[3], false,
                                        kq + r + 7000u, st, cnt, A.leaf_batch);
         if (h.tri != NONE && h.back) back += 1.0f;
     }
@@ -332,7 +332,18 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_di
         {
             ItemIn tx = load_item(A.items, valid ? i : 0u);
             lit = dot3(sun, tx.n) > 0.0f;
+            // where the occlusion and inside-test rays start (entry_node): their origins lie
+            // within 0.45 of the texel's axes from its centre, or PUSH along the normal, all
+            // lifted; they reach at most aoReach or the inside reach
+            frame_t fi = frame(tx.n);
+            f3 c = add(tx.pos, scl(tx.n, P.reach[2]));
+            float r = vmax(P.reach[0], P.ground[3]) * 1.001f + 1e-4f;
+            f3 e = mk(0.45f * (fabsf(tx.dx.x) + fabsf(tx.dy.x)) + fabsf(fi.n.x) * PUSH + r,
+                      0.45f * (fabsf(tx.dx.y) + fabsf(tx.dy.y)) + fabsf(fi.n.y) * PUSH + r,
+                      0.45f * (fabsf(tx.dx.z) + fabsf(tx.dy.z)) + fabsf(fi.n.z) * PUSH + r);
+            uint32_t entry = entry_node<LAYOUT>(A.S, sub(c, e), add(c, e));
             if (lig == 0u) {
+                slot_lds[35] = entry;
                 uint32_t key = (A.key_base + i) * 7919u;
                 frame_t f = frame(tx.n), sf = frame(sun);
                 put_f3(slot_lds, tx.pos);
@@ -415,7 +426,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_di
                 uint32_t r = slot - n_sun;
                 r = az_slot(r, rays, s2);
                 f3 d = cosine(fract(((float)r + 0.5f) / (float)rays + s1), fract(radical(r) + s2), f);
-                ray_start(R, A.S.root, add(at, lift), d, P.reach[0], false, key + r + 2000u);
+                ray_start(R, slot_lds[35], add(at, lift), d, P.reach[0], false, key + r + 2000u);
             }
         };
         auto shade = [&](uint32_t slot, const Ray &R) {

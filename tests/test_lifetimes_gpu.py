@@ -35,23 +35,28 @@ def _close(a, b, tol=1e-4):
     return float(err.max())
 
 
-def test_graph_recaptured_when_spill_area_regrows(gpu):
+def test_graph_recaptured_when_spill_area_regrows():
     """A direct-pass tuning with a shallower LDS stack regrows the GPU-wide spill area in the
     next execute's direct launch; the bounce graph, which spills into that area, is captured
-    again instead of replayed, and the light stays within the lock-step tolerance."""
+    again instead of replayed, and the light stays within the lock-step tolerance. The GPU is
+    opened here, so its spill area holds only what this bake's launches asked for (a shared
+    one may have grown past the new need in earlier launches)."""
     print("graph key: spill area regrown between executes", flush=True)
-    sess = BakeSession(scene2(), gpu, form=N.FORM_TUNED, graph=True)
+    g = Gpu()
+    g.set_tuning(256, 4, 16, N.NODES_QUANT, 32, 0, leaf_batch=16)
+    sess = BakeSession(scene2(), g, form=N.FORM_TUNED, graph=True)
     sess.execute()
     first = _light(sess)
     assert sess.info()["graph_captures"] == 1
     sess.execute()
     assert sess.info()["graph_captures"] == 1, "same key: the graph is replayed"
     assert np.array_equal(_light(sess), first)
-    gpu.set_tuning(pass_=Pass.DIRECT, lds_stack=8)
+    g.set_tuning(pass_=Pass.DIRECT, lds_stack=8)
     sess.execute()
     assert sess.info()["graph_captures"] == 2, "a regrown spill area captures again"
     assert _close(_light(sess), first) < 1e-4
     sess.close()
+    g.close()
 
 
 def test_graph_recaptured_on_every_gather_knob(gpu):
