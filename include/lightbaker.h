@@ -197,9 +197,10 @@ int lb_get_pass_tuning(const lb_gpu *gpu, int pass, lb_tuning *tuning);
 
 /* No counterpart in the reference (its nodes are the 32 B records alone). The TUNED form's node
  * records lb_scene_create derives from a BVH, built on the host with no device: per node index,
- * the QUANT record (8 u32: for each axis the origin's FP32 bits with the axis's step exponent
- * (step 2^(e - 127)) in the low byte; child a's reference; 12 corner bytes, child a's min xyz and
- * max xyz then child b's, each corner origin + byte * step; child b's reference) and the WIDE
+ * the QUANT record (8 u32: for each axis the origin's FP32 bits, their low 9 bits zero, with
+ * e + 151 in the low byte for the axis's step 2^e, so the word shifted left by 23 reads as 2^24
+ * times the step; child a's reference; 12 corner bytes, child a's min xyz and max xyz then child
+ * b's, each corner origin + byte * step; child b's reference) and the WIDE
  * record (16 f32: both children's boxes, then their references as bits). Leaves' records are
  * zero. A reference is an inner child's node index, or bit 31 with a leaf's count in bits 26..30
  * and first triangle in bits 0..25 (count 31: the leaf's node index instead). Either output may

@@ -98,8 +98,6 @@ bool validate_bvh(const lb_node *nodes, size_t n_nodes, const lb_tri *tris, size
     return true;
 }
 
-// The power-of-two step of a quantized axis and the outward-rounded 8-bit corners of a child.
-// Returns false if 255 steps do not reach (the caller widens the step).
 // The largest FP32 value with its low 9 bits zero that is at most x (x finite).
 static float origin_below(float x) {
     uint32_t u = bits(x);
@@ -108,6 +106,8 @@ static float origin_below(float x) {
     return fl((u & ~0x1ffu) + 0x200u);  // a larger magnitude: further below
 }
 
+// The outward-rounded 8-bit corners of a child on one axis, for the step 2^(ebits - 127).
+// Returns false if 255 steps do not reach (the caller widens the step).
 static bool quantize_axis(float origin, uint32_t ebits, float lo, float hi, uint8_t *qlo,
                           uint8_t *qhi) {
     float step = fl(ebits << 23);
