@@ -76,7 +76,7 @@ struct Ray {
     uint32_t n;    // the node to visit next
     uint32_t sp;   // stack entries
     uint32_t key;  // stops()'s ray key
-    uint32_t qsel[3];  // quant_slabs' byte selectors: near x y, near z far x, far y z
+    uint32_t qsel[6];  // quant_slabs' byte selectors: near x y, near z far x, far y z; b's
     bool back, any;
 };
 
@@ -99,6 +99,9 @@ __device__ __forceinline__ void ray_start(Ray &R, uint32_t root, f3 o, f3 d, flo
     R.qsel[0] = (px ? 0u : 3u) | (py ? 1u : 4u) << 16 | 0x0c000c00u;
     R.qsel[1] = (pz ? 2u : 5u) | (px ? 3u : 0u) << 16 | 0x0c000c00u;
     R.qsel[2] = (py ? 4u : 1u) | (pz ? 5u : 2u) << 16 | 0x0c000c00u;
+    R.qsel[3] = R.qsel[0] + 0x00020002u;
+    R.qsel[4] = R.qsel[1] + 0x00020002u;
+    R.qsel[5] = R.qsel[2] + 0x00020002u;
 }
 
 // The triangles of leaf R.n (trace.wgsl:91-104); true when an any-hit ray accepted one.
@@ -171,11 +174,11 @@ __device__ __forceinline__ void quant_slabs(const SceneView &S, const Ray &R, fl
           cz = (bitsf(q0.z & 0xffffff00u) - R.o.z) * R.inv.z;
     // as slab_hit: the entry max(0, nearest planes) in one chain, and lo < t as l0 < t (t > 0).
     // Child a's bytes are selector values 0-5 of q1.y:q1.x; child b's, 2-7 of q1.z:q1.y: its
-    // selectors are a's with 2 added to both data bytes.
-    auto slab = [&](uint32_t hi_w, uint32_t lo_w, uint32_t add, bool *hit) {
-        uint32_t w1 = __builtin_amdgcn_perm(hi_w, lo_w, R.qsel[0] + add);
-        uint32_t w2 = __builtin_amdgcn_perm(hi_w, lo_w, R.qsel[1] + add);
-        uint32_t w3 = __builtin_amdgcn_perm(hi_w, lo_w, R.qsel[2] + add);
+    // selectors (qsel[3..5]) are a's with 2 added to both data bytes.
+    auto slab = [&](uint32_t hi_w, uint32_t lo_w, int b, bool *hit) {
+        uint32_t w1 = __builtin_amdgcn_perm(hi_w, lo_w, R.qsel[3 * b]);
+        uint32_t w2 = __builtin_amdgcn_perm(hi_w, lo_w, R.qsel[3 * b + 1]);
+        uint32_t w3 = __builtin_amdgcn_perm(hi_w, lo_w, R.qsel[3 * b + 2]);
         float l0 = vmax(vmax(vmax(0.0f, __builtin_fmaf(half_lo(w1), kx, cx)),
                              __builtin_fmaf(half_hi(w1), ky, cy)),
                         __builtin_fmaf(half_lo(w2), kz, cz));
@@ -185,8 +188,8 @@ __device__ __forceinline__ void quant_slabs(const SceneView &S, const Ray &R, fl
         *hit = hi >= l0 && l0 < R.t;
         return l0;
     };
-    la = slab(q1.y, q1.x, 0u, &ha);
-    lb = slab(q1.z, q1.y, 0x00020002u, &hb);
+    la = slab(q1.y, q1.x, 0, &ha);
+    lb = slab(q1.z, q1.y, 1, &hb);
     ra = q0.w;
     rb = q1.w;
 }
