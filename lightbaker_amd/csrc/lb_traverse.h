@@ -28,14 +28,16 @@ struct StackView {
     lds_u32 *lds;    // this lane's entry 0; entry k at lds[k]
     glb_u32 *spill;  // this lane's entry `depth`; entry k at spill[k - depth]
     uint32_t depth;
+    // The LDS access is taken whatever k (an entry past the LDS part goes to word `depth`, the
+    // lane's spare: lane_stride(depth) > depth), so only the rare spill takes a branch.
     __device__ __forceinline__ void put(uint32_t k, uint32_t v) const {
-        if (k < depth)
-            lds[k] = v;
-        else
-            spill[k - depth] = v;
+        lds[k < depth ? k : depth] = v;
+        if (k >= depth) spill[k - depth] = v;
     }
     __device__ __forceinline__ uint32_t get(uint32_t k) const {
-        return k < depth ? lds[k] : spill[k - depth];
+        uint32_t v = lds[k < depth ? k : depth];
+        if (k >= depth) v = spill[k - depth];
+        return v;
     }
 };
 
