@@ -118,7 +118,9 @@ __device__ __forceinline__ bool leaf_tests(const SceneView &S, const lb_params &
     }
     if (STATS) cnt.leaves++;
     for (uint32_t i = first; i < first + count; i++) {
-        float4 v0 = S.tris_t[i * 3u], e1 = S.tris_t[i * 3u + 1u], e2 = S.tris_t[i * 3u + 2u];
+        // one address, the three records at immediate offsets
+        const float4 *tp = S.tris_t + (size_t)i * 3u;
+        float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
         // the three loads are issued together and waited for once: without this the compiler
         // sinks v0's load past the determinant test, a second memory round trip
         asm volatile("" ::"v"(v0.x), "v"(v0.y), "v"(v0.z), "v"(e2.w));
