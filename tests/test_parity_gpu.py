@@ -4,8 +4,8 @@ items, keys and scene, through the C-ABI.
 Bar: the PLAIN form reproduces every output bit for bit (hit-derived bits, counts and FP32 sums in
 the reference's order). The TUNED form reproduces every hit-derived quantity exactly (shadow
 bits, pick, sun/soft/back counts) and the ray sums within 2e-6 relative + 1e-6 absolute (its
-sums are Kahan-compensated in a different order); with 8-bit quantized nodes the hit sets stay
-the same unless a ray grazes a box edge, so at most 0.5% of items may differ in counts.
+sums are Kahan-compensated in a different order); with 8-bit quantized nodes the same bar holds
+against the oracle traversing the same 8-bit records (oracle.run_quant), no item excepted.
 """
 from __future__ import annotations
 
