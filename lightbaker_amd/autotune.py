@@ -24,7 +24,7 @@ RANGES = {
     "rays_per_lane": [1, 2, 4, 8, 16],
     "refill": [16, 32, 48, 56],
     "leaf_batch": [8, 16, 24, 32, 48, 64],
-    "workgroup": [64, 128, 256, 512],
+    "workgroup": [64, 128, 192, 256, 512],
     "lds_stack": [8, 16, 24, 32, 48],
     "nodes": [N.NODES_WIDE, N.NODES_QUANT],
     "waves": [0, 6, 8],
