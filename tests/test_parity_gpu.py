@@ -103,7 +103,8 @@ def test_tuned_matches(gpu, setup, pass_, nodes):
 
 
 @pytest.mark.parametrize("cfg", [(64, 1, 8, 0, 64), (128, 4, 24, 6, 16), (512, 8, 24, 8, 1),
-                                 (256, 2, 32, 0, 32), (256, 8, 48, 6, 8), (256, 4, 16, 8, 24)])
+                                 (256, 2, 32, 0, 32), (256, 8, 48, 6, 8), (256, 4, 16, 8, 24),
+                                 (192, 4, 16, 8, 16)])
 def test_tuned_configs(gpu, setup, cfg):
     """Every autotuner setting gives the same results (workgroup, rays/lane, LDS stack, waves
     a SIMD, leaf batch), on every pass."""
@@ -187,7 +188,7 @@ def test_pooled_bit_exact(gpu, setup, pass_):
 
 
 @pytest.mark.parametrize("cfg", [(64, 8, 1, 0, 1), (512, 48, 64, 0, 64), (256, 16, 32, 5000, 8),
-                                 (128, 24, 8, 777, 32)])
+                                 (128, 24, 8, 777, 32), (192, 16, 32, 0, 16)])
 def test_pooled_configs(gpu, setup, cfg):
     """Workgroup, LDS stack, refill threshold, leaf batch and chunking (777 rays: a few items a
     chunk) leave every output bit as it is."""
