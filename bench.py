@@ -8,8 +8,10 @@ all on inputs already resident in HBM. `value` is the rays those passes trace pe
 (counted by the kernels' own counters in a separate run), over the wall time of K such steps.
 
 Also reported: the roofline of the dominant kernel (the gather pass), the PLAIN form's time per
-step on the same map, and the oracle (the C restatement of the passes) timed on the host's cores
-on a bounded sample of the gather pass.
+step on the same map, the gather pass through the reference's host-buffer boundary (lb_run, the
+PCIe copies inside the call) beside the same pass on resident buffers (`config.host_boundary`),
+and the oracle (the C restatement of the passes) timed on the host's cores on a bounded sample of
+the gather pass.
 
 `--workload rebake` measures a rebake instead: lightbaker_amd.scenes.rebake_scene(3), a 200 x
 200 m map in 64 tiles of 25 m, each laid out at its own density of 4-16 texels/m (texel counts
@@ -232,6 +234,51 @@ def cpu_baseline(sess, gpu, volume_rays: int, target_s: float = 12.0,
     base["deps_filter"] = {"value": len(zones) * len(pos) / dt / 1e9, "unit": "Gpairs/s",
                            "cores": 1, "sample": f"4 zones x {len(pos)} triangles"}
     return base
+
+
+def host_boundary(sess, gpu, form: int, reps: int = 3) -> dict:
+    """The gather pass through the reference's own boundary, lb_run (gpu/mod.rs:140-196: host
+    texels and light in, host results out, so the copies over PCIe are inside the call), beside
+    lb_run_device on the same items and light already in HBM. The bake's gather items and final
+    light atlas, the bake's tuning, unranked (lb_run has no first bounce to rank by); the median
+    of `reps` calls after one warm-up each way."""
+    from lightbaker_amd import _native as N
+    from lightbaker_amd.gpu import Bvh, DeviceArray, Pass
+    texels = sess.array(N.BAKE_TEXELS, N.TEXEL)
+    items = np.ascontiguousarray(texels[sess.array(N.BAKE_GATHER_INDEX, np.uint32)])
+    light = np.ascontiguousarray(sess.array(N.BAKE_LIGHT, np.float32).reshape(-1, 4))
+    params = sess.array(N.BAKE_PARAMS, N.PARAMS)
+    sb = gpu.scene(Bvh(sess.array(N.BAKE_NODES, N.NODE), sess.array(N.BAKE_TRIS, N.TRI)),
+                   sess.array(N.BAKE_MATERIALS, np.float32).reshape(-1, 4),
+                   sess.array(N.BAKE_CHARTS, np.float32).reshape(-1, 8))
+    gpu.set_form(form)
+    n = len(items)
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            gpu.sync()
+            t0 = time.perf_counter()
+            fn()
+            gpu.sync()
+            ts.append((time.perf_counter() - t0) * 1000.0)
+        return float(np.median(ts))
+
+    ms_host = timed(lambda: gpu.run(Pass.GATHER, sb, params, items, light))
+    d_items, d_light = DeviceArray.of(gpu, items), DeviceArray.of(gpu, light)
+    d_out = DeviceArray(gpu, n * 16)
+    ms_dev = timed(lambda: gpu.run_device(Pass.GATHER, form, sb, params, d_items, n, d_light,
+                                          len(light), d_out))
+    for a in (d_items, d_light, d_out):
+        a.free()
+    sb.close()
+    gpu.set_form(N.FORM_PLAIN)
+    rays = n * int(params["counts"][0][2])
+    return {"pass": "gather", "items": n, "rays": rays,
+            "bytes_to_device": items.nbytes + light.nbytes, "bytes_to_host": n * 16,
+            "ms_host_buffers": ms_host, "grays_host_buffers": rays / ms_host / 1e6,
+            "ms_device_resident": ms_dev, "grays_device_resident": rays / ms_dev / 1e6}
 
 
 def tris_of(sess) -> np.ndarray:
@@ -480,6 +527,7 @@ def main() -> int:
     cpu = None if args.no_cpu else cpu_baseline(
         sess, gpu, scene.volume.rays if scene.volume is not None else 1)
     dep = deps_timing(gpu, scene)
+    hostb = host_boundary(sess, gpu, form) if args.workload == "map" else None
     if args.workload == "map":
         metric = "Grays/s, 100x100 m 3-bounce bake (direct + 3 gather bounces + volume)"
         workload = "map_scene(2): 100x100 m, 8 texels/m, 256 GI rays, 3 bounces, 16 tiles"
@@ -538,6 +586,7 @@ def main() -> int:
                                             + stats[2]["rays"]) if replay_ms else None,
             "replay_vs_plain": (plain_ms / replay_ms) if plain_ms and replay_ms else None,
             "deps": dep,
+            "host_boundary": hostb,
         },
         "roofline": roof,
         "cpu_baseline": cpu,
