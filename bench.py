@@ -102,10 +102,11 @@ def roofline(kernel: str, ms: float, algorithmic_bytes: float, record: dict | No
     return out
 
 
-def cpu_baseline(sess, gpu, volume_rays: int, target_s: float = 12.0,
+def cpu_baseline(sess, gpu, volume_rays: int, target_s: float = 5.0,
                  max_items: int = 500000) -> dict:
     """The CPU leg: the oracle (oracle/trace_ref.c, FP32, OpenMP over items) timed on a sample of
-    the gather pass, and the error of the PLAIN form and of the TUNED form with each node layout
+    the gather pass (sized to about `target_s` seconds, then the median of 3 runs after that
+    warm-up), and the error of the PLAIN form and of the TUNED form with each node layout
     against the oracle's FP64 tracer on identical rays (gather: the gather items, a random
     `max_items` of them where there are more; direct: 200 000 random covered texels; volume:
     20 000 random probes), with the final light atlas."""
@@ -137,12 +138,14 @@ def cpu_baseline(sess, gpu, volume_rays: int, target_s: float = 12.0,
         n = min(len(items), n * 4)
         dt = run(n)
     n = int(min(len(items), max(256, n * target_s / max(dt, 1e-3))))
-    dt = run(n)
+    runs = sorted(run(n) for _ in range(3))
+    dt = runs[1]
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     base = {"value": n * rays_per_item / dt / 1e9, "unit": "Grays/s", "cores": threads,
             "kind": "port",
             "sample": f"{n} gather items x {rays_per_item} rays of the last bounce "
-                      f"(random subset, final light atlas), {dt:.1f} s"}
+                      f"(random subset, final light atlas), median of 3 runs "
+                      f"({', '.join(f'{t:.2f}' for t in runs)} s)"}
 
     # error against FP64 on identical rays; the key of an item is its index in the pass
     sb = gpu.scene(Bvh(nodes, tris), mats, charts)
