@@ -687,7 +687,10 @@ int bake_execute(lb_bake *B, lb_stats *per_pass) {
                 int rc = 0;
                 for (uint32_t b = 0; b < sc.gi_bounces && !rc; b++) rc = bounce(B, nullptr, b);
                 hipError_t ce = hipStreamEndCapture(st, &gr);
-                if (rc) return 1;
+                if (rc) {
+                    if (ce == hipSuccess) hipGraphDestroy(gr);
+                    return 1;
+                }
                 if (ce != hipSuccess) return hip_fail(ce, "graph capture");
                 hipError_t ie = hipGraphInstantiate(&B->graph, gr, nullptr, nullptr, 0);
                 hipGraphDestroy(gr);
