@@ -43,11 +43,14 @@ __device__ __forceinline__ f3 get_f3(const lds_u32 *s) {
 // trace.wgsl:83-130 over the tuned node records, to the end, the lanes of a wave in step: they
 // go through inner nodes until leaf_batch of the lanes still tracing stand at a leaf (64: until
 // none is at an inner node), then those take their leaves, and again. The count is the exec
-// mask's population, a scalar operation, so a step costs no extra vector instruction. Each lane
-// still takes its own ray's steps in the reference's order.
-template <int LAYOUT, bool STATS, typename C>
+// mask's population, a scalar operation, so a step costs no extra vector instruction; it is
+// taken every VISITS inner visits (a lane that reaches a leaf before then waits for the count),
+// which saves the loop's scalar bookkeeping between them. Each lane still takes its own ray's
+// steps in the reference's order.
+template <int LAYOUT, bool STATS, int VISITS, typename C>
 __device__ __forceinline__ void trace_locked(const SceneView &S, const lb_params &P, Ray &R,
                                              const StackView &st, C &cnt, int leaf_batch) {
+    static_assert(VISITS >= 1 && VISITS <= 4, "1 to 4 visits between counts");
     for (;;) {
         bool done = false;
         // lanes that may stay at inner nodes before the wave turns to the leaves
@@ -57,6 +60,28 @@ __device__ __forceinline__ void trace_locked(const SceneView &S, const lb_params
                 done = true;
                 break;
             }
+            // the further visits written out: a loop over them compiles to slower code
+            if constexpr (VISITS >= 2) {
+                if (R.n & LEAF_BIT) break;
+                if (!inner_visit<LAYOUT, STATS>(S, R, st, cnt) && pop(R, st)) {
+                    done = true;
+                    break;
+                }
+            }
+            if constexpr (VISITS >= 3) {
+                if (R.n & LEAF_BIT) break;
+                if (!inner_visit<LAYOUT, STATS>(S, R, st, cnt) && pop(R, st)) {
+                    done = true;
+                    break;
+                }
+            }
+            if constexpr (VISITS >= 4) {
+                if (R.n & LEAF_BIT) break;
+                if (!inner_visit<LAYOUT, STATS>(S, R, st, cnt) && pop(R, st)) {
+                    done = true;
+                    break;
+                }
+            }
             if (__popcll(__ballot(1)) <= keep) break;
         }
         if (done) break;
@@ -64,16 +89,22 @@ __device__ __forceinline__ void trace_locked(const SceneView &S, const lb_params
     }
 }
 
-template <int LAYOUT, bool STATS, typename C>
+template <int LAYOUT, bool STATS, int VISITS, typename C>
 __device__ Hit trace_t(const SceneView &S, const lb_params &P, uint32_t root, f3 o, f3 d,
                        float tmax, bool any, uint32_t ray, const StackView &st, C &cnt,
                        int leaf_batch) {
     Ray R;
     ray_start(R, root, o, d, tmax, any, ray);
     if (STATS) cnt.rays++;
-    trace_locked<LAYOUT, STATS>(S, P, R, st, cnt, leaf_batch);
+    trace_locked<LAYOUT, STATS, VISITS>(S, P, R, st, cnt, leaf_batch);
     return Hit{R.t, R.tri, R.back};
 }
+
+// Inner visits between the lock-step loop's counts of its lanes (trace_locked): 3 took the map's
+// ranked gather launch from 10.84 to 10.23 ms and its direct pass from 11.40 to 10.73 ms (2 and
+// 4 gave less on the city). The gather's counting launch, whose rays are unranked and part
+// sooner, keeps a count every visit: 2 made it slower.
+constexpr int LOCKED_VISITS = 3;
 
 // ---- group reductions -------------------------------------------------------------------------
 
@@ -216,7 +247,7 @@ __device__ __forceinline__ ItemIn load_item(const lb_texel *items, uint32_t i) {
 
 // trace.wgsl:159-168 with the 16 rays over the group, from point q of the item whose state is in
 // the group's LDS slot (the texel centre, an offset, or the centre pushed along the normal)
-template <int LAYOUT, bool STATS>
+template <int LAYOUT, bool STATS, int VISITS>
 __device__ __forceinline__ bool inside_g(const TunedArgs &A, const StackView &st,
                                          const lds_u32 *slot, uint32_t q, uint32_t lig, int L,
                                          Counts &cnt) {
@@ -241,8 +272,9 @@ __device__ __forceinline__ bool inside_g(const TunedArgs &A, const StackView &st
             kq = key + 331u;
         }
         f3 d = cosine(((float)r + 0.5f) / 16.0f, radical(r), f);
-        Hit h = trace_t<LAYOUT, STATS>(A.S, A.P, slot[35], add(p, lift), d, A.P.ground[3], false,
-                                       kq + r + 7000u, st, cnt, A.leaf_batch);
+        Hit h = trace_t<LAYOUT, STATS, VISITS>(A.S, A.P, slot[35], add(p, lift), d,
+                                               A.P.ground[3], false, kq + r + 7000u, st, cnt,
+                                               A.leaf_batch);
         if (h.tri != NONE && h.back) back += 1.0f;
     }
     back = group_sum(back, L);
@@ -290,7 +322,7 @@ __device__ __forceinline__ uint32_t el_az_slot(uint32_t s, uint32_t total, float
 // The slot loop of one item's rays: each lane traces its slot to the end (trace_locked), then
 // takes slot + L. start(slot, R) sets up slot's ray; shade(slot, R) folds its result into the
 // lane's sums.
-template <int LAYOUT, bool STATS, typename C, typename Start, typename Shade>
+template <int LAYOUT, bool STATS, int VISITS, typename C, typename Start, typename Shade>
 __device__ __forceinline__ void run_slots(const TunedArgs &A, const StackView &st, uint32_t first,
                                           uint32_t total, uint32_t L, C &cnt, Start start,
                                           Shade shade) {
@@ -298,7 +330,7 @@ __device__ __forceinline__ void run_slots(const TunedArgs &A, const StackView &s
     for (uint32_t slot = first; slot < total; slot += L) {
         start(slot, R);
         if (STATS) cnt.rays++;
-        trace_locked<LAYOUT, STATS>(A.S, A.P, R, st, cnt, A.leaf_batch);
+        trace_locked<LAYOUT, STATS, VISITS>(A.S, A.P, R, st, cnt, A.leaf_batch);
         shade(slot, R);
     }
 }
@@ -364,9 +396,9 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_di
         // trace.wgsl:174-193: the point the rays leave from, 0 (the centre) unless the centre
         // lies inside a solid
         uint32_t pick = 0u;
-        if (valid && inside_g<LAYOUT, STATS>(A, st, slot_lds, 0u, lig, L, cnt)) {
+        if (valid && inside_g<LAYOUT, STATS, LOCKED_VISITS>(A, st, slot_lds, 0u, lig, L, cnt)) {
             for (uint32_t q = 1u; q < 10u; q++)
-                if (!inside_g<LAYOUT, STATS>(A, st, slot_lds, q, lig, L, cnt)) {
+                if (!inside_g<LAYOUT, STATS, LOCKED_VISITS>(A, st, slot_lds, q, lig, L, cnt)) {
                     pick = q;
                     break;
                 }
@@ -443,7 +475,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_di
                 if (R.back && R.t < P.ground[3]) back += 1.0f;
             }
         };
-        run_slots<LAYOUT, STATS>(A, st, lig, total, (uint32_t)L, cnt, start, shade);
+        run_slots<LAYOUT, STATS, LOCKED_VISITS>(A, st, lig, total, (uint32_t)L, cnt, start,
+                                                shade);
         hard = group_or(hard, L);
         soft = group_sum(soft, L);
         back = group_sum(back, L);
@@ -563,8 +596,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
                 sb.add(l.z);
             }
         };
-        run_slots<LAYOUT, STATS>(A, st, lig, valid ? rays : 0u, (uint32_t)L, cnt, start,
-                                        shade);
+        run_slots<LAYOUT, STATS, MODE == 1 ? 1 : LOCKED_VISITS>(
+            A, st, lig, valid ? rays : 0u, (uint32_t)L, cnt, start, shade);
         group_ksum(sr, L);
         group_ksum(sg, L);
         group_ksum(sb, L);
@@ -656,8 +689,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_vo
                 sun += 1.0f;
             }
         };
-        run_slots<LAYOUT, STATS>(A, st, lig, valid ? rays + gg : 0u, (uint32_t)L, cnt,
-                                        start, shade);
+        run_slots<LAYOUT, STATS, LOCKED_VISITS>(A, st, lig, valid ? rays + gg : 0u, (uint32_t)L,
+                                                cnt, start, shade);
 #pragma unroll
         for (int k = 0; k < 12; k++) group_ksum(c[k], L);
         back = group_sum(back, L);
