@@ -393,7 +393,7 @@ def main() -> int:
     gpu.set_tuning(256, 4, 16, layout, 32, args.schedule, leaf_batch=16, waves=6)
     # the autotuner's usual picks on this map, so a run without it uses the same kernels
     gpu.set_tuning(pass_=Pass.GATHER, rays_per_lane=8, waves=8, xcd_split=1)
-    gpu.set_tuning(pass_=Pass.DIRECT, rays_per_lane=8, waves=8)
+    gpu.set_tuning(pass_=Pass.DIRECT, rays_per_lane=8, waves=8, leaf_batch=8)
     if args.xcd_split is not None:
         for p in Pass:
             gpu.set_tuning(pass_=p, xcd_split=args.xcd_split)
