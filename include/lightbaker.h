@@ -146,7 +146,7 @@ int lb_run_device(lb_gpu *gpu, int pass, int form, const lb_scene *scene, const 
 /* No counterpart in the reference. For callers that cannot allocate device memory: code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. As lb_run_device, but
  * the TUNED form's lock-step launch takes its work-queue heads and traversal spill area from a
- * static per-device scratch of 64 MiB (1 KiB of queue heads, the rest spill) instead of buffers
+ * static per-device scratch (1 KiB of queue heads, then 64 MiB of spill) instead of buffers
  * the library grows, and no call allocates or frees device memory. The caller still passes every
  * input and output. The PLAIN and WATERTIGHT forms use no scratch and run as lb_run_device.
  * Errors: the pooled schedule (its hits need memory sized by the launch), and a launch whose

@@ -18,12 +18,14 @@ namespace {
 thread_local std::string g_error;
 
 // lb_run_device_noalloc's scratch: the TUNED lock-step launch's work-queue heads (the first
-// 1 KiB), then its traversal spill area. 64 MiB holds the spill of every lane a 256-CU device
-// keeps resident (32 waves a CU) with 16 or more of the 48 stack entries in LDS. Each device
-// holds its own copy for the life of the process; host code reaches it only through the address
-// hipGetSymbolAddress gives on that device (lb_gpu.scratch), never through this name.
-constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+// 1 KiB), then its traversal spill area of 64 MiB, which holds the spill of every lane a 256-CU
+// device keeps resident (32 waves a CU: 524 288 lanes x 32 entries x 4 B) with 16 or more of
+// the 48 stack entries in LDS. Each device holds its own copy for the life of the process; host
+// code reaches it only through the address hipGetSymbolAddress gives on that device
+// (lb_gpu.scratch), never through this name.
 constexpr size_t SCRATCH_QUEUE_BYTES = 1024;
+constexpr size_t SCRATCH_SPILL_BYTES = size_t(64) << 20;
+constexpr size_t SCRATCH_BYTES = SCRATCH_QUEUE_BYTES + SCRATCH_SPILL_BYTES;
 static __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];
 }
 
@@ -268,7 +270,7 @@ int lb_run_device_noalloc(lb_gpu *gpu, int pass, int form, const lb_scene *scene
     hipSetDevice(gpu->device);
     size_t need = 0;
     HIP_TRY(tuned_spill_bytes(gpu, pass, *params, (uint32_t)n, &need), "occupancy");
-    const size_t have = SCRATCH_BYTES - SCRATCH_QUEUE_BYTES;
+    const size_t have = SCRATCH_SPILL_BYTES;
     if (need > have)
         return fail("lb_run_device_noalloc: this launch spills %zu bytes of traversal stack, the "
                     "static scratch holds %zu: keep more stack entries in LDS (lds_stack) or use "

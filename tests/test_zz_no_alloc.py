@@ -61,6 +61,24 @@ def test_noalloc_equals_run_device(gpu, setup, pass_, form):
         assert np.array_equal(kit.bits(got), kit.bits(want)), (pass_.name, n)
 
 
+def test_noalloc_full_residency(gpu, setup):
+    """The benchmark's gather settings (8 waves a SIMD, 32 lanes an item of 256 rays, 16 LDS
+    entries) over enough items to fill every CU: 524 288 resident lanes spilling 32 entries each
+    is the whole 64 MiB spill area, which the launch must take, not refuse."""
+    gpu.set_tuning(pass_=Pass.GATHER, workgroup=256, rays_per_lane=8, lds_stack=16, waves=8,
+                   schedule=0)
+    params = np.ascontiguousarray(setup["params"], dtype=N.PARAMS).copy()
+    params["counts"][0][2] = 256
+    rng = np.random.default_rng(29)
+    idx = rng.integers(0, len(setup["items"]), 20000)
+    items = np.ascontiguousarray(setup["items"][idx])
+    s = dict(setup, params=params)
+    print(f"no-alloc full residency: {len(items)} items x 256 rays", flush=True)
+    want = _run(gpu, s, "lb_run_device", Pass.GATHER, N.FORM_TUNED, items)
+    got = _run(gpu, s, "lb_run_device_noalloc", Pass.GATHER, N.FORM_TUNED, items)
+    assert np.array_equal(kit.bits(got), kit.bits(want))
+
+
 def test_noalloc_shallow_lds_stack(gpu, setup):
     """Eight entries in LDS: 40 spill a lane. The launch either fits the scratch and equals
     lb_run_device, or is refused with the scratch's size named, never run over it."""
