@@ -103,7 +103,7 @@ __device__ Hit trace_t(const SceneView &S, const lb_params &P, uint32_t root, f3
 // Inner visits between the lock-step loop's counts of its lanes (trace_locked): 3 took the map's
 // ranked gather launch from 10.84 to 10.23 ms and its direct pass from 11.40 to 10.73 ms (2 and
 // 4 gave less on the city). The gather's counting launch, whose rays are unranked and part
-// sooner, keeps a count every visit: 2 made it slower.
+// sooner, keeps a count every visit: 2 made it slower, 3 slower still (15.3 -> 17.3 ms).
 constexpr int LOCKED_VISITS = 3;
 
 // ---- group reductions -------------------------------------------------------------------------
