@@ -117,7 +117,8 @@ __device__ __forceinline__ bool leaf_tests(const SceneView &S, const lb_params &
         count = li.y;
     }
     if (STATS) cnt.leaves++;
-    for (uint32_t i = first; i < first + count; i++) {
+    uint32_t end = first + count;
+    for (uint32_t i = first; i < end; i++) {
         // one address, the three records at immediate offsets
         const float4 *tp = S.tris_t + (size_t)i * 3u;
         float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
@@ -135,11 +136,13 @@ __device__ __forceinline__ bool leaf_tests(const SceneView &S, const lb_params &
                 R.t = t;
                 R.tri = i;
                 R.back = det < 0.0f;
-                if (R.any) return true;
+                // an any-hit ray ends at its first accepted hit: the loop stops there (no
+                // return from inside it, which costs the loop's exec-mask bookkeeping)
+                if (R.any) end = i;
             }
         }
     }
-    return false;
+    return R.any && R.tri != NONE;
 }
 
 // Inner node R.n (trace.wgsl:105-123): true when the ray moved on to its near child (the far one
