@@ -406,6 +406,7 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     TRY(B->d_f4.reserve(Tn * 16));
     TRY(B->d_m1.reserve(Tn));
     TRY(B->d_m2.reserve(Tn));
+    TRY(B->d_ring.reserve(Tn));
     TRY(B->d_items.reserve(std::max<size_t>(B->ncov, 1) * sizeof(lb_texel)));
     TRY(B->d_res.reserve(std::max<size_t>(B->ncov, 1) * 16));
     TRY(B->d_light.reserve((size_t)aw * ah * 16));
@@ -420,6 +421,7 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
     TRY(put(B->d_wg, gauss((int)B->rg), st));
     B->rings = L.padding + 4;                  // bake.rs:243
     B->grings = L.padding + 2 * B->ds + 2;     // bake.rs:320
+    if (B->grings >= lbk::RING_NEVER) return fail("bake: too many fill rings for the bounce fill");
     if (B->npd) {
         TRY(put(B->d_pitems, pitems, st));
         TRY(B->d_pres.reserve(B->npd * 64));
@@ -455,18 +457,24 @@ int bake_prepare(lb_gpu *gpu, const lb_scene_desc &sc, const lb_bake_options &op
 
 namespace {
 
-// filter.rs:58-94 over `rings` rings into `v` (C channels, stride 1 or 4) from `valid0`
-void fill(lb_bake *B, int channels, float *v, uint32_t rings, const uint8_t *valid0,
-          uint8_t *final_mask) {
+// filter.rs:58-94 over grings rings into the indirect light from the gather set, as bake.rs:320
+// fills it after each bounce. Every bounce starts from the same mask, so the rings fill the same
+// texels: the first bounce files each texel's ring in d_ring, the later ones take each ring over
+// the texels filed for it (the same values bit for bit, without a pass over the unfilled texels).
+void fill_bounce(lb_bake *B, uint32_t k) {
     hipStream_t st = B->gpu->stream;
-    uint32_t T = (uint32_t)B->T;
+    float *v = (float *)B->V.indirect;
+    uint8_t *ring = (uint8_t *)B->d_ring.ptr;
+    if (k > 0) {
+        for (uint32_t r = 0; r < B->grings; r++) tp_fill_ring_filed(B->V, v, ring, r, st);
+        return;
+    }
     uint8_t *ma = (uint8_t *)B->d_m1.ptr, *mb = (uint8_t *)B->d_m2.ptr;
-    tp_copy_u8(valid0, ma, T, st);
-    for (uint32_t r = 0; r < rings; r++) {
-        tp_fill_ring(B->V, channels, v, ma, mb, st);
+    tp_copy_u8((const uint8_t *)B->d_gmask.ptr, ma, (uint32_t)B->T, st);
+    for (uint32_t r = 0; r < B->grings; r++) {
+        tp_fill_ring_file(B->V, v, ma, mb, ring, r, st);
         std::swap(ma, mb);
     }
-    if (final_mask) tp_copy_u8(ma, final_mask, T, st);
 }
 
 // the direct phase's three fills (bake.rs:253-255): the occlusion, the sun field and the direct
@@ -573,8 +581,7 @@ int bounce(lb_bake *B, lb_stats *acc, uint32_t k) {
                irr, st);
     tp_smooth(V, 4, (const float *)irr, (float *)V.indirect, (const uint8_t *)B->d_gmask.ptr,
               (int)B->rg, (const float *)B->d_wg.ptr, st);
-    fill(B, 3, (float *)V.indirect, B->grings,
-         (const uint8_t *)B->d_gmask.ptr, nullptr);
+    fill_bounce(B, k);
     tp_write_light(V, (float4 *)B->d_light.ptr, B->aw, st);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "bounce");

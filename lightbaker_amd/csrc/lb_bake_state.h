@@ -70,7 +70,8 @@ struct lb_bake {
     lbk::DevBuf d_tex0, d_tex, d_tile_of, d_tiles, d_chart, d_normal, d_valid, d_known, d_state,
         d_hard, d_soft, d_ao, d_field, d_direct, d_indirect, d_cov, d_items, d_res, d_light, d_w1,
         d_wg, d_f1, d_f4, d_m1, d_m2, d_covered, d_gat, d_gmask, d_gitems, d_gres, d_filled,
-        d_pitems, d_pres, d_counts, d_total, d_light0, d_cost, d_perm, d_rec, d_raster, d_owner;
+        d_pitems, d_pres, d_counts, d_total, d_light0, d_cost, d_perm, d_rec, d_raster, d_owner,
+        d_ring;
     // every tile's TileLight (bake.rs:61-69) in the cache order, the atlas' tile map, and the
     // finish's outputs
     lbk::DevBuf d_tl_known, d_tl_state, d_tl_field, d_tl_ao, d_tl_direct, d_tl_indirect, d_atiles,

@@ -73,6 +73,13 @@ void tp_sun_field(const TexelView &V, const uint8_t *covered, uint32_t g, float 
 void tp_direct_light(const TexelView &V, const float *soft, float3 irr, float3 sun, hipStream_t s);
 void tp_fill_ring(const TexelView &V, int channels, float *v, const uint8_t *was, uint8_t *now,
                   hipStream_t s);
+// the bounce fill over 3 channels at stride 4: ring r filing each texel's ring (RING_NEVER: never
+// filled), and ring r over the texels filed for it
+constexpr uint8_t RING_NEVER = 0xff;
+void tp_fill_ring_file(const TexelView &V, float *v, const uint8_t *was, uint8_t *now,
+                       uint8_t *ring, uint32_t r, hipStream_t s);
+void tp_fill_ring_filed(const TexelView &V, float *v, const uint8_t *ring, uint32_t r,
+                        hipStream_t s);
 // one ring of the occlusion (1 channel), sun field (1) and direct light (3, stride 4) together
 void tp_fill_ring_direct(const TexelView &V, float *ao, float *field, float *direct,
                          const uint8_t *was, uint8_t *now, hipStream_t s);

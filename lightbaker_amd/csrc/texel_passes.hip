@@ -220,6 +220,70 @@ __global__ void k_fill_ring(TexelView V, float *v0, float *v1, float *v2,
     now[g] = known;
 }
 
+// One texel's ring step in k_fill_ring's arithmetic (one array): the mean of its 3 x 3 neighbours
+// for which `was(j)` holds, its own chart's first; true when it took a value.
+template <int C, int S, typename Was>
+__device__ __forceinline__ bool fill_texel(const TexelView &V, float *v, uint32_t g, Was was) {
+    Where p = where(V, g);
+    FillArr<C, S> a;
+    a.v = v;
+    a.clear();
+    int no = 0, na = 0;
+    uint32_t ci = V.chart[g];
+    int y0 = max((int)p.y - 1, 0), y1 = min((int)p.y + 2, (int)p.h);
+    int x0 = max((int)p.x - 1, 0), x1 = min((int)p.x + 2, (int)p.w);
+    for (int yy = y0; yy < y1; yy++)
+        for (int xx = x0; xx < x1; xx++) {
+            uint32_t j = p.off + (uint32_t)yy * p.w + (uint32_t)xx;
+            if (!was(j)) continue;
+            if (ci != NONE && V.chart[j] == ci) {
+                a.add_own(j);
+                no++;
+            } else {
+                a.add_any(j);
+                na++;
+            }
+        }
+    if (no > 0) {
+        a.put_own(g, no);
+        return true;
+    }
+    if (na > 0 && ci == NONE) {
+        a.put_any(g, na);
+        return true;
+    }
+    return false;
+}
+
+// Which texels a ring fills follows from the starting mask alone, not from the values, and every
+// bounce fills the indirect light from the same mask (the gather set). So the first bounce's fill
+// files each texel's ring (0: known before the first ring, r + 1: filled by ring r, RING_NEVER:
+// never; ring 0 writes it for every texel, a later ring for the texels it fills), and the later
+// bounces' rings visit only their own texels, reading the neighbours known before them from the
+// filed rings: the same texels filled from the same neighbours in the same order, bit for bit.
+template <int C, int S>
+__global__ void k_fill_ring_file(TexelView V, float *v, const uint8_t *__restrict__ was,
+                                 uint8_t *__restrict__ now, uint8_t *__restrict__ ring, uint32_t r,
+                                 uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    uint8_t known = was[g];
+    if (!known && fill_texel<C, S>(V, v, g, [&](uint32_t j) { return was[j] != 0; })) {
+        known = 1;
+        ring[g] = (uint8_t)(r + 1u);
+    } else if (r == 0u) {
+        ring[g] = known ? 0 : RING_NEVER;
+    }
+    now[g] = known;
+}
+template <int C, int S>
+__global__ void k_fill_ring_filed(TexelView V, float *v, const uint8_t *__restrict__ ring,
+                                  uint32_t r, uint32_t n) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n || ring[g] != r + 1u) return;
+    fill_texel<C, S>(V, v, g, [&](uint32_t j) { return ring[j] <= r; });
+}
+
 // bake.rs:257-265: the state of covered texels filled from neighbours, and known = 2 own, 1 filled
 __global__ void k_known(TexelView V, const uint8_t *__restrict__ filled, uint32_t n) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -405,6 +469,18 @@ void tp_fill_ring(const TexelView &V, int channels, float *v, const uint8_t *was
     else
         hipLaunchKernelGGL((k_fill_ring<3, 4, 0, 1, 0, 1>), grid_for(V.n), dim3(256), 0, s, V, v,
                            no_array, no_array, was, now, V.n);
+}
+void tp_fill_ring_file(const TexelView &V, float *v, const uint8_t *was, uint8_t *now,
+                       uint8_t *ring, uint32_t r, hipStream_t s) {
+    if (V.n)
+        hipLaunchKernelGGL((k_fill_ring_file<3, 4>), grid_for(V.n), dim3(256), 0, s, V, v, was,
+                           now, ring, r, V.n);
+}
+void tp_fill_ring_filed(const TexelView &V, float *v, const uint8_t *ring, uint32_t r,
+                        hipStream_t s) {
+    if (V.n)
+        hipLaunchKernelGGL((k_fill_ring_filed<3, 4>), grid_for(V.n), dim3(256), 0, s, V, v, ring,
+                           r, V.n);
 }
 void tp_fill_ring_direct(const TexelView &V, float *ao, float *field, float *direct,
                          const uint8_t *was, uint8_t *now, hipStream_t s) {

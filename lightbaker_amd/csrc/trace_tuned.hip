@@ -100,8 +100,11 @@ __device__ Hit trace_t(const SceneView &S, const lb_params &P, uint32_t root, f3
 // Inner visits between the lock-step loop's counts of its lanes (trace_locked): 3 took the map's
 // ranked gather launch from 10.84 to 10.23 ms and its direct pass from 11.40 to 10.73 ms (2 and
 // 4 gave less on the city). The gather's counting launch, whose rays are unranked and part
-// sooner, keeps a count every visit: 2 made it slower, 3 slower still (15.3 -> 17.3 ms).
+// sooner, takes COUNTING_VISITS: with a break out of the loops on an ended ray 2 made it slower
+// and 3 slower still (15.3 -> 17.3 ms); with the ended ray's sentinel, 2 shortened the map's
+// bounces by 0.7 ms and the city's by 11 ms.
 constexpr int LOCKED_VISITS = 3;
+constexpr int COUNTING_VISITS = 2;
 
 // ---- group reductions -------------------------------------------------------------------------
 
@@ -593,7 +596,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_ga
                 sb.add(l.z);
             }
         };
-        run_slots<LAYOUT, STATS, MODE == 1 ? 1 : LOCKED_VISITS>(
+        run_slots<LAYOUT, STATS, MODE == 1 ? COUNTING_VISITS : LOCKED_VISITS>(
             A, st, lig, valid ? rays : 0u, (uint32_t)L, cnt, start, shade);
         group_ksum(sr, L);
         group_ksum(sg, L);
@@ -806,7 +809,8 @@ hipError_t tuned_spill_bytes(lb_gpu *gpu, int pass, const lb_params &P, uint32_t
 // chunk's highest lane of each cost then adds the chunk's count to the bin. An exclusive scan of
 // the bins gives each cost's first place. A wave's LDS requests complete in the order it issues
 // them and every wave keeps to its own bins, so no barrier is needed, only the compiler's
-// (`order_lds`) between the phases that read what other lanes wrote.
+// (`order_lds`) between the phases that read what other lanes wrote. With 64 rays or fewer an
+// item is one chunk and the ballots alone give each slot its place.
 constexpr uint32_t RANK_WAVES = 4;
 __device__ __forceinline__ void order_lds() { asm volatile("" ::: "memory"); }
 __global__ __launch_bounds__(RANK_WAVES * 64) void k_rank_slots(const uint8_t *__restrict__ cost,
@@ -819,6 +823,25 @@ __global__ __launch_bounds__(RANK_WAVES * 64) void k_rank_slots(const uint8_t *_
     for (uint32_t item = blockIdx.x * RANK_WAVES + w; item < n_items;
          item += gridDim.x * RANK_WAVES) {
         size_t base = (size_t)item * rays;
+        if (rays <= 64u) {
+            // one chunk: a lane's place is the count of the slots with a lower cost plus those
+            // below it with its own, both from the eight ballots, the cost's bits taken from the
+            // highest (no bins, no scan); slot s = lane l
+            bool v = l < rays;
+            uint32_t x = v ? (uint32_t)cost[base + l] : 0u;
+            uint64_t lt = 0ull, eq = __ballot(v);
+            for (int b = 7; b >= 0; b--) {
+                uint64_t m = __ballot((x >> b) & 1u);
+                if ((x >> b) & 1u) {
+                    lt |= eq & ~m;
+                    eq &= m;
+                } else {
+                    eq &= ~m;
+                }
+            }
+            if (v) perm[base + __popcll(lt) + __popcll(eq & below)] = (uint8_t)l;
+            continue;
+        }
         uint32_t c[4], place[4];
         for (uint32_t j = 0; j < 4u; j++) {
             uint32_t s = j * 64u + l;

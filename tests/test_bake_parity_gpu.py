@@ -107,8 +107,10 @@ def az_slots(total, rot):
 
 @pytest.mark.parametrize("nodes", [N.NODES_WIDE, N.NODES_QUANT], ids=["wide", "quant"])
 @pytest.mark.parametrize("gen,rays", [(lambda: reference_test_scene(8.0), None),
-                                      (lambda: small_scene(1), 100)],
-                         ids=["reference-test-scene", "small-scene-100-rays"])
+                                      (lambda: small_scene(1), 100),
+                                      (lambda: small_scene(1), 48)],
+                         ids=["reference-test-scene", "small-scene-100-rays",
+                              "small-scene-48-rays"])
 def test_ranked_gather_order(gpu, gen, rays, nodes):
     """The TUNED lock-step gather's first bounce files each ray's traversal steps (inner visits +
     triangle tests, 255 at most) by azimuth slot, and the later bounces take each item's slots in
