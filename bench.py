@@ -20,7 +20,8 @@ tiles' light is the cache), then a seeded subset of its tiles and their probe ch
 step is the device part of that rebake, every dirty tile's items in one list per pass.
 
 `--workload city` measures the 400 x 400 m city (lightbaker_amd.scenes.city_scene(4): 2 000
-buildings, 1.24 M triangles, 8 texels/m, 3 bounces) the same way, with the map's tuning; its CPU
+buildings, 1.24 M triangles, 8 texels/m, 3 bounces) the same way, with the autotuner's earlier
+picks on the city (`--autotune` runs it again); its CPU
 leg compares the PLAIN form and the TUNED form with quantized and FP32 nodes against the FP64
 tracer.
 
@@ -386,7 +387,7 @@ def main() -> int:
              "city": lambda: city_scene(4)}[args.workload]()
     if args.workload == "city" and not args.autotune:
         # one city bake is 15 G rays: the autotuner's hundred trial launches take minutes, so by
-        # default the map's picks are used
+        # default the picks it made on the city before are used
         args.no_autotune = True
     t_scene = time.perf_counter() - t_scene
     gpu = Gpu()
@@ -394,6 +395,12 @@ def main() -> int:
     # the autotuner's usual picks on this map, so a run without it uses the same kernels
     gpu.set_tuning(pass_=Pass.GATHER, rays_per_lane=8, waves=8, xcd_split=1)
     gpu.set_tuning(pass_=Pass.DIRECT, rays_per_lane=8, waves=8, leaf_batch=8)
+    if args.workload == "city":
+        # the autotuner's picks on the city, where they differ from the map's: a shallower LDS
+        # stack for the gather (its tree is taller, the spill rarer than the LDS it frees), a
+        # deeper one for the direct pass
+        gpu.set_tuning(pass_=Pass.GATHER, lds_stack=8, leaf_batch=16)
+        gpu.set_tuning(pass_=Pass.DIRECT, lds_stack=16)
     if args.xcd_split is not None:
         for p in Pass:
             gpu.set_tuning(pass_=p, xcd_split=args.xcd_split)
